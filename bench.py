@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Headline benchmark: points/sec of the global periodic Delaunay
+triangulation of 1,048,576 uniform points (box 1024 x 1024, cell size 1) on
+one MI355X, FP32-filtered predicates with FP64 fallback, with the all-FP64
+path timed beside it.
+
+A step is one DelaunayGPU.globalDelaunayTriangulation call (counting-sort
+cell list + one-ring kernel + the status readback of the C ABI) on points
+already resident in HBM.  The timed region covers exactly --steps calls,
+bracketed by device synchronisation.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_POINTS = 1 << 20
+BOX = 1024.0
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes (FP64 coordinates and int32 indices in HBM, C = N cells):
+#   whole triangulation 84 N + 12 C; one-ring kernel 52 B per point
+#   (16 B offset + 4 B index + 4 B cell start + 24 B ring entries + 4 B count).
+RING_BYTES_PER_POINT = 52.0
+TRI_BYTES = lambda n, c: 84.0 * n + 12.0 * c  # noqa: E731
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=N_POINTS)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-runs", type=int, default=3)
+    return ap.parse_args()
+
+
+def time_path(dg, pts, tri, num, steps, warmup, torch):
+    for _ in range(warmup):
+        dg.globalDelaunayTriangulation(pts, tri, num)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ring_ms, cell_ms = [], []
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        dg.globalDelaunayTriangulation(pts, tri, num)
+        st = dg.stats()
+        ring_ms.append(st["ms_ring"])
+        cell_ms.append(st["ms_cells"])
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev0.elapsed_time(ev1) / steps
+    return ms, wall, sum(ring_ms) / steps, sum(cell_ms) / steps, dg.stats()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from gpudelaunaytriangulation_amd import PREC_FP32F, PREC_FP64, inputs, rings
+    from gpudelaunaytriangulation_amd.delaunay import DelaunayGPU, PeriodicBoundaries
+
+    n = args.n
+    L = BOX if n == N_POINTS else float(np.sqrt(n))
+    xy = inputs.uniform_points(n, L, L, seed=2)
+    pts = torch.from_numpy(xy).cuda()
+    out = {}
+    for name, prec in (("fp32f", PREC_FP32F), ("fp64", PREC_FP64)):
+        dg = DelaunayGPU(n, 16, 1.0, PeriodicBoundaries(L, L))
+        dg.setPrecision(prec)
+        dg.ctx.set_config(profile=1)
+        tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device="cuda")
+        num = torch.zeros(n, dtype=torch.int32, device="cuda")
+        ms, wall, ring_ms, cell_ms, st = time_path(dg, pts, tri, num, args.steps, args.warmup, torch)
+        nb = tri.cpu().numpy().reshape(n, dg.MaxSize)
+        cnt = num.cpu().numpy()
+        out[name] = dict(ms=ms, wall=wall, ring_ms=ring_ms, cell_ms=cell_ms, st=st,
+                         hash=rings.ring_hash(nb, cnt), sum=int(cnt.sum()), nb=nb, cnt=cnt)
+        print(f"[bench] {name}: {ms:.3f} ms/step (one-ring {ring_ms:.3f} ms, cell list "
+              f"{cell_ms:.3f} ms), {n / ms * 1e3 / 1e9:.3f} Gpoints/s", file=sys.stderr, flush=True)
+
+    f32, f64 = out["fp32f"], out["fp64"]
+    identical = (f32["hash"] == f64["hash"] and np.array_equal(f32["nb"], f64["nb"])
+                 and np.array_equal(f32["cnt"], f64["cnt"]))
+    golden_ok = None
+    gpath = os.path.join(ROOT, "tests", "golden", "golden.json")
+    if n == N_POINTS and os.path.exists(gpath):
+        with open(gpath) as fh:
+            golden_ok = json.load(fh)["workloads"]["c2"]["ring_sha256"] == f32["hash"]
+
+    value = n / (f32["ms"] * 1e-3)
+    ring_bytes = RING_BYTES_PER_POINT * n
+    achieved = ring_bytes / (f32["ring_ms"] * 1e-3) / 1e9
+    st = f32["st"]
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        oracle.triangulate(xy, L, L, 1.0, 16, nthreads=threads)     # warm-up
+        times = []
+        for _ in range(args.cpu_runs):
+            t0 = time.perf_counter()
+            oracle.triangulate(xy, L, L, 1.0, 16, nthreads=threads)
+            times.append(time.perf_counter() - t0)
+        t = sorted(times)[len(times) // 2]
+        cpu = {"value": n / t, "unit": "points/s", "cores": threads, "kind": "port",
+               "sample": f"the same {n} points, full triangulation incl. cell list, "
+                         f"median of {args.cpu_runs} runs after 1 warm-up, "
+                         f"{threads} OpenMP threads (oracle/gdt_oracle.c)"}
+
+    line = {
+        "metric": "points/sec triangulated, 1M uniform: FP32-filtered vs FP64 GPU, topology exact",
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": f32["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64 fallback (coordinates fp64)",
+        "data": "synthetic: SplitMix64 uniform points, seed 2",
+        "config": {"workload": f"uniform periodic {n} points, box {L:g}x{L:g}, cell 1.0, "
+                               f"FP32-filtered global triangulation",
+                   "n": n, "box": L, "cell_size": 1.0, "block": dg.ctx.config().block},
+        "fp64_points_per_s": n / (f64["ms"] * 1e-3),
+        "fp64_ms_per_step": f64["ms"],
+        "fp32_over_fp64": f64["ms"] / f32["ms"],
+        "topology_identical_fp32_fp64": bool(identical),
+        "matches_golden_oracle_hash": golden_ok,
+        "sum_counts_eq_6N": f32["sum"] == 6 * n,
+        "fallback_fraction": st["fallbacks"] / max(1, st["predicates"]),
+        "predicates_per_point": st["predicates"] / n,
+        "kernel_ms": {"one_ring_fp32f": f32["ring_ms"], "one_ring_fp64": f64["ring_ms"],
+                      "cell_list": f32["cell_ms"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_one_ring<32>", "algorithmic_bytes_per_launch": ring_bytes},
+        "triangulation_roofline_frac": TRI_BYTES(n, n) / (f32["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

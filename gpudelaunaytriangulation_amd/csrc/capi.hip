@@ -1,0 +1,549 @@
+// capi.hip -- host side of the C ABI declared in include/gpudelaunay.h:
+// context (scratch buffers, set table, chunk tables), argument checking and
+// the stream-ordered sequences of kernel launches.
+#include "gdt_kernels.h"
+#include "gpudelaunay.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+namespace {
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n)
+    {
+        if (n <= cap && p) return hipSuccess;
+        if (p) { hipError_t e = hipFree(p); p = nullptr; cap = 0; if (e != hipSuccess) return e; }
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T));
+        if (e == hipSuccess) cap = n;
+        else p = nullptr;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct DevGuard {
+    int prev = -1;
+    explicit DevGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DevGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+struct gdt_context {
+    int device = 0;
+    gdt_config cfg = {256, 1, 1.0, 0};
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+
+    std::vector<GdtSet> sets;
+    int total_n = 0, total_keys = 0;
+    bool cells_valid = false;
+    DBuf<GdtSet> d_sets;
+    DBuf<GdtChunk> pch, kch, rch;
+    int npch = 0, nkch = 0, nrch = 0, rch_block = 0;
+
+    DBuf<uint32_t> pkey, prank, ccount, cstart, scell, scan_tmp, tcount, toff, mark, markscan;
+    DBuf<int> perm, slot_of, worklist, repair, tris;
+    DBuf<gdt_d2> off64;
+    DBuf<gdt_f2> off32;
+    DBuf<int> status;
+    DBuf<unsigned long long> counters;
+    int* h_status = nullptr;
+    unsigned long long* h_counters = nullptr;
+    gdt_stats stats;
+
+    GdtCells cells()
+    {
+        GdtCells c;
+        c.pkey = pkey.p; c.prank = prank.p; c.ccount = ccount.p; c.cstart = cstart.p;
+        c.perm = perm.p; c.slot_of = slot_of.p; c.off64 = off64.p; c.off32 = off32.p;
+        c.scell = scell.p; c.scan_tmp = scan_tmp.p;
+        return c;
+    }
+};
+
+#define GDT_CHECK(expr)                                   \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return GDT_ERR_DEVICE;      \
+    } while (0)
+
+static bool same_sets(const std::vector<GdtSet>& a, const std::vector<GdtSet>& b)
+{
+    if (a.size() != b.size()) return false;
+    return a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(GdtSet)) == 0;
+}
+
+static std::vector<GdtChunk> make_chunks(const std::vector<GdtSet>& sets, int per, bool keys)
+{
+    std::vector<GdtChunk> ch;
+    for (int s = 0; s < (int)sets.size(); ++s) {
+        const int n = keys ? sets[s].nkeys : sets[s].n;
+        for (int st = 0; st < n; st += per) ch.push_back(GdtChunk{s, st});
+    }
+    return ch;
+}
+
+template <class T>
+static hipError_t upload(DBuf<T>& buf, const std::vector<T>& v)
+{
+    hipError_t e = buf.reserve(v.size());
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+// Install a set table: upload it, build the chunk tables and size the buffers.
+static int configure(gdt_context* c, std::vector<GdtSet> sets)
+{
+    int c0 = 0, p0 = 0;
+    for (auto& s : sets) {
+        if (s.n <= 0 || s.xsize > 65535 || s.ysize > 65535 || s.xsize < 3 || s.ysize < 3)
+            return GDT_ERR_INVALID;
+        s.p0 = p0;
+        s.c0 = c0;
+        p0 += s.n;
+        if ((long long)c0 + s.nkeys > 0x7ffffff0LL) return GDT_ERR_INVALID;
+        c0 += s.nkeys;
+    }
+    if (same_sets(sets, c->sets) && c->d_sets.p) return GDT_OK;
+    c->cells_valid = false;
+    c->sets = sets;
+    c->total_n = p0;
+    c->total_keys = c0;
+    GDT_CHECK(upload(c->d_sets, c->sets));
+    auto pch = make_chunks(c->sets, GDT_BLOCK, false);
+    auto kch = make_chunks(c->sets, GDT_BLOCK, true);
+    GDT_CHECK(upload(c->pch, pch));
+    GDT_CHECK(upload(c->kch, kch));
+    c->npch = (int)pch.size();
+    c->nkch = (int)kch.size();
+    c->rch_block = 0;
+    const size_t N = (size_t)c->total_n, K = (size_t)c->total_keys;
+    GDT_CHECK(c->pkey.reserve(N));
+    GDT_CHECK(c->prank.reserve(N));
+    GDT_CHECK(c->perm.reserve(N));
+    GDT_CHECK(c->slot_of.reserve(N));
+    GDT_CHECK(c->off64.reserve(N));
+    GDT_CHECK(c->off32.reserve(N));
+    GDT_CHECK(c->scell.reserve(N));
+    GDT_CHECK(c->ccount.reserve(K));
+    GDT_CHECK(c->cstart.reserve(K + 1));
+    GDT_CHECK(c->scan_tmp.reserve(gdt_scan_tmp_words(std::max(K, N + 1))));
+    return GDT_OK;
+}
+
+static int ring_chunks(gdt_context* c)
+{
+    if (c->rch_block == c->cfg.block && c->rch.p) return GDT_OK;
+    auto rch = make_chunks(c->sets, c->cfg.block, false);
+    GDT_CHECK(upload(c->rch, rch));
+    c->nrch = (int)rch.size();
+    c->rch_block = c->cfg.block;
+    return GDT_OK;
+}
+
+static int single_set(gdt_context* c, int n, double lx, double ly, double cell_size)
+{
+    if (n < 4 || !(lx > 0.) || !(ly > 0.) || !(cell_size > 0.) || !std::isfinite(lx) ||
+        !std::isfinite(ly) || !std::isfinite(cell_size))
+        return GDT_ERR_INVALID;
+    std::vector<GdtSet> sets(1, gdt_make_set(0, n, 0, lx, ly, cell_size * c->cfg.cell_scale,
+                                             c->cfg.tiled));
+    return configure(c, sets);
+}
+
+static int reset_status(gdt_context* c, hipStream_t s)
+{
+    GDT_CHECK(gdt_fill_i32(c->status.p, 0, ST_WORDS, s));
+    GDT_CHECK(gdt_fill_u32(reinterpret_cast<uint32_t*>(c->counters.p), 0u, 2 * CT_WORDS, s));
+    return GDT_OK;
+}
+
+static int read_status(gdt_context* c, hipStream_t s)
+{
+    GDT_CHECK(hipMemcpyAsync(c->h_status, c->status.p, ST_WORDS * sizeof(int), hipMemcpyDeviceToHost, s));
+    GDT_CHECK(hipMemcpyAsync(c->h_counters, c->counters.p, CT_WORDS * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, s));
+    GDT_CHECK(hipStreamSynchronize(s));
+    return GDT_OK;
+}
+
+static int status_from_flags(int flags)
+{
+    if (flags & (ERR_NOCLOSE | ERR_NONEIGHBOUR)) return GDT_ERR_NONGENERIC;
+    if (flags & ERR_SEARCH) return GDT_ERR_SEARCH;
+    return GDT_OK;
+}
+
+static void record_ring_stats(gdt_context* c)
+{
+    c->stats.predicates = c->h_counters[CT_PRED];
+    c->stats.fallbacks = c->h_counters[CT_FALLBACK];
+    c->stats.neighbours = c->h_counters[CT_NBRSUM];
+    c->stats.max_ring = c->h_status[ST_REQUIRED];
+    if (!c->sets.empty()) {
+        c->stats.xsize = c->sets[0].xsize;
+        c->stats.ysize = c->sets[0].ysize;
+        c->stats.bsx = c->sets[0].bsx;
+        c->stats.bsy = c->sets[0].bsy;
+    }
+}
+
+static GdtRingIO ring_io(gdt_context* c, int* nbrs, int stride)
+{
+    GdtRingIO io;
+    io.cstart = c->cstart.p;
+    io.perm = c->perm.p;
+    io.off64 = c->off64.p;
+    io.off32 = c->off32.p;
+    io.scell = c->scell.p;
+    io.nbrs = nbrs;
+    io.stride = stride;
+    return io;
+}
+
+static int build_cells(gdt_context* c, const double* d_xy, hipStream_t s)
+{
+    if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[0], s));
+    GdtCells cl = c->cells();
+    GDT_CHECK(gdt_build_cells(c->d_sets.p, c->pch.p, c->npch, c->kch.p, c->nkch, c->total_n,
+                              c->total_keys, d_xy, cl, s));
+    if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
+    c->cells_valid = true;
+    return GDT_OK;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* gdt_status_string(int st)
+{
+    switch (st) {
+        case GDT_OK: return "ok";
+        case GDT_OVERFLOW: return "ring longer than the row stride";
+        case GDT_ERR_INVALID: return "invalid argument";
+        case GDT_ERR_DEVICE: return "HIP runtime error";
+        case GDT_ERR_NONGENERIC: return "non-generic input (a ring did not close)";
+        case GDT_ERR_EULER: return "not a triangulation of the torus (Euler check)";
+        case GDT_ERR_SEARCH: return "search radius exhausted (box too small)";
+        case GDT_ERR_NOMEM: return "out of memory";
+        default: return "unknown status";
+    }
+}
+
+int gdt_create(gdt_context** out, int device)
+{
+    if (!out) return GDT_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return GDT_ERR_DEVICE;
+    gdt_context* c = new (std::nothrow) gdt_context();
+    if (!c) return GDT_ERR_NOMEM;
+    c->device = device;
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    DevGuard g(device);
+    if (c->status.reserve(ST_WORDS) != hipSuccess || c->counters.reserve(CT_WORDS) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&c->h_status), ST_WORDS * sizeof(int), 0) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&c->h_counters), CT_WORDS * sizeof(unsigned long long), 0) != hipSuccess ||
+        hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess ||
+        hipEventCreate(&c->ev[2]) != hipSuccess) {
+        gdt_destroy(c);
+        return GDT_ERR_DEVICE;
+    }
+    *out = c;
+    return GDT_OK;
+}
+
+void gdt_destroy(gdt_context* c)
+{
+    if (!c) return;
+    DevGuard g(c->device);
+    (void)hipDeviceSynchronize();
+    c->d_sets.release(); c->pch.release(); c->kch.release(); c->rch.release();
+    c->pkey.release(); c->prank.release(); c->ccount.release(); c->cstart.release();
+    c->scell.release(); c->scan_tmp.release(); c->tcount.release(); c->toff.release();
+    c->mark.release(); c->markscan.release(); c->perm.release(); c->slot_of.release();
+    c->worklist.release(); c->repair.release(); c->tris.release(); c->off64.release();
+    c->off32.release(); c->status.release(); c->counters.release();
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->h_status) (void)hipHostFree(c->h_status);
+    if (c->h_counters) (void)hipHostFree(c->h_counters);
+    delete c;
+}
+
+int gdt_set_config(gdt_context* c, const gdt_config* cfg)
+{
+    if (!c || !cfg) return GDT_ERR_INVALID;
+    if (cfg->block < 64 || cfg->block > 1024 || cfg->block % 64 != 0) return GDT_ERR_INVALID;
+    if (!(cfg->cell_scale >= 0.25 && cfg->cell_scale <= 8.0)) return GDT_ERR_INVALID;
+    c->cfg = *cfg;
+    c->cfg.tiled = cfg->tiled ? 1 : 0;
+    c->sets.clear();          // force a new set table (cell size / order may change)
+    c->cells_valid = false;
+    c->rch_block = 0;
+    return GDT_OK;
+}
+
+int gdt_get_config(const gdt_context* c, gdt_config* cfg)
+{
+    if (!c || !cfg) return GDT_ERR_INVALID;
+    *cfg = c->cfg;
+    return GDT_OK;
+}
+
+int gdt_get_stats(const gdt_context* c, gdt_stats* out)
+{
+    if (!c || !out) return GDT_ERR_INVALID;
+    *out = c->stats;
+    return GDT_OK;
+}
+
+int gdt_celllist_build(gdt_context* c, const double* d_xy, int n, double lx, double ly,
+                       double cell_size, void* stream)
+{
+    if (!c || !d_xy) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = single_set(c, n, lx, ly, cell_size);
+    if (rc != GDT_OK) return rc;
+    return build_cells(c, d_xy, s);
+}
+
+int gdt_celllist_info(gdt_context* c, int* xsize, int* ysize, double* bsx, double* bsy,
+                      int* max_occupancy, void* stream)
+{
+    if (!c || !c->cells_valid || c->sets.size() != 1) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    hipStream_t s = (hipStream_t)stream;
+    const GdtSet& S = c->sets[0];
+    if (xsize) *xsize = S.xsize;
+    if (ysize) *ysize = S.ysize;
+    if (bsx) *bsx = S.bsx;
+    if (bsy) *bsy = S.bsy;
+    if (max_occupancy) {
+        GDT_CHECK(gdt_max_occupancy(c->ccount.p, c->total_keys, c->status.p + ST_WORDS - 1, s));
+        int h = 0;
+        GDT_CHECK(hipMemcpyAsync(&h, c->status.p + ST_WORDS - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        GDT_CHECK(hipStreamSynchronize(s));
+        *max_occupancy = h;
+    }
+    return GDT_OK;
+}
+
+int gdt_celllist_export(gdt_context* c, unsigned int* d_cell_sizes, int* d_idxs, int nmax,
+                        void* stream)
+{
+    if (!c || !c->cells_valid || c->sets.size() != 1 || !d_cell_sizes || !d_idxs || nmax < 1)
+        return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    GdtCells cl = c->cells();
+    GDT_CHECK(gdt_export_cells(c->d_sets.p, c->sets[0], cl, d_cell_sizes, d_idxs, nmax,
+                               (hipStream_t)stream));
+    return GDT_OK;
+}
+
+static int finish_rings(gdt_context* c, int stride, int* required, hipStream_t s)
+{
+    int rc = read_status(c, s);
+    if (rc != GDT_OK) return rc;
+    record_ring_stats(c);
+    const int req = c->h_status[ST_REQUIRED];
+    if (required) *required = req;
+    rc = status_from_flags(c->h_status[ST_ERRFLAGS]);
+    if (rc != GDT_OK) return rc;
+    return req > stride ? GDT_OVERFLOW : GDT_OK;
+}
+
+int gdt_triangulate(gdt_context* c, const double* d_xy, int n, double lx, double ly,
+                    double cell_size, int stride, int* d_nbrs, int* d_counts, int precision,
+                    int* required, void* stream)
+{
+    if (!c || !d_xy || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
+    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = single_set(c, n, lx, ly, cell_size);
+    if (rc != GDT_OK) return rc;
+    if ((rc = ring_chunks(c)) != GDT_OK) return rc;
+    if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
+    if ((rc = reset_status(c, s)) != GDT_OK) return rc;
+    if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
+    GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), c->rch.p, c->nrch,
+                                  c->cfg.block, nullptr, nullptr, 0, ring_io(c, d_nbrs, stride),
+                                  d_counts, c->status.p, c->counters.p, s));
+    if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[2], s));
+    rc = finish_rings(c, stride, required, s);
+    if (c->cfg.profile) {
+        float a = 0.f, b = 0.f;
+        GDT_CHECK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+        GDT_CHECK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+        c->stats.ms_cells = a;
+        c->stats.ms_ring = b;
+    }
+    return rc;
+}
+
+// Owned triangles into `tris` (capacity cap); count stays on the device at toff[N].
+static int enumerate(gdt_context* c, const int* d_nbrs, const int* d_counts, int stride,
+                     int* tris, int cap, hipStream_t s)
+{
+    GDT_CHECK(c->tcount.reserve((size_t)c->total_n));
+    GDT_CHECK(c->toff.reserve((size_t)c->total_n + 1));
+    GDT_CHECK(gdt_launch_triangles(c->d_sets.p, c->pch.p, c->npch, c->total_n, d_nbrs, d_counts,
+                                   stride, c->tcount.p, c->toff.p, tris, cap, c->scan_tmp.p,
+                                   c->counters.p, c->status.p, s));
+    return GDT_OK;
+}
+
+int gdt_triangles(gdt_context* c, const int* d_nbrs, const int* d_counts, int n, int stride,
+                  int* d_tris, int cap, int* ntri, long long* total_nbrs, void* stream)
+{
+    if (!c || !d_nbrs || !d_counts || !d_tris || stride < 3 || cap < 0) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    hipStream_t s = (hipStream_t)stream;
+    if (c->sets.empty() || c->total_n != n) {
+        // geometry is irrelevant for enumeration: use a unit-density placeholder set
+        const double L = std::sqrt((double)std::max(n, 16));
+        int rc = single_set(c, n, L, L, 1.0);
+        if (rc != GDT_OK) return rc;
+    }
+    int rc = reset_status(c, s);
+    if (rc != GDT_OK) return rc;
+    if ((rc = enumerate(c, d_nbrs, d_counts, stride, d_tris, cap, s)) != GDT_OK) return rc;
+    unsigned int h_nt = 0;
+    GDT_CHECK(hipMemcpyAsync(&h_nt, c->toff.p + c->total_n, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+    if ((rc = read_status(c, s)) != GDT_OK) return rc;
+    if (ntri) *ntri = (int)h_nt;
+    if (total_nbrs) *total_nbrs = (long long)c->h_counters[CT_INSUM];
+    return (int)h_nt > cap ? GDT_ERR_INVALID : GDT_OK;
+}
+
+int gdt_test_circumcircles(gdt_context* c, const double* d_xy, int n, const int* d_tris, int ntri,
+                           int* d_repair, int* nflagged, void* stream)
+{
+    if (!c || !d_xy || !d_tris || !d_repair || ntri < 0) return GDT_ERR_INVALID;
+    if (!c->cells_valid || c->total_n != n) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    hipStream_t s = (hipStream_t)stream;
+    GDT_CHECK(gdt_fill_i32(d_repair, -1, (size_t)n, s));
+    GDT_CHECK(gdt_fill_u32(reinterpret_cast<uint32_t*>(c->status.p + ST_NTRI), (uint32_t)ntri, 1, s));
+    GdtCells cl = c->cells();
+    GDT_CHECK(gdt_launch_test(c->d_sets.p, (int)c->sets.size(), d_tris,
+                              reinterpret_cast<const uint32_t*>(c->status.p + ST_NTRI), ntri,
+                              c->total_n, d_xy, cl, d_repair, s));
+    if (nflagged) {
+        GDT_CHECK(c->mark.reserve((size_t)n));
+        GDT_CHECK(c->markscan.reserve((size_t)n));
+        GDT_CHECK(c->worklist.reserve((size_t)n));
+        GDT_CHECK(gdt_launch_compact(d_repair, c->slot_of.p, n, c->mark.p, c->markscan.p,
+                                     c->worklist.p, reinterpret_cast<uint32_t*>(c->status.p + ST_NFLAGGED),
+                                     c->scan_tmp.p, s));
+        int h = 0;
+        GDT_CHECK(hipMemcpyAsync(&h, c->status.p + ST_NFLAGGED, sizeof(int), hipMemcpyDeviceToHost, s));
+        GDT_CHECK(hipStreamSynchronize(s));
+        *nflagged = h;
+        c->stats.flagged = h;
+    }
+    return GDT_OK;
+}
+
+// Compact the flags and run the one-ring kernel over the flagged slots.
+static int repair_flagged(gdt_context* c, const int* d_repair, int stride, int* d_nbrs,
+                          int* d_counts, int precision, hipStream_t s)
+{
+    const size_t n = (size_t)c->total_n;
+    GDT_CHECK(c->mark.reserve(n));
+    GDT_CHECK(c->markscan.reserve(n));
+    GDT_CHECK(c->worklist.reserve(n));
+    uint32_t* worklen = reinterpret_cast<uint32_t*>(c->status.p + ST_NFLAGGED);
+    GDT_CHECK(gdt_launch_compact(d_repair, c->slot_of.p, c->total_n, c->mark.p, c->markscan.p,
+                                 c->worklist.p, worklen, c->scan_tmp.p, s));
+    GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), nullptr, 0,
+                                  c->cfg.block, c->worklist.p, worklen, c->total_n,
+                                  ring_io(c, d_nbrs, stride), d_counts, c->status.p,
+                                  c->counters.p, s));
+    return GDT_OK;
+}
+
+int gdt_repair(gdt_context* c, const double* d_xy, int n, const int* d_repair, int stride,
+               int* d_nbrs, int* d_counts, int precision, int* required, void* stream)
+{
+    if (!c || !d_xy || !d_repair || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
+    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    if (!c->cells_valid || c->total_n != n) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = reset_status(c, s);
+    if (rc != GDT_OK) return rc;
+    if ((rc = repair_flagged(c, d_repair, stride, d_nbrs, d_counts, precision, s)) != GDT_OK) return rc;
+    rc = finish_rings(c, stride, required, s);
+    c->stats.flagged = c->h_status[ST_NFLAGGED];
+    return rc;
+}
+
+int gdt_test_and_repair(gdt_context* c, const double* d_xy, int n, double lx, double ly,
+                        double cell_size, int stride, int* d_nbrs, int* d_counts, int precision,
+                        int* nflagged, int* required, void* stream)
+{
+    if (!c || !d_xy || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
+    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = single_set(c, n, lx, ly, cell_size);
+    if (rc != GDT_OK) return rc;
+    const int cap = 2 * n + 64;
+    GDT_CHECK(c->tris.reserve(3 * (size_t)cap));
+    GDT_CHECK(c->repair.reserve((size_t)n));
+    if ((rc = reset_status(c, s)) != GDT_OK) return rc;
+    // 1. triangles of the current rings (getCircumcirclesGPU, DelaunayGPU.cpp:650-666)
+    if ((rc = enumerate(c, d_nbrs, d_counts, stride, c->tris.p, cap, s)) != GDT_OK) return rc;
+    // 2. cell list of the moved points (DelaunayGPU.cpp:580-586)
+    if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
+    // 3. empty-circle test (testTriangulation, DelaunayGPU.cpp:607-648)
+    GDT_CHECK(gdt_fill_i32(c->repair.p, -1, (size_t)n, s));
+    GdtCells cl = c->cells();
+    GDT_CHECK(gdt_launch_test(c->d_sets.p, (int)c->sets.size(), c->tris.p, c->toff.p + n, cap,
+                              c->total_n, d_xy, cl, c->repair.p, s));
+    // 4. repair the flagged points (locallyRepairDelaunayTriangulation, DelaunayGPU.cpp:87-115)
+    if ((rc = repair_flagged(c, c->repair.p, stride, d_nbrs, d_counts, precision, s)) != GDT_OK) return rc;
+    unsigned int h_nt = 0;
+    GDT_CHECK(hipMemcpyAsync(&h_nt, c->toff.p + n, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+    if ((rc = read_status(c, s)) != GDT_OK) return rc;
+    const int nf = c->h_status[ST_NFLAGGED];
+    if (nflagged) *nflagged = nf;
+    c->stats.flagged = nf;
+    c->stats.predicates = c->h_counters[CT_PRED];
+    c->stats.fallbacks = c->h_counters[CT_FALLBACK];
+    const int req = c->h_status[ST_REQUIRED];
+    if (required) *required = req;
+    // Euler check on the input triangulation (getCircumcirclesCPU, DelaunayGPU.cpp:701-705)
+    if (h_nt != (unsigned)(2 * n) || c->h_counters[CT_INSUM] != 6ULL * (unsigned long long)n)
+        return GDT_ERR_EULER;
+    rc = status_from_flags(c->h_status[ST_ERRFLAGS]);
+    if (rc != GDT_OK) return rc;
+    return req > stride ? GDT_OVERFLOW : GDT_OK;
+}
+
+}  // extern "C"

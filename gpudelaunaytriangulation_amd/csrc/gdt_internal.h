@@ -1,0 +1,210 @@
+// gdt_internal.h -- shared device-side definitions of the gfx950 triangulation
+// library: per-set geometry, cell keys, cell-local displacements and the
+// FP64 / certified-FP32 predicates.
+//
+// Coordinates are never subtracted globally.  Every point is stored as (cell,
+// offset inside the cell); the displacement of a candidate p from a point v is
+//     d = (dcx * bs - off_v) + off_p          (per axis)
+// with dcx the (unwrapped) cell distance.  The FP64 path evaluates exactly this
+// expression in FP64; the filtered path evaluates it in FP32 from offsets
+// rounded to FP32 and certifies every orientation / in-circle / distance
+// decision against a forward error bound that includes the difference between
+// the two inputs; undecided cases re-evaluate the FP64 expression with the same
+// operation order, so both paths take identical decisions.
+#pragma once
+
+#include <stdint.h>
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define GDT_HD __host__ __device__
+#else
+// Host-only build of the per-point routines (CPU emulation tests).
+#include <cmath>
+#include <algorithm>
+#define GDT_HD
+#endif
+
+#define GDT_BLOCK 256
+#define GDT_HARD_CAP 256      // ring walk cap: a longer ring means non-generic input
+
+// Device status word layout (gdt_status[...]).
+enum {
+    ST_REQUIRED = 0,     // largest ring size seen (atomicMax)
+    ST_ERRFLAGS = 1,     // bit 0: search radius exhausted, bit 1: ring did not close, bit 2: no neighbour
+    ST_NFLAGGED = 2,     // number of points flagged by the circumcircle test
+    ST_NTRI = 3,         // number of triangles enumerated
+    ST_WORDS = 8
+};
+enum { ERR_SEARCH = 1, ERR_NOCLOSE = 2, ERR_NONEIGHBOUR = 4 };
+// 64-bit counters (gdt_counters[...]).
+// CT_NBRSUM: sum of ring lengths written; CT_INSUM: sum of input counts seen by
+// the triangle enumeration (Euler check).
+enum { CT_PRED = 0, CT_FALLBACK = 1, CT_NBRSUM = 2, CT_INSUM = 3, CT_WORDS = 4 };
+
+// One independent periodic point set.  Points of set s occupy the range
+// [p0, p0+n) both in the caller's (original) order and in cell (sorted) order;
+// its cells occupy keys [c0, c0+nkeys).
+struct GdtSet {
+    int p0, n;
+    int c0, nkeys;
+    int xsize, ysize;
+    int tiles_x, tiled;
+    int rmax;            // largest Chebyshev search radius in cells
+    int pad0;
+    double lx, ly;       // box
+    double bsx, bsy;     // cell sides
+    float bsx32, bsy32;
+    float bsmax32, bsmin32;
+};
+
+struct GdtChunk { int set; int start; };   // a block's work: set id and first local item
+
+GDT_HD inline int gdt_cell_key(const GdtSet& s, int cx, int cy)
+{
+    if (s.tiled)
+        return ((((cy >> 3) * s.tiles_x) + (cx >> 3)) << 6) | ((cy & 7) << 3) | (cx & 7);
+    return cy * s.xsize + cx;
+}
+
+GDT_HD inline int gdt_nkeys(int xsize, int ysize, int tiled)
+{
+    if (!tiled) return xsize * ysize;
+    return ((xsize + 7) / 8) * ((ysize + 7) / 8) * 64;
+}
+
+// Offsets (dx,dy) of the i-th cell of the Chebyshev ring of radius r (8r cells).
+GDT_HD inline void gdt_ring_cell(int r, int i, int& dx, int& dy)
+{
+    if (r == 0) { dx = 0; dy = 0; return; }
+    const int side = i / (2 * r), t = i - side * 2 * r;
+    switch (side) {
+        case 0: dx = -r + t; dy = -r; break;
+        case 1: dx = r; dy = -r + t; break;
+        case 2: dx = r - t; dy = r; break;
+        default: dx = -r; dy = r - t; break;
+    }
+}
+
+GDT_HD inline int gdt_wrap(int x, int n)
+{
+    if (x < 0) x += n;
+    if (x >= n) x -= n;
+    return x;
+}
+
+// Set of a global point index (binary search over set starts).
+GDT_HD inline int gdt_set_of(const GdtSet* sets, int nsets, int p)
+{
+    int lo = 0, hi = nsets - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (sets[mid].p0 <= p) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Wrap a point into [0,lx) x [0,ly), find its cell and its FP64 offset inside
+// the cell.  Every kernel that needs a point's cell calls this one routine, so
+// all of them agree bit for bit.
+GDT_HD inline void gdt_bin_point(const GdtSet& S, double x, double y, int& cx, int& cy,
+                                 double& ox, double& oy)
+{
+    if (!(x >= 0. && x < S.lx)) {
+        x -= floor(x / S.lx) * S.lx;
+        if (x >= S.lx) x -= S.lx;
+        if (!(x >= 0.)) x = 0.;
+    }
+    if (!(y >= 0. && y < S.ly)) {
+        y -= floor(y / S.ly) * S.ly;
+        if (y >= S.ly) y -= S.ly;
+        if (!(y >= 0.)) y = 0.;
+    }
+    cx = (int)floor(x / S.bsx);
+    cy = (int)floor(y / S.bsy);
+    if (cx > S.xsize - 1) cx = S.xsize - 1;
+    if (cy > S.ysize - 1) cy = S.ysize - 1;
+    if (cx < 0) cx = 0;
+    if (cy < 0) cy = 0;
+    ox = x - (double)cx * S.bsx;
+    oy = y - (double)cy * S.bsy;
+}
+
+// Cell grid of a box (cellListGPU::setGridSize, cellListGPU.cpp:97-117): the
+// count per axis is floor(L/a) rounded up to even; sides are L/count per axis.
+GDT_HD inline void gdt_grid(double lx, double ly, double a, int& xsize, int& ysize)
+{
+    xsize = (int)floor(lx / a);
+    if (xsize % 2 == 1) xsize += 1;
+    ysize = (int)floor(ly / a);
+    if (ysize % 2 == 1) ysize += 1;
+    if (xsize < 2) xsize = 2;
+    if (ysize < 2) ysize = 2;
+}
+
+// Geometry of one set; c0 is its first cell key.  rmax keeps every searched
+// cell distinct on the torus (2*rmax+1 <= cells per axis).
+inline GdtSet gdt_make_set(int p0, int n, int c0, double lx, double ly, double cell_size, int tiled)
+{
+    GdtSet s;
+    s.p0 = p0; s.n = n; s.c0 = c0;
+    gdt_grid(lx, ly, cell_size, s.xsize, s.ysize);
+    s.tiled = tiled ? 1 : 0;
+    s.tiles_x = (s.xsize + 7) / 8;
+    s.nkeys = gdt_nkeys(s.xsize, s.ysize, s.tiled);
+    const int rx = (s.xsize - 1) / 2, ry = (s.ysize - 1) / 2;
+    s.rmax = rx < ry ? rx : ry;
+    s.pad0 = 0;
+    s.lx = lx; s.ly = ly;
+    s.bsx = lx / s.xsize;
+    s.bsy = ly / s.ysize;
+    s.bsx32 = (float)s.bsx;
+    s.bsy32 = (float)s.bsy;
+    s.bsmax32 = (float)(s.bsx > s.bsy ? s.bsx : s.bsy);
+    s.bsmin32 = (float)(s.bsx < s.bsy ? s.bsx : s.bsy);
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 displacement and predicates (one fixed operation order; the library is
+// compiled with -ffp-contract=off, and fused operations are written out).
+// ---------------------------------------------------------------------------
+GDT_HD inline double gdt_base64(int dc, double bs, double ov)
+{
+    return (double)dc * bs - ov;
+}
+GDT_HD inline double gdt_disp64(double base, double op) { return base + op; }
+
+// a x c  (> 0: c strictly left of the ray from the origin through a)
+GDT_HD inline double gdt_orient64(double ax, double ay, double cx, double cy)
+{
+    return ax * cy - ay * cx;
+}
+
+// In-circle coefficients of the circle through the origin, a and b:
+//   P(c) = K0 |c|^2 + Kx cx + Ky cy = K0 (|c - q|^2 - R^2),
+// q = -(Kx,Ky)/(2 K0) the centre: P < 0 iff c strictly inside (K0 > 0).
+struct Circ64 { double k0, kx, ky; };
+GDT_HD inline Circ64 gdt_circ64(double ax, double ay, double bx, double by)
+{
+    const double aa = ax * ax + ay * ay;
+    const double bb = bx * bx + by * by;
+    Circ64 c;
+    c.k0 = ax * by - ay * bx;
+    c.kx = bb * ay - aa * by;
+    c.ky = aa * bx - bb * ax;
+    return c;
+}
+GDT_HD inline double gdt_incircle64(const Circ64& k, double cx, double cy)
+{
+    const double cc = cx * cx + cy * cy;
+    return k.k0 * cc + (k.kx * cx + k.ky * cy);
+}
+GDT_HD inline double gdt_dist2_64(double x, double y) { return x * x + y * y; }
+
+// ---------------------------------------------------------------------------
+// FP32 filter constants.  u = 2^-24.  The FP32 displacement differs from the
+// FP64 one by at most EPS1*|d| + EPS0*bs per coordinate (rounding of the two
+// offsets, of dcx*bs and of the two sums); all bounds below include a factor 2
+// of slack for the rounding of the bound itself.
+// ---------------------------------------------------------------------------
+#define GDT_U32 5.9604644775390625e-08f

@@ -1,0 +1,60 @@
+// gdt_kernels.h -- host-side launchers of the gfx950 kernels (one per .hip file).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "gdt_point.h"
+
+// Arrays of the cell list (cell order), owned by the context.
+struct GdtCells {
+    uint32_t* pkey;     // [N] cell key of each caller-order point
+    uint32_t* prank;    // [N] arrival rank inside its cell
+    uint32_t* ccount;   // [K] points per key
+    uint32_t* cstart;   // [K+1] exclusive scan of ccount
+    int* perm;          // [N] slot -> caller index
+    int* slot_of;       // [N] caller index -> slot
+    gdt_d2* off64;      // [N]
+    gdt_f2* off32;      // [N]
+    uint32_t* scell;    // [N]
+    uint32_t* scan_tmp; // scan scratch (gdt_scan_tmp_words(K) words)
+};
+
+size_t gdt_scan_tmp_words(size_t n);
+// out[0..n) = exclusive prefix sums of in, *total = sum (device pointer).
+hipError_t gdt_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total,
+                        uint32_t* tmp, hipStream_t s);
+
+hipError_t gdt_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+hipError_t gdt_fill_i32(int* p, int v, size_t n, hipStream_t s);
+
+// Cell list over all sets.  pchunks: blocks of 256 points, kchunks: blocks of 256 keys.
+hipError_t gdt_build_cells(const GdtSet* d_sets, const GdtChunk* pchunks, int npch,
+                           const GdtChunk* kchunks, int nkch, int total_n, int total_keys,
+                           const double* d_xy, GdtCells& c, hipStream_t s);
+
+hipError_t gdt_export_cells(const GdtSet* d_sets, const GdtSet& h_set, const GdtCells& c,
+                            unsigned int* cell_sizes, int* idxs, int nmax, hipStream_t s);
+hipError_t gdt_max_occupancy(const uint32_t* ccount, int total_keys, int* d_out, hipStream_t s);
+
+// One-ring of every point (chunks of `block` points) or of the worklist slots.
+hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
+                               const GdtChunk* chunks, int nch, int block, const int* worklist,
+                               const uint32_t* worklen, int max_work, const GdtRingIO& io,
+                               int* counts, int* status, unsigned long long* counters,
+                               hipStream_t s);
+
+// Triangles owned by their smallest vertex, written at toff (deterministic).
+hipError_t gdt_launch_triangles(const GdtSet* d_sets, const GdtChunk* pchunks, int npch,
+                                int total_n, const int* nbrs, const int* counts, int stride,
+                                uint32_t* tcount, uint32_t* toff, int* tris, int cap,
+                                uint32_t* scan_tmp, unsigned long long* counters, int* status,
+                                hipStream_t s);
+
+// Empty-circle test of *ntri triangles; repair must be -1 on entry.
+hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, const uint32_t* ntri,
+                           int max_tri, int total_n, const double* d_xy, const GdtCells& c, int* repair,
+                           hipStream_t s);
+
+// Slots of the flagged points -> worklist; count -> *worklen (device).
+hipError_t gdt_launch_compact(const int* repair, const int* slot_of, int total_n, uint32_t* mark,
+                              uint32_t* markscan, int* worklist, uint32_t* worklen,
+                              uint32_t* scan_tmp, hipStream_t s);

@@ -1,0 +1,124 @@
+// onering.hip -- per-point Delaunay one-ring kernels (FP64 and FP32-filtered).
+//
+// One lane per point.  Lanes of a block take consecutive cell-ordered slots,
+// so a wavefront covers a compact patch of cells (8x8-cell tiles by default)
+// and the candidate offsets it reads come from the same few cache lines.
+// Replaces gpu_voronoi_calc_global_kernel + gpu_get_neighbors_global_kernel
+// (DelaunayGPU.cu:649-675, 981-1006) and their masked repair variants
+// (DelaunayGPU.cu:611-640, 945-975), which instead launch every point and
+// mask most of them out.
+#include "gdt_kernels.h"
+
+__device__ inline unsigned wave_sum_u32(unsigned v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline int wave_max_i32(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline int wave_or_i32(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int PREC>
+__device__ inline void one_ring_lane(bool active, const GdtSet& S, int j, const GdtRingIO& io,
+                                     int* __restrict__ counts, int* __restrict__ status,
+                                     unsigned long long* __restrict__ counters)
+{
+    GdtRingResult r = {0, 0, 0u, 0u};
+    if (active) {
+        r = (PREC == 32) ? gdt_one_ring_f32(S, j, io) : gdt_one_ring_f64(S, j, io);
+        counts[io.perm[j]] = r.count;
+    }
+    // every lane of the wave reaches the reductions below
+    const int mx = wave_max_i32(r.count);
+    const int er = wave_or_i32(r.err);
+    const unsigned np = wave_sum_u32(r.npred);
+    const unsigned nf = wave_sum_u32(r.nfb);
+    const unsigned ns = wave_sum_u32(active ? (unsigned)r.count : 0u);
+    if ((threadIdx.x & 63) == 0) {
+        if (mx > 0) atomicMax(&status[ST_REQUIRED], mx);
+        if (er) atomicOr(&status[ST_ERRFLAGS], er);
+        if (np) atomicAdd(&counters[CT_PRED], (unsigned long long)np);
+        if (nf) atomicAdd(&counters[CT_FALLBACK], (unsigned long long)nf);
+        if (ns) atomicAdd(&counters[CT_NBRSUM], (unsigned long long)ns);
+    }
+}
+
+// All points: block b handles chunk b (one set, blockDim consecutive slots).
+template <int PREC>
+__global__ __launch_bounds__(1024) void k_one_ring(const GdtSet* __restrict__ sets,
+                                                   const GdtChunk* __restrict__ chunks,
+                                                   GdtRingIO io, int* __restrict__ counts,
+                                                   int* __restrict__ status,
+                                                   unsigned long long* __restrict__ counters)
+{
+    const GdtChunk ch = chunks[blockIdx.x];
+    const GdtSet S = sets[ch.set];
+    const int li = ch.start + (int)threadIdx.x;
+    const bool active = li < S.n;
+    one_ring_lane<PREC>(active, S, S.p0 + li, io, counts, status, counters);
+}
+
+// Worklist of slots (count in device memory): grid-stride, set found per lane.
+template <int PREC>
+__global__ __launch_bounds__(1024) void k_one_ring_list(const GdtSet* __restrict__ sets, int nsets,
+                                                        const int* __restrict__ worklist,
+                                                        const uint32_t* __restrict__ worklen,
+                                                        GdtRingIO io, int* __restrict__ counts,
+                                                        int* __restrict__ status,
+                                                        unsigned long long* __restrict__ counters)
+{
+    const int len = (int)*worklen;
+    const int stride = gridDim.x * blockDim.x;
+    const int first = blockIdx.x * blockDim.x + threadIdx.x;
+    // uniform trip count per wave so the reductions see every lane
+    const int wave_first = first - (int)(threadIdx.x & 63);
+    for (int base = wave_first; base < len; base += stride) {
+        const int t = base + (int)(threadIdx.x & 63);
+        const bool active = t < len;
+        int j = 0, s = 0;
+        if (active) {
+            j = worklist[t];
+            s = gdt_set_of(sets, nsets, j);
+        }
+        const GdtSet S = sets[s];
+        one_ring_lane<PREC>(active, S, j, io, counts, status, counters);
+    }
+}
+
+hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
+                               const GdtChunk* chunks, int nch, int block, const int* worklist,
+                               const uint32_t* worklen, int max_work, const GdtRingIO& io,
+                               int* counts, int* status, unsigned long long* counters,
+                               hipStream_t s)
+{
+    if (!worklist) {
+        if (nch <= 0) return hipSuccess;
+        if (precision == 32)
+            hipLaunchKernelGGL(k_one_ring<32>, dim3(nch), dim3(block), 0, s, d_sets, chunks, io,
+                               counts, status, counters);
+        else
+            hipLaunchKernelGGL(k_one_ring<64>, dim3(nch), dim3(block), 0, s, d_sets, chunks, io,
+                               counts, status, counters);
+        return hipGetLastError();
+    }
+    if (max_work <= 0) return hipSuccess;
+    int grid = (max_work + block - 1) / block;
+    if (grid > 4096) grid = 4096;
+    if (precision == 32)
+        hipLaunchKernelGGL(k_one_ring_list<32>, dim3(grid), dim3(block), 0, s, d_sets, nsets,
+                           worklist, worklen, io, counts, status, counters);
+    else
+        hipLaunchKernelGGL(k_one_ring_list<64>, dim3(grid), dim3(block), 0, s, d_sets, nsets,
+                           worklist, worklen, io, counts, status, counters);
+    return hipGetLastError();
+}

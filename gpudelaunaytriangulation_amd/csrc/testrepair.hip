@@ -1,0 +1,219 @@
+// testrepair.hip -- triangle enumeration, empty-circle test and flag
+// compaction for the device-only test-and-repair pipeline.
+//
+// Replaces gpu_get_circumcircles_kernel (DelaunayGPU.cu:1008-1028), which
+// appends triangles through one global atomic counter (order varies run to
+// run), by a count + exclusive scan + write (deterministic order), and
+// gpu_test_circumcircles_kernel (DelaunayGPU.cu:290-313).
+#include "gdt_kernels.h"
+
+__global__ __launch_bounds__(GDT_BLOCK) void k_tri_count(const GdtSet* __restrict__ sets,
+                                                         const GdtChunk* __restrict__ chunks,
+                                                         const int* __restrict__ nbrs,
+                                                         const int* __restrict__ counts, int stride,
+                                                         uint32_t* __restrict__ tcount,
+                                                         unsigned long long* __restrict__ counters,
+                                                         int* __restrict__ status)
+{
+    const GdtChunk ch = chunks[blockIdx.x];
+    const GdtSet S = sets[ch.set];
+    const int li = ch.start + (int)threadIdx.x;
+    unsigned cnt = 0, tot = 0;
+    int bad = 0;
+    if (li < S.n) {
+        const int i = S.p0 + li;
+        int c = counts[i];
+        tot = c > 0 ? (unsigned)c : 0u;
+        if (c > stride || c < 3) { bad = 1; c = c > stride ? stride : (c < 0 ? 0 : c); }
+        if (c > 0) {
+            const int* row = nbrs + (size_t)i * stride;
+            int prev = row[c - 1];
+            for (int m = 0; m < c; ++m) {
+                const int cur = row[m];
+                cnt += (i < prev && i < cur) ? 1u : 0u;
+                prev = cur;
+            }
+        }
+        tcount[i] = cnt;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        tot += __shfl_xor(tot, o, 64);
+        bad |= __shfl_xor(bad, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (tot) atomicAdd(&counters[CT_INSUM], (unsigned long long)tot);
+        if (bad) atomicOr(&status[ST_ERRFLAGS], ERR_NOCLOSE);
+    }
+}
+
+__global__ __launch_bounds__(GDT_BLOCK) void k_tri_write(const GdtSet* __restrict__ sets,
+                                                         const GdtChunk* __restrict__ chunks,
+                                                         const int* __restrict__ nbrs,
+                                                         const int* __restrict__ counts, int stride,
+                                                         const uint32_t* __restrict__ toff,
+                                                         int* __restrict__ tris, int cap)
+{
+    const GdtChunk ch = chunks[blockIdx.x];
+    const GdtSet S = sets[ch.set];
+    const int li = ch.start + (int)threadIdx.x;
+    if (li >= S.n) return;
+    const int i = S.p0 + li;
+    int c = counts[i];
+    if (c > stride) c = stride;
+    if (c <= 0) return;
+    const int* row = nbrs + (size_t)i * stride;
+    uint32_t t = toff[i];
+    int prev = row[c - 1];
+    for (int m = 0; m < c; ++m) {
+        const int cur = row[m];
+        if (i < prev && i < cur) {
+            if (t < (uint32_t)cap) {
+                tris[3 * (size_t)t] = i;
+                tris[3 * (size_t)t + 1] = prev;
+                tris[3 * (size_t)t + 2] = cur;
+            }
+            ++t;
+        }
+        prev = cur;
+    }
+}
+
+hipError_t gdt_launch_triangles(const GdtSet* d_sets, const GdtChunk* pchunks, int npch,
+                                int total_n, const int* nbrs, const int* counts, int stride,
+                                uint32_t* tcount, uint32_t* toff, int* tris, int cap,
+                                uint32_t* scan_tmp, unsigned long long* counters, int* status,
+                                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tri_count, dim3(npch), dim3(GDT_BLOCK), 0, s, d_sets, pchunks, nbrs,
+                       counts, stride, tcount, counters, status);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = gdt_scan_u32(tcount, toff, (size_t)total_n, toff + total_n, scan_tmp, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tri_write, dim3(npch), dim3(GDT_BLOCK), 0, s, d_sets, pchunks, nbrs,
+                       counts, stride, toff, tris, cap);
+    return hipGetLastError();
+}
+
+// Empty-circle test.  The circle through the triangle (circumcentre q, radius
+// R, both FP64 relative to the first vertex) is scanned by Chebyshev rings of
+// cells around q's cell until the covered square contains the whole circle.
+// The test is conservative: a point within relative 1e-9 of the circle counts
+// as inside, and a circle that cannot be covered is reported, so no invalid
+// triangle escapes (a flagged valid one is merely re-triangulated to itself).
+__global__ __launch_bounds__(GDT_BLOCK) void k_test(const GdtSet* __restrict__ sets, int nsets,
+                                                    const int* __restrict__ tris,
+                                                    const uint32_t* __restrict__ ntri, int cap,
+                                                    int total_n, const gdt_d2* __restrict__ xy,
+                                                    const uint32_t* __restrict__ cstart,
+                                                    const int* __restrict__ slot_of,
+                                                    const int* __restrict__ perm,
+                                                    const gdt_d2* __restrict__ off64,
+                                                    int* __restrict__ repair)
+{
+    const int nt = min((int)*ntri, cap);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+        const int i = tris[3 * (size_t)t], a = tris[3 * (size_t)t + 1], b = tris[3 * (size_t)t + 2];
+        if ((unsigned)i >= (unsigned)total_n || (unsigned)a >= (unsigned)total_n ||
+            (unsigned)b >= (unsigned)total_n)
+            continue;   // malformed ring entry: the Euler check reports it
+        const GdtSet S = sets[gdt_set_of(sets, nsets, i)];
+        const gdt_d2 v = xy[i], pa = xy[a], pb = xy[b];
+        double ax = pa.x - v.x, ay = pa.y - v.y, bx = pb.x - v.x, by = pb.y - v.y;
+        ax -= S.lx * rint(ax / S.lx);
+        ay -= S.ly * rint(ay / S.ly);
+        bx -= S.lx * rint(bx / S.lx);
+        by -= S.ly * rint(by / S.ly);
+        const double D = ax * by - ay * bx;
+        bool bad = !(D > 0.);
+        int intruder = -1;
+        if (!bad) {
+            const double aa = ax * ax + ay * ay, bb = bx * bx + by * by;
+            const double qx = (aa * by - bb * ay) / (2. * D);
+            const double qy = (bb * ax - aa * bx) / (2. * D);
+            const double R2 = qx * qx + qy * qy;
+            const double lim = R2 * (1. + 1e-9);
+            int qcx, qcy;
+            double oqx, oqy;
+            gdt_bin_point(S, v.x + qx, v.y + qy, qcx, qcy, oqx, oqy);
+            const double covb = fmin(fmin(oqx, S.bsx - oqx), fmin(oqy, S.bsy - oqy));
+            const double bsmin = fmin(S.bsx, S.bsy);
+            const int si = slot_of[i], sa = slot_of[a], sb = slot_of[b];
+            bool covered = false;
+            for (int r = 0; r <= S.rmax && intruder < 0; ++r) {
+                const int ncell = r ? 8 * r : 1;
+                for (int q = 0; q < ncell && intruder < 0; ++q) {
+                    int dx, dy;
+                    gdt_ring_cell(r, q, dx, dy);
+                    const int x = gdt_wrap(qcx + dx, S.xsize), y = gdt_wrap(qcy + dy, S.ysize);
+                    const int key = S.c0 + gdt_cell_key(S, x, y);
+                    const uint32_t cb = cstart[key], ce = cstart[key + 1];
+                    const double ebx = gdt_base64(dx, S.bsx, oqx), eby = gdt_base64(dy, S.bsy, oqy);
+                    for (uint32_t k = cb; k < ce; ++k) {
+                        if ((int)k == si || (int)k == sa || (int)k == sb) continue;
+                        const gdt_d2 o = off64[k];
+                        const double ex = gdt_disp64(ebx, o.x), ey = gdt_disp64(eby, o.y);
+                        if (ex * ex + ey * ey < lim) { intruder = (int)k; break; }
+                    }
+                }
+                const double cov = (double)r * bsmin + covb;
+                if (intruder < 0 && cov > 0. && cov * cov >= R2 * (1. + 4e-9)) { covered = true; break; }
+            }
+            bad = intruder >= 0 || !covered;
+        }
+        if (bad) {
+            repair[i] = i;
+            repair[a] = a;
+            repair[b] = b;
+            if (intruder >= 0) {
+                const int p = perm[intruder];
+                repair[p] = p;
+            }
+        }
+    }
+}
+
+hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, const uint32_t* ntri,
+                           int max_tri, int total_n, const double* d_xy, const GdtCells& c, int* repair,
+                           hipStream_t s)
+{
+    if (max_tri <= 0) return hipSuccess;
+    int grid = (max_tri + GDT_BLOCK - 1) / GDT_BLOCK;
+    if (grid > 8192) grid = 8192;
+    hipLaunchKernelGGL(k_test, dim3(grid), dim3(GDT_BLOCK), 0, s, d_sets, nsets, tris, ntri,
+                       max_tri, total_n, reinterpret_cast<const gdt_d2*>(d_xy), c.cstart, c.slot_of, c.perm, c.off64,
+                       repair);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(GDT_BLOCK) void k_mark(const int* __restrict__ repair, int n,
+                                                    uint32_t* __restrict__ mark)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) mark[i] = repair[i] >= 0 ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(GDT_BLOCK) void k_compact(const int* __restrict__ repair, int n,
+                                                       const uint32_t* __restrict__ pos,
+                                                       const int* __restrict__ slot_of,
+                                                       int* __restrict__ worklist)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && repair[i] >= 0) worklist[pos[i]] = slot_of[i];
+}
+
+hipError_t gdt_launch_compact(const int* repair, const int* slot_of, int total_n, uint32_t* mark,
+                              uint32_t* markscan, int* worklist, uint32_t* worklen,
+                              uint32_t* scan_tmp, hipStream_t s)
+{
+    const int g = (total_n + GDT_BLOCK - 1) / GDT_BLOCK;
+    hipLaunchKernelGGL(k_mark, dim3(g), dim3(GDT_BLOCK), 0, s, repair, total_n, mark);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = gdt_scan_u32(mark, markscan, (size_t)total_n, worklen, scan_tmp, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_compact, dim3(g), dim3(GDT_BLOCK), 0, s, repair, total_n, markscan,
+                       slot_of, worklist);
+    return hipGetLastError();
+}

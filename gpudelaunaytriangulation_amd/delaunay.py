@@ -1,0 +1,388 @@
+"""Python mirror of the reference's host API over the gfx950 C ABI.
+
+Names, arguments and error behaviour follow the reference classes
+``DelaunayGPU`` (src/model/DelaunayGPU.h:18-130), ``cellListGPU``
+(src/utilities/cellListGPU.h:17-121) and ``periodicBoundaries``
+(src/utilities/periodicBoundaries.h:25-73).  Arrays are torch tensors on a
+HIP device, standing in for the reference's ``GPUArray``:
+
+* points            float64 [N, 2]        (GPUArray<double2>)
+* GPUTriangulation  int32   [N*MaxSize]   (GPUArray<int>, read as [N][MaxSize])
+* cellNeighborNum   int32   [N]
+
+As in the reference, the triangulation call may grow ``MaxSize`` and resize
+the caller's ``GPUTriangulation`` in place (DelaunayGPU.cpp:130, 154).
+Unlike the reference, ring overflow is always detected, whatever
+``setSafetyMode`` says (the reference silently truncates when it is off,
+DelaunayGPU.cu:874-879).  There is no CPU mode: ``setGPUcompute(False)``
+raises, because the reference's OpenMP path is not part of this library.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+
+try:
+    import torch
+except ImportError:  # pragma: no cover - torch is part of the image
+    torch = None
+
+
+def _stream_ptr(stream=None) -> int:
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return int(stream.cuda_stream)
+
+
+def _dev_ptr(t, dtype, name: str) -> int:
+    if torch is None or not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must live on a HIP device")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return int(t.data_ptr())
+
+
+class PeriodicBoundaries:
+    """Rectangular periodic box (periodicBoundaries::setSquare, periodicBoundaries.h:75-84)."""
+
+    def __init__(self, x: float = 1.0, y: float | None = None):
+        self.setSquare(x, x if y is None else y)
+
+    def setSquare(self, x: float, y: float) -> None:
+        if not (x > 0 and y > 0):
+            raise ValueError("box sides must be positive")
+        self.x11, self.x22 = float(x), float(y)
+
+    def getBoxDims(self):
+        return self.x11, 0.0, 0.0, self.x22
+
+    def isBoxSquare(self) -> bool:
+        return True
+
+    def minDist(self, p1, p2):
+        """Nearest-image displacement p1 - p2 (periodicBoundaries.h:154-164)."""
+        d = np.asarray(p1, dtype=np.float64) - np.asarray(p2, dtype=np.float64)
+        L = np.array([self.x11, self.x22])
+        return d - L * np.round(d / L)
+
+    def putInBoxReal(self, p):
+        """Wrap into [0,Lx) x [0,Ly) (periodicBoundaries.h:131-152)."""
+        p = np.asarray(p, dtype=np.float64)
+        L = np.array([self.x11, self.x22])
+        q = p - np.floor(p / L) * L
+        return np.where(q >= L, q - L, q)
+
+
+PeriodicBoxPtr = PeriodicBoundaries
+
+
+class Context:
+    """Owns one gdt_context (device scratch buffers) on a HIP device."""
+
+    def __init__(self, device: int = 0):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(L.gdt_create(ctypes.byref(h), int(device)), "gdt_create")
+        self._h = h
+        self.device = int(device)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.load().gdt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def config(self) -> _lib.Config:
+        c = _lib.Config()
+        _lib.check(_lib.load().gdt_get_config(self._h, ctypes.byref(c)), "gdt_get_config")
+        return c
+
+    def set_config(self, block: int | None = None, tiled: int | None = None,
+                   cell_scale: float | None = None, profile: int | None = None) -> None:
+        c = self.config()
+        if profile is not None:
+            c.profile = int(profile)
+        if block is not None:
+            c.block = int(block)
+        if tiled is not None:
+            c.tiled = int(tiled)
+        if cell_scale is not None:
+            c.cell_scale = float(cell_scale)
+        _lib.check(_lib.load().gdt_set_config(self._h, ctypes.byref(c)), "gdt_set_config")
+
+    def stats(self) -> dict:
+        s = _lib.Stats()
+        _lib.check(_lib.load().gdt_get_stats(self._h, ctypes.byref(s)), "gdt_get_stats")
+        return s.as_dict()
+
+
+class CellListGPU:
+    """Cell list over a periodic box (cellListGPU, cellListGPU.h:17-121).
+
+    The list is a counting sort (cell-ordered arrays); ``cell_sizes`` and the
+    padded ``idxs`` of the reference layout are produced on request."""
+
+    def __init__(self, box: PeriodicBoundaries | None = None, context: Context | None = None,
+                 device: int = 0):
+        self.Box = box if box is not None else PeriodicBoundaries(1.0)
+        self.ctx = context if context is not None else Context(device)
+        self.Np = 0
+        self.a = 1.0
+        self.xsize = self.ysize = 0
+        self.boxsize = 0.0
+        self.Nmax = 0
+        self._cell_sizes = None
+        self._idxs = None
+
+    def setNp(self, nn: int) -> None:
+        self.Np = int(nn)
+
+    def setBox(self, bx: PeriodicBoundaries) -> None:
+        self.Box = bx
+
+    def setGridSize(self, a: float) -> None:
+        """cellListGPU::setGridSize (cellListGPU.cpp:97-117)."""
+        x11, _, _, x22 = self.Box.getBoxDims()
+        self.a = float(a)
+        self.xsize = int(math.floor(x11 / a))
+        self.xsize += self.xsize % 2
+        self.ysize = int(math.floor(x22 / a))
+        self.ysize += self.ysize % 2
+        self.boxsize = x11 / self.xsize
+
+    def getNmax(self) -> int:
+        return self.Nmax
+
+    def getXsize(self) -> int:
+        return self.xsize
+
+    def getYsize(self) -> int:
+        return self.ysize
+
+    def getBoxsize(self) -> float:
+        return self.boxsize
+
+    def positionToCellIndex(self, x: float, y: float) -> int:
+        """cellListGPU.cpp:183-189."""
+        bsy = self.Box.getBoxDims()[3] / self.ysize
+        bx = max(0, min(self.xsize - 1, int(math.floor(x / self.boxsize))))
+        by = max(0, min(self.ysize - 1, int(math.floor(y / bsy))))
+        return by * self.xsize + bx
+
+    def getCellNeighbors(self, cellIndex: int, width: int) -> list[int]:
+        """All cells within `width` of a cell (cellListGPU.cpp:196-212, row index
+        taken as cellIndex // xsize)."""
+        w = min(width, self.xsize // 2)
+        cx, cy = cellIndex % self.xsize, cellIndex // self.xsize
+        return [((cy + i) % self.ysize) * self.xsize + (cx + j) % self.xsize
+                for i in range(-w, w + 1) for j in range(-w, w + 1)]
+
+    def getCellShellNeighbors(self, cellIndex: int, width: int) -> list[int]:
+        """Cells on the square shell at distance `width` (cellListGPU.cpp:221-237)."""
+        w = min(width, self.xsize)
+        cx, cy = cellIndex % self.xsize, cellIndex // self.xsize
+        return [((cy + i) % self.ysize) * self.xsize + (cx + j) % self.xsize
+                for i in range(-w, w + 1) for j in range(-w, w + 1)
+                if i in (-w, w) or j in (-w, w)]
+
+    def computeGPU(self, points, stream=None) -> None:
+        """Bin the points (cellListGPU::computeGPU(points), cellListGPU.cpp:404-460)."""
+        L = _lib.load()
+        n = int(points.shape[0])
+        x11, _, _, x22 = self.Box.getBoxDims()
+        if self.xsize == 0:
+            self.setGridSize(self.a)
+        _lib.check(L.gdt_celllist_build(self.ctx.handle, _dev_ptr(points, torch.float64, "points"),
+                                        n, x11, x22, self.a, _stream_ptr(stream)),
+                   "gdt_celllist_build")
+        self.Np = n
+        xs, ys, mo = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        bx, by = ctypes.c_double(), ctypes.c_double()
+        _lib.check(L.gdt_celllist_info(self.ctx.handle, ctypes.byref(xs), ctypes.byref(ys),
+                                       ctypes.byref(bx), ctypes.byref(by), ctypes.byref(mo),
+                                       _stream_ptr(stream)), "gdt_celllist_info")
+        self.xsize, self.ysize, self.boxsize = xs.value, ys.value, bx.value
+        self.Nmax = max(self.Nmax, mo.value)
+        self._cell_sizes = self._idxs = None
+
+    # The reference's CPU binning (cellListGPU.cpp:289-330) yields the same
+    # buckets in the same order; here both names run the device build.
+    compute = computeGPU
+
+    def _export(self):
+        L = _lib.load()
+        dev = torch.device("cuda", self.ctx.device)
+        nc = self.xsize * self.ysize
+        nmax = max(1, self.Nmax)
+        self._cell_sizes = torch.zeros(nc, dtype=torch.int32, device=dev)
+        self._idxs = torch.zeros(nc * nmax, dtype=torch.int32, device=dev)
+        _lib.check(L.gdt_celllist_export(self.ctx.handle, self._cell_sizes.data_ptr(),
+                                         self._idxs.data_ptr(), nmax, _stream_ptr()),
+                   "gdt_celllist_export")
+
+    @property
+    def cell_sizes(self):
+        """Points per cell, cells row-major (cell_indexer(cx,cy) = cy*xsize+cx)."""
+        if self._cell_sizes is None:
+            self._export()
+        return self._cell_sizes
+
+    @property
+    def idxs(self):
+        """Padded bucket contents: idxs[cell*Nmax + k] (cell_list_indexer)."""
+        if self._idxs is None:
+            self._export()
+        return self._idxs
+
+    def getCellSizeArray(self):
+        return self.cell_sizes
+
+    def getIdxArray(self):
+        return self.idxs
+
+
+class DelaunayGPU:
+    """Localized periodic Delaunay triangulation (DelaunayGPU, DelaunayGPU.h:18-130)."""
+
+    def __init__(self, N: int | None = None, maximumNeighborsGuess: int = 16,
+                 cellSize: float = 1.0, bx: PeriodicBoundaries | None = None, device: int = 0):
+        self.ctx = Context(device)
+        self.device = int(device)
+        self.safetyMode = False
+        self.GPUcompute = True
+        self.OMPThreadsNum = 1
+        self.precision = _lib.PREC_FP64
+        self.Ncells = 0
+        self.MaxSize = 4
+        self.cellsize = 1.10           # reference default (DelaunayGPU.cpp:7)
+        self.Box = PeriodicBoundaries(1.0)
+        self.cList = CellListGPU(self.Box, self.ctx)
+        self.last_repaired = 0
+        if N is not None:
+            self.initialize(N, maximumNeighborsGuess, cellSize, bx)
+
+    # --- set-up (DelaunayGPU.cpp:16-62) -----------------------------------
+    def initialize(self, N: int, maximumNeighborsGuess: int, cellSize: float,
+                   bx: PeriodicBoundaries | None) -> None:
+        self.Ncells = int(N)
+        self.MaxSize = max(4, int(maximumNeighborsGuess))
+        self.cellsize = float(cellSize)
+        if bx is not None:
+            self.setBox(bx)
+        self._init_cell_list()
+
+    def _init_cell_list(self) -> None:
+        self.cList.setNp(self.Ncells)
+        self.cList.setBox(self.Box)
+        self.cList.setGridSize(self.cellsize)
+
+    def resize(self, nmax: int) -> None:
+        self.MaxSize = int(nmax)
+
+    def setCellListSize(self, csize: float) -> None:
+        self.cellsize = float(csize)
+        self._init_cell_list()
+
+    def updateList(self, points) -> None:
+        self.cList.computeGPU(points)
+
+    def setBox(self, bx: PeriodicBoundaries) -> None:
+        self.Box = bx
+        self.cList.setBox(bx)
+
+    def setSafetyMode(self, sm: bool) -> None:
+        self.safetyMode = bool(sm)
+
+    def setGPUcompute(self, flag: bool) -> None:
+        if not flag:
+            raise NotImplementedError("CPU mode is not part of this library (GPU only)")
+        self.GPUcompute = True
+
+    def setOMPthreads(self, num: int) -> None:
+        self.OMPThreadsNum = int(num)
+
+    def setPrecision(self, precision: int) -> None:
+        """64: all-FP64 predicates; 32: FP32 with certified bound + FP64 fallback."""
+        if precision not in (_lib.PREC_FP64, _lib.PREC_FP32F):
+            raise ValueError("precision must be 64 or 32")
+        self.precision = int(precision)
+
+    def stats(self) -> dict:
+        return self.ctx.stats()
+
+    # --- main entry points ------------------------------------------------
+    def _check_args(self, points, tri, num):
+        n = int(points.shape[0])
+        if points.dim() != 2 or points.shape[1] != 2:
+            raise ValueError("points must be [N, 2]")
+        _dev_ptr(points, torch.float64, "points")
+        _dev_ptr(tri, torch.int32, "GPUTriangulation")
+        _dev_ptr(num, torch.int32, "cellNeighborNum")
+        if num.numel() != n:
+            raise ValueError("cellNeighborNum must have N entries")
+        return n
+
+    def globalDelaunayTriangulation(self, points, GPUTriangulation, cellNeighborNum,
+                                    stream=None) -> None:
+        """DelaunayGPU::globalDelaunayTriangulation (DelaunayGPU.cpp:118-157)."""
+        n = int(points.shape[0])
+        if n == 0:
+            print("No points in GPU DT")
+            return
+        if n != self.Ncells or GPUTriangulation.numel() != n * self.MaxSize:
+            self.Ncells = n
+            GPUTriangulation.resize_(n * self.MaxSize)
+            self._init_cell_list()
+        self._check_args(points, GPUTriangulation, cellNeighborNum)
+        L = _lib.load()
+        x11, _, _, x22 = self.Box.getBoxDims()
+        while True:
+            req = ctypes.c_int(0)
+            st = L.gdt_triangulate(self.ctx.handle, points.data_ptr(), n, x11, x22, self.cellsize,
+                                   self.MaxSize, GPUTriangulation.data_ptr(),
+                                   cellNeighborNum.data_ptr(), self.precision, ctypes.byref(req),
+                                   _stream_ptr(stream))
+            if st == _lib.GDT_OVERFLOW:
+                self.resize(req.value)
+                GPUTriangulation.resize_(n * self.MaxSize)
+                continue
+            _lib.check(st, "globalDelaunayTriangulation")
+            return
+
+    def testAndRepairDelaunayTriangulation(self, points, GPUTriangulation, cellNeighborNum,
+                                           stream=None) -> None:
+        """DelaunayGPU::testAndRepairDelaunayTriangulation (DelaunayGPU.cpp:567-602)."""
+        n = self._check_args(points, GPUTriangulation, cellNeighborNum)
+        if GPUTriangulation.numel() != n * self.MaxSize:
+            raise ValueError("GPUTriangulation must hold N*MaxSize entries")
+        L = _lib.load()
+        x11, _, _, x22 = self.Box.getBoxDims()
+        nf, req = ctypes.c_int(0), ctypes.c_int(0)
+        st = L.gdt_test_and_repair(self.ctx.handle, points.data_ptr(), n, x11, x22, self.cellsize,
+                                   self.MaxSize, GPUTriangulation.data_ptr(),
+                                   cellNeighborNum.data_ptr(), self.precision, ctypes.byref(nf),
+                                   ctypes.byref(req), _stream_ptr(stream))
+        self.last_repaired = nf.value
+        if st == _lib.GDT_OVERFLOW:
+            # a repaired ring outgrew the row stride: re-triangulate globally
+            self.resize(req.value)
+            self.globalDelaunayTriangulation(points, GPUTriangulation, cellNeighborNum, stream)
+            return
+        _lib.check(st, "testAndRepairDelaunayTriangulation")
