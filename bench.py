@@ -40,6 +40,13 @@ def parse():
     ap.add_argument("--n", type=int, default=N_POINTS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--block", type=int, default=None, help="threads per block (one-ring kernel)")
+    ap.add_argument("--tiled", type=int, default=None, help="1: 16x16-cell tiles (LDS kernel), 0: row-major")
+    ap.add_argument("--cell-scale", type=float, default=None, help="cell size multiplier")
+    ap.add_argument("--precisions", default="fp32f,fp64", help="comma list of fp32f,fp64")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="after the bench line, run a plain device copy of MIB MiB three "
+                         "times (reference kernel for scaling rocprofv3 FETCH/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -81,10 +88,13 @@ def main():
     xy = inputs.uniform_points(n, L, L, seed=2)
     pts = torch.from_numpy(xy).cuda()
     out = {}
+    wanted = args.precisions.split(",")
     for name, prec in (("fp32f", PREC_FP32F), ("fp64", PREC_FP64)):
+        if name not in wanted and not (name == "fp32f"):
+            continue
         dg = DelaunayGPU(n, 16, 1.0, PeriodicBoundaries(L, L))
         dg.setPrecision(prec)
-        dg.ctx.set_config(profile=1)
+        dg.ctx.set_config(profile=1, block=args.block, tiled=args.tiled, cell_scale=args.cell_scale)
         tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device="cuda")
         num = torch.zeros(n, dtype=torch.int32, device="cuda")
         ms, wall, ring_ms, cell_ms, st = time_path(dg, pts, tri, num, args.steps, args.warmup, torch)
@@ -93,9 +103,11 @@ def main():
         out[name] = dict(ms=ms, wall=wall, ring_ms=ring_ms, cell_ms=cell_ms, st=st,
                          hash=rings.ring_hash(nb, cnt), sum=int(cnt.sum()), nb=nb, cnt=cnt)
         print(f"[bench] {name}: {ms:.3f} ms/step (one-ring {ring_ms:.3f} ms, cell list "
-              f"{cell_ms:.3f} ms), {n / ms * 1e3 / 1e9:.3f} Gpoints/s", file=sys.stderr, flush=True)
+              f"{cell_ms:.3f} ms), {n / ms * 1e3 / 1e9:.3f} Gpoints/s, deferred {st['deferred']}, "
+              f"pred/pt {st['predicates'] / n:.1f}", file=sys.stderr, flush=True)
 
-    f32, f64 = out["fp32f"], out["fp64"]
+    f32 = out["fp32f"]
+    f64 = out.get("fp64", f32)
     identical = (f32["hash"] == f64["hash"] and np.array_equal(f32["nb"], f64["nb"])
                  and np.array_equal(f32["cnt"], f64["cnt"]))
     golden_ok = None
@@ -149,6 +161,7 @@ def main():
         "sum_counts_eq_6N": f32["sum"] == 6 * n,
         "fallback_fraction": st["fallbacks"] / max(1, st["predicates"]),
         "predicates_per_point": st["predicates"] / n,
+        "deferred_points": st["deferred"],
         "kernel_ms": {"one_ring_fp32f": f32["ring_ms"], "one_ring_fp64": f64["ring_ms"],
                       "cell_list": f32["cell_ms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -158,6 +171,16 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+    if args.calibrate_copy > 0:
+        nbytes = args.calibrate_copy << 20
+        src = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy calibration: 3 x {nbytes} bytes read + written", file=sys.stderr,
+              flush=True)
 
 
 if __name__ == "__main__":

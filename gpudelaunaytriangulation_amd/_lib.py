@@ -54,6 +54,7 @@ class Stats(ctypes.Structure):
         ("neighbours", ctypes.c_ulonglong), ("max_ring", ctypes.c_int), ("flagged", ctypes.c_int),
         ("xsize", ctypes.c_int), ("ysize", ctypes.c_int), ("bsx", ctypes.c_double),
         ("bsy", ctypes.c_double), ("ms_cells", ctypes.c_float), ("ms_ring", ctypes.c_float),
+        ("deferred", ctypes.c_int),
     ]
 
     def as_dict(self) -> dict:

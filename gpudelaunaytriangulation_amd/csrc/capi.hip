@@ -52,18 +52,18 @@ struct DevGuard {
 
 struct gdt_context {
     int device = 0;
-    gdt_config cfg = {256, 1, 1.0, 0};
+    gdt_config cfg = {320, 1, 1.0, 0};
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 
     std::vector<GdtSet> sets;
     int total_n = 0, total_keys = 0;
     bool cells_valid = false;
     DBuf<GdtSet> d_sets;
-    DBuf<GdtChunk> pch, kch, rch;
-    int npch = 0, nkch = 0, nrch = 0, rch_block = 0;
+    DBuf<GdtChunk> pch, kch, rch, tch;
+    int npch = 0, nkch = 0, nrch = 0, rch_block = 0, ntch = 0;
 
     DBuf<uint32_t> pkey, prank, ccount, cstart, scell, scan_tmp, tcount, toff, mark, markscan;
-    DBuf<int> perm, slot_of, worklist, repair, tris;
+    DBuf<int> perm, slot_of, worklist, repair, tris, defer;
     DBuf<gdt_d2> off64;
     DBuf<gdt_f2> off32;
     DBuf<int> status;
@@ -137,6 +137,13 @@ static int configure(gdt_context* c, std::vector<GdtSet> sets)
     GDT_CHECK(upload(c->kch, kch));
     c->npch = (int)pch.size();
     c->nkch = (int)kch.size();
+    // one entry per 16x16-cell tile (tiled cell order only)
+    std::vector<GdtChunk> tch;
+    for (int si = 0; si < (int)c->sets.size(); ++si)
+        if (c->sets[si].tiled)
+            for (int t = 0; t < c->sets[si].nkeys / GDT_TILE_CELLS; ++t) tch.push_back(GdtChunk{si, t});
+    GDT_CHECK(upload(c->tch, tch));
+    c->ntch = (int)tch.size();
     c->rch_block = 0;
     const size_t N = (size_t)c->total_n, K = (size_t)c->total_keys;
     GDT_CHECK(c->pkey.reserve(N));
@@ -146,6 +153,7 @@ static int configure(gdt_context* c, std::vector<GdtSet> sets)
     GDT_CHECK(c->off64.reserve(N));
     GDT_CHECK(c->off32.reserve(N));
     GDT_CHECK(c->scell.reserve(N));
+    GDT_CHECK(c->defer.reserve(N));
     GDT_CHECK(c->ccount.reserve(K));
     GDT_CHECK(c->cstart.reserve(K + 1));
     GDT_CHECK(c->scan_tmp.reserve(gdt_scan_tmp_words(std::max(K, N + 1))));
@@ -201,6 +209,7 @@ static void record_ring_stats(gdt_context* c)
     c->stats.fallbacks = c->h_counters[CT_FALLBACK];
     c->stats.neighbours = c->h_counters[CT_NBRSUM];
     c->stats.max_ring = c->h_status[ST_REQUIRED];
+    c->stats.deferred = c->h_status[ST_NDEFER];
     if (!c->sets.empty()) {
         c->stats.xsize = c->sets[0].xsize;
         c->stats.ysize = c->sets[0].ysize;
@@ -280,11 +289,11 @@ void gdt_destroy(gdt_context* c)
     if (!c) return;
     DevGuard g(c->device);
     (void)hipDeviceSynchronize();
-    c->d_sets.release(); c->pch.release(); c->kch.release(); c->rch.release();
+    c->d_sets.release(); c->pch.release(); c->kch.release(); c->rch.release(); c->tch.release();
     c->pkey.release(); c->prank.release(); c->ccount.release(); c->cstart.release();
     c->scell.release(); c->scan_tmp.release(); c->tcount.release(); c->toff.release();
     c->mark.release(); c->markscan.release(); c->perm.release(); c->slot_of.release();
-    c->worklist.release(); c->repair.release(); c->tris.release(); c->off64.release();
+    c->worklist.release(); c->repair.release(); c->tris.release(); c->defer.release(); c->off64.release();
     c->off32.release(); c->status.release(); c->counters.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -390,9 +399,15 @@ int gdt_triangulate(gdt_context* c, const double* d_xy, int n, double lx, double
     if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
     if ((rc = reset_status(c, s)) != GDT_OK) return rc;
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
-    GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), c->rch.p, c->nrch,
-                                  c->cfg.block, nullptr, nullptr, 0, ring_io(c, d_nbrs, stride),
-                                  d_counts, c->status.p, c->counters.p, s));
+    if (c->cfg.tiled)
+        GDT_CHECK(gdt_launch_one_ring_tile(precision, c->d_sets.p, (int)c->sets.size(), c->tch.p,
+                                           c->ntch, c->cfg.block, c->total_n,
+                                           ring_io(c, d_nbrs, stride), d_counts, c->status.p,
+                                           c->counters.p, c->defer.p, s));
+    else
+        GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), c->rch.p, c->nrch,
+                                      c->cfg.block, nullptr, nullptr, 0, ring_io(c, d_nbrs, stride),
+                                      d_counts, c->status.p, c->counters.p, s));
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[2], s));
     rc = finish_rings(c, stride, required, s);
     if (c->cfg.profile) {

@@ -33,9 +33,11 @@ enum {
     ST_ERRFLAGS = 1,     // bit 0: search radius exhausted, bit 1: ring did not close, bit 2: no neighbour
     ST_NFLAGGED = 2,     // number of points flagged by the circumcircle test
     ST_NTRI = 3,         // number of triangles enumerated
+    ST_NDEFER = 4,       // points the tile kernel deferred to the direct routine
     ST_WORDS = 8
 };
-enum { ERR_SEARCH = 1, ERR_NOCLOSE = 2, ERR_NONEIGHBOUR = 4 };
+// ERR_DEFER is internal: the LDS routine hands the point to the direct routine.
+enum { ERR_SEARCH = 1, ERR_NOCLOSE = 2, ERR_NONEIGHBOUR = 4, ERR_DEFER = 8 };
 // 64-bit counters (gdt_counters[...]).
 // CT_NBRSUM: sum of ring lengths written; CT_INSUM: sum of input counts seen by
 // the triangle enumeration (Euler check).
@@ -59,17 +61,26 @@ struct GdtSet {
 
 struct GdtChunk { int set; int start; };   // a block's work: set id and first local item
 
+// Cell order.  Tiled: 16x16-cell tiles in row-major tile order, cells
+// row-major inside a tile, so one tile's points are one contiguous slot range
+// (the unit of work of the LDS-staged one-ring kernel).  Untiled: row-major.
+#define GDT_TILE_SHIFT 4
+#define GDT_TILE (1 << GDT_TILE_SHIFT)
+#define GDT_TILE_CELLS (GDT_TILE * GDT_TILE)
+
 GDT_HD inline int gdt_cell_key(const GdtSet& s, int cx, int cy)
 {
     if (s.tiled)
-        return ((((cy >> 3) * s.tiles_x) + (cx >> 3)) << 6) | ((cy & 7) << 3) | (cx & 7);
+        return ((((cy >> GDT_TILE_SHIFT) * s.tiles_x) + (cx >> GDT_TILE_SHIFT))
+                << (2 * GDT_TILE_SHIFT)) |
+               ((cy & (GDT_TILE - 1)) << GDT_TILE_SHIFT) | (cx & (GDT_TILE - 1));
     return cy * s.xsize + cx;
 }
 
 GDT_HD inline int gdt_nkeys(int xsize, int ysize, int tiled)
 {
     if (!tiled) return xsize * ysize;
-    return ((xsize + 7) / 8) * ((ysize + 7) / 8) * 64;
+    return ((xsize + GDT_TILE - 1) / GDT_TILE) * ((ysize + GDT_TILE - 1) / GDT_TILE) * GDT_TILE_CELLS;
 }
 
 // Offsets (dx,dy) of the i-th cell of the Chebyshev ring of radius r (8r cells).
@@ -149,7 +160,7 @@ inline GdtSet gdt_make_set(int p0, int n, int c0, double lx, double ly, double c
     s.p0 = p0; s.n = n; s.c0 = c0;
     gdt_grid(lx, ly, cell_size, s.xsize, s.ysize);
     s.tiled = tiled ? 1 : 0;
-    s.tiles_x = (s.xsize + 7) / 8;
+    s.tiles_x = (s.xsize + GDT_TILE - 1) / GDT_TILE;
     s.nkeys = gdt_nkeys(s.xsize, s.ysize, s.tiled);
     const int rx = (s.xsize - 1) / 2, ry = (s.ysize - 1) / 2;
     s.rmax = rx < ry ? rx : ry;

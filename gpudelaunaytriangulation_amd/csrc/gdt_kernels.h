@@ -42,6 +42,14 @@ hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
                                int* counts, int* status, unsigned long long* counters,
                                hipStream_t s);
 
+// One-ring of every point, one block per 16x16-cell tile (tiled cell order).
+// Points of windows that do not fit are deferred to the direct routine
+// (second launch over `defer`, size total_n, count in status[ST_NDEFER]).
+hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nsets,
+                                    const GdtChunk* tiles, int ntiles, int block, int total_n,
+                                    const GdtRingIO& io, int* counts, int* status,
+                                    unsigned long long* counters, int* defer, hipStream_t s);
+
 // Triangles owned by their smallest vertex, written at toff (deterministic).
 hipError_t gdt_launch_triangles(const GdtSet* d_sets, const GdtChunk* pchunks, int npch,
                                 int total_n, const int* nbrs, const int* counts, int stride,

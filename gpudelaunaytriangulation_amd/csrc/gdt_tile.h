@@ -1,0 +1,512 @@
+// gdt_tile.h -- LDS-staged one-ring construction (host/device routines).
+//
+// A block owns one 16x16-cell tile (a contiguous range of cell-ordered
+// slots).  It stages the tile's window -- the tile plus a halo of GDT_HALO
+// cells on every side, wrapped on the torus -- into LDS, with every point
+// written as a coordinate in the window frame,
+//     X = f * bs + off,   f = (window cell index) - GDT_FRAME,
+// in FP64 and (filtered path) FP32.  A lane then walks its point's one-ring
+// exactly as in gdt_point.h (nearest neighbour, then gift wrapping with
+// certified stops), reading the Chebyshev rings r <= GDT_HALO as row
+// segments of the LDS window and rings beyond it from the global cell list,
+// with the same frame formula.  A displacement is d = X(c) - X(v).
+//
+// The filtered path certifies against the FP64 displacement of the same
+// frame formula.  FP32 input error per coordinate (u = 2^-24):
+//   |X32 - X64| <= 3u|X| + 3u*bs   =>   |d32 - d64| <= 5u|d| + 6u(|Xv| + bs),
+// i.e. eps1 = 5u (relative) and eps0 = 7.5u(|Xv|_inf + bs) (absolute, with
+// margin).  The orientation, in-circle and distance bounds are the ones of
+// gdt_point.h written for general (eps1, eps0), doubled for slack.
+#pragma once
+
+#include "gdt_point.h"
+
+#define GDT_HALO 3
+#define GDT_WIN (GDT_TILE + 2 * GDT_HALO)
+#define GDT_WIN_CELLS (GDT_WIN * GDT_WIN)
+#define GDT_WIN_MAXPTS 768
+#define GDT_FRAME (GDT_WIN / 2)
+#define GDT_RINGREG 16      // ring entries held in registers before a row is written
+
+struct GdtTileWin {
+    int tx, ty;          // tile coordinates
+    int gx0, gy0;        // unwrapped global cell of window cell (0,0)
+};
+
+struct GdtTileLds {
+    uint32_t* cs;        // [GDT_WIN_CELLS+1] first LDS entry of each window cell
+    int* gstart;         // [GDT_WIN_CELLS] first global slot of each window cell
+    gdt_d2* X64;         // [GDT_WIN_MAXPTS] FP64 frame coordinates
+    gdt_f2* X32;         // [GDT_WIN_MAXPTS] FP32 frame coordinates (filtered path)
+    int* orig;           // [GDT_WIN_MAXPTS] caller index
+};
+
+GDT_HD inline int gdt_wrapm(int v, int n)
+{
+    v %= n;
+    return v < 0 ? v + n : v;
+}
+
+GDT_HD inline GdtTileWin gdt_tile_window(const GdtSet& S, int t)
+{
+    GdtTileWin w;
+    w.tx = t % S.tiles_x;
+    w.ty = t / S.tiles_x;
+    w.gx0 = w.tx * GDT_TILE - GDT_HALO;
+    w.gy0 = w.ty * GDT_TILE - GDT_HALO;
+    return w;
+}
+
+// Staging, phase 1: global slot range of window cell c.
+GDT_HD inline void gdt_tile_count(const GdtSet& S, const GdtTileWin& w, int c,
+                                  const uint32_t* cstart, int& gstart, uint32_t& cnt)
+{
+    const int wx = c % GDT_WIN, wy = c / GDT_WIN;
+    const int gx = gdt_wrapm(w.gx0 + wx, S.xsize), gy = gdt_wrapm(w.gy0 + wy, S.ysize);
+    const int key = S.c0 + gdt_cell_key(S, gx, gy);
+    const uint32_t b = cstart[key], e = cstart[key + 1];
+    gstart = (int)b;
+    cnt = e - b;
+}
+
+GDT_HD inline double gdt_frame64(int f, double bs, double off) { return (double)f * bs + off; }
+GDT_HD inline float gdt_frame32(int f, float bs, float off) { return (float)f * bs + off; }
+
+// Staging, phase 2: copy window cell c's points to LDS entries [at, at+cnt).
+template <bool F32>
+GDT_HD inline void gdt_tile_copy(const GdtSet& S, int c, int gstart, uint32_t at, uint32_t cnt,
+                                 const GdtRingIO& io, const GdtTileLds& L)
+{
+    const int fx = c % GDT_WIN - GDT_FRAME, fy = c / GDT_WIN - GDT_FRAME;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        const int s = gstart + (int)k;
+        const gdt_d2 o = io.off64[s];
+        gdt_d2 X;
+        X.x = gdt_frame64(fx, S.bsx, o.x);
+        X.y = gdt_frame64(fy, S.bsy, o.y);
+        L.X64[at + k] = X;
+        L.orig[at + k] = io.perm[s];
+        if (F32) {
+            const gdt_f2 o32 = io.off32[s];
+            gdt_f2 Y;
+            Y.x = gdt_frame32(fx, S.bsx32, o32.x);
+            Y.y = gdt_frame32(fy, S.bsy32, o32.y);
+            L.X32[at + k] = Y;
+        }
+    }
+}
+
+// LDS entries of Chebyshev ring r (r <= GDT_HALO) around window cell (wx,wy):
+// top and bottom rows as one segment each, the side cells of the rows between.
+template <class Fn>
+GDT_HD inline void gdt_lds_ring(const GdtTileLds& L, int wx, int wy, int r, Fn& fn)
+{
+    if (r == 0) {
+        const int c = wy * GDT_WIN + wx;
+        for (uint32_t k = L.cs[c]; k < L.cs[c + 1]; ++k) fn((int)k);
+        return;
+    }
+    for (int side = 0; side < 2; ++side) {
+        const int c0 = (side ? wy + r : wy - r) * GDT_WIN + wx - r;
+        const uint32_t e = L.cs[c0 + 2 * r + 1];
+        for (uint32_t k = L.cs[c0]; k < e; ++k) fn((int)k);
+    }
+    for (int y = wy - r + 1; y < wy + r; ++y) {
+        const int cl = y * GDT_WIN + wx - r, cr = cl + 2 * r;
+        for (uint32_t k = L.cs[cl]; k < L.cs[cl + 1]; ++k) fn((int)k);
+        for (uint32_t k = L.cs[cr]; k < L.cs[cr + 1]; ++k) fn((int)k);
+    }
+}
+
+// Global cells of Chebyshev ring r around global cell (cx,cy): fn(slot, dx, dy).
+template <class Fn>
+GDT_HD inline void gdt_global_ring(const GdtSet& S, int cx, int cy, int r, const uint32_t* cstart,
+                                   Fn& fn)
+{
+    const int ncell = r ? 8 * r : 1;
+    for (int q = 0; q < ncell; ++q) {
+        int dx, dy;
+        gdt_ring_cell(r, q, dx, dy);
+        const int x = gdt_wrap(cx + dx, S.xsize), y = gdt_wrap(cy + dy, S.ysize);
+        const int key = S.c0 + gdt_cell_key(S, x, y);
+        const uint32_t b = cstart[key], e = cstart[key + 1];
+        for (uint32_t s = b; s < e; ++s) fn((int)s, dx, dy);
+    }
+}
+
+// Ring entries: the first GDT_RINGREG in a register shift queue, written as
+// one burst at the end (one write-back per row), later ones directly.
+struct GdtRingOut {
+    int q[GDT_RINGREG];
+    int count;
+    GDT_HD inline void push(int x, int* row, int stride)
+    {
+        if (count < GDT_RINGREG) {
+#pragma unroll
+            for (int m = 0; m < GDT_RINGREG - 1; ++m) q[m] = q[m + 1];
+            q[GDT_RINGREG - 1] = x;
+        } else if (count < stride) {
+            row[count] = x;
+        }
+        ++count;
+    }
+    GDT_HD inline void flush(int* row, int stride) const
+    {
+        const int held = count < GDT_RINGREG ? count : GDT_RINGREG;
+#pragma unroll
+        for (int m = 0; m < GDT_RINGREG; ++m) {
+            const int i = m - (GDT_RINGREG - held);
+            if (i >= 0 && i < stride) row[i] = q[m];
+        }
+    }
+};
+
+// LDS entries of the block of cells [x0,x1] x [y0,y1] (offsets from window
+// cell (wx,wy), inside the window) minus the already scanned square
+// [-hole,hole]^2 (hole < 0: none), visited as one flat loop over row segments
+// (each row of cells is one contiguous LDS range; a row crossing the hole is
+// two segments), so the trip count is the number of points, not cells.
+template <class Fn>
+GDT_HD inline void gdt_lds_block(const GdtTileLds& L, int wx, int wy, int x0, int x1, int y0,
+                                 int y1, int hole, Fn& fn)
+{
+    // segment sg: row y0 + sg/2, side sg&1
+    auto seg = [&](int sg, uint32_t& b, uint32_t& e) {
+        const int y = y0 + (sg >> 1);
+        int a = x0, z = x1;
+        if (y >= -hole && y <= hole) {
+            if (sg & 1) { a = a > hole + 1 ? a : hole + 1; }
+            else { z = z < -hole - 1 ? z : -hole - 1; }
+        } else if (sg & 1) {
+            z = a - 1;     // rows outside the hole are one segment
+        }
+        if (z < a) { b = e = 0; return; }
+        const int c = (wy + y) * GDT_WIN + wx;
+        b = L.cs[c + a];
+        e = L.cs[c + z + 1];
+    };
+    const int nseg = 2 * (y1 - y0 + 1);
+    int total = 0;
+    for (int sg = 0; sg < nseg; ++sg) {
+        uint32_t b, e;
+        seg(sg, b, e);
+        total += (int)(e - b);
+    }
+    int sg = 0;
+    uint32_t k, e;
+    seg(0, k, e);
+    for (int t = 0; t < total; ++t) {
+        while (k == e) seg(++sg, k, e);
+        fn((int)k);
+        ++k;
+    }
+}
+
+// Cell-offset bounding box of the circle through the origin, a and b, given
+// centre bounds [qxlo,qxhi] x [qylo,qyhi] and radius bound R, for a point whose
+// offset inside its cell is (ox,oy); `margin` is in cells.
+template <class T>
+GDT_HD inline void gdt_circle_cells(T qxlo, T qxhi, T qylo, T qyhi, T R, T ox, T oy, T bsx, T bsy,
+                                    T margin, int& dx0, int& dx1, int& dy0, int& dy1)
+{
+    dx0 = (int)floor((double)((ox + (qxlo - R)) / bsx - margin));
+    dx1 = (int)floor((double)((ox + (qxhi + R)) / bsx + margin));
+    dy0 = (int)floor((double)((oy + (qylo - R)) / bsy - margin));
+    dy1 = (int)floor((double)((oy + (qyhi + R)) / bsy + margin));
+}
+
+// ---------------------------------------------------------------------------
+// Per point, one search step at a time (the kernel interleaves the points of
+// a tile across its lanes step by step).  Every step is one scan of the 7x7
+// block of cells [-HALO,HALO]^2 around v's cell, one tight loop per row of
+// cells (a row is one contiguous LDS range); the body evaluates the predicates
+// of the lane's current search and takes a single (rare) branch when the
+// candidate wins or a filtered predicate is undecided.
+//   step 0 (nearest neighbour): certified when its distance is below cov(3);
+//   step k (successor of edge (v,a)): the final circle (v, a, best) holds no
+//     scanned point, so it is certified when its conservative bounding box
+//     lies inside the block, or 2R <= cov(3).
+// A step that cannot be certified returns ERR_DEFER: the kernel hands the
+// point to the direct global-memory routine (gdt_point.h), which rewrites its
+// whole row.  Candidates are LDS entries k.
+// ---------------------------------------------------------------------------
+template <class Real>
+struct GdtTileLane {
+    int j;            // slot of v
+    int iv;           // LDS entry of v
+    int c0;           // window cell of v
+    Real vx, vy;      // v's window-frame coordinates
+    Real covb;        // distance of v to its cell walls
+    int ak, n0;       // LDS entries of the current neighbour a and of the first one
+    Real ax, ay;      // displacement of a
+    int count;        // ring entries emitted
+    int* row;
+};
+
+template <class Real>
+GDT_HD inline void gdt_tile_lane_init(const GdtTileWin& w, const GdtTileLds& L, int j,
+                                      const GdtRingIO& io, const GdtSet& S, GdtTileLane<Real>& st);
+
+template <>
+GDT_HD inline void gdt_tile_lane_init<double>(const GdtTileWin& w, const GdtTileLds& L, int j,
+                                              const GdtRingIO& io, const GdtSet& S,
+                                              GdtTileLane<double>& st)
+{
+    const uint32_t pc = io.scell[j];
+    const int wxv = (int)(pc & 0xffffu) - w.gx0, wyv = (int)(pc >> 16) - w.gy0;
+    st.j = j;
+    st.c0 = wyv * GDT_WIN + wxv;
+    st.iv = (int)L.cs[st.c0] + (j - L.gstart[st.c0]);
+    const gdt_d2 X = L.X64[st.iv];
+    st.vx = X.x;
+    st.vy = X.y;
+    const gdt_d2 ov = io.off64[j];
+    st.covb = GDT_FMIN(GDT_FMIN(ov.x, S.bsx - ov.x), GDT_FMIN(ov.y, S.bsy - ov.y));
+    st.ak = st.n0 = -1;
+    st.ax = st.ay = 0.;
+    st.count = 0;
+    st.row = io.nbrs + (size_t)L.orig[st.iv] * (size_t)io.stride;
+}
+
+template <>
+GDT_HD inline void gdt_tile_lane_init<float>(const GdtTileWin& w, const GdtTileLds& L, int j,
+                                             const GdtRingIO& io, const GdtSet& S,
+                                             GdtTileLane<float>& st)
+{
+    const uint32_t pc = io.scell[j];
+    const int wxv = (int)(pc & 0xffffu) - w.gx0, wyv = (int)(pc >> 16) - w.gy0;
+    st.j = j;
+    st.c0 = wyv * GDT_WIN + wxv;
+    st.iv = (int)L.cs[st.c0] + (j - L.gstart[st.c0]);
+    const gdt_f2 X = L.X32[st.iv];
+    st.vx = X.x;
+    st.vy = X.y;
+    const gdt_f2 ov = io.off32[j];
+    st.covb = GDT_FMINF(GDT_FMINF(ov.x, S.bsx32 - ov.x), GDT_FMINF(ov.y, S.bsy32 - ov.y));
+    st.ak = st.n0 = -1;
+    st.ax = st.ay = 0.f;
+    st.count = 0;
+    st.row = io.nbrs + (size_t)L.orig[st.iv] * (size_t)io.stride;
+}
+
+// Offset of v inside its cell, recovered from the frame coordinate.
+GDT_HD inline double gdt_cell_offset(double X, int wx, double bs)
+{
+    return X - (double)(wx - GDT_FRAME) * bs;
+}
+
+// Result of one step: 0 running, 1 ring closed, 2 defer, 3 non-generic (no close)
+enum { STEP_RUN = 0, STEP_DONE = 1, STEP_DEFER = 2, STEP_NOCLOSE = 3 };
+
+GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const GdtRingIO& io,
+                                    GdtTileLane<double>& st, unsigned& npred)
+{
+    const bool wrap = st.ak >= 0;
+    int bk = -1;
+    double bx = 0., by = 0., bd2 = 0.;
+    Circ64 K = {0., 0., 0.};
+    const double ax = st.ax, ay = st.ay, vx = st.vx, vy = st.vy;
+    const int iv = st.iv, ak = st.ak;
+#pragma unroll 1
+    for (int ry = -GDT_HALO; ry <= GDT_HALO; ++ry) {
+        const int c = st.c0 + ry * GDT_WIN;
+        const uint32_t e = L.cs[c + GDT_HALO + 1];
+        for (uint32_t kk = L.cs[c - GDT_HALO]; kk < e; ++kk) {
+            const int k = (int)kk;
+            const gdt_d2 X = L.X64[k];
+            const double px = X.x - vx, py = X.y - vy;
+            const double cc = px * px + py * py;
+            const double O = gdt_orient64(ax, ay, px, py);
+            const double P = K.k0 * cc + (K.kx * px + K.ky * py);
+            const bool take = k != iv && (wrap ? (k != ak && O > 0. && (bk < 0 || P < 0.))
+                                               : (bk < 0 || cc < bd2));
+            if (take) {
+                bk = k; bx = px; by = py; bd2 = cc;
+                if (wrap) K = gdt_circ64(ax, ay, px, py);
+            }
+        }
+        npred += e - L.cs[c - GDT_HALO];
+    }
+    const double bsmin = GDT_FMIN(S.bsx, S.bsy);
+    const double cov = (double)GDT_HALO * bsmin + st.covb;
+    bool ok = false;
+    if (!wrap) {
+        ok = bk >= 0 && cov > 0. && bd2 <= cov * cov * (1.0 - 1e-12);
+    } else if (bk >= 0 && K.k0 > 1e-6 * (fabs(ax * by) + fabs(ay * bx))) {
+        const double qx = -K.kx / (2. * K.k0), qy = -K.ky / (2. * K.k0);
+        const double rad = sqrt(qx * qx + qy * qy);
+        const double m = 1e-6 * (fabs(qx) + fabs(qy) + rad) + 1e-9 * S.bsx;
+        const int wx = st.c0 % GDT_WIN, wy = st.c0 / GDT_WIN;
+        const double ox = gdt_cell_offset(vx, wx, S.bsx), oy = gdt_cell_offset(vy, wy, S.bsy);
+        int dx0, dx1, dy0, dy1;
+        gdt_circle_cells<double>(qx - m, qx + m, qy - m, qy + m, rad, ox, oy, S.bsx, S.bsy, 1e-3,
+                                 dx0, dx1, dy0, dy1);
+        ok = (dx0 >= -GDT_HALO && dx1 <= GDT_HALO && dy0 >= -GDT_HALO && dy1 <= GDT_HALO) ||
+             (cov > 0. && K.kx * K.kx + K.ky * K.ky <= cov * cov * (K.k0 * K.k0) * (1.0 - 1e-6));
+    }
+    if (!ok) return STEP_DEFER;
+    if (wrap && bk == st.n0) return STEP_DONE;
+    if (!wrap) st.n0 = bk;
+    if (st.count < io.stride) st.row[st.count] = L.orig[bk];
+    ++st.count;
+    if (st.count > GDT_HARD_CAP) return STEP_NOCLOSE;
+    st.ak = bk; st.ax = bx; st.ay = by;
+    return STEP_RUN;
+}
+
+GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const GdtRingIO& io,
+                                    GdtTileLane<float>& st, unsigned& npred, unsigned& nfb)
+{
+    const float u = GDT_U32;
+    const float bsM = S.bsmax32;
+    const float eps1 = 5.f * u;
+    const float eps0 = 7.5f * u * (GDT_FMAXF(fabsf(st.vx), fabsf(st.vy)) + bsM);
+    const float mc = (float)(GDT_HALO + 1) * bsM * 1.0001f;
+    const float cP = 8.f * (4.f * eps1 + 8.f * u), eP = 8.f * eps0;
+    const bool wrap = st.ak >= 0;
+    const float ax = st.ax, ay = st.ay, vx = st.vx, vy = st.vy;
+    const int iv = st.iv, ak = st.ak;
+    const float ma = GDT_FMAXF(fabsf(ax), fabsf(ay));
+    const float BO = fmaf(4.f * (2.f * eps1 + 2.f * u) * ma + 4.f * eps0, mc, 4.f * eps0 * ma + 1e-30f);
+    const float cE = 2.f * (2.f * eps1 + 3.f * u);
+    const float eabs = 8.f * eps0 * mc + 4.f * eps0 * eps0 + 1e-30f;
+    int bk = -1;
+    float bx = 0.f, by = 0.f, bd2 = 0.f, bE = 0.f;
+    float K0 = 0.f, Kx = 0.f, Ky = 0.f, pab = 0.f, sab = 0.f, BP = 0.f;
+    // FP64 displacement of LDS entry k: the FP64 path's value (fallbacks only)
+    auto x64 = [&](int k, double& x, double& y) {
+        const gdt_d2 V = L.X64[iv], X = L.X64[k];
+        x = X.x - V.x;
+        y = X.y - V.y;
+    };
+#pragma unroll 1
+    for (int ry = -GDT_HALO; ry <= GDT_HALO; ++ry) {
+        const int c = st.c0 + ry * GDT_WIN;
+        const uint32_t e = L.cs[c + GDT_HALO + 1];
+        for (uint32_t kk = L.cs[c - GDT_HALO]; kk < e; ++kk) {
+            const int k = (int)kk;
+            const gdt_f2 Y = L.X32[k];
+            const float px = Y.x - vx, py = Y.y - vy;
+            const float cc = fmaf(px, px, py * py);
+            const float O = fmaf(ax, py, -(ay * px));
+            const float P = fmaf(K0, cc, fmaf(Kx, px, Ky * py));
+            const float E = fmaf(cE, cc, eabs);
+            bool take, und;
+            if (wrap) {
+                const bool okO = O > BO, decO = fabsf(O) > BO;
+                const bool inP = P < -BP, decP = fabsf(P) > BP;
+                take = okO && (bk < 0 || inP);
+                und = !decO || (okO && bk >= 0 && !decP);
+                und = und && k != ak;
+            } else {
+                take = bk < 0 || cc + E < bd2 - bE;
+                und = bk >= 0 && !take && !(cc - E > bd2 + bE);
+            }
+            if (k == iv) { take = false; und = false; }
+            if (wrap && k == ak) take = false;
+            if (take || und) {
+                if (und) {
+                    ++nfb;
+                    double c64x, c64y;
+                    x64(k, c64x, c64y);
+                    if (wrap) {
+                        double a64x, a64y;
+                        x64(ak, a64x, a64y);
+                        const bool left = fabsf(O) > BO ? O > 0.f
+                                                        : gdt_orient64(a64x, a64y, c64x, c64y) > 0.;
+                        if (!left) take = false;
+                        else if (bk < 0) take = true;
+                        else if (fabsf(P) > BP) take = P < 0.f;
+                        else {
+                            double b64x, b64y;
+                            x64(bk, b64x, b64y);
+                            const Circ64 K64 = gdt_circ64(a64x, a64y, b64x, b64y);
+                            take = gdt_incircle64(K64, c64x, c64y) < 0.;
+                        }
+                    } else {
+                        double b64x, b64y;
+                        x64(bk, b64x, b64y);
+                        take = gdt_dist2_64(c64x, c64y) < gdt_dist2_64(b64x, b64y);
+                    }
+                }
+                if (take) {
+                    bk = k; bx = px; by = py; bd2 = cc; bE = E;
+                    if (wrap) {
+                        const float aa = fmaf(ax, ax, ay * ay);
+                        K0 = fmaf(ax, py, -(ay * px));
+                        Kx = fmaf(cc, ay, -(aa * py));
+                        Ky = fmaf(aa, px, -(cc * ax));
+                        const float mb = GDT_FMAXF(fabsf(px), fabsf(py));
+                        sab = ma + mb;
+                        pab = ma * mb;
+                        const float A0 = eP * sab * sab * sab + 1e-30f;
+                        const float A1 = cP * pab * sab + 3.f * eP * sab * sab;
+                        const float A2 = cP * pab + 3.f * eP * sab;
+                        BP = fmaf(mc, fmaf(mc, fmaf(mc, eP, A2), A1), A0);
+                    }
+                }
+            }
+        }
+        npred += e - L.cs[c - GDT_HALO];
+    }
+    const float bsm = S.bsmin32;
+    const float cov = (float)GDT_HALO * bsm + st.covb - 9.5367431640625e-07f * (float)(GDT_HALO + 2) * bsM;
+    bool ok = false;
+    if (!wrap) {
+        ok = bk >= 0 && cov > 0.f && bd2 + bE <= cov * cov * 0.9999f;
+    } else if (bk >= 0) {
+        const float EK0 = 4.f * (2.f * eps1 + 2.f * u) * pab + 4.f * eps0 * sab;
+        const float EK = 4.f * (3.f * eps1 + 6.f * u) * pab * sab + 8.f * eps0 * sab * sab;
+        const float Klo = K0 - EK0, Khi = K0 + EK0;
+        if (Klo > 0.0625f * K0) {
+            const float nxlo = -(Kx + EK), nxhi = -(Kx - EK);
+            const float nylo = -(Ky + EK), nyhi = -(Ky - EK);
+            const float qxlo = GDT_FMINF(nxlo / (2.f * Klo), nxlo / (2.f * Khi));
+            const float qxhi = GDT_FMAXF(nxhi / (2.f * Klo), nxhi / (2.f * Khi));
+            const float qylo = GDT_FMINF(nylo / (2.f * Klo), nylo / (2.f * Khi));
+            const float qyhi = GDT_FMAXF(nyhi / (2.f * Klo), nyhi / (2.f * Khi));
+            const float rad = sqrtf(GDT_FMAXF(qxlo * qxlo, qxhi * qxhi) +
+                                    GDT_FMAXF(qylo * qylo, qyhi * qyhi)) * 1.0001f;
+            const int wx = st.c0 % GDT_WIN, wy = st.c0 / GDT_WIN;
+            const float ox = vx - (float)(wx - GDT_FRAME) * S.bsx32;
+            const float oy = vy - (float)(wy - GDT_FRAME) * S.bsy32;
+            int dx0, dx1, dy0, dy1;
+            gdt_circle_cells<float>(qxlo, qxhi, qylo, qyhi, rad, ox, oy, S.bsx32, S.bsy32, 1e-2f,
+                                    dx0, dx1, dy0, dy1);
+            const float gx = fabsf(Kx) + EK, gy = fabsf(Ky) + EK;
+            ok = (dx0 >= -GDT_HALO && dx1 <= GDT_HALO && dy0 >= -GDT_HALO && dy1 <= GDT_HALO) ||
+                 (cov > 0.f && (gx * gx + gy * gy) * 1.001f <= (cov * cov) * (Klo * Klo));
+        }
+    }
+    if (!ok) return STEP_DEFER;
+    if (wrap && bk == st.n0) return STEP_DONE;
+    if (!wrap) st.n0 = bk;
+    if (st.count < io.stride) st.row[st.count] = L.orig[bk];
+    ++st.count;
+    if (st.count > GDT_HARD_CAP) return STEP_NOCLOSE;
+    st.ak = bk; st.ax = bx; st.ay = by;
+    return STEP_RUN;
+}
+
+// Whole point, one step after another (host emulation and simple callers).
+template <int PREC>
+GDT_HD inline GdtRingResult gdt_tile_ring(const GdtSet& S, const GdtTileWin& w, const GdtTileLds& L,
+                                          int j, const GdtRingIO& io)
+{
+    GdtRingResult res = {0, 0, 0u, 0u};
+    int rc = STEP_RUN;
+    if (PREC == 32) {
+        GdtTileLane<float> st;
+        gdt_tile_lane_init<float>(w, L, j, io, S, st);
+        while (rc == STEP_RUN) rc = gdt_tile_step_f32(S, L, io, st, res.npred, res.nfb);
+        res.count = st.count;
+    } else {
+        GdtTileLane<double> st;
+        gdt_tile_lane_init<double>(w, L, j, io, S, st);
+        while (rc == STEP_RUN) rc = gdt_tile_step_f64(S, L, io, st, res.npred);
+        res.count = st.count;
+    }
+    if (rc == STEP_DEFER) res.err = ERR_DEFER;
+    if (rc == STEP_NOCLOSE) res.err = ERR_NOCLOSE;
+    return res;
+}

@@ -8,6 +8,9 @@
 // (DelaunayGPU.cu:611-640, 945-975), which instead launch every point and
 // mask most of them out.
 #include "gdt_kernels.h"
+#include "gdt_tile.h"
+
+#include <type_traits>
 
 __device__ inline unsigned wave_sum_u32(unsigned v)
 {
@@ -93,6 +96,161 @@ __global__ __launch_bounds__(1024) void k_one_ring_list(const GdtSet* __restrict
         const GdtSet S = sets[s];
         one_ring_lane<PREC>(active, S, j, io, counts, status, counters);
     }
+}
+
+// Exclusive scan of a[0..n) in LDS by the whole block; a[n] = total.
+__device__ inline void block_scan_lds(uint32_t* a, int n, uint32_t* wsum)
+{
+    const int nt = blockDim.x, tid = threadIdx.x;
+    const int ipt = (n + nt - 1) / nt;
+    const int b = tid * ipt, e = min(b + ipt, n);
+    uint32_t s = 0;
+    for (int i = b; i < e; ++i) s += a[i];
+    const int lane = tid & 63, wid = tid >> 6;
+    uint32_t x = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+    for (int q = 0; q < (nt >> 6); ++q) {
+        const uint32_t t = wsum[q];
+        if (q < wid) woff += t;
+        tot += t;
+    }
+    uint32_t run = woff + x - s;
+    for (int i = b; i < e; ++i) {
+        const uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+    if (tid == 0) a[n] = tot;
+}
+
+// One block per 16x16-cell tile: stage the tile window in LDS, then one lane
+// per point of the tile (in rounds of blockDim).  The points of a window that
+// does not fit the LDS budget, or of a grid too small for the halo, are
+// appended to `defer` and handled by the direct routine (k_one_ring_list) in
+// the next launch (the same decision in both precisions).
+template <int PREC>
+__global__ __launch_bounds__(512) void k_one_ring_tile(const GdtSet* __restrict__ sets,
+                                                       const GdtChunk* __restrict__ tiles,
+                                                       GdtRingIO io, int* __restrict__ counts,
+                                                       int* __restrict__ status,
+                                                       unsigned long long* __restrict__ counters,
+                                                       int* __restrict__ defer)
+{
+    __shared__ uint32_t s_cs[GDT_WIN_CELLS + 1];
+    __shared__ int s_gstart[GDT_WIN_CELLS];
+    __shared__ gdt_d2 s_X64[GDT_WIN_MAXPTS];
+    __shared__ gdt_f2 s_X32[PREC == 32 ? GDT_WIN_MAXPTS : 1];
+    __shared__ int s_orig[GDT_WIN_MAXPTS];
+    __shared__ uint32_t s_wsum[16];
+    __shared__ int s_next;
+
+    const GdtChunk ch = tiles[blockIdx.x];
+    const GdtSet S = sets[ch.set];
+    const GdtTileWin w = gdt_tile_window(S, ch.start);
+    GdtTileLds L;
+    L.cs = s_cs; L.gstart = s_gstart; L.X64 = s_X64; L.X32 = s_X32; L.orig = s_orig;
+    const int tid = threadIdx.x, nt = blockDim.x;
+
+    for (int c = tid; c < GDT_WIN_CELLS; c += nt) {
+        int gs;
+        uint32_t cn;
+        gdt_tile_count(S, w, c, io.cstart, gs, cn);
+        s_gstart[c] = gs;
+        s_cs[c] = cn;
+    }
+    __syncthreads();
+    block_scan_lds(s_cs, GDT_WIN_CELLS, s_wsum);
+    __syncthreads();
+    const bool fits = s_cs[GDT_WIN_CELLS] <= GDT_WIN_MAXPTS && S.rmax >= GDT_HALO;
+    if (fits)
+        for (int c = tid; c < GDT_WIN_CELLS; c += nt)
+            gdt_tile_copy<PREC == 32>(S, c, s_gstart[c], s_cs[c], s_cs[c + 1] - s_cs[c], io, L);
+    __syncthreads();
+
+    const int k0 = S.c0 + ch.start * GDT_TILE_CELLS;
+    const int T0 = (int)io.cstart[k0], T1 = (int)io.cstart[k0 + GDT_TILE_CELLS];
+    if (!fits) {
+        for (int j = T0 + tid; j < T1; j += nt) {
+            const uint32_t pos = atomicAdd(reinterpret_cast<unsigned int*>(&status[ST_NDEFER]), 1u);
+            defer[pos] = j;
+        }
+        return;
+    }
+    // Lanes take points of the tile and advance them one search step per
+    // iteration; a lane whose point is finished takes the next one, so a
+    // wave's iterations track the total number of steps, not the longest ring.
+    if (tid == 0) s_next = T0 + nt;
+    __syncthreads();
+    using Real = typename std::conditional<PREC == 32, float, double>::type;
+    GdtTileLane<Real> st;
+    int j = T0 + tid;
+    bool have = j < T1;
+    if (have) gdt_tile_lane_init<Real>(w, L, j, io, S, st);
+    unsigned npred = 0, nfb = 0, nsum = 0;
+    int mx = 0, er = 0;
+    while (__ballot(have) != 0) {
+        if (have) {
+            int rc;
+            if constexpr (PREC == 32)
+                rc = gdt_tile_step_f32(S, L, io, st, npred, nfb);
+            else
+                rc = gdt_tile_step_f64(S, L, io, st, npred);
+            if (rc != STEP_RUN) {
+                if (rc == STEP_DEFER) {
+                    const uint32_t pos = atomicAdd(reinterpret_cast<unsigned int*>(&status[ST_NDEFER]), 1u);
+                    defer[pos] = j;
+                } else {
+                    counts[L.orig[st.iv]] = st.count;
+                    mx = max(mx, st.count);
+                    nsum += (unsigned)st.count;
+                    if (rc == STEP_NOCLOSE) er |= ERR_NOCLOSE;
+                }
+                j = atomicAdd(&s_next, 1);
+                have = j < T1;
+                if (have) gdt_tile_lane_init<Real>(w, L, j, io, S, st);
+            }
+        }
+    }
+    mx = wave_max_i32(mx);
+    er = wave_or_i32(er);
+    npred = wave_sum_u32(npred);
+    nfb = wave_sum_u32(nfb);
+    nsum = wave_sum_u32(nsum);
+    if ((tid & 63) == 0) {
+        if (mx > 0) atomicMax(&status[ST_REQUIRED], mx);
+        if (er) atomicOr(&status[ST_ERRFLAGS], er);
+        if (npred) atomicAdd(&counters[CT_PRED], (unsigned long long)npred);
+        if (nfb) atomicAdd(&counters[CT_FALLBACK], (unsigned long long)nfb);
+        if (nsum) atomicAdd(&counters[CT_NBRSUM], (unsigned long long)nsum);
+    }
+}
+
+hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nsets,
+                                    const GdtChunk* tiles, int ntiles, int block, int total_n,
+                                    const GdtRingIO& io, int* counts, int* status,
+                                    unsigned long long* counters, int* defer, hipStream_t s)
+{
+    if (ntiles <= 0) return hipSuccess;
+    if (block > 512) block = 512;
+    if (precision == 32)
+        hipLaunchKernelGGL(k_one_ring_tile<32>, dim3(ntiles), dim3(block), 0, s, d_sets, tiles, io,
+                           counts, status, counters, defer);
+    else
+        hipLaunchKernelGGL(k_one_ring_tile<64>, dim3(ntiles), dim3(block), 0, s, d_sets, tiles, io,
+                           counts, status, counters, defer);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // deferred points: direct routine over the list (count read on the device)
+    return gdt_launch_one_ring(precision, d_sets, nsets, nullptr, 0, 256, defer,
+                               reinterpret_cast<const uint32_t*>(status + ST_NDEFER), total_n, io,
+                               counts, status, counters, s);
 }
 
 hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,

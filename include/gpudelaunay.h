@@ -65,6 +65,7 @@ typedef struct {
     double bsx, bsy;                 /* cell sides */
     float ms_cells;                  /* last cell-list build (profile = 1), ms */
     float ms_ring;                   /* last one-ring kernel (profile = 1), ms */
+    int deferred;                    /* points the LDS tile kernel handed to the direct kernel */
 } gdt_stats;
 
 /* ---- context ---------------------------------------------------------- */

@@ -1,9 +1,11 @@
-// Host build of the per-point device routines (gdt_point.h) for CPU tests.
+// Host build of the per-point device routines for CPU tests.
 // Builds the cell-ordered arrays the GPU cell-list kernels build, then runs
-// the FP64 or FP32-filtered one-ring routine for every point.
-// Usage: emul <points.bin> <n> <lx> <ly> <cell_size> <tiled> <precision 64|32> <stride> <out.bin>
+// either the direct per-point routine (gdt_point.h) or the LDS-staged tile
+// routine (gdt_tile.h, one emulated block per tile) for every point.
+// Usage: emul <points.bin> <n> <lx> <ly> <cell_size> <tiled> <precision 64|32> <stride>
+//             <out.bin> [mode 0=direct 1=tile]
 // out.bin: int32 counts[n], int32 nbrs[n*stride], uint64 npred, uint64 nfb, int32 errflags
-#include "gdt_point.h"
+#include "gdt_tile.h"
 
 #include <cstdio>
 #include <cstdlib>
@@ -11,10 +13,12 @@
 
 int main(int argc, char** argv)
 {
-    if (argc != 10) { std::fprintf(stderr, "bad args\n"); return 2; }
+    if (argc != 10 && argc != 11) { std::fprintf(stderr, "bad args\n"); return 2; }
     const int n = std::atoi(argv[2]);
     const double lx = std::atof(argv[3]), ly = std::atof(argv[4]), a = std::atof(argv[5]);
     const int tiled = std::atoi(argv[6]), prec = std::atoi(argv[7]), stride = std::atoi(argv[8]);
+    const int mode = argc == 11 ? std::atoi(argv[10]) : 0;
+    if (mode == 1 && !tiled) { std::fprintf(stderr, "tile mode needs tiled order\n"); return 2; }
     std::vector<double> xy(2 * (size_t)n);
     FILE* f = std::fopen(argv[1], "rb");
     if (!f || std::fread(xy.data(), sizeof(double), xy.size(), f) != xy.size()) return 3;
@@ -49,12 +53,52 @@ int main(int argc, char** argv)
     io.cstart = cstart.data(); io.perm = perm.data(); io.off64 = off64.data();
     io.off32 = off32.data(); io.scell = scell.data(); io.nbrs = nbrs.data(); io.stride = stride;
     unsigned long long npred = 0, nfb = 0;
-    int err = 0;
-    for (int j = 0; j < n; ++j) {
-        GdtRingResult r = prec == 32 ? gdt_one_ring_f32(S, j, io) : gdt_one_ring_f64(S, j, io);
+    int err = 0, ndefer = 0;
+    auto acc = [&](int j, const GdtRingResult& r) {
         counts[perm[j]] = r.count;
         npred += r.npred; nfb += r.nfb; err |= r.err;
+    };
+    if (mode == 0) {
+        for (int j = 0; j < n; ++j)
+            acc(j, prec == 32 ? gdt_one_ring_f32(S, j, io) : gdt_one_ring_f64(S, j, io));
+    } else {
+        // one emulated block per tile: stage counts, scan, copy, then every point
+        std::vector<uint32_t> cs(GDT_WIN_CELLS + 1);
+        std::vector<int> gstart(GDT_WIN_CELLS), orig(GDT_WIN_MAXPTS);
+        std::vector<gdt_d2> X64(GDT_WIN_MAXPTS);
+        std::vector<gdt_f2> X32(GDT_WIN_MAXPTS);
+        GdtTileLds L = {cs.data(), gstart.data(), X64.data(), X32.data(), orig.data()};
+        const int ntiles = S.nkeys / GDT_TILE_CELLS;
+        for (int t = 0; t < ntiles; ++t) {
+            const GdtTileWin w = gdt_tile_window(S, t);
+            std::vector<uint32_t> c_cnt(GDT_WIN_CELLS);
+            for (int c = 0; c < GDT_WIN_CELLS; ++c)
+                gdt_tile_count(S, w, c, io.cstart, gstart[c], c_cnt[c]);
+            uint32_t run = 0;
+            for (int c = 0; c < GDT_WIN_CELLS; ++c) { cs[c] = run; run += c_cnt[c]; }
+            cs[GDT_WIN_CELLS] = run;
+            const bool fits = run <= GDT_WIN_MAXPTS && S.rmax >= GDT_HALO;
+            if (fits)
+                for (int c = 0; c < GDT_WIN_CELLS; ++c) {
+                    if (prec == 32) gdt_tile_copy<true>(S, c, gstart[c], cs[c], c_cnt[c], io, L);
+                    else gdt_tile_copy<false>(S, c, gstart[c], cs[c], c_cnt[c], io, L);
+                }
+            const int k0 = S.c0 + t * GDT_TILE_CELLS;
+            for (int j = (int)io.cstart[k0]; j < (int)io.cstart[k0 + GDT_TILE_CELLS]; ++j) {
+                GdtRingResult r;
+                if (fits) r = prec == 32 ? gdt_tile_ring<32>(S, w, L, j, io) : gdt_tile_ring<64>(S, w, L, j, io);
+                if (!fits || (r.err & ERR_DEFER)) {
+                    const unsigned np = fits ? r.npred : 0u, nf = fits ? r.nfb : 0u;
+                    ++ndefer;
+                    r = prec == 32 ? gdt_one_ring_f32(S, j, io) : gdt_one_ring_f64(S, j, io);
+                    r.npred += np;
+                    r.nfb += nf;
+                }
+                acc(j, r);
+            }
+        }
     }
+    if (mode == 1) std::fprintf(stderr, "deferred %d of %d points\n", ndefer, n);
     f = std::fopen(argv[9], "wb");
     if (!f) return 4;
     std::fwrite(counts.data(), sizeof(int), n, f);

@@ -1,7 +1,9 @@
-"""The per-point device routines (gdt_point.h) compiled for the host with
-g++ under AddressSanitizer/UBSan and run on the CPU: FP64 and FP32-filtered
-paths must reproduce the oracle's rings bit-exactly and identically to each
-other, on uniform, near-degenerate (jittered lattice) and rectangular inputs."""
+"""The device routines compiled for the host with g++ under AddressSanitizer
+and UBSan and run on the CPU: the direct per-point routine (gdt_point.h) and
+the LDS-staged tile routine (gdt_tile.h, one emulated block per tile).  FP64
+and FP32-filtered paths must reproduce the oracle's rings bit-exactly and be
+byte-identical to each other, on uniform, near-degenerate (jittered lattice),
+rectangular and small inputs."""
 import os
 import subprocess
 
@@ -25,12 +27,12 @@ def emul(tmp_path_factory):
     return exe
 
 
-def run(exe, tmp, xy, lx, ly, tiled, prec, stride=32):
+def run(exe, tmp, xy, lx, ly, tiled, prec, stride=32, mode=0):
     n = xy.shape[0]
     pin, pout = os.path.join(tmp, "p.bin"), os.path.join(tmp, "o.bin")
     xy.astype(np.float64).tofile(pin)
     subprocess.run([exe, pin, str(n), repr(lx), repr(ly), "1.0", str(tiled), str(prec),
-                    str(stride), pout], check=True)
+                    str(stride), pout, str(mode)], check=True)
     raw = np.fromfile(pout, dtype=np.uint8)
     counts = raw[:4 * n].view(np.int32)
     nbrs = raw[4 * n:4 * n + 4 * n * stride].view(np.int32).reshape(n, stride)
@@ -44,6 +46,8 @@ CASES = {
     "uniform4096": (lambda: inputs.uniform_points(4096, 64.0, 64.0, seed=1), 64.0, 64.0),
     "lattice64": (lambda: inputs.lattice_points(64, 64, 1e-5, seed=3), *inputs.lattice_box(64, 64)),
     "rect1000": (lambda: inputs.uniform_points(1000, 40.0, 25.0, seed=7), 40.0, 25.0),
+    "small300": (lambda: inputs.uniform_points(300, 17.0, 17.0, seed=9), 17.0, 17.0),
+    "tiny60": (lambda: inputs.uniform_points(60, 8.0, 8.0, seed=9), 8.0, 8.0),
 }
 
 
@@ -54,18 +58,18 @@ def test_emulated_kernels_match_oracle(emul, tmp_path, case):
     on, oc = oracle.triangulate(xy, lx, ly, 1.0, 16)
     ref = rings.ring_hash(on, oc)
     outs = {}
-    for prec in (64, 32):
-        for tiled in (1, 0):
-            nb, c, npred, nfb, err = run(emul, str(tmp_path), xy, lx, ly, tiled, prec)
+    # (tiled order, mode): direct routine in both cell orders, tile routine
+    for tiled, mode in ((1, 0), (0, 0), (1, 1)):
+        for prec in (64, 32):
+            nb, c, npred, nfb, err = run(emul, str(tmp_path), xy, lx, ly, tiled, prec, mode=mode)
             assert err == 0
-            assert rings.ring_hash(nb, c) == ref, (prec, tiled)
+            assert rings.ring_hash(nb, c) == ref, (prec, tiled, mode)
             if prec == 64:
                 assert nfb == 0
-            outs[(prec, tiled)] = (nb, c)
-    # identical decisions: FP32-filtered rows equal FP64 rows byte for byte
-    for tiled in (1, 0):
-        assert np.array_equal(outs[(64, tiled)][1], outs[(32, tiled)][1])
-        n64, c64 = outs[(64, tiled)]
-        n32, _ = outs[(32, tiled)]
+            outs[(prec, tiled, mode)] = (nb, c)
+        # identical decisions: FP32-filtered rows equal FP64 rows byte for byte
+        n64, c64 = outs[(64, tiled, mode)]
+        n32, c32 = outs[(32, tiled, mode)]
+        assert np.array_equal(c64, c32)
         for i in range(len(c64)):
             assert np.array_equal(n64[i, :c64[i]], n32[i, :c64[i]])

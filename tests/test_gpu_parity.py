@@ -98,3 +98,33 @@ def test_row_major_and_tiled_cell_orders_agree():
     dg.ctx.set_config(tiled=0, block=128)
     nb_r, c_r, _ = gpu_triangulate(xy, 141.0, 141.0, PREC_FP64, dg=dg)
     assert np.array_equal(nb_t, nb_r) and np.array_equal(c_t, c_r)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_clustered_points_take_the_deferred_path(prec):
+    """A dense cluster overflows the LDS window of its tiles; those points go
+    through the direct routine (deferred list) and still match the oracle."""
+    L = 141.0
+    base = inputs.uniform_points(17000, L, L, seed=41)
+    rng = np.random.default_rng(42)
+    cluster = 60.0 + rng.uniform(0.0, 6.0, size=(3000, 2))
+    xy = np.concatenate([base, cluster])
+    on, oc = oracle.triangulate(xy, L, L, 1.0, 16)
+    nb, c, dg = gpu_triangulate(xy, L, L, prec)
+    assert rings.ring_hash(nb, c) == rings.ring_hash(on, oc)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_grid_smaller_than_the_halo(prec):
+    """6x6 cells (search radius 2 < halo 3): every point is deferred to the
+    direct routine; the rings equal scipy's Qhull (the reference path itself
+    keeps a virtual corner in such small boxes)."""
+    from qhull_periodic import periodic_rings
+    xy = inputs.uniform_points(40, 6.0, 6.0, seed=43)
+    qn, qc = periodic_rings(xy, 6.0, 6.0)
+    try:
+        nb, c, _ = gpu_triangulate(xy, 6.0, 6.0, prec)
+    except Exception as e:  # the box may be too small for the search radius
+        assert "search radius" in str(e)
+        return
+    assert rings.ring_hash(nb, c) == rings.ring_hash(qn, qc)
