@@ -406,7 +406,7 @@ int gdt_triangulate(gdt_context* c, const double* d_xy, int n, double lx, double
                                            c->counters.p, c->defer.p, s));
     else
         GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), c->rch.p, c->nrch,
-                                      c->cfg.block, nullptr, nullptr, 0, ring_io(c, d_nbrs, stride),
+                                      c->cfg.block, nullptr, nullptr, 0, 0, ring_io(c, d_nbrs, stride),
                                       d_counts, c->status.p, c->counters.p, s));
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[2], s));
     rc = finish_rings(c, stride, required, s);
@@ -496,7 +496,7 @@ static int repair_flagged(gdt_context* c, const int* d_repair, int stride, int* 
     GDT_CHECK(gdt_launch_compact(d_repair, c->slot_of.p, c->total_n, c->mark.p, c->markscan.p,
                                  c->worklist.p, worklen, c->scan_tmp.p, s));
     GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), nullptr, 0,
-                                  c->cfg.block, c->worklist.p, worklen, c->total_n,
+                                  c->cfg.block, c->worklist.p, worklen, 0, c->total_n,
                                   ring_io(c, d_nbrs, stride), d_counts, c->status.p,
                                   c->counters.p, s));
     return GDT_OK;

@@ -64,7 +64,9 @@ struct GdtChunk { int set; int start; };   // a block's work: set id and first l
 // Cell order.  Tiled: 16x16-cell tiles in row-major tile order, cells
 // row-major inside a tile, so one tile's points are one contiguous slot range
 // (the unit of work of the LDS-staged one-ring kernel).  Untiled: row-major.
+#ifndef GDT_TILE_SHIFT
 #define GDT_TILE_SHIFT 4
+#endif
 #define GDT_TILE (1 << GDT_TILE_SHIFT)
 #define GDT_TILE_CELLS (GDT_TILE * GDT_TILE)
 
@@ -176,8 +178,10 @@ inline GdtSet gdt_make_set(int p0, int n, int c0, double lx, double ly, double c
 }
 
 // ---------------------------------------------------------------------------
-// FP64 displacement and predicates (one fixed operation order; the library is
-// compiled with -ffp-contract=off, and fused operations are written out).
+// FP64 displacement and predicates.  One fixed operation order with explicit
+// fused multiply-adds (the library is compiled with -ffp-contract=off), used
+// by every FP64 decision of both paths, so the filtered path's fallback
+// reproduces the FP64 path bit for bit.
 // ---------------------------------------------------------------------------
 GDT_HD inline double gdt_base64(int dc, double bs, double ov)
 {
@@ -188,8 +192,10 @@ GDT_HD inline double gdt_disp64(double base, double op) { return base + op; }
 // a x c  (> 0: c strictly left of the ray from the origin through a)
 GDT_HD inline double gdt_orient64(double ax, double ay, double cx, double cy)
 {
-    return ax * cy - ay * cx;
+    return fma(ax, cy, -(ay * cx));
 }
+
+GDT_HD inline double gdt_dist2_64(double x, double y) { return fma(x, x, y * y); }
 
 // In-circle coefficients of the circle through the origin, a and b:
 //   P(c) = K0 |c|^2 + Kx cx + Ky cy = K0 (|c - q|^2 - R^2),
@@ -197,20 +203,22 @@ GDT_HD inline double gdt_orient64(double ax, double ay, double cx, double cy)
 struct Circ64 { double k0, kx, ky; };
 GDT_HD inline Circ64 gdt_circ64(double ax, double ay, double bx, double by)
 {
-    const double aa = ax * ax + ay * ay;
-    const double bb = bx * bx + by * by;
+    const double aa = gdt_dist2_64(ax, ay);
+    const double bb = gdt_dist2_64(bx, by);
     Circ64 c;
-    c.k0 = ax * by - ay * bx;
-    c.kx = bb * ay - aa * by;
-    c.ky = aa * bx - bb * ax;
+    c.k0 = fma(ax, by, -(ay * bx));
+    c.kx = fma(bb, ay, -(aa * by));
+    c.ky = fma(aa, bx, -(bb * ax));
     return c;
+}
+GDT_HD inline double gdt_incircle_cc64(const Circ64& k, double cx, double cy, double cc)
+{
+    return fma(k.k0, cc, fma(k.kx, cx, k.ky * cy));
 }
 GDT_HD inline double gdt_incircle64(const Circ64& k, double cx, double cy)
 {
-    const double cc = cx * cx + cy * cy;
-    return k.k0 * cc + (k.kx * cx + k.ky * cy);
+    return gdt_incircle_cc64(k, cx, cy, gdt_dist2_64(cx, cy));
 }
-GDT_HD inline double gdt_dist2_64(double x, double y) { return x * x + y * y; }
 
 // ---------------------------------------------------------------------------
 // FP32 filter constants.  u = 2^-24.  The FP32 displacement differs from the

@@ -35,16 +35,22 @@ hipError_t gdt_export_cells(const GdtSet* d_sets, const GdtSet& h_set, const Gdt
                             unsigned int* cell_sizes, int* idxs, int nmax, hipStream_t s);
 hipError_t gdt_max_occupancy(const uint32_t* ccount, int total_keys, int* d_out, hipStream_t s);
 
-// One-ring of every point (chunks of `block` points) or of the worklist slots.
+// One-ring of every point (chunks of `block` points) or of the worklist slots
+// [skip, *worklen) (max_work bounds *worklen).
 hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
                                const GdtChunk* chunks, int nch, int block, const int* worklist,
-                               const uint32_t* worklen, int max_work, const GdtRingIO& io,
-                               int* counts, int* status, unsigned long long* counters,
-                               hipStream_t s);
+                               const uint32_t* worklen, int skip, int max_work,
+                               const GdtRingIO& io, int* counts, int* status,
+                               unsigned long long* counters, hipStream_t s);
+
+// Deferred points: the first GDT_WAVE_CAP one wavefront each (GDT_WAVE_BLOCKS
+// blocks of 256 threads), the rest one lane each.
+#define GDT_WAVE_CAP 8192
+#define GDT_WAVE_BLOCKS 512
 
 // One-ring of every point, one block per 16x16-cell tile (tiled cell order).
-// Points of windows that do not fit are deferred to the direct routine
-// (second launch over `defer`, size total_n, count in status[ST_NDEFER]).
+// Points the tile kernel cannot finish from LDS are deferred (second and
+// third launches over `defer`, size total_n, count in status[ST_NDEFER]).
 hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nsets,
                                     const GdtChunk* tiles, int ntiles, int block, int total_n,
                                     const GdtRingIO& io, int* counts, int* status,

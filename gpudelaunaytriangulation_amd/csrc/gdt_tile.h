@@ -6,10 +6,11 @@
 // written as a coordinate in the window frame,
 //     X = f * bs + off,   f = (window cell index) - GDT_FRAME,
 // in FP64 and (filtered path) FP32.  A lane then walks its point's one-ring
-// exactly as in gdt_point.h (nearest neighbour, then gift wrapping with
-// certified stops), reading the Chebyshev rings r <= GDT_HALO as row
-// segments of the LDS window and rings beyond it from the global cell list,
-// with the same frame formula.  A displacement is d = X(c) - X(v).
+// as in gdt_point.h (nearest neighbour, then gift wrapping with certified
+// stops), each search step scanning the block of cells [-GDT_SCAN,GDT_SCAN]^2
+// around the point and, when that does not certify the step, the rest of the
+// step's circle bounding box while it stays inside the window.  A
+// displacement is d = X(c) - X(v).
 //
 // The filtered path certifies against the FP64 displacement of the same
 // frame formula.  FP32 input error per coordinate (u = 2^-24):
@@ -21,12 +22,16 @@
 
 #include "gdt_point.h"
 
-#define GDT_HALO 3
+#define GDT_SCAN 3          // half-width (cells) of the block every search step scans
+#ifndef GDT_HALO
+#define GDT_HALO 4          // window halo (cells): reach of the circle-box extension
+#endif
 #define GDT_WIN (GDT_TILE + 2 * GDT_HALO)
 #define GDT_WIN_CELLS (GDT_WIN * GDT_WIN)
-#define GDT_WIN_MAXPTS 768
+#ifndef GDT_WIN_MAXPTS
+#define GDT_WIN_MAXPTS 1024 // LDS entries per window (a fuller window defers its tile)
+#endif
 #define GDT_FRAME (GDT_WIN / 2)
-#define GDT_RINGREG 16      // ring entries held in registers before a row is written
 
 struct GdtTileWin {
     int tx, ty;          // tile coordinates
@@ -96,79 +101,14 @@ GDT_HD inline void gdt_tile_copy(const GdtSet& S, int c, int gstart, uint32_t at
     }
 }
 
-// LDS entries of Chebyshev ring r (r <= GDT_HALO) around window cell (wx,wy):
-// top and bottom rows as one segment each, the side cells of the rows between.
-template <class Fn>
-GDT_HD inline void gdt_lds_ring(const GdtTileLds& L, int wx, int wy, int r, Fn& fn)
-{
-    if (r == 0) {
-        const int c = wy * GDT_WIN + wx;
-        for (uint32_t k = L.cs[c]; k < L.cs[c + 1]; ++k) fn((int)k);
-        return;
-    }
-    for (int side = 0; side < 2; ++side) {
-        const int c0 = (side ? wy + r : wy - r) * GDT_WIN + wx - r;
-        const uint32_t e = L.cs[c0 + 2 * r + 1];
-        for (uint32_t k = L.cs[c0]; k < e; ++k) fn((int)k);
-    }
-    for (int y = wy - r + 1; y < wy + r; ++y) {
-        const int cl = y * GDT_WIN + wx - r, cr = cl + 2 * r;
-        for (uint32_t k = L.cs[cl]; k < L.cs[cl + 1]; ++k) fn((int)k);
-        for (uint32_t k = L.cs[cr]; k < L.cs[cr + 1]; ++k) fn((int)k);
-    }
-}
-
-// Global cells of Chebyshev ring r around global cell (cx,cy): fn(slot, dx, dy).
-template <class Fn>
-GDT_HD inline void gdt_global_ring(const GdtSet& S, int cx, int cy, int r, const uint32_t* cstart,
-                                   Fn& fn)
-{
-    const int ncell = r ? 8 * r : 1;
-    for (int q = 0; q < ncell; ++q) {
-        int dx, dy;
-        gdt_ring_cell(r, q, dx, dy);
-        const int x = gdt_wrap(cx + dx, S.xsize), y = gdt_wrap(cy + dy, S.ysize);
-        const int key = S.c0 + gdt_cell_key(S, x, y);
-        const uint32_t b = cstart[key], e = cstart[key + 1];
-        for (uint32_t s = b; s < e; ++s) fn((int)s, dx, dy);
-    }
-}
-
-// Ring entries: the first GDT_RINGREG in a register shift queue, written as
-// one burst at the end (one write-back per row), later ones directly.
-struct GdtRingOut {
-    int q[GDT_RINGREG];
-    int count;
-    GDT_HD inline void push(int x, int* row, int stride)
-    {
-        if (count < GDT_RINGREG) {
-#pragma unroll
-            for (int m = 0; m < GDT_RINGREG - 1; ++m) q[m] = q[m + 1];
-            q[GDT_RINGREG - 1] = x;
-        } else if (count < stride) {
-            row[count] = x;
-        }
-        ++count;
-    }
-    GDT_HD inline void flush(int* row, int stride) const
-    {
-        const int held = count < GDT_RINGREG ? count : GDT_RINGREG;
-#pragma unroll
-        for (int m = 0; m < GDT_RINGREG; ++m) {
-            const int i = m - (GDT_RINGREG - held);
-            if (i >= 0 && i < stride) row[i] = q[m];
-        }
-    }
-};
-
 // LDS entries of the block of cells [x0,x1] x [y0,y1] (offsets from window
 // cell (wx,wy), inside the window) minus the already scanned square
 // [-hole,hole]^2 (hole < 0: none), visited as one flat loop over row segments
 // (each row of cells is one contiguous LDS range; a row crossing the hole is
 // two segments), so the trip count is the number of points, not cells.
 template <class Fn>
-GDT_HD inline void gdt_lds_block(const GdtTileLds& L, int wx, int wy, int x0, int x1, int y0,
-                                 int y1, int hole, Fn& fn)
+GDT_HD inline unsigned gdt_lds_block(const GdtTileLds& L, int wx, int wy, int x0, int x1, int y0,
+                                     int y1, int hole, Fn& fn)
 {
     // segment sg: row y0 + sg/2, side sg&1
     auto seg = [&](int sg, uint32_t& b, uint32_t& e) {
@@ -200,6 +140,41 @@ GDT_HD inline void gdt_lds_block(const GdtTileLds& L, int wx, int wy, int x0, in
         fn((int)k);
         ++k;
     }
+    return (unsigned)total;
+}
+
+// The block [-GDT_SCAN,GDT_SCAN]^2 around window cell c0 as one flat loop over
+// its rows (each a contiguous LDS range), so that a wave waits for its slowest
+// lane once per step rather than once per row.  Returns the entries visited.
+template <class Fn>
+GDT_HD inline unsigned gdt_scan_block(const GdtTileLds& L, int c0, Fn& fn)
+{
+    int c = c0 - GDT_SCAN * GDT_WIN;
+    const int cend = c0 + (GDT_SCAN + 1) * GDT_WIN;
+    uint32_t k = L.cs[c - GDT_SCAN], e = L.cs[c + GDT_SCAN + 1];
+    unsigned n = 0;
+    for (;;) {
+        if (k == e) {
+            c += GDT_WIN;
+            if (c == cend) break;
+            k = L.cs[c - GDT_SCAN];
+            e = L.cs[c + GDT_SCAN + 1];
+            continue;
+        }
+        fn((int)k);
+        ++k;
+        ++n;
+    }
+    return n;
+}
+
+// Whether the cell block [dx0,dx1] x [dy0,dy1] around window cell c0 lies in
+// the staged window and within the grid's search radius (no cell twice).
+GDT_HD inline bool gdt_in_window(const GdtSet& S, int c0, int dx0, int dx1, int dy0, int dy1)
+{
+    const int wx = c0 % GDT_WIN, wy = c0 / GDT_WIN;
+    return wx + dx0 >= 0 && wx + dx1 < GDT_WIN && wy + dy0 >= 0 && wy + dy1 < GDT_WIN &&
+           dx0 >= -S.rmax && dx1 <= S.rmax && dy0 >= -S.rmax && dy1 <= S.rmax;
 }
 
 // Cell-offset bounding box of the circle through the origin, a and b, given
@@ -217,18 +192,23 @@ GDT_HD inline void gdt_circle_cells(T qxlo, T qxhi, T qylo, T qyhi, T R, T ox, T
 
 // ---------------------------------------------------------------------------
 // Per point, one search step at a time (the kernel interleaves the points of
-// a tile across its lanes step by step).  Every step is one scan of the 7x7
-// block of cells [-HALO,HALO]^2 around v's cell, one tight loop per row of
-// cells (a row is one contiguous LDS range); the body evaluates the predicates
-// of the lane's current search and takes a single (rare) branch when the
-// candidate wins or a filtered predicate is undecided.
-//   step 0 (nearest neighbour): certified when its distance is below cov(3);
+// a tile across its lanes step by step).  A step scans the block of cells
+// [-GDT_SCAN,GDT_SCAN]^2 around v's cell; the loop body evaluates the
+// predicates of the lane's current search and takes a single (rare) branch
+// when the candidate wins or a filtered predicate is undecided.
+//   step 0 (nearest neighbour): certified when its distance is below
+//     cov = GDT_SCAN*bs_min + (distance of v to its cell walls);
 //   step k (successor of edge (v,a)): the final circle (v, a, best) holds no
 //     scanned point, so it is certified when its conservative bounding box
-//     lies inside the block, or 2R <= cov(3).
-// A step that cannot be certified returns ERR_DEFER: the kernel hands the
-// point to the direct global-memory routine (gdt_point.h), which rewrites its
-// whole row.  Candidates are LDS entries k.
+//     lies inside the block, or 2R <= cov.
+// Otherwise the step scans the rest of the bounding box of its current
+// circle (nearest neighbour: of the disk through the current best), provided
+// the box lies in the window.  The circles through v and a with centres on
+// the left of v->a are nested along the bisector, so every later best's
+// circle lies inside the scanned box and the step is then certified.  A box
+// leaving the window returns STEP_DEFER: the kernel hands the point to the
+// wavefront routine (gdt_wave.h), which rewrites its whole row.  Candidates
+// are LDS entries k.
 // ---------------------------------------------------------------------------
 template <class Real>
 struct GdtTileLane {
@@ -298,53 +278,11 @@ GDT_HD inline double gdt_cell_offset(double X, int wx, double bs)
 // Result of one step: 0 running, 1 ring closed, 2 defer, 3 non-generic (no close)
 enum { STEP_RUN = 0, STEP_DONE = 1, STEP_DEFER = 2, STEP_NOCLOSE = 3 };
 
-GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const GdtRingIO& io,
-                                    GdtTileLane<double>& st, unsigned& npred)
+// Emit the step's neighbour and advance the lane (shared tail of both paths).
+template <class Real>
+GDT_HD inline int gdt_tile_advance(const GdtTileLds& L, const GdtRingIO& io, GdtTileLane<Real>& st,
+                                   bool wrap, int bk, Real bx, Real by)
 {
-    const bool wrap = st.ak >= 0;
-    int bk = -1;
-    double bx = 0., by = 0., bd2 = 0.;
-    Circ64 K = {0., 0., 0.};
-    const double ax = st.ax, ay = st.ay, vx = st.vx, vy = st.vy;
-    const int iv = st.iv, ak = st.ak;
-#pragma unroll 1
-    for (int ry = -GDT_HALO; ry <= GDT_HALO; ++ry) {
-        const int c = st.c0 + ry * GDT_WIN;
-        const uint32_t e = L.cs[c + GDT_HALO + 1];
-        for (uint32_t kk = L.cs[c - GDT_HALO]; kk < e; ++kk) {
-            const int k = (int)kk;
-            const gdt_d2 X = L.X64[k];
-            const double px = X.x - vx, py = X.y - vy;
-            const double cc = px * px + py * py;
-            const double O = gdt_orient64(ax, ay, px, py);
-            const double P = K.k0 * cc + (K.kx * px + K.ky * py);
-            const bool take = k != iv && (wrap ? (k != ak && O > 0. && (bk < 0 || P < 0.))
-                                               : (bk < 0 || cc < bd2));
-            if (take) {
-                bk = k; bx = px; by = py; bd2 = cc;
-                if (wrap) K = gdt_circ64(ax, ay, px, py);
-            }
-        }
-        npred += e - L.cs[c - GDT_HALO];
-    }
-    const double bsmin = GDT_FMIN(S.bsx, S.bsy);
-    const double cov = (double)GDT_HALO * bsmin + st.covb;
-    bool ok = false;
-    if (!wrap) {
-        ok = bk >= 0 && cov > 0. && bd2 <= cov * cov * (1.0 - 1e-12);
-    } else if (bk >= 0 && K.k0 > 1e-6 * (fabs(ax * by) + fabs(ay * bx))) {
-        const double qx = -K.kx / (2. * K.k0), qy = -K.ky / (2. * K.k0);
-        const double rad = sqrt(qx * qx + qy * qy);
-        const double m = 1e-6 * (fabs(qx) + fabs(qy) + rad) + 1e-9 * S.bsx;
-        const int wx = st.c0 % GDT_WIN, wy = st.c0 / GDT_WIN;
-        const double ox = gdt_cell_offset(vx, wx, S.bsx), oy = gdt_cell_offset(vy, wy, S.bsy);
-        int dx0, dx1, dy0, dy1;
-        gdt_circle_cells<double>(qx - m, qx + m, qy - m, qy + m, rad, ox, oy, S.bsx, S.bsy, 1e-3,
-                                 dx0, dx1, dy0, dy1);
-        ok = (dx0 >= -GDT_HALO && dx1 <= GDT_HALO && dy0 >= -GDT_HALO && dy1 <= GDT_HALO) ||
-             (cov > 0. && K.kx * K.kx + K.ky * K.ky <= cov * cov * (K.k0 * K.k0) * (1.0 - 1e-6));
-    }
-    if (!ok) return STEP_DEFER;
     if (wrap && bk == st.n0) return STEP_DONE;
     if (!wrap) st.n0 = bk;
     if (st.count < io.stride) st.row[st.count] = L.orig[bk];
@@ -354,6 +292,64 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
     return STEP_RUN;
 }
 
+GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const GdtRingIO& io,
+                                    GdtTileLane<double>& st, unsigned& npred)
+{
+    const bool wrap = st.ak >= 0;
+    int bk = -1;
+    double bx = 0., by = 0., bd2 = 0.;
+    Circ64 K = {0., 0., 0.};
+    const double ax = st.ax, ay = st.ay, vx = st.vx, vy = st.vy;
+    const int iv = st.iv, ak = st.ak;
+    auto visit = [&](int k) {
+        const gdt_d2 X = L.X64[k];
+        const double px = X.x - vx, py = X.y - vy;
+        const double cc = gdt_dist2_64(px, py);
+        const double O = gdt_orient64(ax, ay, px, py);
+        const double P = gdt_incircle_cc64(K, px, py, cc);
+        // predicates combined without branches; one branch for a new best
+        const bool none = bk < 0;
+        const bool take_w = (O > 0.) & ((P < 0.) | none) & (k != ak);
+        const bool take_n = none | (cc < bd2);
+        const bool take = (k != iv) & (wrap ? take_w : take_n);
+        if (take) {
+            bk = k; bx = px; by = py; bd2 = cc;
+            K = gdt_circ64(ax, ay, px, py);
+        }
+    };
+    npred += gdt_scan_block(L, st.c0, visit);
+
+    const double cov = (double)GDT_SCAN * GDT_FMIN(S.bsx, S.bsy) + st.covb;
+    const int wx = st.c0 % GDT_WIN, wy = st.c0 / GDT_WIN;
+    const double ox = gdt_cell_offset(vx, wx, S.bsx), oy = gdt_cell_offset(vy, wy, S.bsy);
+    bool ok = false, boxed = false;
+    int dx0 = 0, dx1 = 0, dy0 = 0, dy1 = 0;
+    if (!wrap) {
+        if (bk >= 0) {
+            ok = cov > 0. && bd2 <= cov * cov * (1.0 - 1e-12);
+            const double rad = sqrt(bd2) * (1.0 + 1e-9);
+            gdt_circle_cells<double>(0., 0., 0., 0., rad, ox, oy, S.bsx, S.bsy, 1e-3, dx0, dx1, dy0,
+                                     dy1);
+            boxed = true;
+        }
+    } else if (bk >= 0 && K.k0 > 1e-6 * (fabs(ax * by) + fabs(ay * bx))) {
+        const double qx = -K.kx / (2. * K.k0), qy = -K.ky / (2. * K.k0);
+        const double rad = sqrt(qx * qx + qy * qy);
+        const double m = 1e-6 * (fabs(qx) + fabs(qy) + rad) + 1e-9 * S.bsx;
+        gdt_circle_cells<double>(qx - m, qx + m, qy - m, qy + m, rad, ox, oy, S.bsx, S.bsy, 1e-3,
+                                 dx0, dx1, dy0, dy1);
+        ok = (dx0 >= -GDT_SCAN && dx1 <= GDT_SCAN && dy0 >= -GDT_SCAN && dy1 <= GDT_SCAN) ||
+             (cov > 0. && K.kx * K.kx + K.ky * K.ky <= cov * cov * (K.k0 * K.k0) * (1.0 - 1e-6));
+        boxed = true;
+    }
+    if (!ok && boxed && gdt_in_window(S, st.c0, dx0, dx1, dy0, dy1)) {
+        npred += gdt_lds_block(L, wx, wy, dx0, dx1, dy0, dy1, GDT_SCAN, visit);
+        ok = true;
+    }
+    if (!ok) return STEP_DEFER;
+    return gdt_tile_advance<double>(L, io, st, wrap, bk, bx, by);
+}
+
 GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const GdtRingIO& io,
                                     GdtTileLane<float>& st, unsigned& npred, unsigned& nfb)
 {
@@ -361,99 +357,111 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
     const float bsM = S.bsmax32;
     const float eps1 = 5.f * u;
     const float eps0 = 7.5f * u * (GDT_FMAXF(fabsf(st.vx), fabsf(st.vy)) + bsM);
-    const float mc = (float)(GDT_HALO + 1) * bsM * 1.0001f;
     const float cP = 8.f * (4.f * eps1 + 8.f * u), eP = 8.f * eps0;
     const bool wrap = st.ak >= 0;
     const float ax = st.ax, ay = st.ay, vx = st.vx, vy = st.vy;
     const int iv = st.iv, ak = st.ak;
     const float ma = GDT_FMAXF(fabsf(ax), fabsf(ay));
-    const float BO = fmaf(4.f * (2.f * eps1 + 2.f * u) * ma + 4.f * eps0, mc, 4.f * eps0 * ma + 1e-30f);
     const float cE = 2.f * (2.f * eps1 + 3.f * u);
-    const float eabs = 8.f * eps0 * mc + 4.f * eps0 * eps0 + 1e-30f;
     int bk = -1;
     float bx = 0.f, by = 0.f, bd2 = 0.f, bE = 0.f;
     float K0 = 0.f, Kx = 0.f, Ky = 0.f, pab = 0.f, sab = 0.f, BP = 0.f;
+    // Bounds that depend on mc, a bound on the max-norm of the displacements
+    // of the region being scanned: (h+1) cells for offsets |dx|,|dy| <= h.
+    float mc = 0.f, BO = 0.f, eabs = 0.f;
+    auto set_bp = [&]() {
+        const float A0 = eP * sab * sab * sab + 1e-30f;
+        const float A1 = cP * pab * sab + 3.f * eP * sab * sab;
+        const float A2 = cP * pab + 3.f * eP * sab;
+        BP = fmaf(mc, fmaf(mc, fmaf(mc, eP, A2), A1), A0);
+    };
+    auto set_region = [&](int h) {
+        mc = (float)(h + 1) * bsM * 1.0001f;
+        BO = fmaf(4.f * (2.f * eps1 + 2.f * u) * ma + 4.f * eps0, mc, 4.f * eps0 * ma + 1e-30f);
+        eabs = 8.f * eps0 * mc + 4.f * eps0 * eps0 + 1e-30f;
+        if (bk >= 0) set_bp();
+    };
+    set_region(GDT_SCAN);
     // FP64 displacement of LDS entry k: the FP64 path's value (fallbacks only)
     auto x64 = [&](int k, double& x, double& y) {
         const gdt_d2 V = L.X64[iv], X = L.X64[k];
         x = X.x - V.x;
         y = X.y - V.y;
     };
-#pragma unroll 1
-    for (int ry = -GDT_HALO; ry <= GDT_HALO; ++ry) {
-        const int c = st.c0 + ry * GDT_WIN;
-        const uint32_t e = L.cs[c + GDT_HALO + 1];
-        for (uint32_t kk = L.cs[c - GDT_HALO]; kk < e; ++kk) {
-            const int k = (int)kk;
-            const gdt_f2 Y = L.X32[k];
-            const float px = Y.x - vx, py = Y.y - vy;
-            const float cc = fmaf(px, px, py * py);
-            const float O = fmaf(ax, py, -(ay * px));
-            const float P = fmaf(K0, cc, fmaf(Kx, px, Ky * py));
-            const float E = fmaf(cE, cc, eabs);
-            bool take, und;
-            if (wrap) {
-                const bool okO = O > BO, decO = fabsf(O) > BO;
-                const bool inP = P < -BP, decP = fabsf(P) > BP;
-                take = okO && (bk < 0 || inP);
-                und = !decO || (okO && bk >= 0 && !decP);
-                und = und && k != ak;
-            } else {
-                take = bk < 0 || cc + E < bd2 - bE;
-                und = bk >= 0 && !take && !(cc - E > bd2 + bE);
-            }
-            if (k == iv) { take = false; und = false; }
-            if (wrap && k == ak) take = false;
-            if (take || und) {
-                if (und) {
-                    ++nfb;
-                    double c64x, c64y;
-                    x64(k, c64x, c64y);
-                    if (wrap) {
-                        double a64x, a64y;
-                        x64(ak, a64x, a64y);
-                        const bool left = fabsf(O) > BO ? O > 0.f
-                                                        : gdt_orient64(a64x, a64y, c64x, c64y) > 0.;
-                        if (!left) take = false;
-                        else if (bk < 0) take = true;
-                        else if (fabsf(P) > BP) take = P < 0.f;
-                        else {
-                            double b64x, b64y;
-                            x64(bk, b64x, b64y);
-                            const Circ64 K64 = gdt_circ64(a64x, a64y, b64x, b64y);
-                            take = gdt_incircle64(K64, c64x, c64y) < 0.;
-                        }
-                    } else {
+    auto visit = [&](int k) {
+        const gdt_f2 Y = L.X32[k];
+        const float px = Y.x - vx, py = Y.y - vy;
+        const float cc = fmaf(px, px, py * py);
+        const float O = fmaf(ax, py, -(ay * px));
+        const float P = fmaf(K0, cc, fmaf(Kx, px, Ky * py));
+        const float E = fmaf(cE, cc, eabs);
+        // predicates combined without branches; one branch when the
+        // candidate wins or a bound is not cleared
+        const bool none = bk < 0;
+        const bool okO = O > BO, decO = fabsf(O) > BO;
+        const bool inP = P < -BP, decP = fabsf(P) > BP;
+        const bool w_take = okO & (none | inP);
+        const bool w_und = (!decO) | (okO & !none & !decP);
+        const bool n_lt = cc + E < bd2 - bE, n_gt = cc - E > bd2 + bE;
+        const bool n_take = none | n_lt;
+        const bool n_und = !none & !n_lt & !n_gt;
+        const bool valid = (k != iv) & (!wrap | (k != ak));
+        bool take = valid & (wrap ? w_take : n_take);
+        const bool und = valid & (wrap ? w_und : n_und);
+        if (take | und) {
+            if (und) {
+                ++nfb;
+                double c64x, c64y;
+                x64(k, c64x, c64y);
+                if (wrap) {
+                    double a64x, a64y;
+                    x64(ak, a64x, a64y);
+                    const bool left = decO ? okO : gdt_orient64(a64x, a64y, c64x, c64y) > 0.;
+                    if (!left) take = false;
+                    else if (none) take = true;
+                    else if (decP) take = inP;
+                    else {
                         double b64x, b64y;
                         x64(bk, b64x, b64y);
-                        take = gdt_dist2_64(c64x, c64y) < gdt_dist2_64(b64x, b64y);
+                        const Circ64 K64 = gdt_circ64(a64x, a64y, b64x, b64y);
+                        take = gdt_incircle64(K64, c64x, c64y) < 0.;
                     }
-                }
-                if (take) {
-                    bk = k; bx = px; by = py; bd2 = cc; bE = E;
-                    if (wrap) {
-                        const float aa = fmaf(ax, ax, ay * ay);
-                        K0 = fmaf(ax, py, -(ay * px));
-                        Kx = fmaf(cc, ay, -(aa * py));
-                        Ky = fmaf(aa, px, -(cc * ax));
-                        const float mb = GDT_FMAXF(fabsf(px), fabsf(py));
-                        sab = ma + mb;
-                        pab = ma * mb;
-                        const float A0 = eP * sab * sab * sab + 1e-30f;
-                        const float A1 = cP * pab * sab + 3.f * eP * sab * sab;
-                        const float A2 = cP * pab + 3.f * eP * sab;
-                        BP = fmaf(mc, fmaf(mc, fmaf(mc, eP, A2), A1), A0);
-                    }
+                } else {
+                    double b64x, b64y;
+                    x64(bk, b64x, b64y);
+                    take = gdt_dist2_64(c64x, c64y) < gdt_dist2_64(b64x, b64y);
                 }
             }
+            if (take) {
+                bk = k; bx = px; by = py; bd2 = cc; bE = E;
+                const float aa = fmaf(ax, ax, ay * ay);
+                K0 = fmaf(ax, py, -(ay * px));
+                Kx = fmaf(cc, ay, -(aa * py));
+                Ky = fmaf(aa, px, -(cc * ax));
+                const float mb = GDT_FMAXF(fabsf(px), fabsf(py));
+                sab = ma + mb;
+                pab = ma * mb;
+                set_bp();
+            }
         }
-        npred += e - L.cs[c - GDT_HALO];
-    }
+    };
+    npred += gdt_scan_block(L, st.c0, visit);
+
     const float bsm = S.bsmin32;
-    const float cov = (float)GDT_HALO * bsm + st.covb - 9.5367431640625e-07f * (float)(GDT_HALO + 2) * bsM;
-    bool ok = false;
+    const float cov = (float)GDT_SCAN * bsm + st.covb - 9.5367431640625e-07f * (float)(GDT_SCAN + 2) * bsM;
+    const int wx = st.c0 % GDT_WIN, wy = st.c0 / GDT_WIN;
+    const float ox = vx - (float)(wx - GDT_FRAME) * S.bsx32;
+    const float oy = vy - (float)(wy - GDT_FRAME) * S.bsy32;
+    bool ok = false, boxed = false;
+    int dx0 = 0, dx1 = 0, dy0 = 0, dy1 = 0;
     if (!wrap) {
-        ok = bk >= 0 && cov > 0.f && bd2 + bE <= cov * cov * 0.9999f;
+        if (bk >= 0) {
+            ok = cov > 0.f && bd2 + bE <= cov * cov * 0.9999f;
+            const float rad = sqrtf(bd2 + bE) * 1.0001f;
+            gdt_circle_cells<float>(0.f, 0.f, 0.f, 0.f, rad, ox, oy, S.bsx32, S.bsy32, 1e-2f, dx0,
+                                    dx1, dy0, dy1);
+            boxed = true;
+        }
     } else if (bk >= 0) {
         const float EK0 = 4.f * (2.f * eps1 + 2.f * u) * pab + 4.f * eps0 * sab;
         const float EK = 4.f * (3.f * eps1 + 6.f * u) * pab * sab + 8.f * eps0 * sab * sab;
@@ -467,25 +475,22 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
             const float qyhi = GDT_FMAXF(nyhi / (2.f * Klo), nyhi / (2.f * Khi));
             const float rad = sqrtf(GDT_FMAXF(qxlo * qxlo, qxhi * qxhi) +
                                     GDT_FMAXF(qylo * qylo, qyhi * qyhi)) * 1.0001f;
-            const int wx = st.c0 % GDT_WIN, wy = st.c0 / GDT_WIN;
-            const float ox = vx - (float)(wx - GDT_FRAME) * S.bsx32;
-            const float oy = vy - (float)(wy - GDT_FRAME) * S.bsy32;
-            int dx0, dx1, dy0, dy1;
             gdt_circle_cells<float>(qxlo, qxhi, qylo, qyhi, rad, ox, oy, S.bsx32, S.bsy32, 1e-2f,
                                     dx0, dx1, dy0, dy1);
             const float gx = fabsf(Kx) + EK, gy = fabsf(Ky) + EK;
-            ok = (dx0 >= -GDT_HALO && dx1 <= GDT_HALO && dy0 >= -GDT_HALO && dy1 <= GDT_HALO) ||
+            ok = (dx0 >= -GDT_SCAN && dx1 <= GDT_SCAN && dy0 >= -GDT_SCAN && dy1 <= GDT_SCAN) ||
                  (cov > 0.f && (gx * gx + gy * gy) * 1.001f <= (cov * cov) * (Klo * Klo));
+            boxed = true;
         }
     }
+    if (!ok && boxed && gdt_in_window(S, st.c0, dx0, dx1, dy0, dy1)) {
+        const int h0 = dx0 < dy0 ? dx0 : dy0, h1 = dx1 > dy1 ? dx1 : dy1;
+        set_region(-h0 > h1 ? -h0 : h1);
+        npred += gdt_lds_block(L, wx, wy, dx0, dx1, dy0, dy1, GDT_SCAN, visit);
+        ok = true;
+    }
     if (!ok) return STEP_DEFER;
-    if (wrap && bk == st.n0) return STEP_DONE;
-    if (!wrap) st.n0 = bk;
-    if (st.count < io.stride) st.row[st.count] = L.orig[bk];
-    ++st.count;
-    if (st.count > GDT_HARD_CAP) return STEP_NOCLOSE;
-    st.ak = bk; st.ax = bx; st.ay = by;
-    return STEP_RUN;
+    return gdt_tile_advance<float>(L, io, st, wrap, bk, bx, by);
 }
 
 // Whole point, one step after another (host emulation and simple callers).

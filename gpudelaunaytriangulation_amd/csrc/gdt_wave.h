@@ -1,0 +1,171 @@
+// gdt_wave.h -- one-ring construction of a single point by a whole wavefront
+// (host/device routines), for the points the tile kernel hands back (circles
+// reaching past the staged window, windows too full for LDS).
+//
+// The search is the one of the direct routine (gdt_point.h): nearest
+// neighbour, then gift wrapping, each search visiting the Chebyshev rings of
+// cells r = 0, 1, ... around v's cell until its answer is certified.  Here the
+// cells of a ring are dealt to the lanes of a wavefront; each lane keeps a best
+// candidate, and the ring ends with a reduction of the lane bests to the wave's
+// best, which every lane then starts the next ring from.  Arithmetic is FP64
+// with the displacement formula of the direct routine, so both precision paths
+// give these points the same row.
+//
+// Reduction order does not change the answer: "c lies inside circle (v,a,b)"
+// orders the candidates strictly left of v->a by the centre of their circle
+// along the bisector of (v,a), a total order for generic input, so any
+// reduction tree keeps the candidate a sequential scan keeps.  Nearest
+// neighbour: minimum of (squared distance, slot).
+//
+// The routine is written against a wave policy W with
+//     GdtWaveBest W::ring(const GdtWaveBest& start, bool wrap, double ax,
+//                         double ay, Fn& lane_fn)
+// which runs lane_fn(lane, best) on every lane (each starting from `start`)
+// and returns the reduction of the lane bests, identical on every lane: a
+// device policy with cross-lane shuffles, and a host policy that runs the
+// lanes one after another and reduces with the same tree.
+#pragma once
+
+#include "gdt_point.h"
+
+#define GDT_WAVE 64
+
+struct GdtWaveBest {
+    int k;             // slot, -1 none
+    double x, y;       // displacement from v
+    double d2;         // squared distance (nearest-neighbour search)
+};
+
+GDT_HD inline GdtWaveBest gdt_wave_none()
+{
+    GdtWaveBest b;
+    b.k = -1;
+    b.x = b.y = b.d2 = 0.;
+    return b;
+}
+
+// Whether candidate c beats the current best b.
+GDT_HD inline bool gdt_wave_better(bool wrap, double ax, double ay, const GdtWaveBest& b,
+                                   const GdtWaveBest& c)
+{
+    if (c.k < 0) return false;
+    if (b.k < 0) return true;
+    if (!wrap) return c.d2 < b.d2 || (c.d2 == b.d2 && c.k < b.k);
+    const Circ64 K = gdt_circ64(ax, ay, b.x, b.y);
+    return gdt_incircle64(K, c.x, c.y) < 0.;
+}
+
+// Reduction tree of the lane bests (the device policy's shuffle-down order):
+// after the pass with offset `off`, lane i < off holds the better of lanes i
+// and i + off.
+struct GdtWaveHost {
+    template <class Fn>
+    GdtWaveBest ring(const GdtWaveBest& start, bool wrap, double ax, double ay, Fn& lane_fn)
+    {
+        GdtWaveBest lb[GDT_WAVE];
+        for (int l = 0; l < GDT_WAVE; ++l) {
+            lb[l] = start;
+            lane_fn(l, lb[l]);
+        }
+        for (int off = GDT_WAVE / 2; off > 0; off >>= 1)
+            for (int i = 0; i < off; ++i)
+                if (gdt_wave_better(wrap, ax, ay, lb[i], lb[i + off])) lb[i] = lb[i + off];
+        return lb[0];
+    }
+};
+
+// One point's ring.  `lead` is true on the one lane that writes the row.
+// npred counts the predicates this lane evaluated.
+template <class W>
+GDT_HD inline GdtRingResult gdt_wave_ring(const GdtSet& S, int j, const GdtRingIO& io, W& w,
+                                          bool lead)
+{
+    GdtRingResult res = {0, 0, 0u, 0u};
+    const uint32_t pc = io.scell[j];
+    const int cx = (int)(pc & 0xffffu), cy = (int)(pc >> 16);
+    const gdt_d2 ov = io.off64[j];
+    const double covb = GDT_FMIN(GDT_FMIN(ov.x, S.bsx - ov.x), GDT_FMIN(ov.y, S.bsy - ov.y));
+    const double bsmin = GDT_FMIN(S.bsx, S.bsy);
+    const size_t row = (size_t)io.perm[j] * (size_t)io.stride;
+    unsigned npred = 0;
+
+    // One search: nearest neighbour (wrap false) or successor of edge (v,a).
+    auto search = [&](bool wrap, int a, double ax, double ay, GdtWaveBest& g) -> bool {
+        g = gdt_wave_none();
+        for (int r = 0; r <= S.rmax; ++r) {
+            const int ncell = r ? 8 * r : 1;
+            auto lane_fn = [&](int lane, GdtWaveBest& m) {
+                Circ64 K = {0., 0., 0.};
+                if (wrap && m.k >= 0) K = gdt_circ64(ax, ay, m.x, m.y);
+                for (int q = lane; q < ncell; q += GDT_WAVE) {
+                    int dx, dy;
+                    gdt_ring_cell(r, q, dx, dy);
+                    const int x = gdt_wrap(cx + dx, S.xsize), y = gdt_wrap(cy + dy, S.ysize);
+                    const int key = S.c0 + gdt_cell_key(S, x, y);
+                    const uint32_t b = io.cstart[key], e = io.cstart[key + 1];
+                    const double bx = gdt_base64(dx, S.bsx, ov.x), by = gdt_base64(dy, S.bsy, ov.y);
+                    for (uint32_t k = b; k < e; ++k) {
+                        if ((int)k == j || (int)k == a) continue;
+                        const gdt_d2 o = io.off64[k];
+                        GdtWaveBest c;
+                        c.k = (int)k;
+                        c.x = gdt_disp64(bx, o.x);
+                        c.y = gdt_disp64(by, o.y);
+                        c.d2 = gdt_dist2_64(c.x, c.y);
+                        ++npred;
+                        if (!wrap) {
+                            if (gdt_wave_better(false, 0., 0., m, c)) m = c;
+                            continue;
+                        }
+                        if (!(gdt_orient64(ax, ay, c.x, c.y) > 0.)) continue;
+                        if (m.k >= 0) {
+                            ++npred;
+                            if (!(gdt_incircle64(K, c.x, c.y) < 0.)) continue;
+                        }
+                        m = c;
+                        K = gdt_circ64(ax, ay, c.x, c.y);
+                    }
+                }
+            };
+            g = w.ring(g, wrap, ax, ay, lane_fn);
+            if (g.k < 0) continue;
+            const double cov = (double)r * bsmin + covb;
+            if (!(cov > 0.)) continue;
+            if (!wrap) {
+                if (g.d2 <= cov * cov * (1.0 - 1e-12)) return true;
+                continue;
+            }
+            const Circ64 K = gdt_circ64(ax, ay, g.x, g.y);
+            const double k0min = 1e-6 * (fabs(ax * g.y) + fabs(ay * g.x));
+            if (K.k0 > k0min && K.kx * K.kx + K.ky * K.ky <= cov * cov * (K.k0 * K.k0) * (1.0 - 1e-6))
+                return true;
+        }
+        return false;
+    };
+
+    GdtWaveBest g;
+    if (!search(false, -1, 0., 0., g)) {
+        res.err = g.k < 0 ? ERR_NONEIGHBOUR : ERR_SEARCH;
+        res.npred = npred;
+        return res;
+    }
+    const int nn = g.k;
+    int a = nn, count = 0;
+    double ax = g.x, ay = g.y;
+    for (;;) {
+        if (lead && count < io.stride) io.nbrs[row + count] = io.perm[a];
+        ++count;
+        if (count > GDT_HARD_CAP) { res.err |= ERR_NOCLOSE; break; }
+        if (!search(true, a, ax, ay, g)) {
+            res.err |= g.k < 0 ? ERR_NONEIGHBOUR : ERR_SEARCH;
+            break;
+        }
+        if (g.k == nn) break;
+        a = g.k;
+        ax = g.x;
+        ay = g.y;
+    }
+    res.count = count;
+    res.npred = npred;
+    return res;
+}

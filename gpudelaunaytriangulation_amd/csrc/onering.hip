@@ -9,6 +9,7 @@
 // mask most of them out.
 #include "gdt_kernels.h"
 #include "gdt_tile.h"
+#include "gdt_wave.h"
 
 #include <type_traits>
 
@@ -72,15 +73,18 @@ __global__ __launch_bounds__(1024) void k_one_ring(const GdtSet* __restrict__ se
 }
 
 // Worklist of slots (count in device memory): grid-stride, set found per lane.
+// Entries [skip, *worklen) of the list are processed.
 template <int PREC>
 __global__ __launch_bounds__(1024) void k_one_ring_list(const GdtSet* __restrict__ sets, int nsets,
                                                         const int* __restrict__ worklist,
                                                         const uint32_t* __restrict__ worklen,
-                                                        GdtRingIO io, int* __restrict__ counts,
+                                                        int skip, GdtRingIO io,
+                                                        int* __restrict__ counts,
                                                         int* __restrict__ status,
                                                         unsigned long long* __restrict__ counters)
 {
-    const int len = (int)*worklen;
+    const int len = max((int)*worklen - skip, 0);
+    worklist += skip;
     const int stride = gridDim.x * blockDim.x;
     const int first = blockIdx.x * blockDim.x + threadIdx.x;
     // uniform trip count per wave so the reductions see every lane
@@ -95,6 +99,75 @@ __global__ __launch_bounds__(1024) void k_one_ring_list(const GdtSet* __restrict
         }
         const GdtSet S = sets[s];
         one_ring_lane<PREC>(active, S, j, io, counts, status, counters);
+    }
+}
+
+// Wavefront policy of gdt_wave_ring: every lane scans its share of a ring,
+// then a shuffle-down tree leaves the best at lane 0, which is broadcast.
+struct GdtWaveDev {
+    int lane;
+    template <class Fn>
+    __device__ GdtWaveBest ring(const GdtWaveBest& start, bool wrap, double ax, double ay, Fn& fn)
+    {
+        GdtWaveBest m = start;
+        fn(lane, m);
+#pragma unroll
+        for (int off = GDT_WAVE / 2; off > 0; off >>= 1) {
+            GdtWaveBest p;
+            p.k = __shfl_down(m.k, off, 64);
+            p.x = __shfl_down(m.x, off, 64);
+            p.y = __shfl_down(m.y, off, 64);
+            p.d2 = __shfl_down(m.d2, off, 64);
+            if (lane < off && gdt_wave_better(wrap, ax, ay, m, p)) m = p;
+        }
+        GdtWaveBest g;
+        g.k = __shfl(m.k, 0, 64);
+        g.x = __shfl(m.x, 0, 64);
+        g.y = __shfl(m.y, 0, 64);
+        g.d2 = __shfl(m.d2, 0, 64);
+        return g;
+    }
+};
+
+// Deferred points, one wavefront per point (FP64 in both precision paths):
+// list entries [0, min(*worklen, cap)), grid-stride over the waves.
+__global__ __launch_bounds__(256) void k_one_ring_wave(const GdtSet* __restrict__ sets, int nsets,
+                                                       const int* __restrict__ worklist,
+                                                       const uint32_t* __restrict__ worklen,
+                                                       int cap, GdtRingIO io,
+                                                       int* __restrict__ counts,
+                                                       int* __restrict__ status,
+                                                       unsigned long long* __restrict__ counters)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int wid = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nw = (int)((gridDim.x * blockDim.x) >> 6);
+    const int len = min((int)*worklen, cap);
+    GdtWaveDev w;
+    w.lane = lane;
+    unsigned npred = 0, nsum = 0;
+    int mx = 0, er = 0;
+    for (int t = wid; t < len; t += nw) {
+        const int j = worklist[t];
+        const GdtSet S = sets[gdt_set_of(sets, nsets, j)];
+        const GdtRingResult r = gdt_wave_ring(S, j, io, w, lane == 0);
+        npred += r.npred;
+        if (lane == 0) {
+            counts[io.perm[j]] = r.count;
+            mx = max(mx, r.count);
+            er |= r.err;
+            nsum += (unsigned)r.count;
+        }
+    }
+    mx = wave_max_i32(mx);
+    er = wave_or_i32(er);
+    npred = wave_sum_u32(npred);
+    nsum = wave_sum_u32(nsum);
+    if (lane == 0) {
+        if (mx > 0) atomicMax(&status[ST_REQUIRED], mx);
+        if (er) atomicOr(&status[ST_ERRFLAGS], er);
+        if (npred) atomicAdd(&counters[CT_PRED], (unsigned long long)npred);
+        if (nsum) atomicAdd(&counters[CT_NBRSUM], (unsigned long long)nsum);
     }
 }
 
@@ -130,13 +203,13 @@ __device__ inline void block_scan_lds(uint32_t* a, int n, uint32_t* wsum)
     if (tid == 0) a[n] = tot;
 }
 
-// One block per 16x16-cell tile: stage the tile window in LDS, then one lane
-// per point of the tile (in rounds of blockDim).  The points of a window that
-// does not fit the LDS budget, or of a grid too small for the halo, are
-// appended to `defer` and handled by the direct routine (k_one_ring_list) in
-// the next launch (the same decision in both precisions).
+// One block per 16x16-cell tile: stage the tile window in LDS, then the lanes
+// advance the tile's points one search step at a time.  The points of a
+// window that does not fit the LDS budget, or of a grid too small for the
+// scan block, and points with a step whose circle leaves the window, are
+// appended to `defer` and handled by k_one_ring_wave in the next launch.
 template <int PREC>
-__global__ __launch_bounds__(512) void k_one_ring_tile(const GdtSet* __restrict__ sets,
+__global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict__ sets,
                                                        const GdtChunk* __restrict__ tiles,
                                                        GdtRingIO io, int* __restrict__ counts,
                                                        int* __restrict__ status,
@@ -168,7 +241,7 @@ __global__ __launch_bounds__(512) void k_one_ring_tile(const GdtSet* __restrict_
     __syncthreads();
     block_scan_lds(s_cs, GDT_WIN_CELLS, s_wsum);
     __syncthreads();
-    const bool fits = s_cs[GDT_WIN_CELLS] <= GDT_WIN_MAXPTS && S.rmax >= GDT_HALO;
+    const bool fits = s_cs[GDT_WIN_CELLS] <= GDT_WIN_MAXPTS && S.rmax >= GDT_SCAN;
     if (fits)
         for (int c = tid; c < GDT_WIN_CELLS; c += nt)
             gdt_tile_copy<PREC == 32>(S, c, s_gstart[c], s_cs[c], s_cs[c + 1] - s_cs[c], io, L);
@@ -238,7 +311,7 @@ hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nse
                                     unsigned long long* counters, int* defer, hipStream_t s)
 {
     if (ntiles <= 0) return hipSuccess;
-    if (block > 512) block = 512;
+    if (block > 1024) block = 1024;
     if (precision == 32)
         hipLaunchKernelGGL(k_one_ring_tile<32>, dim3(ntiles), dim3(block), 0, s, d_sets, tiles, io,
                            counts, status, counters, defer);
@@ -247,17 +320,22 @@ hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nse
                            counts, status, counters, defer);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // deferred points: direct routine over the list (count read on the device)
-    return gdt_launch_one_ring(precision, d_sets, nsets, nullptr, 0, 256, defer,
-                               reinterpret_cast<const uint32_t*>(status + ST_NDEFER), total_n, io,
-                               counts, status, counters, s);
+    // Deferred points (count read on the device): the first GDT_WAVE_CAP one
+    // wavefront each, any beyond (dense clusters) one lane each.
+    const uint32_t* ndefer = reinterpret_cast<const uint32_t*>(status + ST_NDEFER);
+    hipLaunchKernelGGL(k_one_ring_wave, dim3(GDT_WAVE_BLOCKS), dim3(256), 0, s, d_sets, nsets, defer,
+                       ndefer, GDT_WAVE_CAP, io, counts, status, counters);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return gdt_launch_one_ring(precision, d_sets, nsets, nullptr, 0, 256, defer, ndefer,
+                               GDT_WAVE_CAP, total_n, io, counts, status, counters, s);
 }
 
 hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
                                const GdtChunk* chunks, int nch, int block, const int* worklist,
-                               const uint32_t* worklen, int max_work, const GdtRingIO& io,
-                               int* counts, int* status, unsigned long long* counters,
-                               hipStream_t s)
+                               const uint32_t* worklen, int skip, int max_work,
+                               const GdtRingIO& io, int* counts, int* status,
+                               unsigned long long* counters, hipStream_t s)
 {
     if (!worklist) {
         if (nch <= 0) return hipSuccess;
@@ -269,14 +347,15 @@ hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
                                counts, status, counters);
         return hipGetLastError();
     }
+    max_work -= skip;
     if (max_work <= 0) return hipSuccess;
     int grid = (max_work + block - 1) / block;
     if (grid > 4096) grid = 4096;
     if (precision == 32)
         hipLaunchKernelGGL(k_one_ring_list<32>, dim3(grid), dim3(block), 0, s, d_sets, nsets,
-                           worklist, worklen, io, counts, status, counters);
+                           worklist, worklen, skip, io, counts, status, counters);
     else
         hipLaunchKernelGGL(k_one_ring_list<64>, dim3(grid), dim3(block), 0, s, d_sets, nsets,
-                           worklist, worklen, io, counts, status, counters);
+                           worklist, worklen, skip, io, counts, status, counters);
     return hipGetLastError();
 }

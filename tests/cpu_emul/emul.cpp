@@ -1,11 +1,14 @@
 // Host build of the per-point device routines for CPU tests.
 // Builds the cell-ordered arrays the GPU cell-list kernels build, then runs
-// either the direct per-point routine (gdt_point.h) or the LDS-staged tile
-// routine (gdt_tile.h, one emulated block per tile) for every point.
+// the direct per-point routine (gdt_point.h), the LDS-staged tile routine
+// (gdt_tile.h, one emulated block per tile, deferred points through the
+// wavefront routine) or the wavefront routine (gdt_wave.h, 64 emulated lanes)
+// for every point.
 // Usage: emul <points.bin> <n> <lx> <ly> <cell_size> <tiled> <precision 64|32> <stride>
-//             <out.bin> [mode 0=direct 1=tile]
+//             <out.bin> [mode 0=direct 1=tile 2=wave]
 // out.bin: int32 counts[n], int32 nbrs[n*stride], uint64 npred, uint64 nfb, int32 errflags
 #include "gdt_tile.h"
+#include "gdt_wave.h"
 
 #include <cstdio>
 #include <cstdlib>
@@ -58,9 +61,12 @@ int main(int argc, char** argv)
         counts[perm[j]] = r.count;
         npred += r.npred; nfb += r.nfb; err |= r.err;
     };
+    GdtWaveHost wave;
     if (mode == 0) {
         for (int j = 0; j < n; ++j)
             acc(j, prec == 32 ? gdt_one_ring_f32(S, j, io) : gdt_one_ring_f64(S, j, io));
+    } else if (mode == 2) {
+        for (int j = 0; j < n; ++j) acc(j, gdt_wave_ring(S, j, io, wave, true));
     } else {
         // one emulated block per tile: stage counts, scan, copy, then every point
         std::vector<uint32_t> cs(GDT_WIN_CELLS + 1);
@@ -77,7 +83,7 @@ int main(int argc, char** argv)
             uint32_t run = 0;
             for (int c = 0; c < GDT_WIN_CELLS; ++c) { cs[c] = run; run += c_cnt[c]; }
             cs[GDT_WIN_CELLS] = run;
-            const bool fits = run <= GDT_WIN_MAXPTS && S.rmax >= GDT_HALO;
+            const bool fits = run <= GDT_WIN_MAXPTS && S.rmax >= GDT_SCAN;
             if (fits)
                 for (int c = 0; c < GDT_WIN_CELLS; ++c) {
                     if (prec == 32) gdt_tile_copy<true>(S, c, gstart[c], cs[c], c_cnt[c], io, L);
@@ -90,7 +96,7 @@ int main(int argc, char** argv)
                 if (!fits || (r.err & ERR_DEFER)) {
                     const unsigned np = fits ? r.npred : 0u, nf = fits ? r.nfb : 0u;
                     ++ndefer;
-                    r = prec == 32 ? gdt_one_ring_f32(S, j, io) : gdt_one_ring_f64(S, j, io);
+                    r = gdt_wave_ring(S, j, io, wave, true);
                     r.npred += np;
                     r.nfb += nf;
                 }

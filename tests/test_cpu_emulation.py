@@ -1,9 +1,12 @@
 """The device routines compiled for the host with g++ under AddressSanitizer
-and UBSan and run on the CPU: the direct per-point routine (gdt_point.h) and
-the LDS-staged tile routine (gdt_tile.h, one emulated block per tile).  FP64
-and FP32-filtered paths must reproduce the oracle's rings bit-exactly and be
-byte-identical to each other, on uniform, near-degenerate (jittered lattice),
-rectangular and small inputs."""
+and UBSan and run on the CPU: the direct per-point routine (gdt_point.h), the
+LDS-staged tile routine (gdt_tile.h, one emulated block per tile, deferred
+points through the wavefront routine) and the wavefront routine (gdt_wave.h,
+64 emulated lanes and the shuffle reduction tree).  FP64 and FP32-filtered
+paths must reproduce the oracle's rings bit-exactly and be byte-identical to
+each other, on uniform, near-degenerate (jittered lattice), rectangular, small,
+fine-grid (cells much smaller than the spacing: circle-box extensions and
+deferrals) and clustered (windows too full for LDS) inputs."""
 import os
 import subprocess
 
@@ -27,11 +30,11 @@ def emul(tmp_path_factory):
     return exe
 
 
-def run(exe, tmp, xy, lx, ly, tiled, prec, stride=32, mode=0):
+def run(exe, tmp, xy, lx, ly, tiled, prec, stride=32, mode=0, cell=1.0):
     n = xy.shape[0]
     pin, pout = os.path.join(tmp, "p.bin"), os.path.join(tmp, "o.bin")
     xy.astype(np.float64).tofile(pin)
-    subprocess.run([exe, pin, str(n), repr(lx), repr(ly), "1.0", str(tiled), str(prec),
+    subprocess.run([exe, pin, str(n), repr(lx), repr(ly), repr(cell), str(tiled), str(prec),
                     str(stride), pout, str(mode)], check=True)
     raw = np.fromfile(pout, dtype=np.uint8)
     counts = raw[:4 * n].view(np.int32)
@@ -42,26 +45,39 @@ def run(exe, tmp, xy, lx, ly, tiled, prec, stride=32, mode=0):
     return nbrs.copy(), counts.copy(), int(npred), int(nfb), err
 
 
+def clustered():
+    """2048 uniform points in a 64x64 box plus 2048 in a 3x3 square."""
+    xy = inputs.uniform_points(4096, 64.0, 64.0, seed=11)
+    xy[2048:] = 30.0 + xy[2048:] * (3.0 / 64.0)
+    return xy
+
+
+# name: (points, lx, ly, cell size)
 CASES = {
-    "uniform4096": (lambda: inputs.uniform_points(4096, 64.0, 64.0, seed=1), 64.0, 64.0),
-    "lattice64": (lambda: inputs.lattice_points(64, 64, 1e-5, seed=3), *inputs.lattice_box(64, 64)),
-    "rect1000": (lambda: inputs.uniform_points(1000, 40.0, 25.0, seed=7), 40.0, 25.0),
-    "small300": (lambda: inputs.uniform_points(300, 17.0, 17.0, seed=9), 17.0, 17.0),
-    "tiny60": (lambda: inputs.uniform_points(60, 8.0, 8.0, seed=9), 8.0, 8.0),
+    "uniform4096": (lambda: inputs.uniform_points(4096, 64.0, 64.0, seed=1), 64.0, 64.0, 1.0),
+    "lattice64": (lambda: inputs.lattice_points(64, 64, 1e-5, seed=3), *inputs.lattice_box(64, 64),
+                  1.0),
+    "rect1000": (lambda: inputs.uniform_points(1000, 40.0, 25.0, seed=7), 40.0, 25.0, 1.0),
+    "small300": (lambda: inputs.uniform_points(300, 17.0, 17.0, seed=9), 17.0, 17.0, 1.0),
+    "tiny60": (lambda: inputs.uniform_points(60, 8.0, 8.0, seed=9), 8.0, 8.0, 1.0),
+    "finecells": (lambda: inputs.uniform_points(2048, 45.0, 45.0, seed=5), 45.0, 45.0, 0.4),
+    "clustered": (clustered, 64.0, 64.0, 1.0),
 }
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_emulated_kernels_match_oracle(emul, tmp_path, case):
-    gen, lx, ly = CASES[case]
+    gen, lx, ly, cell = CASES[case]
     xy = gen()
     on, oc = oracle.triangulate(xy, lx, ly, 1.0, 16)
     ref = rings.ring_hash(on, oc)
     outs = {}
-    # (tiled order, mode): direct routine in both cell orders, tile routine
-    for tiled, mode in ((1, 0), (0, 0), (1, 1)):
+    # (tiled order, mode): direct routine in both cell orders, tile routine,
+    # wavefront routine in both cell orders
+    for tiled, mode in ((1, 0), (0, 0), (1, 1), (1, 2), (0, 2)):
         for prec in (64, 32):
-            nb, c, npred, nfb, err = run(emul, str(tmp_path), xy, lx, ly, tiled, prec, mode=mode)
+            nb, c, npred, nfb, err = run(emul, str(tmp_path), xy, lx, ly, tiled, prec, mode=mode,
+                                         cell=cell)
             assert err == 0
             assert rings.ring_hash(nb, c) == ref, (prec, tiled, mode)
             if prec == 64:
