@@ -26,8 +26,9 @@ N_POINTS = 1 << 20
 BOX = 1024.0
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes (FP64 coordinates and int32 indices in HBM, C = N cells):
-#   whole triangulation 84 N + 12 C; one-ring kernel 52 B per point
-#   (16 B offset + 4 B index + 4 B cell start + 24 B ring entries + 4 B count).
+#   whole triangulation 84 N + 12 C; one-ring tile kernel 52 B per point
+#   (16 B cell offset + 4 B caller index read, 4 B cell start read, 24 B of
+#   ring entries (6 per point on the torus) + 4 B count written).
 RING_BYTES_PER_POINT = 52.0
 TRI_BYTES = lambda n, c: 84.0 * n + 12.0 * c  # noqa: E731
 
@@ -56,7 +57,7 @@ def time_path(dg, pts, tri, num, steps, warmup, torch):
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ring_ms, cell_ms = [], []
+    ring_ms, cell_ms, defer_ms = [], [], []
     t0 = time.perf_counter()
     ev0.record(stream)
     for _ in range(steps):
@@ -64,11 +65,13 @@ def time_path(dg, pts, tri, num, steps, warmup, torch):
         st = dg.stats()
         ring_ms.append(st["ms_ring"])
         cell_ms.append(st["ms_cells"])
+        defer_ms.append(st["ms_defer"])
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = ev0.elapsed_time(ev1) / steps
-    return ms, wall, sum(ring_ms) / steps, sum(cell_ms) / steps, dg.stats()
+    return (ms, wall, sum(ring_ms) / steps, sum(cell_ms) / steps, sum(defer_ms) / steps,
+            dg.stats())
 
 
 def main():
@@ -97,14 +100,16 @@ def main():
         dg.ctx.set_config(profile=1, block=args.block, tiled=args.tiled, cell_scale=args.cell_scale)
         tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device="cuda")
         num = torch.zeros(n, dtype=torch.int32, device="cuda")
-        ms, wall, ring_ms, cell_ms, st = time_path(dg, pts, tri, num, args.steps, args.warmup, torch)
+        ms, wall, ring_ms, cell_ms, defer_ms, st = time_path(dg, pts, tri, num, args.steps,
+                                                             args.warmup, torch)
         nb = tri.cpu().numpy().reshape(n, dg.MaxSize)
         cnt = num.cpu().numpy()
-        out[name] = dict(ms=ms, wall=wall, ring_ms=ring_ms, cell_ms=cell_ms, st=st,
-                         hash=rings.ring_hash(nb, cnt), sum=int(cnt.sum()), nb=nb, cnt=cnt)
-        print(f"[bench] {name}: {ms:.3f} ms/step (one-ring {ring_ms:.3f} ms, cell list "
-              f"{cell_ms:.3f} ms), {n / ms * 1e3 / 1e9:.3f} Gpoints/s, deferred {st['deferred']}, "
-              f"pred/pt {st['predicates'] / n:.1f}", file=sys.stderr, flush=True)
+        out[name] = dict(ms=ms, wall=wall, ring_ms=ring_ms, cell_ms=cell_ms, defer_ms=defer_ms,
+                         st=st, hash=rings.ring_hash(nb, cnt), sum=int(cnt.sum()), nb=nb, cnt=cnt)
+        print(f"[bench] {name}: {ms:.3f} ms/step (one-ring {ring_ms:.3f} ms, deferred "
+              f"{defer_ms:.3f} ms, cell list {cell_ms:.3f} ms), {n / ms * 1e3 / 1e9:.3f} Gpoints/s, "
+              f"deferred {st['deferred']}, pred/pt {st['predicates'] / n:.1f}", file=sys.stderr,
+              flush=True)
 
     f32 = out["fp32f"]
     f64 = out.get("fp64", f32)
@@ -162,11 +167,12 @@ def main():
         "fallback_fraction": st["fallbacks"] / max(1, st["predicates"]),
         "predicates_per_point": st["predicates"] / n,
         "deferred_points": st["deferred"],
-        "kernel_ms": {"one_ring_fp32f": f32["ring_ms"], "one_ring_fp64": f64["ring_ms"],
+        "kernel_ms": {"one_ring_tile_fp32f": f32["ring_ms"], "one_ring_tile_fp64": f64["ring_ms"],
+                      "deferred_fp32f": f32["defer_ms"], "deferred_fp64": f64["defer_ms"],
                       "cell_list": f32["cell_ms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_one_ring<32>", "algorithmic_bytes_per_launch": ring_bytes},
+                     "kernel": "k_one_ring_tile<32>", "algorithmic_bytes_per_launch": ring_bytes},
         "triangulation_roofline_frac": TRI_BYTES(n, n) / (f32["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "cpu_baseline": cpu,
     }

@@ -14,13 +14,14 @@ from ._lib import (GDT_OK, GDT_OVERFLOW, PREC_FP32F, PREC_FP64, GdtError, Librar
 __all__ = [
     "inputs", "rings", "GDT_OK", "GDT_OVERFLOW", "PREC_FP32F", "PREC_FP64", "GdtError",
     "LibraryNotBuilt", "load_library", "DelaunayGPU", "CellListGPU", "PeriodicBoundaries",
-    "Context",
+    "Context", "BatchedDelaunayGPU",
 ]
 
 
 def __getattr__(name):
     # torch-backed classes load lazily so the CPU-only tools import without torch
-    if name in ("DelaunayGPU", "CellListGPU", "PeriodicBoundaries", "Context"):
+    if name in ("DelaunayGPU", "CellListGPU", "PeriodicBoundaries", "Context",
+                "BatchedDelaunayGPU"):
         from . import delaunay
         return getattr(delaunay, name)
     raise AttributeError(name)

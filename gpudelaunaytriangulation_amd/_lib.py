@@ -29,7 +29,9 @@ EXPORTS = (
     "gdt_create", "gdt_destroy", "gdt_set_config", "gdt_get_config", "gdt_get_stats",
     "gdt_status_string", "gdt_celllist_build", "gdt_celllist_info", "gdt_celllist_export",
     "gdt_triangulate", "gdt_triangles", "gdt_test_circumcircles", "gdt_repair",
-    "gdt_test_and_repair",
+    "gdt_test_and_repair", "gdt_voronoi_forward", "gdt_voronoi_adjoint",
+    "gdt_voronoi_forward_batch", "gdt_voronoi_adjoint_batch", "gdt_triangulate_batch",
+    "gdt_test_and_repair_batch", "gdt_md_run",
 )
 
 
@@ -54,11 +56,17 @@ class Stats(ctypes.Structure):
         ("neighbours", ctypes.c_ulonglong), ("max_ring", ctypes.c_int), ("flagged", ctypes.c_int),
         ("xsize", ctypes.c_int), ("ysize", ctypes.c_int), ("bsx", ctypes.c_double),
         ("bsy", ctypes.c_double), ("ms_cells", ctypes.c_float), ("ms_ring", ctypes.c_float),
-        ("deferred", ctypes.c_int),
+        ("deferred", ctypes.c_int), ("ms_defer", ctypes.c_float),
     ]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MdReport(ctypes.Structure):
+    _fields_ = [("steps_done", ctypes.c_int), ("first_bad_step", ctypes.c_int),
+                ("required", ctypes.c_int), ("max_flagged", ctypes.c_int),
+                ("flagged_total", ctypes.c_longlong)]
 
 
 _lib = None
@@ -102,6 +110,20 @@ def load():
     _sig(L, "gdt_test_circumcircles", [_vp, _vp, _i, _vp, _i, _vp, _ip, _vp])
     _sig(L, "gdt_repair", [_vp, _vp, _i, _vp, _i, _vp, _vp, _i, _ip, _vp])
     _sig(L, "gdt_test_and_repair", [_vp, _vp, _i, _d, _d, _d, _i, _vp, _vp, _i, _ip, _ip, _vp])
+    _sig(L, "gdt_voronoi_forward", [_vp, _vp, _i, _d, _d, _vp, _vp, _i, _d, _d, _i, _vp, _vp, _vp,
+                                    _vp])
+    _sig(L, "gdt_voronoi_adjoint", [_vp, _vp, _i, _d, _d, _vp, _vp, _i, _d, _d, _i, _vp, _vp, _vp,
+                                    _vp, _vp])
+    _sig(L, "gdt_voronoi_forward_batch", [_vp, _vp, _i, _ip, ctypes.POINTER(_d), _vp, _vp, _i, _d,
+                                          _d, _i, _vp, _vp, _vp, _vp])
+    _sig(L, "gdt_voronoi_adjoint_batch", [_vp, _vp, _i, _ip, ctypes.POINTER(_d), _vp, _vp, _i, _d,
+                                          _d, _i, _vp, _vp, _vp, _vp, _vp])
+    _sig(L, "gdt_triangulate_batch", [_vp, _vp, _i, _ip, ctypes.POINTER(_d), _d, _i, _vp, _vp, _i,
+                                      _ip, _vp])
+    _sig(L, "gdt_test_and_repair_batch", [_vp, _vp, _i, _ip, ctypes.POINTER(_d), _d, _i, _vp, _vp,
+                                          _i, _ip, _ip, _vp])
+    _sig(L, "gdt_md_run", [_vp, _vp, _i, _ip, ctypes.POINTER(_d), _d, _i, _vp, _vp, _i, _d,
+                           ctypes.c_ulonglong, _i, _i, _i, _i, ctypes.POINTER(MdReport), _vp])
     _lib = L
     return L
 

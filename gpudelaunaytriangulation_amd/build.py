@@ -22,7 +22,8 @@ LIB = os.path.join(PKG, "libgpudelaunay.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GDT_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["celllist.hip", "onering.hip", "testrepair.hip", "capi.hip"]
+SOURCES = ["celllist.hip", "onering.hip", "testrepair.hip", "voronoi.hip", "mdstep.hip",
+           "capi.hip"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

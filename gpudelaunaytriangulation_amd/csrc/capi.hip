@@ -53,7 +53,7 @@ struct DevGuard {
 struct gdt_context {
     int device = 0;
     gdt_config cfg = {320, 1, 1.0, 0};
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
     std::vector<GdtSet> sets;
     int total_n = 0, total_keys = 0;
@@ -68,6 +68,14 @@ struct gdt_context {
     DBuf<gdt_f2> off32;
     DBuf<int> status;
     DBuf<unsigned long long> counters;
+    // Voronoi energy: its own set table (only p0, n and the box are used)
+    std::vector<GdtSet> vsets;
+    DBuf<GdtSet> d_vsets;
+    DBuf<double> ecell, partial, vslots;
+    // Brownian loop: device state, its pinned mirror, a stream for capture
+    DBuf<GdtMdState> md;
+    GdtMdState* h_md = nullptr;
+    hipStream_t md_stream = nullptr;
     int* h_status = nullptr;
     unsigned long long* h_counters = nullptr;
     gdt_stats stats;
@@ -180,6 +188,26 @@ static int single_set(gdt_context* c, int n, double lx, double ly, double cell_s
     return configure(c, sets);
 }
 
+// Set s = caller points [offsets[s], offsets[s+1]) in a boxes[2s] x boxes[2s+1] box.
+static int batch_sets(gdt_context* c, int nsets, const int* offsets, const double* boxes,
+                      double cell_size)
+{
+    if (nsets <= 0 || !offsets || !boxes || offsets[0] != 0 || !(cell_size > 0.) ||
+        !std::isfinite(cell_size))
+        return GDT_ERR_INVALID;
+    std::vector<GdtSet> sets;
+    sets.reserve((size_t)nsets);
+    for (int k = 0; k < nsets; ++k) {
+        const int n = offsets[k + 1] - offsets[k];
+        const double lx = boxes[2 * k], ly = boxes[2 * k + 1];
+        if (n < 4 || !(lx > 0.) || !(ly > 0.) || !std::isfinite(lx) || !std::isfinite(ly))
+            return GDT_ERR_INVALID;
+        sets.push_back(gdt_make_set(offsets[k], n, 0, lx, ly, cell_size * c->cfg.cell_scale,
+                                    c->cfg.tiled));
+    }
+    return configure(c, sets);
+}
+
 static int reset_status(gdt_context* c, hipStream_t s)
 {
     GDT_CHECK(gdt_fill_i32(c->status.p, 0, ST_WORDS, s));
@@ -276,7 +304,7 @@ int gdt_create(gdt_context** out, int device)
         hipHostMalloc(reinterpret_cast<void**>(&c->h_status), ST_WORDS * sizeof(int), 0) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void**>(&c->h_counters), CT_WORDS * sizeof(unsigned long long), 0) != hipSuccess ||
         hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess ||
-        hipEventCreate(&c->ev[2]) != hipSuccess) {
+        hipEventCreate(&c->ev[2]) != hipSuccess || hipEventCreate(&c->ev[3]) != hipSuccess) {
         gdt_destroy(c);
         return GDT_ERR_DEVICE;
     }
@@ -295,6 +323,10 @@ void gdt_destroy(gdt_context* c)
     c->mark.release(); c->markscan.release(); c->perm.release(); c->slot_of.release();
     c->worklist.release(); c->repair.release(); c->tris.release(); c->defer.release(); c->off64.release();
     c->off32.release(); c->status.release(); c->counters.release();
+    c->d_vsets.release(); c->ecell.release(); c->partial.release(); c->vslots.release();
+    c->md.release();
+    if (c->h_md) (void)hipHostFree(c->h_md);
+    if (c->md_stream) (void)hipStreamDestroy(c->md_stream);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->h_status) (void)hipHostFree(c->h_status);
@@ -385,6 +417,41 @@ static int finish_rings(gdt_context* c, int stride, int* required, hipStream_t s
     return req > stride ? GDT_OVERFLOW : GDT_OK;
 }
 
+// Global triangulation of the installed set table.
+static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int* d_nbrs,
+                            int* d_counts, int precision, int* required, hipStream_t s)
+{
+    int rc;
+    if ((rc = ring_chunks(c)) != GDT_OK) return rc;
+    if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
+    if ((rc = reset_status(c, s)) != GDT_OK) return rc;
+    if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
+    if (c->cfg.tiled) {
+        GDT_CHECK(gdt_launch_one_ring_tile(precision, c->d_sets.p, (int)c->sets.size(), c->tch.p,
+                                           c->ntch, c->cfg.block, c->total_n,
+                                           ring_io(c, d_nbrs, stride), d_counts, c->status.p,
+                                           c->counters.p, c->defer.p,
+                                           c->cfg.profile ? c->ev[2] : nullptr, s));
+    } else {
+        GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), c->rch.p, c->nrch,
+                                      c->cfg.block, nullptr, nullptr, 0, 0, ring_io(c, d_nbrs, stride),
+                                      d_counts, c->status.p, c->counters.p, s));
+        if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[2], s));
+    }
+    if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[3], s));
+    rc = finish_rings(c, stride, required, s);
+    if (c->cfg.profile) {
+        float a = 0.f, b = 0.f, d = 0.f;
+        GDT_CHECK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+        GDT_CHECK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+        GDT_CHECK(hipEventElapsedTime(&d, c->ev[2], c->ev[3]));
+        c->stats.ms_cells = a;
+        c->stats.ms_ring = b;
+        c->stats.ms_defer = d;
+    }
+    return rc;
+}
+
 int gdt_triangulate(gdt_context* c, const double* d_xy, int n, double lx, double ly,
                     double cell_size, int stride, int* d_nbrs, int* d_counts, int precision,
                     int* required, void* stream)
@@ -392,32 +459,23 @@ int gdt_triangulate(gdt_context* c, const double* d_xy, int n, double lx, double
     if (!c || !d_xy || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
     if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
     DevGuard g(c->device);
-    hipStream_t s = (hipStream_t)stream;
     int rc = single_set(c, n, lx, ly, cell_size);
     if (rc != GDT_OK) return rc;
-    if ((rc = ring_chunks(c)) != GDT_OK) return rc;
-    if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
-    if ((rc = reset_status(c, s)) != GDT_OK) return rc;
-    if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
-    if (c->cfg.tiled)
-        GDT_CHECK(gdt_launch_one_ring_tile(precision, c->d_sets.p, (int)c->sets.size(), c->tch.p,
-                                           c->ntch, c->cfg.block, c->total_n,
-                                           ring_io(c, d_nbrs, stride), d_counts, c->status.p,
-                                           c->counters.p, c->defer.p, s));
-    else
-        GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), c->rch.p, c->nrch,
-                                      c->cfg.block, nullptr, nullptr, 0, 0, ring_io(c, d_nbrs, stride),
-                                      d_counts, c->status.p, c->counters.p, s));
-    if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[2], s));
-    rc = finish_rings(c, stride, required, s);
-    if (c->cfg.profile) {
-        float a = 0.f, b = 0.f;
-        GDT_CHECK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-        GDT_CHECK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
-        c->stats.ms_cells = a;
-        c->stats.ms_ring = b;
-    }
-    return rc;
+    return triangulate_sets(c, d_xy, stride, d_nbrs, d_counts, precision, required,
+                            (hipStream_t)stream);
+}
+
+int gdt_triangulate_batch(gdt_context* c, const double* d_xy, int nsets, const int* offsets,
+                          const double* boxes, double cell_size, int stride, int* d_nbrs,
+                          int* d_counts, int precision, int* required, void* stream)
+{
+    if (!c || !d_xy || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
+    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    int rc = batch_sets(c, nsets, offsets, boxes, cell_size);
+    if (rc != GDT_OK) return rc;
+    return triangulate_sets(c, d_xy, stride, d_nbrs, d_counts, precision, required,
+                            (hipStream_t)stream);
 }
 
 // Owned triangles into `tris` (capacity cap); count stays on the device at toff[N].
@@ -518,19 +576,16 @@ int gdt_repair(gdt_context* c, const double* d_xy, int n, const int* d_repair, i
     return rc;
 }
 
-int gdt_test_and_repair(gdt_context* c, const double* d_xy, int n, double lx, double ly,
-                        double cell_size, int stride, int* d_nbrs, int* d_counts, int precision,
-                        int* nflagged, int* required, void* stream)
+// Triangles -> cell list -> test -> repair for the installed set table.
+static int test_and_repair_sets(gdt_context* c, const double* d_xy, int stride, int* d_nbrs,
+                                int* d_counts, int precision, int* nflagged, int* required,
+                                hipStream_t s)
 {
-    if (!c || !d_xy || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
-    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
-    DevGuard g(c->device);
-    hipStream_t s = (hipStream_t)stream;
-    int rc = single_set(c, n, lx, ly, cell_size);
-    if (rc != GDT_OK) return rc;
+    const int n = c->total_n;
     const int cap = 2 * n + 64;
     GDT_CHECK(c->tris.reserve(3 * (size_t)cap));
     GDT_CHECK(c->repair.reserve((size_t)n));
+    int rc;
     if ((rc = reset_status(c, s)) != GDT_OK) return rc;
     // 1. triangles of the current rings (getCircumcirclesGPU, DelaunayGPU.cpp:650-666)
     if ((rc = enumerate(c, d_nbrs, d_counts, stride, c->tris.p, cap, s)) != GDT_OK) return rc;
@@ -559,6 +614,255 @@ int gdt_test_and_repair(gdt_context* c, const double* d_xy, int n, double lx, do
     rc = status_from_flags(c->h_status[ST_ERRFLAGS]);
     if (rc != GDT_OK) return rc;
     return req > stride ? GDT_OVERFLOW : GDT_OK;
+}
+
+int gdt_test_and_repair(gdt_context* c, const double* d_xy, int n, double lx, double ly,
+                        double cell_size, int stride, int* d_nbrs, int* d_counts, int precision,
+                        int* nflagged, int* required, void* stream)
+{
+    if (!c || !d_xy || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
+    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    int rc = single_set(c, n, lx, ly, cell_size);
+    if (rc != GDT_OK) return rc;
+    return test_and_repair_sets(c, d_xy, stride, d_nbrs, d_counts, precision, nflagged, required,
+                                (hipStream_t)stream);
+}
+
+int gdt_test_and_repair_batch(gdt_context* c, const double* d_xy, int nsets, const int* offsets,
+                              const double* boxes, double cell_size, int stride, int* d_nbrs,
+                              int* d_counts, int precision, int* nflagged, int* required,
+                              void* stream)
+{
+    if (!c || !d_xy || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
+    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    int rc = batch_sets(c, nsets, offsets, boxes, cell_size);
+    if (rc != GDT_OK) return rc;
+    return test_and_repair_sets(c, d_xy, stride, d_nbrs, d_counts, precision, nflagged, required,
+                                (hipStream_t)stream);
+}
+
+// ---- Brownian test-and-repair loop ------------------------------------------
+}  // extern "C"
+
+// One step on stream s, device only (capturable): every buffer it touches is
+// reserved beforehand by gdt_md_run.
+static int md_enqueue_step(gdt_context* c, double* d_xy, int stride, int* d_nbrs, int* d_counts,
+                           int precision, double sigma, unsigned long long seed, hipStream_t s)
+{
+    const int n = c->total_n, cap = 2 * n + 64;
+    int rc;
+    GDT_CHECK(gdt_launch_md_displace(c->d_sets.p, c->pch.p, c->npch, d_xy, sigma, seed, c->md.p, s));
+    if ((rc = reset_status(c, s)) != GDT_OK) return rc;
+    if ((rc = enumerate(c, d_nbrs, d_counts, stride, c->tris.p, cap, s)) != GDT_OK) return rc;
+    if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
+    GDT_CHECK(gdt_fill_i32(c->repair.p, -1, (size_t)n, s));
+    GdtCells cl = c->cells();
+    GDT_CHECK(gdt_launch_test(c->d_sets.p, (int)c->sets.size(), c->tris.p, c->toff.p + n, cap,
+                              c->total_n, d_xy, cl, c->repair.p, s));
+    if ((rc = repair_flagged(c, c->repair.p, stride, d_nbrs, d_counts, precision, s)) != GDT_OK) return rc;
+    GDT_CHECK(gdt_launch_md_step_end(c->status.p, c->counters.p, c->toff.p + n, n, c->md.p, s));
+    return GDT_OK;
+}
+
+// Restores the profile flag (events are not recorded inside the loop).
+struct ProfileOff {
+    gdt_context* c;
+    int saved;
+    explicit ProfileOff(gdt_context* cc) : c(cc), saved(cc->cfg.profile) { c->cfg.profile = 0; }
+    ~ProfileOff() { c->cfg.profile = saved; }
+};
+
+extern "C" {
+
+int gdt_md_run(gdt_context* c, double* d_xy, int nsets, const int* offsets, const double* boxes,
+               double cell_size, int stride, int* d_nbrs, int* d_counts, int precision,
+               double sigma, unsigned long long seed, int step0, int nsteps, int use_graph,
+               int check_every, gdt_md_report* rep, void* stream)
+{
+    if (!c || !d_xy || !d_nbrs || !d_counts || !rep || stride < 3 || nsteps < 0 || step0 < 0 ||
+        check_every < 1 || !(sigma >= 0.) || !std::isfinite(sigma))
+        return GDT_ERR_INVALID;
+    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    std::memset(rep, 0, sizeof(*rep));
+    rep->first_bad_step = -1;
+    DevGuard g(c->device);
+    int rc = batch_sets(c, nsets, offsets, boxes, cell_size);
+    if (rc != GDT_OK) return rc;
+    const size_t n = (size_t)c->total_n, cap = 2 * n + 64;
+    GDT_CHECK(c->tris.reserve(3 * cap));
+    GDT_CHECK(c->repair.reserve(n));
+    GDT_CHECK(c->tcount.reserve(n));
+    GDT_CHECK(c->toff.reserve(n + 1));
+    GDT_CHECK(c->mark.reserve(n));
+    GDT_CHECK(c->markscan.reserve(n));
+    GDT_CHECK(c->worklist.reserve(n));
+    GDT_CHECK(c->md.reserve(1));
+    if (!c->h_md)
+        GDT_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_md), sizeof(GdtMdState), 0));
+    hipStream_t s = (hipStream_t)stream;
+    if (use_graph && !s) {
+        // the legacy null stream cannot be captured: run on the context's own
+        if (!c->md_stream) GDT_CHECK(hipStreamCreateWithFlags(&c->md_stream, hipStreamNonBlocking));
+        GDT_CHECK(hipDeviceSynchronize());
+        s = c->md_stream;
+    }
+    ProfileOff poff(c);
+    GDT_CHECK(gdt_launch_md_init(c->md.p, step0, s));
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (use_graph) {
+        GDT_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        rc = md_enqueue_step(c, d_xy, stride, d_nbrs, d_counts, precision, sigma, seed, s);
+        const hipError_t ec = hipStreamEndCapture(s, &graph);
+        if (rc == GDT_OK && ec == hipSuccess &&
+            hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+            rc = GDT_ERR_DEVICE;
+        if (rc == GDT_OK && ec != hipSuccess) rc = GDT_ERR_DEVICE;
+        if (rc != GDT_OK) {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+    }
+    int done = 0;
+    hipError_t he = hipSuccess;
+    while (done < nsteps && rc == GDT_OK && he == hipSuccess) {
+        const int chunk = std::min(check_every, nsteps - done);
+        for (int t = 0; t < chunk && rc == GDT_OK && he == hipSuccess; ++t) {
+            if (use_graph) he = hipGraphLaunch(exec, s);
+            else rc = md_enqueue_step(c, d_xy, stride, d_nbrs, d_counts, precision, sigma, seed, s);
+        }
+        done += chunk;
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(c->h_md, c->md.p, sizeof(GdtMdState), hipMemcpyDeviceToHost, s);
+        if (he == hipSuccess) he = hipStreamSynchronize(s);
+        if (he == hipSuccess && rc == GDT_OK &&
+            (c->h_md->errflags || c->h_md->euler || c->h_md->required > stride))
+            break;
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc != GDT_OK) return rc;
+    if (he != hipSuccess) return GDT_ERR_DEVICE;
+    const GdtMdState& m = *c->h_md;
+    rep->steps_done = m.step - step0;
+    rep->first_bad_step = m.first_bad;
+    rep->required = m.required;
+    rep->max_flagged = m.max_flagged;
+    rep->flagged_total = (long long)m.flagged_total;
+    c->stats.flagged = m.max_flagged;
+    if (m.euler) return GDT_ERR_EULER;
+    if ((rc = status_from_flags(m.errflags)) != GDT_OK) return rc;
+    return m.required > stride ? GDT_OVERFLOW : GDT_OK;
+}
+
+// ---- Voronoi energy and adjoint -------------------------------------------
+}  // extern "C"
+
+static int voronoi_sets(gdt_context* c, int nsets, const int* offsets, const double* boxes)
+{
+    if (nsets <= 0 || !offsets || !boxes || offsets[0] != 0) return GDT_ERR_INVALID;
+    std::vector<GdtSet> v((size_t)nsets);
+    for (int s = 0; s < nsets; ++s) {
+        GdtSet S;
+        std::memset(&S, 0, sizeof(S));
+        S.p0 = offsets[s];
+        S.n = offsets[s + 1] - offsets[s];
+        S.lx = boxes[2 * s];
+        S.ly = boxes[2 * s + 1];
+        if (S.n < 3 || !(S.lx > 0.) || !(S.ly > 0.) || !std::isfinite(S.lx) || !std::isfinite(S.ly))
+            return GDT_ERR_INVALID;
+        v[(size_t)s] = S;
+    }
+    if (same_sets(v, c->vsets) && c->d_vsets.p) return GDT_OK;
+    c->vsets = v;
+    GDT_CHECK(upload(c->d_vsets, c->vsets));
+    return GDT_OK;
+}
+
+static int voronoi_forward(gdt_context* c, const double* d_xy, int nsets, const int* offsets,
+                           const double* boxes, const int* d_nbrs, const int* d_counts, int stride,
+                           double a0, double p0, int precision, double* d_area, double* d_perim,
+                           double* d_energy, void* stream)
+{
+    if (!c || !d_xy || !d_nbrs || !d_counts || !d_area || !d_perim || !d_energy || stride < 3)
+        return GDT_ERR_INVALID;
+    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    int rc = voronoi_sets(c, nsets, offsets, boxes);
+    if (rc != GDT_OK) return rc;
+    const int total = offsets[nsets];
+    GDT_CHECK(c->ecell.reserve((size_t)total));
+    GDT_CHECK(c->partial.reserve((size_t)nsets * GDT_SUM_PARTS));
+    GDT_CHECK(gdt_launch_voronoi_forward(precision, c->d_vsets.p, nsets, total, d_xy, d_nbrs,
+                                         d_counts, stride, a0, p0, d_area, d_perim, c->ecell.p,
+                                         c->partial.p, d_energy, (hipStream_t)stream));
+    return GDT_OK;
+}
+
+static int voronoi_adjoint(gdt_context* c, const double* d_xy, int nsets, const int* offsets,
+                           const double* boxes, const int* d_nbrs, const int* d_counts, int stride,
+                           double a0, double p0, int precision, const double* d_area,
+                           const double* d_perim, const double* d_gscale, double* d_grad,
+                           void* stream)
+{
+    if (!c || !d_xy || !d_nbrs || !d_counts || !d_area || !d_perim || !d_grad || stride < 3)
+        return GDT_ERR_INVALID;
+    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    int rc = voronoi_sets(c, nsets, offsets, boxes);
+    if (rc != GDT_OK) return rc;
+    const int total = offsets[nsets];
+    GDT_CHECK(c->vslots.reserve(2 * (size_t)total * (size_t)stride));
+    GDT_CHECK(gdt_launch_voronoi_adjoint(precision, c->d_vsets.p, nsets, total, d_xy, d_nbrs,
+                                         d_counts, stride, a0, p0, d_area, d_perim, d_gscale,
+                                         c->vslots.p, d_grad, (hipStream_t)stream));
+    return GDT_OK;
+}
+
+extern "C" {
+
+int gdt_voronoi_forward(gdt_context* c, const double* d_xy, int n, double lx, double ly,
+                        const int* d_nbrs, const int* d_counts, int stride, double a0, double p0,
+                        int precision, double* d_area, double* d_perim, double* d_energy,
+                        void* stream)
+{
+    const int off[2] = {0, n};
+    const double box[2] = {lx, ly};
+    return voronoi_forward(c, d_xy, 1, off, box, d_nbrs, d_counts, stride, a0, p0, precision,
+                           d_area, d_perim, d_energy, stream);
+}
+
+int gdt_voronoi_adjoint(gdt_context* c, const double* d_xy, int n, double lx, double ly,
+                        const int* d_nbrs, const int* d_counts, int stride, double a0, double p0,
+                        int precision, const double* d_area, const double* d_perim,
+                        const double* d_gscale, double* d_grad, void* stream)
+{
+    const int off[2] = {0, n};
+    const double box[2] = {lx, ly};
+    return voronoi_adjoint(c, d_xy, 1, off, box, d_nbrs, d_counts, stride, a0, p0, precision,
+                           d_area, d_perim, d_gscale, d_grad, stream);
+}
+
+int gdt_voronoi_forward_batch(gdt_context* c, const double* d_xy, int nsets, const int* offsets,
+                              const double* boxes, const int* d_nbrs, const int* d_counts,
+                              int stride, double a0, double p0, int precision, double* d_area,
+                              double* d_perim, double* d_energy, void* stream)
+{
+    return voronoi_forward(c, d_xy, nsets, offsets, boxes, d_nbrs, d_counts, stride, a0, p0,
+                           precision, d_area, d_perim, d_energy, stream);
+}
+
+int gdt_voronoi_adjoint_batch(gdt_context* c, const double* d_xy, int nsets, const int* offsets,
+                              const double* boxes, const int* d_nbrs, const int* d_counts,
+                              int stride, double a0, double p0, int precision,
+                              const double* d_area, const double* d_perim, const double* d_gscale,
+                              double* d_grad, void* stream)
+{
+    return voronoi_adjoint(c, d_xy, nsets, offsets, boxes, d_nbrs, d_counts, stride, a0, p0,
+                           precision, d_area, d_perim, d_gscale, d_grad, stream);
 }
 
 }  // extern "C"

@@ -54,7 +54,8 @@ hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
 hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nsets,
                                     const GdtChunk* tiles, int ntiles, int block, int total_n,
                                     const GdtRingIO& io, int* counts, int* status,
-                                    unsigned long long* counters, int* defer, hipStream_t s);
+                                    unsigned long long* counters, int* defer,
+                                    hipEvent_t after_tile, hipStream_t s);
 
 // Triangles owned by their smallest vertex, written at toff (deterministic).
 hipError_t gdt_launch_triangles(const GdtSet* d_sets, const GdtChunk* pchunks, int npch,
@@ -72,3 +73,35 @@ hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, con
 hipError_t gdt_launch_compact(const int* repair, const int* slot_of, int total_n, uint32_t* mark,
                               uint32_t* markscan, int* worklist, uint32_t* worklen,
                               uint32_t* scan_tmp, hipStream_t s);
+
+// Voronoi energy (per-set sums in energy[nsets]; partial holds
+// nsets * GDT_SUM_PARTS doubles, ecell total_n) and its adjoint (G holds
+// 2 * total_n * stride doubles; gscale[nsets] scales each set, null = 1).
+#define GDT_SUM_PARTS 256
+hipError_t gdt_launch_voronoi_forward(int precision, const GdtSet* d_sets, int nsets, int total_n,
+                                      const double* xy, const int* nbrs, const int* counts,
+                                      int stride, double a0, double p0, double* area, double* perim,
+                                      double* ecell, double* partial, double* energy,
+                                      hipStream_t s);
+hipError_t gdt_launch_voronoi_adjoint(int precision, const GdtSet* d_sets, int nsets, int total_n,
+                                      const double* xy, const int* nbrs, const int* counts,
+                                      int stride, double a0, double p0, const double* area,
+                                      const double* perim, const double* gscale, double* G,
+                                      double* grad, hipStream_t s);
+
+// Brownian test-and-repair loop: state kept on the device across steps.
+struct GdtMdState {
+    int step;                         // step index of the next displacement
+    int errflags;                     // OR of the steps' ERR_* flags
+    int euler;                        // 1 if a step started from a non-triangulation
+    int required;                     // longest ring seen
+    int first_bad;                    // first step with an error, -1 none
+    int max_flagged;                  // most points repaired in one step
+    unsigned long long flagged_total; // points repaired, summed over steps
+};
+hipError_t gdt_launch_md_init(GdtMdState* md, int step0, hipStream_t s);
+hipError_t gdt_launch_md_displace(const GdtSet* d_sets, const GdtChunk* pchunks, int npch,
+                                  double* xy, double sigma, unsigned long long seed,
+                                  const GdtMdState* md, hipStream_t s);
+hipError_t gdt_launch_md_step_end(const int* status, const unsigned long long* counters,
+                                  const uint32_t* ntri, int total_n, GdtMdState* md, hipStream_t s);

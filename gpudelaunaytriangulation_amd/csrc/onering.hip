@@ -308,7 +308,8 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
 hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nsets,
                                     const GdtChunk* tiles, int ntiles, int block, int total_n,
                                     const GdtRingIO& io, int* counts, int* status,
-                                    unsigned long long* counters, int* defer, hipStream_t s)
+                                    unsigned long long* counters, int* defer,
+                                    hipEvent_t after_tile, hipStream_t s)
 {
     if (ntiles <= 0) return hipSuccess;
     if (block > 1024) block = 1024;
@@ -320,6 +321,7 @@ hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nse
                            counts, status, counters, defer);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (after_tile && (e = hipEventRecord(after_tile, s)) != hipSuccess) return e;
     // Deferred points (count read on the device): the first GDT_WAVE_CAP one
     // wavefront each, any beyond (dense clusters) one lane each.
     const uint32_t* ndefer = reinterpret_cast<const uint32_t*>(status + ST_NDEFER);

@@ -386,3 +386,129 @@ class DelaunayGPU:
             self.globalDelaunayTriangulation(points, GPUTriangulation, cellNeighborNum, stream)
             return
         _lib.check(st, "testAndRepairDelaunayTriangulation")
+
+
+class BatchedDelaunayGPU:
+    """Many independent periodic sets in one launch per kernel (no reference
+    counterpart: the reference runs one DelaunayGPU per set).
+
+    Set s is rows ``offsets[s]:offsets[s+1]`` of the concatenated points, in a
+    ``boxes[s] = (lx, ly)`` box; ring entries index the concatenated array.
+    Same triangulation / test-and-repair semantics as DelaunayGPU, including
+    the growth of ``MaxSize`` (the row stride shared by every set)."""
+
+    def __init__(self, sizes, boxes, maximumNeighborsGuess: int = 16, cellSize: float = 1.0,
+                 device: int = 0, precision: int = _lib.PREC_FP64):
+        sizes = [int(s) for s in sizes]
+        if len(sizes) != len(boxes) or not sizes:
+            raise ValueError("one box per set")
+        self.offsets = [0]
+        for s in sizes:
+            self.offsets.append(self.offsets[-1] + s)
+        self.boxes = [(float(b[0]), float(b[1])) for b in boxes]
+        self.N = self.offsets[-1]
+        self.nsets = len(sizes)
+        self.MaxSize = max(4, int(maximumNeighborsGuess))
+        self.cellsize = float(cellSize)
+        self.ctx = Context(device)
+        self.precision = int(precision)
+        self.last_repaired = 0
+        self._off = (ctypes.c_int * len(self.offsets))(*self.offsets)
+        flat = [v for b in self.boxes for v in b]
+        self._box = (ctypes.c_double * len(flat))(*flat)
+
+    def setPrecision(self, precision: int) -> None:
+        if precision not in (_lib.PREC_FP64, _lib.PREC_FP32F):
+            raise ValueError("precision must be 64 or 32")
+        self.precision = int(precision)
+
+    def stats(self) -> dict:
+        return self.ctx.stats()
+
+    def _check(self, points, tri, num):
+        if points.dim() != 2 or points.shape[0] != self.N or points.shape[1] != 2:
+            raise ValueError(f"points must be [{self.N}, 2]")
+        _dev_ptr(points, torch.float64, "points")
+        _dev_ptr(tri, torch.int32, "GPUTriangulation")
+        _dev_ptr(num, torch.int32, "cellNeighborNum")
+        if num.numel() != self.N:
+            raise ValueError("cellNeighborNum must have N entries")
+
+    def globalDelaunayTriangulation(self, points, GPUTriangulation, cellNeighborNum,
+                                    stream=None) -> None:
+        if GPUTriangulation.numel() != self.N * self.MaxSize:
+            GPUTriangulation.resize_(self.N * self.MaxSize)
+        self._check(points, GPUTriangulation, cellNeighborNum)
+        L = _lib.load()
+        while True:
+            req = ctypes.c_int(0)
+            st = L.gdt_triangulate_batch(self.ctx.handle, points.data_ptr(), self.nsets, self._off,
+                                         self._box, self.cellsize, self.MaxSize,
+                                         GPUTriangulation.data_ptr(), cellNeighborNum.data_ptr(),
+                                         self.precision, ctypes.byref(req), _stream_ptr(stream))
+            if st == _lib.GDT_OVERFLOW:
+                self.MaxSize = int(req.value)
+                GPUTriangulation.resize_(self.N * self.MaxSize)
+                continue
+            _lib.check(st, "globalDelaunayTriangulation (batch)")
+            return
+
+    def testAndRepairDelaunayTriangulation(self, points, GPUTriangulation, cellNeighborNum,
+                                           stream=None) -> None:
+        self._check(points, GPUTriangulation, cellNeighborNum)
+        if GPUTriangulation.numel() != self.N * self.MaxSize:
+            raise ValueError("GPUTriangulation must hold N*MaxSize entries")
+        L = _lib.load()
+        nf, req = ctypes.c_int(0), ctypes.c_int(0)
+        st = L.gdt_test_and_repair_batch(self.ctx.handle, points.data_ptr(), self.nsets, self._off,
+                                         self._box, self.cellsize, self.MaxSize,
+                                         GPUTriangulation.data_ptr(), cellNeighborNum.data_ptr(),
+                                         self.precision, ctypes.byref(nf), ctypes.byref(req),
+                                         _stream_ptr(stream))
+        self.last_repaired = nf.value
+        if st == _lib.GDT_OVERFLOW:
+            self.MaxSize = int(req.value)
+            self.globalDelaunayTriangulation(points, GPUTriangulation, cellNeighborNum, stream)
+            return
+        _lib.check(st, "testAndRepairDelaunayTriangulation (batch)")
+
+    # --- Brownian test-and-repair loop -----------------------------------
+    def md_run(self, points, GPUTriangulation, cellNeighborNum, nsteps: int, sigma: float = 0.01,
+               seed: int = 1, step0: int = 0, use_graph: bool = True, check_every: int = 100,
+               stream=None) -> dict:
+        """``nsteps`` steps of: move every point by sigma*N(0,1) per coordinate
+        (counter-based noise keyed by (seed, step, set, point)), then
+        test-and-repair, all on the device (one hipGraph replay per step when
+        ``use_graph``).  The rings must triangulate ``points`` on entry.  A
+        step that overflows ``MaxSize`` or fails is recovered at the next
+        check by a global triangulation of the current positions (the
+        positions do not depend on the rings), then the loop continues.
+        Returns the totals (steps, repaired points, recoveries)."""
+        self._check(points, GPUTriangulation, cellNeighborNum)
+        if GPUTriangulation.numel() != self.N * self.MaxSize:
+            raise ValueError("GPUTriangulation must hold N*MaxSize entries")
+        L = _lib.load()
+        out = dict(steps=0, flagged_total=0, max_flagged=0, recoveries=0)
+        step, left = int(step0), int(nsteps)
+        while left > 0:
+            rep = _lib.MdReport()
+            st = L.gdt_md_run(self.ctx.handle, points.data_ptr(), self.nsets, self._off, self._box,
+                              self.cellsize, self.MaxSize, GPUTriangulation.data_ptr(),
+                              cellNeighborNum.data_ptr(), self.precision, float(sigma),
+                              int(seed) & 0xFFFFFFFFFFFFFFFF, step, left, int(bool(use_graph)),
+                              int(check_every), ctypes.byref(rep), _stream_ptr(stream))
+            step += rep.steps_done
+            left -= rep.steps_done
+            out["steps"] += rep.steps_done
+            out["flagged_total"] += rep.flagged_total
+            out["max_flagged"] = max(out["max_flagged"], rep.max_flagged)
+            if st == _lib.GDT_OK:
+                break
+            if st not in (_lib.GDT_OVERFLOW, _lib.GDT_ERR_NONGENERIC, _lib.GDT_ERR_EULER):
+                _lib.check(st, "md_run")
+            if st == _lib.GDT_OVERFLOW:
+                self.MaxSize = max(self.MaxSize, int(rep.required))
+            out["recoveries"] += 1
+            self.globalDelaunayTriangulation(points, GPUTriangulation, cellNeighborNum, stream)
+        self.last_repaired = out["max_flagged"]
+        return out

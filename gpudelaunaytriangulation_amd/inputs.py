@@ -146,6 +146,45 @@ def batch_points(sizes: np.ndarray, seed: int = 4) -> tuple[list[np.ndarray], li
     return pts, boxes
 
 
+MD_ATTEMPTS = 64
+
+
+def md_key(seed: int, step: int, set_index: int) -> int:
+    """Stream key of (seed, step, set) for the Brownian loop (gdt_rng.h)."""
+    c = np.array([(int(step) << 20) | int(set_index)], dtype=np.uint64)
+    return int(splitmix64(seed, c)[0])
+
+
+def md_noise(seed: int, step: int, set_index: int, n: int) -> np.ndarray:
+    """[n, 2] standard normals of one set at one step of the Brownian loop:
+    point i takes the first accepted polar-method attempt a < 64 over the
+    counters 2*(i*64 + a) and 2*(i*64 + a) + 1 of its stream; none accepted
+    gives (0, 0).  Bit-identical to gdt_gauss_pair (gdt_rng.h)."""
+    key = md_key(seed, step, set_index)
+    g = np.zeros((n, 2), dtype=np.float64)
+    todo = np.arange(n, dtype=np.uint64)
+    for a in range(MD_ATTEMPTS):
+        if todo.size == 0:
+            break
+        c = todo * np.uint64(MD_ATTEMPTS) + np.uint64(a)
+        v1 = 2.0 * uniform01(key, np.uint64(2) * c) - 1.0
+        v2 = 2.0 * uniform01(key, np.uint64(2) * c + np.uint64(1)) - 1.0
+        s = v1 * v1 + v2 * v2
+        ok = (s > 0.0) & (s < 1.0)
+        so = s[ok]
+        f = np.sqrt(-2.0 * det_log(so) / so)
+        g[todo[ok], 0] = v1[ok] * f
+        g[todo[ok], 1] = v2[ok] * f
+        todo = todo[~ok]
+    return g
+
+
+def md_step(xy: np.ndarray, lx: float, ly: float, sigma: float, seed: int, step: int,
+            set_index: int) -> np.ndarray:
+    """One Brownian displacement of one set, wrapped into the box."""
+    return wrap_into_box(xy + sigma * md_noise(seed, step, set_index, xy.shape[0]), lx, ly)
+
+
 def sha256(a: np.ndarray) -> str:
     """SHA-256 of an array's bytes in C order."""
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()

@@ -65,7 +65,8 @@ typedef struct {
     double bsx, bsy;                 /* cell sides */
     float ms_cells;                  /* last cell-list build (profile = 1), ms */
     float ms_ring;                   /* last one-ring kernel (profile = 1), ms */
-    int deferred;                    /* points the LDS tile kernel handed to the direct kernel */
+    int deferred;                    /* points the LDS tile kernel handed back */
+    float ms_defer;                  /* kernels of the handed-back points (profile = 1), ms */
 } gdt_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -137,6 +138,83 @@ int gdt_repair(gdt_context* ctx, const double* d_xy, int n, const int* d_repair,
 int gdt_test_and_repair(gdt_context* ctx, const double* d_xy, int n, double lx, double ly,
                         double cell_size, int stride, int* d_nbrs, int* d_counts, int precision,
                         int* nflagged, int* required, void* stream);
+
+/* ---- batches of independent sets (extension) ---------------------------
+ *
+ * nsets periodic sets concatenated in d_xy: set s is the caller points
+ * [offsets[s], offsets[s+1]) in a boxes[2s] x boxes[2s+1] box (offsets and
+ * boxes are host arrays, offsets[0] = 0).  One launch per kernel covers every
+ * set; rows of set s hold indices into the concatenated array (its own range).
+ * The reference has no batch API: a caller runs one DelaunayGPU per set
+ * (DelaunayGPU.cpp:512-565, 567-602 per set).  Status as the single-set calls;
+ * *required is the longest ring over all sets. */
+int gdt_triangulate_batch(gdt_context* ctx, const double* d_xy, int nsets, const int* offsets,
+                          const double* boxes, double cell_size, int stride, int* d_nbrs,
+                          int* d_counts, int precision, int* required, void* stream);
+int gdt_test_and_repair_batch(gdt_context* ctx, const double* d_xy, int nsets, const int* offsets,
+                              const double* boxes, double cell_size, int stride, int* d_nbrs,
+                              int* d_counts, int precision, int* nflagged, int* required,
+                              void* stream);
+
+/* ---- Brownian test-and-repair loop (extension) --------------------------
+ *
+ * nsteps steps on a batch whose rings are a triangulation of d_xy: each step
+ * moves every point by sigma * N(0,1) per coordinate (counter-based noise of
+ * (seed, step, set, point); wrapped into its box), then runs test-and-repair
+ * (triangles -> cell list -> empty-circle test -> repair) entirely on the
+ * device.  use_graph = 1 captures one step in a hipGraph and replays it.  The
+ * device state is read back every check_every steps; the run stops at the
+ * first check that finds an error (GDT_OVERFLOW with report.required,
+ * GDT_ERR_NONGENERIC, GDT_ERR_EULER): positions are then `steps_done` steps
+ * ahead and the rings must be rebuilt by gdt_triangulate_batch.  Replaces the
+ * reference driver's loop of host displacement + testAndRepair per step
+ * (triangulation.cpp:349-415; DelaunayGPU.cpp:567-602).  A null stream runs on
+ * a stream of the context (synchronised with the device first). */
+typedef struct {
+    int steps_done;         /* steps applied to the positions */
+    int first_bad_step;     /* first step whose status had an error, -1 none */
+    int required;           /* longest ring seen */
+    int max_flagged;        /* most points repaired in one step */
+    long long flagged_total;/* points repaired, summed over the steps */
+} gdt_md_report;
+
+int gdt_md_run(gdt_context* ctx, double* d_xy, int nsets, const int* offsets, const double* boxes,
+               double cell_size, int stride, int* d_nbrs, int* d_counts, int precision,
+               double sigma, unsigned long long seed, int step0, int nsteps, int use_graph,
+               int check_every, gdt_md_report* report, void* stream);
+
+/* ---- Voronoi energy (extension; no reference counterpart) --------------
+ *
+ * For the tessellation dual to a ring table: cell areas A_i, perimeters P_i
+ * and E = sum_i (A_i - a0)^2 + (P_i - p0)^2, then the adjoint
+ * grad[i] = gscale * dE/dx_i (FP64 output in both precisions; gscale is a
+ * device scalar, null = 1).  The Voronoi vertices are the circumcentres the
+ * reference keeps per point in GPUVoroCur (DelaunayGPU.cu:921-922,
+ * DelaunayGPU.h:57-58).  Rows that are not rings of a triangulation give NaN.
+ * Asynchronous: no host synchronisation; `area` and `perim` of the forward
+ * call are inputs of the adjoint. */
+int gdt_voronoi_forward(gdt_context* ctx, const double* d_xy, int n, double lx, double ly,
+                        const int* d_nbrs, const int* d_counts, int stride, double a0, double p0,
+                        int precision, double* d_area, double* d_perim, double* d_energy,
+                        void* stream);
+int gdt_voronoi_adjoint(gdt_context* ctx, const double* d_xy, int n, double lx, double ly,
+                        const int* d_nbrs, const int* d_counts, int stride, double a0, double p0,
+                        int precision, const double* d_area, const double* d_perim,
+                        const double* d_gscale, double* d_grad, void* stream);
+
+/* Batched form: nsets sets concatenated in d_xy, set s = points
+ * [offsets[s], offsets[s+1]) in an boxes[2s] x boxes[2s+1] box (host arrays);
+ * ring entries are indices into the concatenated array; d_energy[s] and
+ * d_gscale[s] per set. */
+int gdt_voronoi_forward_batch(gdt_context* ctx, const double* d_xy, int nsets, const int* offsets,
+                              const double* boxes, const int* d_nbrs, const int* d_counts,
+                              int stride, double a0, double p0, int precision, double* d_area,
+                              double* d_perim, double* d_energy, void* stream);
+int gdt_voronoi_adjoint_batch(gdt_context* ctx, const double* d_xy, int nsets, const int* offsets,
+                              const double* boxes, const int* d_nbrs, const int* d_counts,
+                              int stride, double a0, double p0, int precision,
+                              const double* d_area, const double* d_perim, const double* d_gscale,
+                              double* d_grad, void* stream);
 
 #ifdef __cplusplus
 }

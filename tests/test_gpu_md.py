@@ -1,0 +1,113 @@
+"""Brownian test-and-repair loop on the device (gdt_md_run), eager and as
+hipGraph replays:
+
+* positions after K steps are bit-identical to the numpy restatement of the
+  counter-based noise (inputs.md_step), and the rings equal the oracle's
+  triangulation of those positions (canonical, per set);
+* the graph run (K replays of one captured step, with readbacks between
+  checks) equals the eager run for K = 1, 3 and 12;
+* on the 256-set workload (about 3.97 M points, sigma = 0.01), 100 replayed
+  steps end with rings equal to a fresh global triangulation of the final
+  positions."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import oracle  # noqa: E402
+from gpudelaunaytriangulation_amd import PREC_FP32F, PREC_FP64, inputs, rings  # noqa: E402
+from gpudelaunaytriangulation_amd.delaunay import BatchedDelaunayGPU  # noqa: E402
+
+SIGMA, SEED = 0.01, 2024
+
+
+def canonical_sets(bd, tri, num):
+    nb = tri.cpu().numpy().reshape(bd.N, bd.MaxSize)
+    cnt = num.cpu().numpy()
+    out = []
+    for s in range(bd.nsets):
+        a, b = bd.offsets[s], bd.offsets[s + 1]
+        cs = cnt[a:b]
+        loc = np.where(np.arange(bd.MaxSize)[None, :] < cs[:, None], nb[a:b] - a, -1)
+        out.append((cs.copy(), rings.canonical(loc, cs)))
+    return out
+
+
+def same_sets(x, y):
+    for (c1, k1), (c2, k2) in zip(x, y):
+        if not np.array_equal(c1, c2):
+            return False
+        w = min(k1.shape[1], k2.shape[1])
+        if not np.array_equal(k1[:, :w], k2[:, :w]):
+            return False
+    return True
+
+
+def small_batch():
+    sizes = [1024, 1500, 2048, 3000, 4096, 1100]
+    pts = [inputs.uniform_points(n, float(np.sqrt(n)), seed=50 + s) for s, n in enumerate(sizes)]
+    boxes = [float(np.sqrt(n)) for n in sizes]
+    return sizes, pts, boxes
+
+
+def start(sizes, pts, boxes, prec=PREC_FP64):
+    bd = BatchedDelaunayGPU(sizes, [(L, L) for L in boxes], 16, 1.0, precision=prec)
+    X = torch.from_numpy(np.concatenate(pts)).cuda()
+    tri = torch.zeros(bd.N * bd.MaxSize, dtype=torch.int32, device="cuda")
+    num = torch.zeros(bd.N, dtype=torch.int32, device="cuda")
+    bd.globalDelaunayTriangulation(X, tri, num)
+    torch.cuda.synchronize()
+    return bd, X, tri, num
+
+
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32F])
+def test_positions_and_rings_match_restatement(prec):
+    sizes, pts, boxes = small_batch()
+    K = 12
+    bd, X, tri, num = start(sizes, pts, boxes, prec)
+    rep = bd.md_run(X, tri, num, K, SIGMA, SEED, use_graph=True, check_every=4)
+    torch.cuda.synchronize()
+    assert rep["steps"] == K and rep["recoveries"] == 0 and rep["flagged_total"] > 0
+    ref_pos, ref_rings = [], []
+    for s, (xy, L) in enumerate(zip(pts, boxes)):
+        for t in range(K):
+            xy = inputs.md_step(xy, L, L, SIGMA, SEED, t, s)
+        ref_pos.append(xy)
+        nb, c = oracle.triangulate(xy, L, L, 1.0, 20)
+        ref_rings.append((c, rings.canonical(nb, c)))
+    got = X.cpu().numpy()
+    assert np.array_equal(got.view(np.uint64), np.concatenate(ref_pos).view(np.uint64))
+    assert same_sets(canonical_sets(bd, tri, num), ref_rings)
+
+
+@pytest.mark.parametrize("K", [1, 3, 12])
+def test_graph_replays_equal_eager(K):
+    sizes, pts, boxes = small_batch()
+    bd_g, Xg, tg, ng = start(sizes, pts, boxes)
+    bd_e, Xe, te, ne = start(sizes, pts, boxes)
+    rg = bd_g.md_run(Xg, tg, ng, K, SIGMA, SEED, use_graph=True, check_every=2)
+    re = bd_e.md_run(Xe, te, ne, K, SIGMA, SEED, use_graph=False, check_every=2)
+    torch.cuda.synchronize()
+    assert rg == re
+    assert torch.equal(Xg, Xe)
+    assert torch.equal(ng, ne)
+    assert same_sets(canonical_sets(bd_g, tg, ng), canonical_sets(bd_e, te, ne))
+
+
+def test_c4_hundred_steps_equal_fresh_triangulation():
+    sizes = inputs.batch_sizes(256)
+    pts, boxes = inputs.batch_points(sizes)
+    bd, X, tri, num = start(sizes, pts, boxes)
+    rep = bd.md_run(X, tri, num, 100, SIGMA, SEED, use_graph=True, check_every=50)
+    torch.cuda.synchronize()
+    assert rep["steps"] == 100
+    assert 0 < rep["flagged_total"] < 100 * bd.N
+    fresh = BatchedDelaunayGPU(sizes, [(L, L) for L in boxes], 16, 1.0)
+    tri2 = torch.zeros(bd.N * fresh.MaxSize, dtype=torch.int32, device="cuda")
+    num2 = torch.zeros(bd.N, dtype=torch.int32, device="cuda")
+    fresh.globalDelaunayTriangulation(X, tri2, num2)
+    torch.cuda.synchronize()
+    assert int(num.sum().item()) == 6 * bd.N
+    assert same_sets(canonical_sets(bd, tri, num), canonical_sets(fresh, tri2, num2))
