@@ -31,7 +31,7 @@ EXPORTS = (
     "gdt_triangulate", "gdt_triangles", "gdt_test_circumcircles", "gdt_repair",
     "gdt_test_and_repair", "gdt_voronoi_forward", "gdt_voronoi_adjoint",
     "gdt_voronoi_forward_batch", "gdt_voronoi_adjoint_batch", "gdt_triangulate_batch",
-    "gdt_test_and_repair_batch", "gdt_md_run",
+    "gdt_test_and_repair_batch", "gdt_md_run", "gdt_voronoi_vertices",
 )
 
 
@@ -122,6 +122,7 @@ def load():
                                       _ip, _vp])
     _sig(L, "gdt_test_and_repair_batch", [_vp, _vp, _i, _ip, ctypes.POINTER(_d), _d, _i, _vp, _vp,
                                           _i, _ip, _ip, _vp])
+    _sig(L, "gdt_voronoi_vertices", [_vp, _vp, _i, _d, _d, _vp, _vp, _i, _vp, _vp])
     _sig(L, "gdt_md_run", [_vp, _vp, _i, _ip, ctypes.POINTER(_d), _d, _i, _vp, _vp, _i, _d,
                            ctypes.c_ulonglong, _i, _i, _i, _i, ctypes.POINTER(MdReport), _vp])
     _lib = L

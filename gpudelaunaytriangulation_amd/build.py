@@ -44,8 +44,34 @@ def _compile(src: str) -> str:
     return obj
 
 
+CXX = os.environ.get("CXX", "g++")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+BIN = os.path.join(PKG, "bin")
+DROPIN_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_check.cpp")
+DROPIN_BIN = os.path.join(BIN, "dropin_check")
+
+
+def cxx_dropin_cmd(src: str, out: str) -> list[str]:
+    """Host compile + link of a C++ caller of include/gpudelaunay.hpp."""
+    return [CXX, "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
+            f"-I{INCLUDE}", src, f"-L{PKG}", "-lgpudelaunay", f"-L{ROCM}/lib", "-lamdhip64",
+            "-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath,{ROCM}/lib", "-o", out]
+
+
+def build_dropin_check() -> str:
+    """The C++ drop-in check program (run by the GPU tests)."""
+    os.makedirs(BIN, exist_ok=True)
+    r = subprocess.run(cxx_dropin_cmd(DROPIN_SRC, DROPIN_BIN + ".tmp"), capture_output=True,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ drop-in check failed to build:\n{r.stderr}")
+    os.replace(DROPIN_BIN + ".tmp", DROPIN_BIN)
+    return DROPIN_BIN
+
+
 def build(verbose: bool = False) -> str:
-    """Compile every HIP source and link the shared library; returns its path."""
+    """Compile every HIP source and link the shared library (and the C++
+    drop-in check program against it); returns the library's path."""
     os.makedirs(BUILD, exist_ok=True)
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
@@ -55,6 +81,8 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    if os.path.exists(DROPIN_SRC):
+        build_dropin_check()
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -340,7 +340,7 @@ int gdt_set_config(gdt_context* c, const gdt_config* cfg)
     if (cfg->block < 64 || cfg->block > 1024 || cfg->block % 64 != 0) return GDT_ERR_INVALID;
     if (!(cfg->cell_scale >= 0.25 && cfg->cell_scale <= 8.0)) return GDT_ERR_INVALID;
     c->cfg = *cfg;
-    c->cfg.tiled = cfg->tiled ? 1 : 0;
+    c->cfg.tiled = cfg->tiled == 2 ? 2 : (cfg->tiled ? 1 : 0);
     c->sets.clear();          // force a new set table (cell size / order may change)
     c->cells_valid = false;
     c->rch_block = 0;
@@ -823,6 +823,21 @@ static int voronoi_adjoint(gdt_context* c, const double* d_xy, int nsets, const 
 }
 
 extern "C" {
+
+int gdt_voronoi_vertices(gdt_context* c, const double* d_xy, int n, double lx, double ly,
+                         const int* d_nbrs, const int* d_counts, int stride, double* d_vertices,
+                         void* stream)
+{
+    if (!c || !d_xy || !d_nbrs || !d_counts || !d_vertices || stride < 3) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    const int off[2] = {0, n};
+    const double box[2] = {lx, ly};
+    int rc = voronoi_sets(c, 1, off, box);
+    if (rc != GDT_OK) return rc;
+    GDT_CHECK(gdt_launch_voronoi_vertices(c->d_vsets.p, 1, n, d_xy, d_nbrs, d_counts, stride,
+                                          d_vertices, (hipStream_t)stream));
+    return GDT_OK;
+}
 
 int gdt_voronoi_forward(gdt_context* c, const double* d_xy, int n, double lx, double ly,
                         const int* d_nbrs, const int* d_counts, int stride, double a0, double p0,

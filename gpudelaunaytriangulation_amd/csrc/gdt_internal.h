@@ -61,21 +61,36 @@ struct GdtSet {
 
 struct GdtChunk { int set; int start; };   // a block's work: set id and first local item
 
-// Cell order.  Tiled: 16x16-cell tiles in row-major tile order, cells
+// Cell order.  tiled = 1: 16x16-cell tiles in row-major tile order, cells
 // row-major inside a tile, so one tile's points are one contiguous slot range
-// (the unit of work of the LDS-staged one-ring kernel).  Untiled: row-major.
+// (the unit of work of the LDS-staged one-ring kernel); tiled = 2: the same
+// tiles with the cells of a tile in Morton (Z) order, so that consecutive
+// slots (the points of one wavefront) form square patches; tiled = 0:
+// row-major cells.
 #ifndef GDT_TILE_SHIFT
 #define GDT_TILE_SHIFT 4
 #endif
 #define GDT_TILE (1 << GDT_TILE_SHIFT)
 #define GDT_TILE_CELLS (GDT_TILE * GDT_TILE)
 
+// Bits of v spread to the even positions (v < 2^GDT_TILE_SHIFT <= 2^8).
+GDT_HD inline int gdt_spread_bits(int v)
+{
+    v = (v | (v << 4)) & 0x0f0f;
+    v = (v | (v << 2)) & 0x3333;
+    v = (v | (v << 1)) & 0x5555;
+    return v;
+}
+
 GDT_HD inline int gdt_cell_key(const GdtSet& s, int cx, int cy)
 {
-    if (s.tiled)
-        return ((((cy >> GDT_TILE_SHIFT) * s.tiles_x) + (cx >> GDT_TILE_SHIFT))
-                << (2 * GDT_TILE_SHIFT)) |
-               ((cy & (GDT_TILE - 1)) << GDT_TILE_SHIFT) | (cx & (GDT_TILE - 1));
+    if (s.tiled) {
+        const int tile = ((cy >> GDT_TILE_SHIFT) * s.tiles_x) + (cx >> GDT_TILE_SHIFT);
+        const int ix = cx & (GDT_TILE - 1), iy = cy & (GDT_TILE - 1);
+        const int inner = s.tiled == 2 ? (gdt_spread_bits(ix) | (gdt_spread_bits(iy) << 1))
+                                       : ((iy << GDT_TILE_SHIFT) | ix);
+        return (tile << (2 * GDT_TILE_SHIFT)) | inner;
+    }
     return cy * s.xsize + cx;
 }
 
@@ -161,7 +176,7 @@ inline GdtSet gdt_make_set(int p0, int n, int c0, double lx, double ly, double c
     GdtSet s;
     s.p0 = p0; s.n = n; s.c0 = c0;
     gdt_grid(lx, ly, cell_size, s.xsize, s.ysize);
-    s.tiled = tiled ? 1 : 0;
+    s.tiled = tiled == 2 ? 2 : (tiled ? 1 : 0);
     s.tiles_x = (s.xsize + GDT_TILE - 1) / GDT_TILE;
     s.nkeys = gdt_nkeys(s.xsize, s.ysize, s.tiled);
     const int rx = (s.xsize - 1) / 2, ry = (s.ysize - 1) / 2;

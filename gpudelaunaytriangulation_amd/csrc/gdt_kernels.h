@@ -78,6 +78,9 @@ hipError_t gdt_launch_compact(const int* repair, const int* slot_of, int total_n
 // nsets * GDT_SUM_PARTS doubles, ecell total_n) and its adjoint (G holds
 // 2 * total_n * stride doubles; gscale[nsets] scales each set, null = 1).
 #define GDT_SUM_PARTS 256
+hipError_t gdt_launch_voronoi_vertices(const GdtSet* d_sets, int nsets, int total_n,
+                                       const double* xy, const int* nbrs, const int* counts,
+                                       int stride, double* out, hipStream_t s);
 hipError_t gdt_launch_voronoi_forward(int precision, const GdtSet* d_sets, int nsets, int total_n,
                                       const double* xy, const int* nbrs, const int* counts,
                                       int stride, double a0, double p0, double* area, double* perim,

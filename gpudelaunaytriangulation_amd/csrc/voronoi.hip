@@ -310,6 +310,46 @@ __global__ __launch_bounds__(256) void k_voronoi_gather(const GdtSet* __restrict
     grad[2 * (size_t)p + 1] = gy;
 }
 
+// Voronoi vertices relative to each point, one per ring slot:
+// out[i*stride + m] = circumcentre of (x_i, x_{n_m}, x_{n_{m+1}}) - x_i
+// (the layout of the reference's GPUVoroCur); invalid rows are NaN.
+__global__ __launch_bounds__(256) void k_voronoi_vertices(const GdtSet* __restrict__ sets,
+                                                          int nsets, int total_n,
+                                                          const double* __restrict__ xy,
+                                                          const int* __restrict__ nbrs,
+                                                          const int* __restrict__ counts,
+                                                          int stride, double* __restrict__ out)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= total_n) return;
+    const int k = counts[i];
+    const int* row = nbrs + (size_t)i * stride;
+    double* o = out + 2 * (size_t)i * stride;
+    if (!vor_row_ok(row, k, stride, total_n)) {
+        for (int m = 0; m < 2 * stride; ++m) o[m] = __builtin_nan("");
+        return;
+    }
+    const GdtSet S = sets[gdt_set_of(sets, nsets, i)];
+    const V2<double> d0 = vor_round<double>(vor_disp(S, xy, i, row[0]));
+    V2<double> a = d0;
+    for (int m = 0; m < k; ++m) {
+        const V2<double> b = m + 1 < k ? vor_round<double>(vor_disp(S, xy, i, row[m + 1])) : d0;
+        const V2<double> h = vor_circ<double>(a, b);
+        o[2 * m] = h.x;
+        o[2 * m + 1] = h.y;
+        a = b;
+    }
+}
+
+hipError_t gdt_launch_voronoi_vertices(const GdtSet* d_sets, int nsets, int total_n,
+                                       const double* xy, const int* nbrs, const int* counts,
+                                       int stride, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_voronoi_vertices, dim3((total_n + 255) / 256), dim3(256), 0, s, d_sets,
+                       nsets, total_n, xy, nbrs, counts, stride, out);
+    return hipGetLastError();
+}
+
 hipError_t gdt_launch_voronoi_forward(int precision, const GdtSet* d_sets, int nsets, int total_n,
                                       const double* xy, const int* nbrs, const int* counts,
                                       int stride, double a0, double p0, double* area, double* perim,

@@ -49,7 +49,9 @@ typedef struct gdt_context gdt_context;
 /* Tunables (all have defaults; the autotuner writes them). */
 typedef struct {
     int block;          /* threads per block of the one-ring kernel (64..1024) */
-    int tiled;          /* 1: cells stored in 8x8 tiles (Morton-like locality), 0: row-major */
+    int tiled;          /* cell order: 1 = 16x16-cell tiles (LDS tile kernel, default),
+                           2 = the same tiles, Morton order inside, 0 = row-major cells
+                           (direct per-point kernel) */
     double cell_scale;  /* multiplies the caller's cell size (1.0 = as given) */
     int profile;        /* 1: time the cell list and one-ring kernels with HIP events */
 } gdt_config;
@@ -193,6 +195,12 @@ int gdt_md_run(gdt_context* ctx, double* d_xy, int nsets, const int* offsets, co
  * DelaunayGPU.h:57-58).  Rows that are not rings of a triangulation give NaN.
  * Asynchronous: no host synchronisation; `area` and `perim` of the forward
  * call are inputs of the adjoint. */
+/* Voronoi vertex m of every point relative to it, d_vertices[i*stride + m]
+ * (double2 per slot; the GPUVoroCur array of the reference, filled there by
+ * get_oneRing_function, DelaunayGPU.cu:921-922); invalid rows give NaN. */
+int gdt_voronoi_vertices(gdt_context* ctx, const double* d_xy, int n, double lx, double ly,
+                         const int* d_nbrs, const int* d_counts, int stride, double* d_vertices,
+                         void* stream);
 int gdt_voronoi_forward(gdt_context* ctx, const double* d_xy, int n, double lx, double ly,
                         const int* d_nbrs, const int* d_counts, int stride, double a0, double p0,
                         int precision, double* d_area, double* d_perim, double* d_energy,

@@ -1,0 +1,537 @@
+// gpudelaunay.hpp -- C++ drop-in for the reference's host classes, over the
+// C ABI of gpudelaunay.h (header only; link libgpudelaunay.so and amdhip64).
+//
+// Class surface kept from the reference so that callers compile unchanged:
+//   Index2D                       indexer.h:32-62
+//   periodicBoundaries, PeriodicBoxPtr   periodicBoundaries.h:25-73, 191
+//   GPUArray<T>, ArrayHandle<T>   gpuarray.h:26-62, 305-328, 429-590,
+//                                 983-1127 (host/device mirror), 1316-1335 (resize)
+//   cellListGPU                   cellListGPU.h:17-121
+//   DelaunayGPU                   DelaunayGPU.h:18-58
+// Errors: a failing HIP call or library status throws std::runtime_error (the
+// reference prints and throws std::exception, std_include.h:61-72).  There is
+// no CPU mode: setGPUcompute(false) throws.  Only rectangular boxes.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <hip/hip_vector_types.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "gpudelaunay.h"
+
+namespace gdt_detail {
+
+inline void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+inline void status_check(int st, const char* what)
+{
+    if (st != GDT_OK) throw std::runtime_error(std::string(what) + ": " + gdt_status_string(st));
+}
+
+}  // namespace gdt_detail
+
+// ---------------------------------------------------------------------------
+//! Row-major 2D indexer: (i, j) -> j*w + i (indexer.h:32-62).
+class Index2D {
+public:
+    explicit Index2D(unsigned int w = 0) : m_w(w), m_h(w) {}
+    Index2D(unsigned int w, unsigned int h) : m_w(w), m_h(h) {}
+    unsigned int operator()(unsigned int i, unsigned int j) const { return j * m_w + i; }
+    unsigned int getNumElements() const { return m_w * m_h; }
+    unsigned int getW() const { return m_w; }
+    unsigned int getH() const { return m_h; }
+
+private:
+    unsigned int m_w, m_h;
+};
+
+// ---------------------------------------------------------------------------
+//! Rectangular periodic box (periodicBoundaries::setSquare / minDist / putInBoxReal).
+class periodicBoundaries {
+public:
+    periodicBoundaries() { setSquare(1.0, 1.0); }
+    periodicBoundaries(double x, double y) { setSquare(x, y); }
+    void setSquare(double x, double y)
+    {
+        if (!(x > 0.) || !(y > 0.)) throw std::invalid_argument("box sides must be positive");
+        x11 = x;
+        x22 = y;
+    }
+    void getBoxDims(double& xx, double& xy, double& yx, double& yy) const
+    {
+        xx = x11;
+        xy = 0.;
+        yx = 0.;
+        yy = x22;
+    }
+    bool isBoxSquare() const { return true; }
+    //! nearest-image displacement p1 - p2
+    void minDist(const double2& p1, const double2& p2, double2& out) const
+    {
+        double dx = p1.x - p2.x, dy = p1.y - p2.y;
+        dx -= x11 * std::nearbyint(dx / x11);
+        dy -= x22 * std::nearbyint(dy / x22);
+        out.x = dx;
+        out.y = dy;
+    }
+    //! wrap into [0, Lx) x [0, Ly)
+    void putInBoxReal(double2& p) const
+    {
+        p.x -= std::floor(p.x / x11) * x11;
+        p.y -= std::floor(p.y / x22) * x22;
+        if (p.x >= x11) p.x -= x11;
+        if (p.y >= x22) p.y -= x22;
+    }
+    double x11 = 1., x22 = 1.;
+};
+typedef std::shared_ptr<periodicBoundaries> PeriodicBoxPtr;
+
+// ---------------------------------------------------------------------------
+struct access_location { enum Enum { host, device }; };
+struct access_mode { enum Enum { read, readwrite, overwrite }; };
+struct data_location { enum Enum { host, device, hostdevice }; };
+
+//! Host/device mirrored array.  Data live where they were last written; an
+//! acquire copies only when the requested side is stale and the mode is not
+//! overwrite (gpuarray.h:983-1127).  One handle at a time (gpuarray.h:991).
+template <class T>
+class GPUArray {
+public:
+    GPUArray() = default;
+    explicit GPUArray(unsigned int n) { allocate(n); }
+    GPUArray(const GPUArray& o) { copy_from(o); }
+    GPUArray& operator=(const GPUArray& o)
+    {
+        if (this != &o) {
+            deallocate();
+            copy_from(o);
+        }
+        return *this;
+    }
+    GPUArray(GPUArray&& o) noexcept { swap(o); }
+    GPUArray& operator=(GPUArray&& o) noexcept
+    {
+        swap(o);
+        return *this;
+    }
+    ~GPUArray() { deallocate(); }
+
+    unsigned int getNumElements() const { return m_n; }
+    bool isNull() const { return m_h == nullptr; }
+
+    void swap(GPUArray& o) noexcept
+    {
+        std::swap(m_n, o.m_n);
+        std::swap(m_h, o.m_h);
+        std::swap(m_d, o.m_d);
+        std::swap(m_loc, o.m_loc);
+        std::swap(m_acquired, o.m_acquired);
+    }
+
+    //! Resize, keeping the first min(old, new) elements (gpuarray.h:1316-1335).
+    void resize(unsigned int n)
+    {
+        if (m_acquired) throw std::logic_error("GPUArray::resize while acquired");
+        if (n == m_n) return;
+        GPUArray<T> t(n);
+        const size_t keep = (size_t)std::min(n, m_n) * sizeof(T);
+        if (keep) {
+            if (m_loc == data_location::device) {
+                gdt_detail::hip_check(hipMemcpy(t.m_d, m_d, keep, hipMemcpyDeviceToDevice),
+                                      "GPUArray::resize");
+                t.m_loc = data_location::device;
+            } else {
+                std::memcpy(t.m_h, m_h, keep);
+                if (m_loc == data_location::host) {
+                    t.m_loc = data_location::host;
+                } else {
+                    gdt_detail::hip_check(hipMemcpy(t.m_d, m_d, keep, hipMemcpyDeviceToDevice),
+                                          "GPUArray::resize");
+                    t.m_loc = data_location::hostdevice;
+                }
+            }
+        }
+        swap(t);
+    }
+
+    T* acquire(access_location::Enum where, access_mode::Enum mode)
+    {
+        if (m_acquired) throw std::logic_error("GPUArray already acquired");
+        if (isNull()) return nullptr;
+        m_acquired = true;
+        const size_t bytes = (size_t)m_n * sizeof(T);
+        if (where == access_location::host) {
+            if (m_loc == data_location::device && mode != access_mode::overwrite)
+                gdt_detail::hip_check(hipMemcpy(m_h, m_d, bytes, hipMemcpyDeviceToHost),
+                                      "GPUArray device->host");
+            const bool both = mode == access_mode::read &&
+                              m_loc != data_location::host;
+            m_loc = both ? data_location::hostdevice : data_location::host;
+            return m_h;
+        }
+        if (m_loc == data_location::host && mode != access_mode::overwrite)
+            gdt_detail::hip_check(hipMemcpy(m_d, m_h, bytes, hipMemcpyHostToDevice),
+                                  "GPUArray host->device");
+        const bool both = mode == access_mode::read && m_loc != data_location::device;
+        m_loc = both ? data_location::hostdevice : data_location::device;
+        return m_d;
+    }
+    void release() { m_acquired = false; }
+
+private:
+    void allocate(unsigned int n)
+    {
+        m_n = n;
+        if (!n) return;
+        const size_t bytes = (size_t)n * sizeof(T);
+        gdt_detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m_h), bytes, 0),
+                              "GPUArray host allocation");
+        gdt_detail::hip_check(hipMalloc(reinterpret_cast<void**>(&m_d), bytes),
+                              "GPUArray device allocation");
+        std::memset(m_h, 0, bytes);
+        gdt_detail::hip_check(hipMemset(m_d, 0, bytes), "GPUArray device clear");
+        m_loc = data_location::hostdevice;
+    }
+    void deallocate()
+    {
+        if (m_h) (void)hipHostFree(m_h);
+        if (m_d) (void)hipFree(m_d);
+        m_h = nullptr;
+        m_d = nullptr;
+        m_n = 0;
+        m_acquired = false;
+        m_loc = data_location::hostdevice;
+    }
+    void copy_from(const GPUArray& o)
+    {
+        allocate(o.m_n);
+        if (!m_n) return;
+        const size_t bytes = (size_t)m_n * sizeof(T);
+        if (o.m_loc != data_location::device) std::memcpy(m_h, o.m_h, bytes);
+        if (o.m_loc != data_location::host)
+            gdt_detail::hip_check(hipMemcpy(m_d, o.m_d, bytes, hipMemcpyDeviceToDevice),
+                                  "GPUArray copy");
+        m_loc = o.m_loc;
+    }
+
+    unsigned int m_n = 0;
+    T* m_h = nullptr;
+    T* m_d = nullptr;
+    data_location::Enum m_loc = data_location::hostdevice;
+    bool m_acquired = false;
+};
+
+//! Scoped access to a GPUArray (gpuarray.h:305-328).
+template <class T>
+class ArrayHandle {
+public:
+    ArrayHandle(GPUArray<T>& a, access_location::Enum where = access_location::host,
+                access_mode::Enum mode = access_mode::readwrite)
+        : data(a.acquire(where, mode)), m_arr(a)
+    {
+    }
+    ArrayHandle(const GPUArray<T>& a, access_location::Enum where = access_location::host,
+                access_mode::Enum mode = access_mode::read)
+        : ArrayHandle(const_cast<GPUArray<T>&>(a), where, mode)
+    {
+    }
+    ~ArrayHandle() { m_arr.release(); }
+    ArrayHandle(const ArrayHandle&) = delete;
+    ArrayHandle& operator=(const ArrayHandle&) = delete;
+    T* const data;
+
+private:
+    GPUArray<T>& m_arr;
+};
+
+// ---------------------------------------------------------------------------
+//! Owns one library context on the current HIP device.
+class gdtContext {
+public:
+    gdtContext()
+    {
+        int dev = 0;
+        gdt_detail::hip_check(hipGetDevice(&dev), "hipGetDevice");
+        gdt_detail::status_check(gdt_create(&m_c, dev), "gdt_create");
+    }
+    ~gdtContext() { gdt_destroy(m_c); }
+    gdtContext(const gdtContext&) = delete;
+    gdtContext& operator=(const gdtContext&) = delete;
+    gdt_context* get() const { return m_c; }
+
+private:
+    gdt_context* m_c = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+//! Cell list (cellListGPU.h:17-121): a counting sort inside the library,
+//! presented as the reference's cell_sizes / padded idxs on request.
+class cellListGPU {
+public:
+    cellListGPU() : Box(std::make_shared<periodicBoundaries>()), m_ctx(std::make_shared<gdtContext>()) {}
+    explicit cellListGPU(std::shared_ptr<gdtContext> ctx)
+        : Box(std::make_shared<periodicBoundaries>()), m_ctx(std::move(ctx))
+    {
+    }
+
+    void setNp(int nn) { Np = nn; }
+    void setBox(periodicBoundaries& bx) { Box = std::make_shared<periodicBoundaries>(bx); }
+    void setBox(PeriodicBoxPtr bx) { Box = std::move(bx); }
+
+    //! cellListGPU::setGridSize (cellListGPU.cpp:97-117): cells per axis
+    //! floor(L/a) rounded up to even; the y side uses Ly/ysize.
+    void setGridSize(double a)
+    {
+        m_a = a;
+        xsize = (int)std::floor(Box->x11 / a);
+        if (xsize % 2) ++xsize;
+        ysize = (int)std::floor(Box->x22 / a);
+        if (ysize % 2) ++ysize;
+        xsize = std::max(xsize, 2);
+        ysize = std::max(ysize, 2);
+        boxsize = Box->x11 / xsize;
+        totalCells = xsize * ysize;
+        cell_indexer = Index2D(xsize, ysize);
+    }
+    int getNmax() { return Nmax; }
+    int getXsize() { return xsize; }
+    int getYsize() { return ysize; }
+    double getBoxsize() { return boxsize; }
+
+    int positionToCellIndex(double x, double y)
+    {
+        const double bsy = Box->x22 / ysize;
+        const int cx = std::min(xsize - 1, std::max(0, (int)std::floor(x / boxsize)));
+        const int cy = std::min(ysize - 1, std::max(0, (int)std::floor(y / bsy)));
+        return cell_indexer(cx, cy);
+    }
+    //! all cells within `width` (the square around the cell)
+    void getCellNeighbors(int cellIndex, int width, std::vector<int>& out)
+    {
+        const int w = std::min(width, xsize / 2);
+        const int cx = cellIndex % xsize, cy = cellIndex / xsize;
+        out.clear();
+        for (int i = -w; i <= w; ++i)
+            for (int j = -w; j <= w; ++j)
+                out.push_back(cell_indexer(wrap(cx + j, xsize), wrap(cy + i, ysize)));
+    }
+    //! cells on the square shell at distance `width`
+    void getCellShellNeighbors(int cellIndex, int width, std::vector<int>& out)
+    {
+        const int w = std::min(width, xsize);
+        const int cx = cellIndex % xsize, cy = cellIndex / xsize;
+        out.clear();
+        for (int i = -w; i <= w; ++i)
+            for (int j = -w; j <= w; ++j)
+                if (i == -w || i == w || j == -w || j == w)
+                    out.push_back(cell_indexer(wrap(cx + j, xsize), wrap(cy + i, ysize)));
+    }
+
+    //! Bin the points on the device; fills cell_sizes and idxs (reference layout).
+    void computeGPU(GPUArray<double2>& points)
+    {
+        const int n = (int)points.getNumElements();
+        if (xsize == 0) setGridSize(m_a);
+        {
+            ArrayHandle<double2> p(points, access_location::device, access_mode::read);
+            gdt_detail::status_check(gdt_celllist_build(m_ctx->get(), reinterpret_cast<const double*>(p.data),
+                                                        n, Box->x11, Box->x22, m_a, nullptr),
+                                     "cellListGPU::computeGPU");
+        }
+        Np = n;
+        int xs = 0, ys = 0, mo = 0;
+        double bx = 0., by = 0.;
+        gdt_detail::status_check(gdt_celllist_info(m_ctx->get(), &xs, &ys, &bx, &by, &mo, nullptr),
+                                 "gdt_celllist_info");
+        xsize = xs;
+        ysize = ys;
+        boxsize = bx;
+        totalCells = xs * ys;
+        Nmax = std::max(Nmax, mo);
+        cell_indexer = Index2D(xsize, ysize);
+        cell_list_indexer = Index2D(Nmax, totalCells);
+        if ((int)cell_sizes.getNumElements() != totalCells) cell_sizes.resize(totalCells);
+        if ((int)idxs.getNumElements() != totalCells * Nmax) idxs.resize(totalCells * Nmax);
+        ArrayHandle<unsigned int> cs(cell_sizes, access_location::device, access_mode::overwrite);
+        ArrayHandle<int> ix(idxs, access_location::device, access_mode::overwrite);
+        gdt_detail::status_check(gdt_celllist_export(m_ctx->get(), cs.data, ix.data, Nmax, nullptr),
+                                 "gdt_celllist_export");
+        gdt_detail::hip_check(hipDeviceSynchronize(), "cellListGPU::computeGPU");
+    }
+    //! The reference's CPU binning gives the same buckets in the same order.
+    void compute(GPUArray<double2>& points) { computeGPU(points); }
+    void computeGPU() { computeGPU(particles); }
+    void compute() { computeGPU(particles); }
+
+    const GPUArray<unsigned int>& getCellSizeArray() const { return cell_sizes; }
+    const GPUArray<int>& getIdxArray() const { return idxs; }
+
+    Index2D cell_indexer;
+    Index2D cell_list_indexer;
+    GPUArray<double2> particles;
+    GPUArray<unsigned int> cell_sizes;
+    GPUArray<int> idxs;
+    PeriodicBoxPtr Box;
+
+protected:
+    static int wrap(int v, int n) { return ((v % n) + n) % n; }
+    std::shared_ptr<gdtContext> m_ctx;
+    int Np = 0, xsize = 0, ysize = 0, totalCells = 0, Nmax = 0;
+    double boxsize = 0., m_a = 1.;
+};
+
+// ---------------------------------------------------------------------------
+//! Periodic Delaunay triangulation (DelaunayGPU.h:18-58).
+class DelaunayGPU {
+public:
+    DelaunayGPU() : m_ctx(std::make_shared<gdtContext>()), cList(m_ctx) {}
+    DelaunayGPU(int N, int maximumNeighborsGuess, double cellSize, PeriodicBoxPtr bx)
+        : DelaunayGPU()
+    {
+        initialize(N, maximumNeighborsGuess, cellSize, std::move(bx));
+    }
+
+    void initialize(int N, int maximumNeighborsGuess, double cellSize, PeriodicBoxPtr bx)
+    {
+        Ncells = N;
+        MaxSize = std::max(4, maximumNeighborsGuess);
+        cellsize = cellSize;
+        setBox(std::move(bx));
+        cList.setNp(N);
+        cList.setGridSize(cellsize);
+    }
+    void resize(const int nmax) { MaxSize = nmax; }
+    void setCellListSize(double csize)
+    {
+        cellsize = csize;
+        cList.setGridSize(cellsize);
+    }
+    void updateList(GPUArray<double2>& points) { cList.computeGPU(points); }
+    void setBox(PeriodicBoxPtr bx)
+    {
+        Box = bx ? std::move(bx) : std::make_shared<periodicBoundaries>();
+        cList.setBox(Box);
+    }
+    void setSafetyMode(bool sm) { safetyMode = sm; }
+    void setGPUcompute(bool flag)
+    {
+        if (!flag) throw std::invalid_argument("DelaunayGPU: CPU mode is not part of this library");
+    }
+    void setOMPthreads(unsigned int num) { OMPThreadsNum = num; }
+    //! 64: all-FP64 predicates; 32: FP32 with a certified bound + FP64 fallback
+    void setPrecision(int p)
+    {
+        if (p != GDT_PREC_FP64 && p != GDT_PREC_FP32F) throw std::invalid_argument("precision");
+        precision = p;
+    }
+    //! GPUVoroCur is filled after every call while this is on (default).
+    void setVoronoiVertices(bool on) { voronoiVertices = on; }
+
+    //! DelaunayGPU::globalDelaunayTriangulation (DelaunayGPU.cpp:118-157):
+    //! grows MaxSize and GPUTriangulation until every ring fits.
+    void globalDelaunayTriangulation(GPUArray<double2>& points, GPUArray<int>& GPUTriangulation,
+                                     GPUArray<int>& cellNeighborNum)
+    {
+        const int n = (int)points.getNumElements();
+        if (n == 0) return;
+        Ncells = n;
+        if ((int)cellNeighborNum.getNumElements() != n) cellNeighborNum.resize(n);
+        for (;;) {
+            if ((int)GPUTriangulation.getNumElements() != n * MaxSize)
+                GPUTriangulation.resize(n * MaxSize);
+            int req = 0, st;
+            {
+                ArrayHandle<double2> p(points, access_location::device, access_mode::read);
+                ArrayHandle<int> t(GPUTriangulation, access_location::device, access_mode::overwrite);
+                ArrayHandle<int> c(cellNeighborNum, access_location::device, access_mode::overwrite);
+                st = gdt_triangulate(m_ctx->get(), reinterpret_cast<const double*>(p.data), n,
+                                     Box->x11, Box->x22, cellsize, MaxSize, t.data, c.data,
+                                     precision, &req, nullptr);
+            }
+            if (st == GDT_OVERFLOW) {
+                MaxSize = req;
+                continue;
+            }
+            gdt_detail::status_check(st, "DelaunayGPU::globalDelaunayTriangulation");
+            break;
+        }
+        fill_voronoi(points, GPUTriangulation, cellNeighborNum);
+    }
+
+    //! DelaunayGPU::testAndRepairDelaunayTriangulation (DelaunayGPU.cpp:567-602).
+    void testAndRepairDelaunayTriangulation(GPUArray<double2>& points,
+                                            GPUArray<int>& GPUTriangulation,
+                                            GPUArray<int>& cellNeighborNum)
+    {
+        const int n = (int)points.getNumElements();
+        if ((int)GPUTriangulation.getNumElements() != n * MaxSize)
+            throw std::invalid_argument("GPUTriangulation must hold N*MaxSize entries");
+        int nf = 0, req = 0, st;
+        {
+            ArrayHandle<double2> p(points, access_location::device, access_mode::read);
+            ArrayHandle<int> t(GPUTriangulation, access_location::device, access_mode::readwrite);
+            ArrayHandle<int> c(cellNeighborNum, access_location::device, access_mode::readwrite);
+            st = gdt_test_and_repair(m_ctx->get(), reinterpret_cast<const double*>(p.data), n,
+                                     Box->x11, Box->x22, cellsize, MaxSize, t.data, c.data,
+                                     precision, &nf, &req, nullptr);
+        }
+        lastRepaired = nf;
+        if (st == GDT_OVERFLOW) {
+            MaxSize = req;
+            globalDelaunayTriangulation(points, GPUTriangulation, cellNeighborNum);
+            return;
+        }
+        gdt_detail::status_check(st, "DelaunayGPU::testAndRepairDelaunayTriangulation");
+        fill_voronoi(points, GPUTriangulation, cellNeighborNum);
+    }
+
+    gdt_stats stats() const
+    {
+        gdt_stats s;
+        gdt_detail::status_check(gdt_get_stats(m_ctx->get(), &s), "gdt_get_stats");
+        return s;
+    }
+
+    PeriodicBoxPtr Box;
+    int MaxSize = 4;
+    //! Voronoi vertex m of point i relative to it at [i*MaxSize + m]
+    GPUArray<double2> GPUVoroCur;
+    int lastRepaired = 0;
+
+protected:
+    void fill_voronoi(GPUArray<double2>& points, GPUArray<int>& tri, GPUArray<int>& num)
+    {
+        if (!voronoiVertices) return;
+        const int n = (int)points.getNumElements();
+        if ((int)GPUVoroCur.getNumElements() != n * MaxSize) GPUVoroCur.resize(n * MaxSize);
+        ArrayHandle<double2> p(points, access_location::device, access_mode::read);
+        ArrayHandle<int> t(tri, access_location::device, access_mode::read);
+        ArrayHandle<int> c(num, access_location::device, access_mode::read);
+        ArrayHandle<double2> v(GPUVoroCur, access_location::device, access_mode::overwrite);
+        gdt_detail::status_check(gdt_voronoi_vertices(m_ctx->get(), reinterpret_cast<const double*>(p.data), n,
+                                                      Box->x11, Box->x22, t.data, c.data, MaxSize,
+                                                      reinterpret_cast<double*>(v.data), nullptr),
+                                 "gdt_voronoi_vertices");
+    }
+
+    std::shared_ptr<gdtContext> m_ctx;
+    cellListGPU cList;
+    int Ncells = 0;
+    double cellsize = 1.10;     // reference default (DelaunayGPU.cpp:7)
+    bool safetyMode = false;
+    unsigned int OMPThreadsNum = 1;
+    int precision = GDT_PREC_FP64;
+    bool voronoiVertices = true;
+};

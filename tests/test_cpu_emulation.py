@@ -72,9 +72,10 @@ def test_emulated_kernels_match_oracle(emul, tmp_path, case):
     on, oc = oracle.triangulate(xy, lx, ly, 1.0, 16)
     ref = rings.ring_hash(on, oc)
     outs = {}
-    # (tiled order, mode): direct routine in both cell orders, tile routine,
-    # wavefront routine in both cell orders
-    for tiled, mode in ((1, 0), (0, 0), (1, 1), (1, 2), (0, 2)):
+    # (cell order, mode): direct routine in tile and row-major order, tile
+    # routine in tile and Morton-in-tile order, wavefront routine in tile and
+    # row-major order
+    for tiled, mode in ((1, 0), (0, 0), (1, 1), (2, 1), (1, 2), (0, 2)):
         for prec in (64, 32):
             nb, c, npred, nfb, err = run(emul, str(tmp_path), xy, lx, ly, tiled, prec, mode=mode,
                                          cell=cell)

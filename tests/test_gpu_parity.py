@@ -100,10 +100,37 @@ def test_row_major_and_tiled_cell_orders_agree():
     assert np.array_equal(nb_t, nb_r) and np.array_equal(c_t, c_r)
 
 
+def test_morton_order_inside_tiles_agrees():
+    xy = inputs.uniform_points(65536, 256.0, 256.0, seed=9)
+    nb_t, c_t, dg = gpu_triangulate(xy, 256.0, 256.0, PREC_FP32F)
+    dg.ctx.set_config(tiled=2)
+    nb_m, c_m, _ = gpu_triangulate(xy, 256.0, 256.0, PREC_FP32F, dg=dg)
+    assert dg.ctx.config().tiled == 2
+    assert np.array_equal(nb_t, nb_m) and np.array_equal(c_t, c_m)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_point_order_does_not_change_the_rings(prec):
+    """A permuted input gives the same triangulation after mapping indices back."""
+    L = 256.0
+    xy = inputs.uniform_points(65536, L, L, seed=10)
+    perm = np.random.default_rng(11).permutation(xy.shape[0])
+    nb, c, _ = gpu_triangulate(xy, L, L, prec)
+    nbp, cp, _ = gpu_triangulate(xy[perm], L, L, prec)
+    # permuted run: row j is original point perm[j] and its entries are
+    # permuted indices; map entries through perm and rows through perm^-1
+    back = np.where(nbp >= 0, perm[np.clip(nbp, 0, None)], -1)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
+    nb2, c2 = back[inv], cp[inv]
+    assert np.array_equal(c2, c)
+    assert np.array_equal(rings.canonical(nb2, c2), rings.canonical(nb, c))
+
+
 @pytest.mark.parametrize("prec", PRECS)
 def test_clustered_points_take_the_deferred_path(prec):
-    """A dense cluster overflows the LDS window of its tiles; those points go
-    through the direct routine (deferred list) and still match the oracle."""
+    """A dense cluster overflows the LDS window of its tiles; those points are
+    deferred (wavefront kernel) and still match the oracle."""
     L = 141.0
     base = inputs.uniform_points(17000, L, L, seed=41)
     rng = np.random.default_rng(42)
@@ -116,9 +143,9 @@ def test_clustered_points_take_the_deferred_path(prec):
 
 @pytest.mark.parametrize("prec", PRECS)
 def test_grid_smaller_than_the_halo(prec):
-    """6x6 cells (search radius 2 < halo 3): every point is deferred to the
-    direct routine; the rings equal scipy's Qhull (the reference path itself
-    keeps a virtual corner in such small boxes)."""
+    """6x6 cells (search radius 2 < the 3-cell scan block): every point is
+    deferred (wavefront kernel); the rings equal scipy's Qhull (the reference
+    path itself keeps a virtual corner in such small boxes)."""
     from qhull_periodic import periodic_rings
     xy = inputs.uniform_points(40, 6.0, 6.0, seed=43)
     qn, qc = periodic_rings(xy, 6.0, 6.0)

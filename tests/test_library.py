@@ -34,3 +34,13 @@ def test_status_strings():
     assert _lib.status_string(0) == "ok"
     assert "stride" in _lib.status_string(_lib.GDT_OVERFLOW)
     assert L.gdt_status_string(12345) == b"unknown status"
+
+
+def test_cpp_dropin_header_compiles(tmp_path):
+    """include/gpudelaunay.hpp (the C++ classes over the C ABI) compiles and
+    links against the library with a plain host compiler."""
+    import subprocess
+    from gpudelaunaytriangulation_amd import build as B
+    out = str(tmp_path / "dropin_check")
+    r = subprocess.run(B.cxx_dropin_cmd(B.DROPIN_SRC, out), capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
