@@ -42,9 +42,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--block", type=int, default=None, help="threads per block (one-ring kernel)")
-    ap.add_argument("--tiled", type=int, default=None, help="1: 16x16-cell tiles (LDS kernel), 0: row-major")
+    ap.add_argument("--tiled", type=int, default=None, help="1: 32x32-cell tiles (LDS kernel), 2: the same with Morton order inside, 0: row-major")
     ap.add_argument("--cell-scale", type=float, default=None, help="cell size multiplier")
     ap.add_argument("--precisions", default="fp32f,fp64", help="comma list of fp32f,fp64")
+    ap.add_argument("--md-steps", type=int, default=1000,
+                    help="steps of the 256-set Brownian test-and-repair loop (0: skip)")
+    ap.add_argument("--no-voronoi", action="store_true", help="skip the Voronoi energy timing")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the bench line, run a plain device copy of MIB MiB three "
                          "times (reference kernel for scaling rocprofv3 FETCH/WRITE_SIZE)")
@@ -72,6 +75,74 @@ def time_path(dg, pts, tri, num, steps, warmup, torch):
     ms = ev0.elapsed_time(ev1) / steps
     return (ms, wall, sum(ring_ms) / steps, sum(cell_ms) / steps, sum(defer_ms) / steps,
             dg.stats())
+
+
+def bench_md(steps, torch):
+    """256 sets, N log-uniform in [1024, 65536], box sqrt(N): one batched
+    triangulation, then `steps` hipGraph-replayed steps of sigma = 0.01
+    displacement -> test -> repair."""
+    import numpy as np
+    from gpudelaunaytriangulation_amd import inputs
+    from gpudelaunaytriangulation_amd.delaunay import BatchedDelaunayGPU
+    sizes = inputs.batch_sizes(256)
+    pts, boxes = inputs.batch_points(sizes)
+    bd = BatchedDelaunayGPU(sizes, [(L, L) for L in boxes], 16, 1.0)
+    X = torch.from_numpy(np.concatenate(pts)).cuda()
+    tri = torch.zeros(bd.N * bd.MaxSize, dtype=torch.int32, device="cuda")
+    num = torch.zeros(bd.N, dtype=torch.int32, device="cuda")
+    bd.globalDelaunayTriangulation(X, tri, num)          # also sizes MaxSize
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bd.globalDelaunayTriangulation(X, tri, num)
+    torch.cuda.synchronize()
+    t_tri = time.perf_counter() - t0
+    bd.md_run(X, tri, num, 5, 0.01, seed=99, step0=0, use_graph=True, check_every=5)   # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep = bd.md_run(X, tri, num, steps, 0.01, seed=99, step0=5, use_graph=True, check_every=100)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    return {"workload": f"256 sets, {bd.N} points, sigma 0.01, hipGraph replay",
+            "points": bd.N, "batch_triangulation_sets_per_s": 256 / t_tri,
+            "batch_triangulation_points_per_s": bd.N / t_tri, "steps": rep["steps"],
+            "steps_per_s": rep["steps"] / t, "sets_steps_per_s": 256 * rep["steps"] / t,
+            "repaired_fraction": rep["flagged_total"] / (rep["steps"] * bd.N),
+            "recoveries": rep["recoveries"],
+            "step_roofline_frac": (48 + 32 + 96 * rep["flagged_total"] / (rep["steps"] * bd.N))
+            * bd.N * rep["steps"] / t / 1e9 / HBM_PEAK_GBS}
+
+
+def bench_voronoi(torch):
+    """Energy + hand-written adjoint, 65,536 points (box 256), FP64 and FP32."""
+    import numpy as np
+    from gpudelaunaytriangulation_amd import PREC_FP32F, PREC_FP64, inputs
+    from gpudelaunaytriangulation_amd.delaunay import Context, DelaunayGPU, PeriodicBoundaries
+    from gpudelaunaytriangulation_amd.voronoi import voronoi_adjoint, voronoi_forward
+    L, n = 256.0, 65536
+    xy = torch.from_numpy(inputs.uniform_points(n, L, L, seed=5)).cuda()
+    dg = DelaunayGPU(n, 16, 1.0, PeriodicBoundaries(L, L))
+    tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device="cuda")
+    num = torch.zeros(n, dtype=torch.int32, device="cuda")
+    dg.globalDelaunayTriangulation(xy, tri, num)
+    ctx = Context(0)
+    out = {}
+    for name, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32F)):
+        def once():
+            a, p, e = voronoi_forward(ctx, xy, tri, num, (L, L), precision=prec)
+            return voronoi_adjoint(ctx, xy, tri, num, (L, L), a, p, precision=prec)
+        once()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            once()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 20
+        out[name] = {"ms_forward_adjoint": ms, "points_per_s": n / ms * 1e3,
+                     "roofline_frac": 60.0 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    out["workload"] = "65536 uniform points, box 256, E = sum (A-1)^2 + (P-3.8)^2"
+    return out
 
 
 def main():
@@ -176,6 +247,18 @@ def main():
         "triangulation_roofline_frac": TRI_BYTES(n, n) / (f32["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "cpu_baseline": cpu,
     }
+    if args.md_steps > 0:
+        try:
+            line["md_c4"] = bench_md(args.md_steps, torch)
+        except Exception as e:  # reported, never hides the headline number
+            line["md_c4"] = {"error": repr(e)}
+        print(f"[bench] md: {line['md_c4']}", file=sys.stderr, flush=True)
+    if not args.no_voronoi:
+        try:
+            line["voronoi_c5"] = bench_voronoi(torch)
+        except Exception as e:
+            line["voronoi_c5"] = {"error": repr(e)}
+        print(f"[bench] voronoi: {line['voronoi_c5']}", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
     if args.calibrate_copy > 0:

@@ -3,7 +3,7 @@
 Tunes what ``gdt_set_config`` exposes: threads per block of the one-ring
 kernels (the reference fixes 128, DelaunayGPU.cu:16), the cell-size scale of
 the internal grid (the reference takes the caller's cell size,
-triangulation.cpp:172) and the cell order (16x16-cell LDS tiles or
+triangulation.cpp:172) and the cell order (32x32-cell LDS tiles or
 row-major).  Every candidate is timed on the caller's own points with HIP
 events; the rings of every candidate must be identical (canonical hash) —
 the output never depends on the configuration — and the fastest is applied

@@ -52,7 +52,7 @@ struct DevGuard {
 
 struct gdt_context {
     int device = 0;
-    gdt_config cfg = {320, 1, 1.0, 0};
+    gdt_config cfg = {512, 1, 1.0, 0};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
     std::vector<GdtSet> sets;
@@ -67,7 +67,7 @@ struct gdt_context {
     DBuf<gdt_d2> off64;
     DBuf<gdt_f2> off32;
     DBuf<int> status;
-    DBuf<unsigned long long> counters;
+    DBuf<unsigned long long> counters, tpart;
     // Voronoi energy: its own set table (only p0, n and the box are used)
     std::vector<GdtSet> vsets;
     DBuf<GdtSet> d_vsets;
@@ -143,9 +143,10 @@ static int configure(gdt_context* c, std::vector<GdtSet> sets)
     auto kch = make_chunks(c->sets, GDT_BLOCK, true);
     GDT_CHECK(upload(c->pch, pch));
     GDT_CHECK(upload(c->kch, kch));
+    GDT_CHECK(c->tpart.reserve(pch.size()));
     c->npch = (int)pch.size();
     c->nkch = (int)kch.size();
-    // one entry per 16x16-cell tile (tiled cell order only)
+    // one entry per tile (tiled cell orders only)
     std::vector<GdtChunk> tch;
     for (int si = 0; si < (int)c->sets.size(); ++si)
         if (c->sets[si].tiled)
@@ -322,7 +323,7 @@ void gdt_destroy(gdt_context* c)
     c->scell.release(); c->scan_tmp.release(); c->tcount.release(); c->toff.release();
     c->mark.release(); c->markscan.release(); c->perm.release(); c->slot_of.release();
     c->worklist.release(); c->repair.release(); c->tris.release(); c->defer.release(); c->off64.release();
-    c->off32.release(); c->status.release(); c->counters.release();
+    c->off32.release(); c->status.release(); c->counters.release(); c->tpart.release();
     c->d_vsets.release(); c->ecell.release(); c->partial.release(); c->vslots.release();
     c->md.release();
     if (c->h_md) (void)hipHostFree(c->h_md);
@@ -486,7 +487,7 @@ static int enumerate(gdt_context* c, const int* d_nbrs, const int* d_counts, int
     GDT_CHECK(c->toff.reserve((size_t)c->total_n + 1));
     GDT_CHECK(gdt_launch_triangles(c->d_sets.p, c->pch.p, c->npch, c->total_n, d_nbrs, d_counts,
                                    stride, c->tcount.p, c->toff.p, tris, cap, c->scan_tmp.p,
-                                   c->counters.p, c->status.p, s));
+                                   c->tpart.p, c->counters.p, c->status.p, s));
     return GDT_OK;
 }
 

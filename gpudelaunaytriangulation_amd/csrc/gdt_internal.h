@@ -61,14 +61,15 @@ struct GdtSet {
 
 struct GdtChunk { int set; int start; };   // a block's work: set id and first local item
 
-// Cell order.  tiled = 1: 16x16-cell tiles in row-major tile order, cells
+// Cell order.  tiled = 1: GDT_TILE x GDT_TILE-cell tiles (32x32) in row-major
+// tile order, cells
 // row-major inside a tile, so one tile's points are one contiguous slot range
 // (the unit of work of the LDS-staged one-ring kernel); tiled = 2: the same
 // tiles with the cells of a tile in Morton (Z) order, so that consecutive
 // slots (the points of one wavefront) form square patches; tiled = 0:
 // row-major cells.
 #ifndef GDT_TILE_SHIFT
-#define GDT_TILE_SHIFT 4
+#define GDT_TILE_SHIFT 5
 #endif
 #define GDT_TILE (1 << GDT_TILE_SHIFT)
 #define GDT_TILE_CELLS (GDT_TILE * GDT_TILE)

@@ -48,7 +48,8 @@ hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
 #define GDT_WAVE_CAP 8192
 #define GDT_WAVE_BLOCKS 512
 
-// One-ring of every point, one block per 16x16-cell tile (tiled cell order).
+// One-ring of every point, one block per GDT_TILE x GDT_TILE-cell tile (tiled
+// cell order).
 // Points the tile kernel cannot finish from LDS are deferred (second and
 // third launches over `defer`, size total_n, count in status[ST_NDEFER]).
 hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nsets,
@@ -57,12 +58,13 @@ hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nse
                                     unsigned long long* counters, int* defer,
                                     hipEvent_t after_tile, hipStream_t s);
 
-// Triangles owned by their smallest vertex, written at toff (deterministic).
+// Triangles owned by their smallest vertex, written at toff (deterministic);
+// counters[CT_INSUM] += sum of counts (partials holds npch words).
 hipError_t gdt_launch_triangles(const GdtSet* d_sets, const GdtChunk* pchunks, int npch,
                                 int total_n, const int* nbrs, const int* counts, int stride,
                                 uint32_t* tcount, uint32_t* toff, int* tris, int cap,
-                                uint32_t* scan_tmp, unsigned long long* counters, int* status,
-                                hipStream_t s);
+                                uint32_t* scan_tmp, unsigned long long* partials,
+                                unsigned long long* counters, int* status, hipStream_t s);
 
 // Empty-circle test of *ntri triangles; repair must be -1 on entry.
 hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, const uint32_t* ntri,

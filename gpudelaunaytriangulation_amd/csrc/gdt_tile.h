@@ -1,6 +1,6 @@
 // gdt_tile.h -- LDS-staged one-ring construction (host/device routines).
 //
-// A block owns one 16x16-cell tile (a contiguous range of cell-ordered
+// A block owns one GDT_TILE x GDT_TILE-cell tile (32x32; a contiguous range of cell-ordered
 // slots).  It stages the tile's window -- the tile plus a halo of GDT_HALO
 // cells on every side, wrapped on the torus -- into LDS, with every point
 // written as a coordinate in the window frame,
@@ -22,14 +22,17 @@
 
 #include "gdt_point.h"
 
-#define GDT_SCAN 3          // half-width (cells) of the block every search step scans
+#ifndef GDT_SCAN0
+#define GDT_SCAN0 2         // half-width (cells) of the block every search step scans first
+#endif
+#define GDT_SCAN 3          // half-width of the block scanned when the first finds nothing
 #ifndef GDT_HALO
 #define GDT_HALO 4          // window halo (cells): reach of the circle-box extension
 #endif
 #define GDT_WIN (GDT_TILE + 2 * GDT_HALO)
 #define GDT_WIN_CELLS (GDT_WIN * GDT_WIN)
 #ifndef GDT_WIN_MAXPTS
-#define GDT_WIN_MAXPTS 1024 // LDS entries per window (a fuller window defers its tile)
+#define GDT_WIN_MAXPTS 2048 // LDS entries per window (a fuller window defers its tile)
 #endif
 #define GDT_FRAME (GDT_WIN / 2)
 
@@ -105,65 +108,92 @@ GDT_HD inline void gdt_tile_copy(const GdtSet& S, int c, int gstart, uint32_t at
 // cell (wx,wy), inside the window) minus the already scanned square
 // [-hole,hole]^2 (hole < 0: none), visited as one flat loop over row segments
 // (each row of cells is one contiguous LDS range; a row crossing the hole is
-// two segments), so the trip count is the number of points, not cells.
+// two segments).  Segments are set up lazily, when the previous one ends, so
+// the loop costs the points it visits plus one short set-up per segment.
 template <class Fn>
 GDT_HD inline unsigned gdt_lds_block(const GdtTileLds& L, int wx, int wy, int x0, int x1, int y0,
                                      int y1, int hole, Fn& fn)
 {
-    // segment sg: row y0 + sg/2, side sg&1
-    auto seg = [&](int sg, uint32_t& b, uint32_t& e) {
-        const int y = y0 + (sg >> 1);
-        int a = x0, z = x1;
-        if (y >= -hole && y <= hole) {
-            if (sg & 1) { a = a > hole + 1 ? a : hole + 1; }
-            else { z = z < -hole - 1 ? z : -hole - 1; }
-        } else if (sg & 1) {
-            z = a - 1;     // rows outside the hole are one segment
-        }
-        if (z < a) { b = e = 0; return; }
-        const int c = (wy + y) * GDT_WIN + wx;
-        b = L.cs[c + a];
-        e = L.cs[c + z + 1];
-    };
-    const int nseg = 2 * (y1 - y0 + 1);
-    int total = 0;
-    for (int sg = 0; sg < nseg; ++sg) {
-        uint32_t b, e;
-        seg(sg, b, e);
-        total += (int)(e - b);
-    }
-    int sg = 0;
-    uint32_t k, e;
-    seg(0, k, e);
-    for (int t = 0; t < total; ++t) {
-        while (k == e) seg(++sg, k, e);
-        fn((int)k);
-        ++k;
-    }
-    return (unsigned)total;
-}
-
-// The block [-GDT_SCAN,GDT_SCAN]^2 around window cell c0 as one flat loop over
-// its rows (each a contiguous LDS range), so that a wave waits for its slowest
-// lane once per step rather than once per row.  Returns the entries visited.
-template <class Fn>
-GDT_HD inline unsigned gdt_scan_block(const GdtTileLds& L, int c0, Fn& fn)
-{
-    int c = c0 - GDT_SCAN * GDT_WIN;
-    const int cend = c0 + (GDT_SCAN + 1) * GDT_WIN;
-    uint32_t k = L.cs[c - GDT_SCAN], e = L.cs[c + GDT_SCAN + 1];
+    const int xl = x1 < -hole - 1 ? x1 : -hole - 1;     // left segment ends here
+    const int xr = x0 > hole + 1 ? x0 : hole + 1;       // right segment starts here
+    int y = y0, side = 0, base = (wy + y0) * GDT_WIN + wx;
+    uint32_t k = 0, e = 0;
     unsigned n = 0;
     for (;;) {
-        if (k == e) {
-            c += GDT_WIN;
-            if (c == cend) break;
-            k = L.cs[c - GDT_SCAN];
-            e = L.cs[c + GDT_SCAN + 1];
-            continue;
+        while (k == e) {
+            // next segment: (row y, side 0 = left or whole row, side 1 = right)
+            if (side == 0) {
+                const bool split = y >= -hole && y <= hole;
+                const int a = x0, z = split ? xl : x1;
+                side = split ? 1 : 2;
+                if (z >= a) { k = L.cs[base + a]; e = L.cs[base + z + 1]; }
+                continue;
+            }
+            if (side == 1) {
+                side = 2;
+                if (x1 >= xr) { k = L.cs[base + xr]; e = L.cs[base + x1 + 1]; }
+                continue;
+            }
+            if (++y > y1) return n;
+            base += GDT_WIN;
+            side = 0;
         }
         fn((int)k);
         ++k;
         ++n;
+    }
+}
+
+// Cursor over the LDS entries of the block [-h,h]^2 around window cell c0:
+// its rows are contiguous LDS ranges, walked as one sequence.
+struct GdtBlockCursor {
+    int c, cend, h;
+    uint32_t k, e;
+};
+
+GDT_HD inline GdtBlockCursor gdt_block_begin(const GdtTileLds& L, int c0, int h)
+{
+    GdtBlockCursor it;
+    it.h = h;
+    it.c = c0 - h * GDT_WIN;
+    it.cend = c0 + (h + 1) * GDT_WIN;
+    it.k = L.cs[it.c - h];
+    it.e = L.cs[it.c + h + 1];
+    return it;
+}
+
+// Next entry of the block, or -1 when it is exhausted.
+GDT_HD inline int gdt_block_next(const GdtTileLds& L, GdtBlockCursor& it)
+{
+    while (it.k == it.e) {
+        it.c += GDT_WIN;
+        if (it.c == it.cend) return -1;
+        it.k = L.cs[it.c - it.h];
+        it.e = L.cs[it.c + it.h + 1];
+    }
+    return (int)it.k++;
+}
+
+// The block as one flat loop (a wave waits for its slowest lane once per step,
+// not once per row), software-pipelined: the coordinates of the next entry are
+// loaded, load(k), before the current one is evaluated, fn(k, coords), so the
+// LDS latency overlaps the predicate arithmetic.  Returns the entries visited.
+template <class Load, class Fn>
+GDT_HD inline unsigned gdt_scan_block(const GdtTileLds& L, int c0, int h, Load& load, Fn& fn)
+{
+    GdtBlockCursor it = gdt_block_begin(L, c0, h);
+    int k = gdt_block_next(L, it);
+    if (k < 0) return 0u;
+    auto cur = load(k);
+    unsigned n = 0;
+    for (;;) {
+        const int kn = gdt_block_next(L, it);
+        const auto nxt = load(kn >= 0 ? kn : k);
+        fn(k, cur);
+        ++n;
+        if (kn < 0) break;
+        k = kn;
+        cur = nxt;
     }
     return n;
 }
@@ -193,11 +223,12 @@ GDT_HD inline void gdt_circle_cells(T qxlo, T qxhi, T qylo, T qyhi, T R, T ox, T
 // ---------------------------------------------------------------------------
 // Per point, one search step at a time (the kernel interleaves the points of
 // a tile across its lanes step by step).  A step scans the block of cells
-// [-GDT_SCAN,GDT_SCAN]^2 around v's cell; the loop body evaluates the
-// predicates of the lane's current search and takes a single (rare) branch
-// when the candidate wins or a filtered predicate is undecided.
+// [-h,h]^2 around v's cell, h = GDT_SCAN0, widened to h = GDT_SCAN when that
+// block holds no candidate; the loop body evaluates the predicates of the
+// lane's current search and takes a single (rare) branch when the candidate
+// wins or a filtered predicate is undecided.
 //   step 0 (nearest neighbour): certified when its distance is below
-//     cov = GDT_SCAN*bs_min + (distance of v to its cell walls);
+//     cov = h*bs_min + (distance of v to its cell walls);
 //   step k (successor of edge (v,a)): the final circle (v, a, best) holds no
 //     scanned point, so it is certified when its conservative bounding box
 //     lies inside the block, or 2R <= cov.
@@ -301,8 +332,9 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
     Circ64 K = {0., 0., 0.};
     const double ax = st.ax, ay = st.ay, vx = st.vx, vy = st.vy;
     const int iv = st.iv, ak = st.ak;
-    auto visit = [&](int k) {
-        const gdt_d2 X = L.X64[k];
+    const int wx = st.c0 % GDT_WIN, wy = st.c0 / GDT_WIN;
+    auto load = [&](int k) { return L.X64[k]; };
+    auto visit = [&](int k, const gdt_d2 X) {
         const double px = X.x - vx, py = X.y - vy;
         const double cc = gdt_dist2_64(px, py);
         const double O = gdt_orient64(ax, ay, px, py);
@@ -317,33 +349,49 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
             K = gdt_circ64(ax, ay, px, py);
         }
     };
-    npred += gdt_scan_block(L, st.c0, visit);
+    auto visit_k = [&](int k) { visit(k, load(k)); };
+    int hs = GDT_SCAN0;
+    npred += gdt_scan_block(L, st.c0, hs, load, visit);
+    if (bk < 0 && hs < GDT_SCAN) {
+        npred += gdt_lds_block(L, wx, wy, -GDT_SCAN, GDT_SCAN, -GDT_SCAN, GDT_SCAN, hs, visit_k);
+        hs = GDT_SCAN;
+    }
 
-    const double cov = (double)GDT_SCAN * GDT_FMIN(S.bsx, S.bsy) + st.covb;
-    const int wx = st.c0 % GDT_WIN, wy = st.c0 / GDT_WIN;
     const double ox = gdt_cell_offset(vx, wx, S.bsx), oy = gdt_cell_offset(vy, wy, S.bsy);
     bool ok = false, boxed = false;
     int dx0 = 0, dx1 = 0, dy0 = 0, dy1 = 0;
-    if (!wrap) {
-        if (bk >= 0) {
-            ok = cov > 0. && bd2 <= cov * cov * (1.0 - 1e-12);
-            const double rad = sqrt(bd2) * (1.0 + 1e-9);
-            gdt_circle_cells<double>(0., 0., 0., 0., rad, ox, oy, S.bsx, S.bsy, 1e-3, dx0, dx1, dy0,
-                                     dy1);
+    // certification against the scanned block [-hs,hs]^2 (sets the box too)
+    auto certify = [&]() {
+        const double cov = (double)hs * GDT_FMIN(S.bsx, S.bsy) + st.covb;
+        ok = boxed = false;
+        if (!wrap) {
+            if (bk >= 0) {
+                ok = cov > 0. && bd2 <= cov * cov * (1.0 - 1e-12);
+                const double rad = sqrt(bd2) * (1.0 + 1e-9);
+                gdt_circle_cells<double>(0., 0., 0., 0., rad, ox, oy, S.bsx, S.bsy, 1e-3, dx0, dx1,
+                                         dy0, dy1);
+                boxed = true;
+            }
+        } else if (bk >= 0 && K.k0 > 1e-6 * (fabs(ax * by) + fabs(ay * bx))) {
+            const double qx = -K.kx / (2. * K.k0), qy = -K.ky / (2. * K.k0);
+            const double rad = sqrt(qx * qx + qy * qy);
+            const double m = 1e-6 * (fabs(qx) + fabs(qy) + rad) + 1e-9 * S.bsx;
+            gdt_circle_cells<double>(qx - m, qx + m, qy - m, qy + m, rad, ox, oy, S.bsx, S.bsy,
+                                     1e-3, dx0, dx1, dy0, dy1);
+            ok = (dx0 >= -hs && dx1 <= hs && dy0 >= -hs && dy1 <= hs) ||
+                 (cov > 0. && K.kx * K.kx + K.ky * K.ky <= cov * cov * (K.k0 * K.k0) * (1.0 - 1e-6));
             boxed = true;
         }
-    } else if (bk >= 0 && K.k0 > 1e-6 * (fabs(ax * by) + fabs(ay * bx))) {
-        const double qx = -K.kx / (2. * K.k0), qy = -K.ky / (2. * K.k0);
-        const double rad = sqrt(qx * qx + qy * qy);
-        const double m = 1e-6 * (fabs(qx) + fabs(qy) + rad) + 1e-9 * S.bsx;
-        gdt_circle_cells<double>(qx - m, qx + m, qy - m, qy + m, rad, ox, oy, S.bsx, S.bsy, 1e-3,
-                                 dx0, dx1, dy0, dy1);
-        ok = (dx0 >= -GDT_SCAN && dx1 <= GDT_SCAN && dy0 >= -GDT_SCAN && dy1 <= GDT_SCAN) ||
-             (cov > 0. && K.kx * K.kx + K.ky * K.ky <= cov * cov * (K.k0 * K.k0) * (1.0 - 1e-6));
-        boxed = true;
+    };
+    certify();
+    // a box leaving the window: the full block first (it lies in the window)
+    if (!ok && hs < GDT_SCAN && !(boxed && gdt_in_window(S, st.c0, dx0, dx1, dy0, dy1))) {
+        npred += gdt_lds_block(L, wx, wy, -GDT_SCAN, GDT_SCAN, -GDT_SCAN, GDT_SCAN, hs, visit_k);
+        hs = GDT_SCAN;
+        certify();
     }
     if (!ok && boxed && gdt_in_window(S, st.c0, dx0, dx1, dy0, dy1)) {
-        npred += gdt_lds_block(L, wx, wy, dx0, dx1, dy0, dy1, GDT_SCAN, visit);
+        npred += gdt_lds_block(L, wx, wy, dx0, dx1, dy0, dy1, hs, visit_k);
         ok = true;
     }
     if (!ok) return STEP_DEFER;
@@ -381,15 +429,16 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
         eabs = 8.f * eps0 * mc + 4.f * eps0 * eps0 + 1e-30f;
         if (bk >= 0) set_bp();
     };
-    set_region(GDT_SCAN);
+    set_region(GDT_SCAN0);
+    const int wx = st.c0 % GDT_WIN, wy = st.c0 / GDT_WIN;
     // FP64 displacement of LDS entry k: the FP64 path's value (fallbacks only)
     auto x64 = [&](int k, double& x, double& y) {
         const gdt_d2 V = L.X64[iv], X = L.X64[k];
         x = X.x - V.x;
         y = X.y - V.y;
     };
-    auto visit = [&](int k) {
-        const gdt_f2 Y = L.X32[k];
+    auto load = [&](int k) { return L.X32[k]; };
+    auto visit = [&](int k, const gdt_f2 Y) {
         const float px = Y.x - vx, py = Y.y - vy;
         const float cc = fmaf(px, px, py * py);
         const float O = fmaf(ax, py, -(ay * px));
@@ -445,48 +494,66 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
             }
         }
     };
-    npred += gdt_scan_block(L, st.c0, visit);
+    auto visit_k = [&](int k) { visit(k, load(k)); };
+    int hs = GDT_SCAN0;
+    npred += gdt_scan_block(L, st.c0, hs, load, visit);
+    if (bk < 0 && hs < GDT_SCAN) {
+        set_region(GDT_SCAN);
+        npred += gdt_lds_block(L, wx, wy, -GDT_SCAN, GDT_SCAN, -GDT_SCAN, GDT_SCAN, hs, visit_k);
+        hs = GDT_SCAN;
+    }
 
     const float bsm = S.bsmin32;
-    const float cov = (float)GDT_SCAN * bsm + st.covb - 9.5367431640625e-07f * (float)(GDT_SCAN + 2) * bsM;
-    const int wx = st.c0 % GDT_WIN, wy = st.c0 / GDT_WIN;
     const float ox = vx - (float)(wx - GDT_FRAME) * S.bsx32;
     const float oy = vy - (float)(wy - GDT_FRAME) * S.bsy32;
     bool ok = false, boxed = false;
     int dx0 = 0, dx1 = 0, dy0 = 0, dy1 = 0;
-    if (!wrap) {
-        if (bk >= 0) {
-            ok = cov > 0.f && bd2 + bE <= cov * cov * 0.9999f;
-            const float rad = sqrtf(bd2 + bE) * 1.0001f;
-            gdt_circle_cells<float>(0.f, 0.f, 0.f, 0.f, rad, ox, oy, S.bsx32, S.bsy32, 1e-2f, dx0,
-                                    dx1, dy0, dy1);
-            boxed = true;
+    // certification against the scanned block [-hs,hs]^2 (sets the box too)
+    auto certify = [&]() {
+        const float cov = (float)hs * bsm + st.covb - 9.5367431640625e-07f * (float)(hs + 2) * bsM;
+        ok = boxed = false;
+        if (!wrap) {
+            if (bk >= 0) {
+                ok = cov > 0.f && bd2 + bE <= cov * cov * 0.9999f;
+                const float rad = sqrtf(bd2 + bE) * 1.0001f;
+                gdt_circle_cells<float>(0.f, 0.f, 0.f, 0.f, rad, ox, oy, S.bsx32, S.bsy32, 1e-2f,
+                                        dx0, dx1, dy0, dy1);
+                boxed = true;
+            }
+        } else if (bk >= 0) {
+            const float EK0 = 4.f * (2.f * eps1 + 2.f * u) * pab + 4.f * eps0 * sab;
+            const float EK = 4.f * (3.f * eps1 + 6.f * u) * pab * sab + 8.f * eps0 * sab * sab;
+            const float Klo = K0 - EK0, Khi = K0 + EK0;
+            if (Klo > 0.0625f * K0) {
+                const float nxlo = -(Kx + EK), nxhi = -(Kx - EK);
+                const float nylo = -(Ky + EK), nyhi = -(Ky - EK);
+                const float qxlo = GDT_FMINF(nxlo / (2.f * Klo), nxlo / (2.f * Khi));
+                const float qxhi = GDT_FMAXF(nxhi / (2.f * Klo), nxhi / (2.f * Khi));
+                const float qylo = GDT_FMINF(nylo / (2.f * Klo), nylo / (2.f * Khi));
+                const float qyhi = GDT_FMAXF(nyhi / (2.f * Klo), nyhi / (2.f * Khi));
+                const float rad = sqrtf(GDT_FMAXF(qxlo * qxlo, qxhi * qxhi) +
+                                        GDT_FMAXF(qylo * qylo, qyhi * qyhi)) * 1.0001f;
+                gdt_circle_cells<float>(qxlo, qxhi, qylo, qyhi, rad, ox, oy, S.bsx32, S.bsy32,
+                                        1e-2f, dx0, dx1, dy0, dy1);
+                const float gx = fabsf(Kx) + EK, gy = fabsf(Ky) + EK;
+                ok = (dx0 >= -hs && dx1 <= hs && dy0 >= -hs && dy1 <= hs) ||
+                     (cov > 0.f && (gx * gx + gy * gy) * 1.001f <= (cov * cov) * (Klo * Klo));
+                boxed = true;
+            }
         }
-    } else if (bk >= 0) {
-        const float EK0 = 4.f * (2.f * eps1 + 2.f * u) * pab + 4.f * eps0 * sab;
-        const float EK = 4.f * (3.f * eps1 + 6.f * u) * pab * sab + 8.f * eps0 * sab * sab;
-        const float Klo = K0 - EK0, Khi = K0 + EK0;
-        if (Klo > 0.0625f * K0) {
-            const float nxlo = -(Kx + EK), nxhi = -(Kx - EK);
-            const float nylo = -(Ky + EK), nyhi = -(Ky - EK);
-            const float qxlo = GDT_FMINF(nxlo / (2.f * Klo), nxlo / (2.f * Khi));
-            const float qxhi = GDT_FMAXF(nxhi / (2.f * Klo), nxhi / (2.f * Khi));
-            const float qylo = GDT_FMINF(nylo / (2.f * Klo), nylo / (2.f * Khi));
-            const float qyhi = GDT_FMAXF(nyhi / (2.f * Klo), nyhi / (2.f * Khi));
-            const float rad = sqrtf(GDT_FMAXF(qxlo * qxlo, qxhi * qxhi) +
-                                    GDT_FMAXF(qylo * qylo, qyhi * qyhi)) * 1.0001f;
-            gdt_circle_cells<float>(qxlo, qxhi, qylo, qyhi, rad, ox, oy, S.bsx32, S.bsy32, 1e-2f,
-                                    dx0, dx1, dy0, dy1);
-            const float gx = fabsf(Kx) + EK, gy = fabsf(Ky) + EK;
-            ok = (dx0 >= -GDT_SCAN && dx1 <= GDT_SCAN && dy0 >= -GDT_SCAN && dy1 <= GDT_SCAN) ||
-                 (cov > 0.f && (gx * gx + gy * gy) * 1.001f <= (cov * cov) * (Klo * Klo));
-            boxed = true;
-        }
+    };
+    certify();
+    // a box leaving the window: the full block first (it lies in the window)
+    if (!ok && hs < GDT_SCAN && !(boxed && gdt_in_window(S, st.c0, dx0, dx1, dy0, dy1))) {
+        set_region(GDT_SCAN);
+        npred += gdt_lds_block(L, wx, wy, -GDT_SCAN, GDT_SCAN, -GDT_SCAN, GDT_SCAN, hs, visit_k);
+        hs = GDT_SCAN;
+        certify();
     }
     if (!ok && boxed && gdt_in_window(S, st.c0, dx0, dx1, dy0, dy1)) {
         const int h0 = dx0 < dy0 ? dx0 : dy0, h1 = dx1 > dy1 ? dx1 : dy1;
         set_region(-h0 > h1 ? -h0 : h1);
-        npred += gdt_lds_block(L, wx, wy, dx0, dx1, dy0, dy1, GDT_SCAN, visit);
+        npred += gdt_lds_block(L, wx, wy, dx0, dx1, dy0, dy1, hs, visit_k);
         ok = true;
     }
     if (!ok) return STEP_DEFER;

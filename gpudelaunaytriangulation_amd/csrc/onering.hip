@@ -203,7 +203,7 @@ __device__ inline void block_scan_lds(uint32_t* a, int n, uint32_t* wsum)
     if (tid == 0) a[n] = tot;
 }
 
-// One block per 16x16-cell tile: stage the tile window in LDS, then the lanes
+// One block per tile (GDT_TILE x GDT_TILE cells): stage the tile window in LDS, then the lanes
 // advance the tile's points one search step at a time.  The points of a
 // window that does not fit the LDS budget, or of a grid too small for the
 // scan block, and points with a step whose circle leaves the window, are
@@ -291,17 +291,37 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
             }
         }
     }
+    // block totals: one set of atomics per block, not per wave
     mx = wave_max_i32(mx);
     er = wave_or_i32(er);
     npred = wave_sum_u32(npred);
     nfb = wave_sum_u32(nfb);
     nsum = wave_sum_u32(nsum);
+    __shared__ unsigned s_red[5][16];
     if ((tid & 63) == 0) {
-        if (mx > 0) atomicMax(&status[ST_REQUIRED], mx);
-        if (er) atomicOr(&status[ST_ERRFLAGS], er);
-        if (npred) atomicAdd(&counters[CT_PRED], (unsigned long long)npred);
-        if (nfb) atomicAdd(&counters[CT_FALLBACK], (unsigned long long)nfb);
-        if (nsum) atomicAdd(&counters[CT_NBRSUM], (unsigned long long)nsum);
+        const int w = tid >> 6;
+        s_red[0][w] = (unsigned)mx;
+        s_red[1][w] = (unsigned)er;
+        s_red[2][w] = npred;
+        s_red[3][w] = nfb;
+        s_red[4][w] = nsum;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long bp = 0, bf = 0, bs = 0;
+        int bm = 0, be = 0;
+        for (int w = 0; w < (nt >> 6); ++w) {
+            bm = max(bm, (int)s_red[0][w]);
+            be |= (int)s_red[1][w];
+            bp += s_red[2][w];
+            bf += s_red[3][w];
+            bs += s_red[4][w];
+        }
+        if (bm > 0) atomicMax(&status[ST_REQUIRED], bm);
+        if (be) atomicOr(&status[ST_ERRFLAGS], be);
+        if (bp) atomicAdd(&counters[CT_PRED], bp);
+        if (bf) atomicAdd(&counters[CT_FALLBACK], bf);
+        if (bs) atomicAdd(&counters[CT_NBRSUM], bs);
     }
 }
 

@@ -7,14 +7,18 @@
 // gpu_test_circumcircles_kernel (DelaunayGPU.cu:290-313).
 #include "gdt_kernels.h"
 
+// Owned triangles per point, the block's sum of ring lengths into
+// partials[block] (summed by k_sum_partials: no contended atomics) and the
+// malformed-row flag.
 __global__ __launch_bounds__(GDT_BLOCK) void k_tri_count(const GdtSet* __restrict__ sets,
                                                          const GdtChunk* __restrict__ chunks,
                                                          const int* __restrict__ nbrs,
                                                          const int* __restrict__ counts, int stride,
                                                          uint32_t* __restrict__ tcount,
-                                                         unsigned long long* __restrict__ counters,
+                                                         unsigned long long* __restrict__ partials,
                                                          int* __restrict__ status)
 {
+    __shared__ unsigned long long s_tot[GDT_BLOCK / 64];
     const GdtChunk ch = chunks[blockIdx.x];
     const GdtSet S = sets[ch.set];
     const int li = ch.start + (int)threadIdx.x;
@@ -42,9 +46,31 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_tri_count(const GdtSet* __restric
         bad |= __shfl_xor(bad, o, 64);
     }
     if ((threadIdx.x & 63) == 0) {
-        if (tot) atomicAdd(&counters[CT_INSUM], (unsigned long long)tot);
+        s_tot[threadIdx.x >> 6] = tot;
         if (bad) atomicOr(&status[ST_ERRFLAGS], ERR_NOCLOSE);
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_tot[w];
+        partials[blockIdx.x] = t;
+    }
+}
+
+// One block: counters[CT_INSUM] += sum of partials[0..n) (fixed order).
+__global__ __launch_bounds__(256) void k_sum_partials(const unsigned long long* __restrict__ partials,
+                                                      int n, unsigned long long* __restrict__ counters)
+{
+    __shared__ unsigned long long s[256];
+    unsigned long long acc = 0;
+    for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) acc += partials[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) counters[CT_INSUM] += s[0];
 }
 
 __global__ __launch_bounds__(GDT_BLOCK) void k_tri_write(const GdtSet* __restrict__ sets,
@@ -82,11 +108,12 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_tri_write(const GdtSet* __restric
 hipError_t gdt_launch_triangles(const GdtSet* d_sets, const GdtChunk* pchunks, int npch,
                                 int total_n, const int* nbrs, const int* counts, int stride,
                                 uint32_t* tcount, uint32_t* toff, int* tris, int cap,
-                                uint32_t* scan_tmp, unsigned long long* counters, int* status,
-                                hipStream_t s)
+                                uint32_t* scan_tmp, unsigned long long* partials,
+                                unsigned long long* counters, int* status, hipStream_t s)
 {
     hipLaunchKernelGGL(k_tri_count, dim3(npch), dim3(GDT_BLOCK), 0, s, d_sets, pchunks, nbrs,
-                       counts, stride, tcount, counters, status);
+                       counts, stride, tcount, partials, status);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partials, npch, counters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = gdt_scan_u32(tcount, toff, (size_t)total_n, toff + total_n, scan_tmp, s);

@@ -49,7 +49,7 @@ typedef struct gdt_context gdt_context;
 /* Tunables (all have defaults; the autotuner writes them). */
 typedef struct {
     int block;          /* threads per block of the one-ring kernel (64..1024) */
-    int tiled;          /* cell order: 1 = 16x16-cell tiles (LDS tile kernel, default),
+    int tiled;          /* cell order: 1 = 32x32-cell tiles (LDS tile kernel, default),
                            2 = the same tiles, Morton order inside, 0 = row-major cells
                            (direct per-point kernel) */
     double cell_scale;  /* multiplies the caller's cell size (1.0 = as given) */

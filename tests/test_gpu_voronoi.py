@@ -104,7 +104,9 @@ def test_autograd_function_gradcheck(ctx):
     g1 = X.grad.clone()
     X.grad = None
     f(X).sum().backward()
-    assert torch.allclose(g1, 3.0 * X.grad, rtol=1e-14, atol=0)
+    # the scale enters before the per-slot sums: rounding differs at ~1e-15 of
+    # the largest term, so compare against the largest entry
+    assert (g1 - 3.0 * X.grad).abs().max() <= 1e-12 * g1.abs().max()
 
 
 def test_rows_that_are_not_rings_give_nan(ctx):
