@@ -54,6 +54,23 @@ def parse():
     return ap.parse_args()
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_one_ring_tile_uniform1M_hbm_traffic.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE passes of this workload, scaled by the copy
+    calibration); None when the summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def time_path(dg, pts, tri, num, steps, warmup, torch):
     for _ in range(warmup):
         dg.globalDelaunayTriangulation(pts, tri, num)
@@ -107,7 +124,7 @@ def bench_md(steps, torch):
             "batch_triangulation_points_per_s": bd.N / t_tri, "steps": rep["steps"],
             "steps_per_s": rep["steps"] / t, "sets_steps_per_s": 256 * rep["steps"] / t,
             "repaired_fraction": rep["flagged_total"] / (rep["steps"] * bd.N),
-            "recoveries": rep["recoveries"],
+            "recoveries": rep["recoveries"], "recovered": rep["recovered"],
             "step_roofline_frac": (48 + 32 + 96 * rep["flagged_total"] / (rep["steps"] * bd.N))
             * bd.N * rep["steps"] / t / 1e9 / HBM_PEAK_GBS}
 
@@ -242,7 +259,9 @@ def main():
                       "deferred_fp32f": f32["defer_ms"], "deferred_fp64": f64["defer_ms"],
                       "cell_list": f32["cell_ms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("k_one_ring_tile<32>") if n == N_POINTS else None,
+                     "traffic_source": "profiles/k_one_ring_tile_uniform1M_hbm_traffic.md",
                      "kernel": "k_one_ring_tile<32>", "algorithmic_bytes_per_launch": ring_bytes},
         "triangulation_roofline_frac": TRI_BYTES(n, n) / (f32["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "cpu_baseline": cpu,

@@ -754,9 +754,11 @@ int gdt_md_run(gdt_context* c, double* d_xy, int nsets, const int* offsets, cons
     rep->max_flagged = m.max_flagged;
     rep->flagged_total = (long long)m.flagged_total;
     c->stats.flagged = m.max_flagged;
+    // a ring longer than the stride truncates its row, which the next step
+    // then reports as a malformed triangulation: the overflow is the cause
+    if (m.required > stride) return GDT_OVERFLOW;
     if (m.euler) return GDT_ERR_EULER;
-    if ((rc = status_from_flags(m.errflags)) != GDT_OK) return rc;
-    return m.required > stride ? GDT_OVERFLOW : GDT_OK;
+    return status_from_flags(m.errflags);
 }
 
 // ---- Voronoi energy and adjoint -------------------------------------------

@@ -388,6 +388,9 @@ class DelaunayGPU:
         _lib.check(st, "testAndRepairDelaunayTriangulation")
 
 
+MD_HEADROOM = 2   # extra row slots after an overflow inside the Brownian loop
+
+
 class BatchedDelaunayGPU:
     """Many independent periodic sets in one launch per kernel (no reference
     counterpart: the reference runs one DelaunayGPU per set).
@@ -488,7 +491,7 @@ class BatchedDelaunayGPU:
         if GPUTriangulation.numel() != self.N * self.MaxSize:
             raise ValueError("GPUTriangulation must hold N*MaxSize entries")
         L = _lib.load()
-        out = dict(steps=0, flagged_total=0, max_flagged=0, recoveries=0)
+        out = dict(steps=0, flagged_total=0, max_flagged=0, recoveries=0, recovered=[])
         step, left = int(step0), int(nsteps)
         while left > 0:
             rep = _lib.MdReport()
@@ -507,8 +510,11 @@ class BatchedDelaunayGPU:
             if st not in (_lib.GDT_OVERFLOW, _lib.GDT_ERR_NONGENERIC, _lib.GDT_ERR_EULER):
                 _lib.check(st, "md_run")
             if st == _lib.GDT_OVERFLOW:
-                self.MaxSize = max(self.MaxSize, int(rep.required))
+                # rings keep changing as the points move: leave headroom
+                self.MaxSize = max(self.MaxSize, int(rep.required)) + MD_HEADROOM
             out["recoveries"] += 1
+            out["recovered"].append(dict(status=_lib.status_string(st), first_bad_step=
+                                         rep.first_bad_step, at_step=step, required=rep.required))
             self.globalDelaunayTriangulation(points, GPUTriangulation, cellNeighborNum, stream)
         self.last_repaired = out["max_flagged"]
         return out

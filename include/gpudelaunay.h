@@ -166,9 +166,10 @@ int gdt_test_and_repair_batch(gdt_context* ctx, const double* d_xy, int nsets, c
  * (triangles -> cell list -> empty-circle test -> repair) entirely on the
  * device.  use_graph = 1 captures one step in a hipGraph and replays it.  The
  * device state is read back every check_every steps; the run stops at the
- * first check that finds an error (GDT_OVERFLOW with report.required,
- * GDT_ERR_NONGENERIC, GDT_ERR_EULER): positions are then `steps_done` steps
- * ahead and the rings must be rebuilt by gdt_triangulate_batch.  Replaces the
+ * first check that finds an error (GDT_OVERFLOW with report.required, which
+ * takes precedence because a truncated row also fails the next step's Euler
+ * check; GDT_ERR_NONGENERIC; GDT_ERR_EULER): positions are then `steps_done`
+ * steps ahead and the rings must be rebuilt by gdt_triangulate_batch.  Replaces the
  * reference driver's loop of host displacement + testAndRepair per step
  * (triangulation.cpp:349-415; DelaunayGPU.cpp:567-602).  A null stream runs on
  * a stream of the context (synchronised with the device first). */

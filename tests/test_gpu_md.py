@@ -108,3 +108,33 @@ def test_c4_hundred_steps_equal_fresh_triangulation():
     torch.cuda.synchronize()
     assert int(num.sum().item()) == 6 * bd.N
     assert same_sets(canonical_sets(bd, tri, num), canonical_sets(fresh, tri2, num2))
+
+
+def test_overflow_inside_the_loop_is_recovered_exactly():
+    """Rows sized to the starting triangulation (guess 4, grown only to the
+    initial longest ring) overflow once large moves lengthen some rings; the
+    loop reports the overflow, grows MaxSize, re-triangulates globally and
+    goes on; the final positions and rings still equal the restatement."""
+    sizes, pts, boxes = small_batch()
+    sigma, K = 0.25, 12
+    bd = BatchedDelaunayGPU(sizes, [(L, L) for L in boxes], 4, 1.0)
+    X = torch.from_numpy(np.concatenate(pts)).cuda()
+    tri = torch.zeros(bd.N * bd.MaxSize, dtype=torch.int32, device="cuda")
+    num = torch.zeros(bd.N, dtype=torch.int32, device="cuda")
+    bd.globalDelaunayTriangulation(X, tri, num)
+    ms0 = bd.MaxSize
+    rep = bd.md_run(X, tri, num, K, sigma, SEED, use_graph=True, check_every=1)
+    torch.cuda.synchronize()
+    assert rep["steps"] == K
+    assert rep["recoveries"] >= 1 and bd.MaxSize > ms0
+    assert all(r["status"] == "ring longer than the row stride" for r in rep["recovered"])
+    ref_pos, ref_rings = [], []
+    for s, (xy, L) in enumerate(zip(pts, boxes)):
+        for t in range(K):
+            xy = inputs.md_step(xy, L, L, sigma, SEED, t, s)
+        ref_pos.append(xy)
+        nb, c = oracle.triangulate(xy, L, L, 1.0, 24)
+        ref_rings.append((c, rings.canonical(nb, c)))
+    got = X.cpu().numpy()
+    assert np.array_equal(got.view(np.uint64), np.concatenate(ref_pos).view(np.uint64))
+    assert same_sets(canonical_sets(bd, tri, num), ref_rings)
