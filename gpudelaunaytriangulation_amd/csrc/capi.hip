@@ -429,7 +429,7 @@ static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int*
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
     if (c->cfg.tiled) {
         GDT_CHECK(gdt_launch_one_ring_tile(precision, c->d_sets.p, (int)c->sets.size(), c->tch.p,
-                                           c->ntch, c->cfg.block, c->total_n,
+                                           c->ntch, c->cfg.block, c->total_n, nullptr, nullptr,
                                            ring_io(c, d_nbrs, stride), d_counts, c->status.p,
                                            c->counters.p, c->defer.p,
                                            c->cfg.profile ? c->ev[2] : nullptr, s));
@@ -529,9 +529,9 @@ int gdt_test_circumcircles(gdt_context* c, const double* d_xy, int n, const int*
                               c->total_n, d_xy, cl, d_repair, s));
     if (nflagged) {
         GDT_CHECK(c->mark.reserve((size_t)n));
-        GDT_CHECK(c->markscan.reserve((size_t)n));
+        GDT_CHECK(c->markscan.reserve((size_t)n + 1));
         GDT_CHECK(c->worklist.reserve((size_t)n));
-        GDT_CHECK(gdt_launch_compact(d_repair, c->slot_of.p, n, c->mark.p, c->markscan.p,
+        GDT_CHECK(gdt_launch_compact(d_repair, c->perm.p, n, c->mark.p, c->markscan.p,
                                      c->worklist.p, reinterpret_cast<uint32_t*>(c->status.p + ST_NFLAGGED),
                                      c->scan_tmp.p, s));
         int h = 0;
@@ -543,17 +543,27 @@ int gdt_test_circumcircles(gdt_context* c, const double* d_xy, int n, const int*
     return GDT_OK;
 }
 
-// Compact the flags and run the one-ring kernel over the flagged slots.
+// Compact the flags (slot order) and redo the rings of the flagged points:
+// with the tiled cell order through the LDS tile kernel, restricted to each
+// tile's flagged slots (tiles without any exit at once), else one lane each.
+// status[ST_NDEFER] must be zero on entry (reset_status).
 static int repair_flagged(gdt_context* c, const int* d_repair, int stride, int* d_nbrs,
                           int* d_counts, int precision, hipStream_t s)
 {
     const size_t n = (size_t)c->total_n;
     GDT_CHECK(c->mark.reserve(n));
-    GDT_CHECK(c->markscan.reserve(n));
+    GDT_CHECK(c->markscan.reserve(n + 1));
     GDT_CHECK(c->worklist.reserve(n));
     uint32_t* worklen = reinterpret_cast<uint32_t*>(c->status.p + ST_NFLAGGED);
-    GDT_CHECK(gdt_launch_compact(d_repair, c->slot_of.p, c->total_n, c->mark.p, c->markscan.p,
+    GDT_CHECK(gdt_launch_compact(d_repair, c->perm.p, c->total_n, c->mark.p, c->markscan.p,
                                  c->worklist.p, worklen, c->scan_tmp.p, s));
+    if (!c->sets.empty() && c->sets[0].tiled && c->ntch > 0) {
+        GDT_CHECK(gdt_launch_one_ring_tile(precision, c->d_sets.p, (int)c->sets.size(), c->tch.p,
+                                           c->ntch, c->cfg.block, c->total_n, c->worklist.p,
+                                           c->markscan.p, ring_io(c, d_nbrs, stride), d_counts,
+                                           c->status.p, c->counters.p, c->defer.p, nullptr, s));
+        return GDT_OK;
+    }
     GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), nullptr, 0,
                                   c->cfg.block, c->worklist.p, worklen, 0, c->total_n,
                                   ring_io(c, d_nbrs, stride), d_counts, c->status.p,
@@ -697,7 +707,7 @@ int gdt_md_run(gdt_context* c, double* d_xy, int nsets, const int* offsets, cons
     GDT_CHECK(c->tcount.reserve(n));
     GDT_CHECK(c->toff.reserve(n + 1));
     GDT_CHECK(c->mark.reserve(n));
-    GDT_CHECK(c->markscan.reserve(n));
+    GDT_CHECK(c->markscan.reserve(n + 1));
     GDT_CHECK(c->worklist.reserve(n));
     GDT_CHECK(c->md.reserve(1));
     if (!c->h_md)

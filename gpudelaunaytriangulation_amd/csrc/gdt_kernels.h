@@ -54,9 +54,9 @@ hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
 // third launches over `defer`, size total_n, count in status[ST_NDEFER]).
 hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nsets,
                                     const GdtChunk* tiles, int ntiles, int block, int total_n,
-                                    const GdtRingIO& io, int* counts, int* status,
-                                    unsigned long long* counters, int* defer,
-                                    hipEvent_t after_tile, hipStream_t s);
+                                    const int* list, const uint32_t* lscan, const GdtRingIO& io,
+                                    int* counts, int* status, unsigned long long* counters,
+                                    int* defer, hipEvent_t after_tile, hipStream_t s);
 
 // Triangles owned by their smallest vertex, written at toff (deterministic);
 // counters[CT_INSUM] += sum of counts (partials holds npch words).
@@ -71,8 +71,10 @@ hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, con
                            int max_tri, int total_n, const double* d_xy, const GdtCells& c, int* repair,
                            hipStream_t s);
 
-// Slots of the flagged points -> worklist; count -> *worklen (device).
-hipError_t gdt_launch_compact(const int* repair, const int* slot_of, int total_n, uint32_t* mark,
+// Slots of the flagged points, in slot order -> worklist; markscan (total_n+1
+// words) = exclusive scan of the flags with the count at markscan[total_n];
+// count -> *worklen (device).
+hipError_t gdt_launch_compact(const int* repair, const int* perm, int total_n, uint32_t* mark,
                               uint32_t* markscan, int* worklist, uint32_t* worklen,
                               uint32_t* scan_tmp, hipStream_t s);
 

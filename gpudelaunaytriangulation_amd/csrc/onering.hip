@@ -204,17 +204,19 @@ __device__ inline void block_scan_lds(uint32_t* a, int n, uint32_t* wsum)
 }
 
 // One block per tile (GDT_TILE x GDT_TILE cells): stage the tile window in LDS, then the lanes
-// advance the tile's points one search step at a time.  The points of a
-// window that does not fit the LDS budget, or of a grid too small for the
-// scan block, and points with a step whose circle leaves the window, are
-// appended to `defer` and handled by k_one_ring_wave in the next launch.
+// advance the tile's points one search step at a time.  With `list` null the
+// tile's points are all its slots [T0, T1); otherwise they are the slots
+// list[lscan[T0] .. lscan[T1]) (a slot-ordered list of the points to redo, with
+// lscan its exclusive scan over slots), and a tile with none exits before
+// staging.  The points of a window that does not fit the LDS budget, or of a
+// grid too small for the scan block, and points with a step whose circle
+// leaves the window, are appended to `defer` and handled by k_one_ring_wave
+// in the next launch.
 template <int PREC>
-__global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict__ sets,
-                                                       const GdtChunk* __restrict__ tiles,
-                                                       GdtRingIO io, int* __restrict__ counts,
-                                                       int* __restrict__ status,
-                                                       unsigned long long* __restrict__ counters,
-                                                       int* __restrict__ defer)
+__global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict__ sets, const GdtChunk* __restrict__ tiles,
+                     const int* __restrict__ list, const uint32_t* __restrict__ lscan,
+                     GdtRingIO io, int* __restrict__ counts, int* __restrict__ status,
+                     unsigned long long* __restrict__ counters, int* __restrict__ defer)
 {
     __shared__ uint32_t s_cs[GDT_WIN_CELLS + 1];
     __shared__ int s_gstart[GDT_WIN_CELLS];
@@ -230,6 +232,11 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
     GdtTileLds L;
     L.cs = s_cs; L.gstart = s_gstart; L.X64 = s_X64; L.X32 = s_X32; L.orig = s_orig;
     const int tid = threadIdx.x, nt = blockDim.x;
+    const int k0 = S.c0 + ch.start * GDT_TILE_CELLS;
+    const int T0 = (int)io.cstart[k0], T1 = (int)io.cstart[k0 + GDT_TILE_CELLS];
+    // the tile's items q in [Q0, Q1): point j = list ? list[q] : q
+    const int Q0 = list ? (int)lscan[T0] : T0, Q1 = list ? (int)lscan[T1] : T1;
+    if (Q0 >= Q1) return;    // block-uniform: nothing to do in this tile
 
     for (int c = tid; c < GDT_WIN_CELLS; c += nt) {
         int gs;
@@ -247,24 +254,23 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
             gdt_tile_copy<PREC == 32>(S, c, s_gstart[c], s_cs[c], s_cs[c + 1] - s_cs[c], io, L);
     __syncthreads();
 
-    const int k0 = S.c0 + ch.start * GDT_TILE_CELLS;
-    const int T0 = (int)io.cstart[k0], T1 = (int)io.cstart[k0 + GDT_TILE_CELLS];
     if (!fits) {
-        for (int j = T0 + tid; j < T1; j += nt) {
+        for (int q = Q0 + tid; q < Q1; q += nt) {
             const uint32_t pos = atomicAdd(reinterpret_cast<unsigned int*>(&status[ST_NDEFER]), 1u);
-            defer[pos] = j;
+            defer[pos] = list ? list[q] : q;
         }
         return;
     }
     // Lanes take points of the tile and advance them one search step per
     // iteration; a lane whose point is finished takes the next one, so a
     // wave's iterations track the total number of steps, not the longest ring.
-    if (tid == 0) s_next = T0 + nt;
+    if (tid == 0) s_next = Q0 + nt;
     __syncthreads();
     using Real = typename std::conditional<PREC == 32, float, double>::type;
     GdtTileLane<Real> st;
-    int j = T0 + tid;
-    bool have = j < T1;
+    int q = Q0 + tid;
+    bool have = q < Q1;
+    int j = have ? (list ? list[q] : q) : 0;
     if (have) gdt_tile_lane_init<Real>(w, L, j, io, S, st);
     unsigned npred = 0, nfb = 0, nsum = 0;
     int mx = 0, er = 0;
@@ -285,9 +291,12 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
                     nsum += (unsigned)st.count;
                     if (rc == STEP_NOCLOSE) er |= ERR_NOCLOSE;
                 }
-                j = atomicAdd(&s_next, 1);
-                have = j < T1;
-                if (have) gdt_tile_lane_init<Real>(w, L, j, io, S, st);
+                q = atomicAdd(&s_next, 1);
+                have = q < Q1;
+                if (have) {
+                    j = list ? list[q] : q;
+                    gdt_tile_lane_init<Real>(w, L, j, io, S, st);
+                }
             }
         }
     }
@@ -327,18 +336,18 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
 
 hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nsets,
                                     const GdtChunk* tiles, int ntiles, int block, int total_n,
-                                    const GdtRingIO& io, int* counts, int* status,
-                                    unsigned long long* counters, int* defer,
-                                    hipEvent_t after_tile, hipStream_t s)
+                                    const int* list, const uint32_t* lscan, const GdtRingIO& io,
+                                    int* counts, int* status, unsigned long long* counters,
+                                    int* defer, hipEvent_t after_tile, hipStream_t s)
 {
     if (ntiles <= 0) return hipSuccess;
     if (block > 1024) block = 1024;
     if (precision == 32)
-        hipLaunchKernelGGL(k_one_ring_tile<32>, dim3(ntiles), dim3(block), 0, s, d_sets, tiles, io,
-                           counts, status, counters, defer);
+        hipLaunchKernelGGL(k_one_ring_tile<32>, dim3(ntiles), dim3(block), 0, s, d_sets, tiles,
+                           list, lscan, io, counts, status, counters, defer);
     else
-        hipLaunchKernelGGL(k_one_ring_tile<64>, dim3(ntiles), dim3(block), 0, s, d_sets, tiles, io,
-                           counts, status, counters, defer);
+        hipLaunchKernelGGL(k_one_ring_tile<64>, dim3(ntiles), dim3(block), 0, s, d_sets, tiles,
+                           list, lscan, io, counts, status, counters, defer);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (after_tile && (e = hipEventRecord(after_tile, s)) != hipSuccess) return e;

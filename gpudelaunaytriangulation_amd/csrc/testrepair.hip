@@ -214,33 +214,37 @@ hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, con
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(GDT_BLOCK) void k_mark(const int* __restrict__ repair, int n,
+// Flags in cell (slot) order: mark[s] = 1 when the point in slot s is flagged.
+__global__ __launch_bounds__(GDT_BLOCK) void k_mark(const int* __restrict__ repair,
+                                                    const int* __restrict__ perm, int n,
                                                     uint32_t* __restrict__ mark)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) mark[i] = repair[i] >= 0 ? 1u : 0u;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) mark[s] = repair[perm[s]] >= 0 ? 1u : 0u;
 }
 
-__global__ __launch_bounds__(GDT_BLOCK) void k_compact(const int* __restrict__ repair, int n,
+// Flagged slots, in slot order, to worklist[pos[s]]; *worklen = pos[n].
+__global__ __launch_bounds__(GDT_BLOCK) void k_compact(const uint32_t* __restrict__ mark, int n,
                                                        const uint32_t* __restrict__ pos,
-                                                       const int* __restrict__ slot_of,
-                                                       int* __restrict__ worklist)
+                                                       int* __restrict__ worklist,
+                                                       uint32_t* __restrict__ worklen)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && repair[i] >= 0) worklist[pos[i]] = slot_of[i];
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n && mark[s]) worklist[pos[s]] = s;
+    if (s == 0) *worklen = pos[n];
 }
 
-hipError_t gdt_launch_compact(const int* repair, const int* slot_of, int total_n, uint32_t* mark,
+hipError_t gdt_launch_compact(const int* repair, const int* perm, int total_n, uint32_t* mark,
                               uint32_t* markscan, int* worklist, uint32_t* worklen,
                               uint32_t* scan_tmp, hipStream_t s)
 {
     const int g = (total_n + GDT_BLOCK - 1) / GDT_BLOCK;
-    hipLaunchKernelGGL(k_mark, dim3(g), dim3(GDT_BLOCK), 0, s, repair, total_n, mark);
+    hipLaunchKernelGGL(k_mark, dim3(g), dim3(GDT_BLOCK), 0, s, repair, perm, total_n, mark);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = gdt_scan_u32(mark, markscan, (size_t)total_n, worklen, scan_tmp, s);
+    e = gdt_scan_u32(mark, markscan, (size_t)total_n, markscan + total_n, scan_tmp, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_compact, dim3(g), dim3(GDT_BLOCK), 0, s, repair, total_n, markscan,
-                       slot_of, worklist);
+    hipLaunchKernelGGL(k_compact, dim3(g), dim3(GDT_BLOCK), 0, s, mark, total_n, markscan,
+                       worklist, worklen);
     return hipGetLastError();
 }
