@@ -479,16 +479,39 @@ int gdt_triangulate_batch(gdt_context* c, const double* d_xy, int nsets, const i
                             (hipStream_t)stream);
 }
 
-// Owned triangles into `tris` (capacity cap); count stays on the device at toff[N].
+// Owned triangles into `tris` (capacity cap); count stays on the device at
+// toff[N].  slot_order: owners in cell-slot order (the current cell list's).
 static int enumerate(gdt_context* c, const int* d_nbrs, const int* d_counts, int stride,
-                     int* tris, int cap, hipStream_t s)
+                     int* tris, int cap, bool slot_order, hipStream_t s)
 {
     GDT_CHECK(c->tcount.reserve((size_t)c->total_n));
     GDT_CHECK(c->toff.reserve((size_t)c->total_n + 1));
-    GDT_CHECK(gdt_launch_triangles(c->d_sets.p, c->pch.p, c->npch, c->total_n, d_nbrs, d_counts,
-                                   stride, c->tcount.p, c->toff.p, tris, cap, c->scan_tmp.p,
-                                   c->tpart.p, c->counters.p, c->status.p, s));
+    GDT_CHECK(gdt_launch_triangles(c->d_sets.p, c->pch.p, c->npch, c->total_n,
+                                   slot_order ? c->perm.p : nullptr, d_nbrs, d_counts, stride,
+                                   c->tcount.p, c->toff.p, tris, cap, c->scan_tmp.p, c->tpart.p,
+                                   c->counters.p, c->status.p, s));
     return GDT_OK;
+}
+
+// Empty-circle test of the triangles enumerate() wrote (slot order when the
+// cell order is tiled: then one LDS block per tile).
+static int test_triangles(gdt_context* c, const double* d_xy, bool tiled, int cap, hipStream_t s)
+{
+    const int n = c->total_n;
+    GDT_CHECK(gdt_fill_i32(c->repair.p, -1, (size_t)n, s));
+    GdtCells cl = c->cells();
+    if (tiled)
+        GDT_CHECK(gdt_launch_test_tile(c->d_sets.p, c->tch.p, c->ntch, c->tris.p, c->toff.p, cap, n,
+                                       d_xy, cl, c->repair.p, s));
+    else
+        GDT_CHECK(gdt_launch_test(c->d_sets.p, (int)c->sets.size(), c->tris.p, c->toff.p + n, cap,
+                                  n, d_xy, cl, c->repair.p, s));
+    return GDT_OK;
+}
+
+static bool tiled_sets(const gdt_context* c)
+{
+    return !c->sets.empty() && c->sets[0].tiled && c->ntch > 0;
 }
 
 int gdt_triangles(gdt_context* c, const int* d_nbrs, const int* d_counts, int n, int stride,
@@ -505,7 +528,7 @@ int gdt_triangles(gdt_context* c, const int* d_nbrs, const int* d_counts, int n,
     }
     int rc = reset_status(c, s);
     if (rc != GDT_OK) return rc;
-    if ((rc = enumerate(c, d_nbrs, d_counts, stride, d_tris, cap, s)) != GDT_OK) return rc;
+    if ((rc = enumerate(c, d_nbrs, d_counts, stride, d_tris, cap, false, s)) != GDT_OK) return rc;
     unsigned int h_nt = 0;
     GDT_CHECK(hipMemcpyAsync(&h_nt, c->toff.p + c->total_n, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
     if ((rc = read_status(c, s)) != GDT_OK) return rc;
@@ -557,7 +580,7 @@ static int repair_flagged(gdt_context* c, const int* d_repair, int stride, int* 
     uint32_t* worklen = reinterpret_cast<uint32_t*>(c->status.p + ST_NFLAGGED);
     GDT_CHECK(gdt_launch_compact(d_repair, c->perm.p, c->total_n, c->mark.p, c->markscan.p,
                                  c->worklist.p, worklen, c->scan_tmp.p, s));
-    if (!c->sets.empty() && c->sets[0].tiled && c->ntch > 0) {
+    if (tiled_sets(c)) {
         GDT_CHECK(gdt_launch_one_ring_tile(precision, c->d_sets.p, (int)c->sets.size(), c->tch.p,
                                            c->ntch, c->cfg.block, c->total_n, c->worklist.p,
                                            c->markscan.p, ring_io(c, d_nbrs, stride), d_counts,
@@ -598,15 +621,13 @@ static int test_and_repair_sets(gdt_context* c, const double* d_xy, int stride, 
     GDT_CHECK(c->repair.reserve((size_t)n));
     int rc;
     if ((rc = reset_status(c, s)) != GDT_OK) return rc;
-    // 1. triangles of the current rings (getCircumcirclesGPU, DelaunayGPU.cpp:650-666)
-    if ((rc = enumerate(c, d_nbrs, d_counts, stride, c->tris.p, cap, s)) != GDT_OK) return rc;
-    // 2. cell list of the moved points (DelaunayGPU.cpp:580-586)
+    const bool tiled = tiled_sets(c);
+    // 1. cell list of the moved points (DelaunayGPU.cpp:580-586)
     if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
+    // 2. triangles of the current rings (getCircumcirclesGPU, DelaunayGPU.cpp:650-666)
+    if ((rc = enumerate(c, d_nbrs, d_counts, stride, c->tris.p, cap, tiled, s)) != GDT_OK) return rc;
     // 3. empty-circle test (testTriangulation, DelaunayGPU.cpp:607-648)
-    GDT_CHECK(gdt_fill_i32(c->repair.p, -1, (size_t)n, s));
-    GdtCells cl = c->cells();
-    GDT_CHECK(gdt_launch_test(c->d_sets.p, (int)c->sets.size(), c->tris.p, c->toff.p + n, cap,
-                              c->total_n, d_xy, cl, c->repair.p, s));
+    if ((rc = test_triangles(c, d_xy, tiled, cap, s)) != GDT_OK) return rc;
     // 4. repair the flagged points (locallyRepairDelaunayTriangulation, DelaunayGPU.cpp:87-115)
     if ((rc = repair_flagged(c, c->repair.p, stride, d_nbrs, d_counts, precision, s)) != GDT_OK) return rc;
     unsigned int h_nt = 0;
@@ -666,12 +687,10 @@ static int md_enqueue_step(gdt_context* c, double* d_xy, int stride, int* d_nbrs
     int rc;
     GDT_CHECK(gdt_launch_md_displace(c->d_sets.p, c->pch.p, c->npch, d_xy, sigma, seed, c->md.p, s));
     if ((rc = reset_status(c, s)) != GDT_OK) return rc;
-    if ((rc = enumerate(c, d_nbrs, d_counts, stride, c->tris.p, cap, s)) != GDT_OK) return rc;
+    const bool tiled = tiled_sets(c);
     if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
-    GDT_CHECK(gdt_fill_i32(c->repair.p, -1, (size_t)n, s));
-    GdtCells cl = c->cells();
-    GDT_CHECK(gdt_launch_test(c->d_sets.p, (int)c->sets.size(), c->tris.p, c->toff.p + n, cap,
-                              c->total_n, d_xy, cl, c->repair.p, s));
+    if ((rc = enumerate(c, d_nbrs, d_counts, stride, c->tris.p, cap, tiled, s)) != GDT_OK) return rc;
+    if ((rc = test_triangles(c, d_xy, tiled, cap, s)) != GDT_OK) return rc;
     if ((rc = repair_flagged(c, c->repair.p, stride, d_nbrs, d_counts, precision, s)) != GDT_OK) return rc;
     GDT_CHECK(gdt_launch_md_step_end(c->status.p, c->counters.p, c->toff.p + n, n, c->md.p, s));
     return GDT_OK;

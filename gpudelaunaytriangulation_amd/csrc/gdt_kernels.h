@@ -59,10 +59,12 @@ hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nse
                                     int* defer, hipEvent_t after_tile, hipStream_t s);
 
 // Triangles owned by their smallest vertex, written at toff (deterministic);
-// counters[CT_INSUM] += sum of counts (partials holds npch words).
+// counters[CT_INSUM] += sum of counts (partials holds npch words).  perm null:
+// owners in caller order (toff over caller indices); otherwise in cell-slot
+// order, owner of slot r = perm[r] (toff over slots).
 hipError_t gdt_launch_triangles(const GdtSet* d_sets, const GdtChunk* pchunks, int npch,
-                                int total_n, const int* nbrs, const int* counts, int stride,
-                                uint32_t* tcount, uint32_t* toff, int* tris, int cap,
+                                int total_n, const int* perm, const int* nbrs, const int* counts,
+                                int stride, uint32_t* tcount, uint32_t* toff, int* tris, int cap,
                                 uint32_t* scan_tmp, unsigned long long* partials,
                                 unsigned long long* counters, int* status, hipStream_t s);
 
@@ -70,6 +72,12 @@ hipError_t gdt_launch_triangles(const GdtSet* d_sets, const GdtChunk* pchunks, i
 hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, const uint32_t* ntri,
                            int max_tri, int total_n, const double* d_xy, const GdtCells& c, int* repair,
                            hipStream_t s);
+
+// The same test, one block per tile, for triangles enumerated in slot order
+// (toff over slots); repair must be -1 on entry.
+hipError_t gdt_launch_test_tile(const GdtSet* d_sets, const GdtChunk* tiles, int ntiles,
+                                const int* tris, const uint32_t* toff, int cap, int total_n,
+                                const double* d_xy, const GdtCells& c, int* repair, hipStream_t s);
 
 // Slots of the flagged points, in slot order -> worklist; markscan (total_n+1
 // words) = exclusive scan of the flags with the count at markscan[total_n];

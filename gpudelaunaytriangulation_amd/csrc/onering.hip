@@ -8,7 +8,7 @@
 // (DelaunayGPU.cu:611-640, 945-975), which instead launch every point and
 // mask most of them out.
 #include "gdt_kernels.h"
-#include "gdt_tile.h"
+#include "gdt_stage.h"
 #include "gdt_wave.h"
 
 #include <type_traits>
@@ -171,38 +171,6 @@ __global__ __launch_bounds__(256) void k_one_ring_wave(const GdtSet* __restrict_
     }
 }
 
-// Exclusive scan of a[0..n) in LDS by the whole block; a[n] = total.
-__device__ inline void block_scan_lds(uint32_t* a, int n, uint32_t* wsum)
-{
-    const int nt = blockDim.x, tid = threadIdx.x;
-    const int ipt = (n + nt - 1) / nt;
-    const int b = tid * ipt, e = min(b + ipt, n);
-    uint32_t s = 0;
-    for (int i = b; i < e; ++i) s += a[i];
-    const int lane = tid & 63, wid = tid >> 6;
-    uint32_t x = s;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
-    if (lane == 63) wsum[wid] = x;
-    __syncthreads();
-    uint32_t woff = 0, tot = 0;
-    for (int q = 0; q < (nt >> 6); ++q) {
-        const uint32_t t = wsum[q];
-        if (q < wid) woff += t;
-        tot += t;
-    }
-    uint32_t run = woff + x - s;
-    for (int i = b; i < e; ++i) {
-        const uint32_t v = a[i];
-        a[i] = run;
-        run += v;
-    }
-    if (tid == 0) a[n] = tot;
-}
-
 // One block per tile (GDT_TILE x GDT_TILE cells): stage the tile window in LDS, then the lanes
 // advance the tile's points one search step at a time.  With `list` null the
 // tile's points are all its slots [T0, T1); otherwise they are the slots
@@ -238,21 +206,7 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
     const int Q0 = list ? (int)lscan[T0] : T0, Q1 = list ? (int)lscan[T1] : T1;
     if (Q0 >= Q1) return;    // block-uniform: nothing to do in this tile
 
-    for (int c = tid; c < GDT_WIN_CELLS; c += nt) {
-        int gs;
-        uint32_t cn;
-        gdt_tile_count(S, w, c, io.cstart, gs, cn);
-        s_gstart[c] = gs;
-        s_cs[c] = cn;
-    }
-    __syncthreads();
-    block_scan_lds(s_cs, GDT_WIN_CELLS, s_wsum);
-    __syncthreads();
-    const bool fits = s_cs[GDT_WIN_CELLS] <= GDT_WIN_MAXPTS && S.rmax >= GDT_SCAN;
-    if (fits)
-        for (int c = tid; c < GDT_WIN_CELLS; c += nt)
-            gdt_tile_copy<PREC == 32>(S, c, s_gstart[c], s_cs[c], s_cs[c + 1] - s_cs[c], io, L);
-    __syncthreads();
+    const bool fits = gdt_stage_window<PREC == 32>(S, w, io, L, s_wsum);
 
     if (!fits) {
         for (int q = Q0 + tid; q < Q1; q += nt) {

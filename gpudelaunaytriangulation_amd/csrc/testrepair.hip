@@ -6,12 +6,15 @@
 // run), by a count + exclusive scan + write (deterministic order), and
 // gpu_test_circumcircles_kernel (DelaunayGPU.cu:290-313).
 #include "gdt_kernels.h"
+#include "gdt_stage.h"
 
 // Owned triangles per point, the block's sum of ring lengths into
 // partials[block] (summed by k_sum_partials: no contended atomics) and the
-// malformed-row flag.
+// malformed-row flag.  Item r = p0 + local index is the point itself (perm
+// null: caller order) or the cell-ordered slot r holding point perm[r].
 __global__ __launch_bounds__(GDT_BLOCK) void k_tri_count(const GdtSet* __restrict__ sets,
                                                          const GdtChunk* __restrict__ chunks,
+                                                         const int* __restrict__ perm,
                                                          const int* __restrict__ nbrs,
                                                          const int* __restrict__ counts, int stride,
                                                          uint32_t* __restrict__ tcount,
@@ -25,7 +28,8 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_tri_count(const GdtSet* __restric
     unsigned cnt = 0, tot = 0;
     int bad = 0;
     if (li < S.n) {
-        const int i = S.p0 + li;
+        const int r = S.p0 + li;
+        const int i = perm ? perm[r] : r;
         int c = counts[i];
         tot = c > 0 ? (unsigned)c : 0u;
         if (c > stride || c < 3) { bad = 1; c = c > stride ? stride : (c < 0 ? 0 : c); }
@@ -38,7 +42,7 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_tri_count(const GdtSet* __restric
                 prev = cur;
             }
         }
-        tcount[i] = cnt;
+        tcount[r] = cnt;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -75,6 +79,7 @@ __global__ __launch_bounds__(256) void k_sum_partials(const unsigned long long* 
 
 __global__ __launch_bounds__(GDT_BLOCK) void k_tri_write(const GdtSet* __restrict__ sets,
                                                          const GdtChunk* __restrict__ chunks,
+                                                         const int* __restrict__ perm,
                                                          const int* __restrict__ nbrs,
                                                          const int* __restrict__ counts, int stride,
                                                          const uint32_t* __restrict__ toff,
@@ -84,12 +89,13 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_tri_write(const GdtSet* __restric
     const GdtSet S = sets[ch.set];
     const int li = ch.start + (int)threadIdx.x;
     if (li >= S.n) return;
-    const int i = S.p0 + li;
+    const int r = S.p0 + li;
+    const int i = perm ? perm[r] : r;
     int c = counts[i];
     if (c > stride) c = stride;
     if (c <= 0) return;
     const int* row = nbrs + (size_t)i * stride;
-    uint32_t t = toff[i];
+    uint32_t t = toff[r];
     int prev = row[c - 1];
     for (int m = 0; m < c; ++m) {
         const int cur = row[m];
@@ -106,19 +112,19 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_tri_write(const GdtSet* __restric
 }
 
 hipError_t gdt_launch_triangles(const GdtSet* d_sets, const GdtChunk* pchunks, int npch,
-                                int total_n, const int* nbrs, const int* counts, int stride,
-                                uint32_t* tcount, uint32_t* toff, int* tris, int cap,
+                                int total_n, const int* perm, const int* nbrs, const int* counts,
+                                int stride, uint32_t* tcount, uint32_t* toff, int* tris, int cap,
                                 uint32_t* scan_tmp, unsigned long long* partials,
                                 unsigned long long* counters, int* status, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_tri_count, dim3(npch), dim3(GDT_BLOCK), 0, s, d_sets, pchunks, nbrs,
+    hipLaunchKernelGGL(k_tri_count, dim3(npch), dim3(GDT_BLOCK), 0, s, d_sets, pchunks, perm, nbrs,
                        counts, stride, tcount, partials, status);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partials, npch, counters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = gdt_scan_u32(tcount, toff, (size_t)total_n, toff + total_n, scan_tmp, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_tri_write, dim3(npch), dim3(GDT_BLOCK), 0, s, d_sets, pchunks, nbrs,
+    hipLaunchKernelGGL(k_tri_write, dim3(npch), dim3(GDT_BLOCK), 0, s, d_sets, pchunks, perm, nbrs,
                        counts, stride, toff, tris, cap);
     return hipGetLastError();
 }
@@ -129,6 +135,81 @@ hipError_t gdt_launch_triangles(const GdtSet* d_sets, const GdtChunk* pchunks, i
 // The test is conservative: a point within relative 1e-9 of the circle counts
 // as inside, and a circle that cannot be covered is reported, so no invalid
 // triangle escapes (a flagged valid one is merely re-triangulated to itself).
+
+struct GdtTriCircle {
+    bool bad;            // not counter-clockwise: invalid whatever else holds
+    double qx, qy;       // circumcentre relative to the first vertex
+    double R2, lim;      // squared radius and the conservative test bound
+};
+
+__device__ inline GdtTriCircle tri_circle(const GdtSet& S, const gdt_d2* __restrict__ xy, int i,
+                                          int a, int b)
+{
+    GdtTriCircle c;
+    const gdt_d2 v = xy[i], pa = xy[a], pb = xy[b];
+    double ax = pa.x - v.x, ay = pa.y - v.y, bx = pb.x - v.x, by = pb.y - v.y;
+    ax -= S.lx * rint(ax / S.lx);
+    ay -= S.ly * rint(ay / S.ly);
+    bx -= S.lx * rint(bx / S.lx);
+    by -= S.ly * rint(by / S.ly);
+    const double D = ax * by - ay * bx;
+    c.bad = !(D > 0.);
+    c.qx = c.qy = c.R2 = c.lim = 0.;
+    if (!c.bad) {
+        const double aa = ax * ax + ay * ay, bb = bx * bx + by * by;
+        c.qx = (aa * by - bb * ay) / (2. * D);
+        c.qy = (bb * ax - aa * bx) / (2. * D);
+        c.R2 = c.qx * c.qx + c.qy * c.qy;
+        c.lim = c.R2 * (1. + 1e-9);
+    }
+    return c;
+}
+
+// Global-memory scan of one circle; returns the intruder's slot (-1 none) and
+// whether the circle was covered.
+__device__ inline int tri_scan_global(const GdtSet& S, const gdt_d2* __restrict__ xy, int i,
+                                      const GdtTriCircle& c, int si, int sa, int sb,
+                                      const uint32_t* __restrict__ cstart,
+                                      const gdt_d2* __restrict__ off64, bool& covered)
+{
+    const gdt_d2 v = xy[i];
+    int qcx, qcy;
+    double oqx, oqy;
+    gdt_bin_point(S, v.x + c.qx, v.y + c.qy, qcx, qcy, oqx, oqy);
+    const double covb = fmin(fmin(oqx, S.bsx - oqx), fmin(oqy, S.bsy - oqy));
+    const double bsmin = fmin(S.bsx, S.bsy);
+    int intruder = -1;
+    covered = false;
+    for (int r = 0; r <= S.rmax && intruder < 0; ++r) {
+        const int ncell = r ? 8 * r : 1;
+        for (int q = 0; q < ncell && intruder < 0; ++q) {
+            int dx, dy;
+            gdt_ring_cell(r, q, dx, dy);
+            const int x = gdt_wrap(qcx + dx, S.xsize), y = gdt_wrap(qcy + dy, S.ysize);
+            const int key = S.c0 + gdt_cell_key(S, x, y);
+            const uint32_t cb = cstart[key], ce = cstart[key + 1];
+            const double ebx = gdt_base64(dx, S.bsx, oqx), eby = gdt_base64(dy, S.bsy, oqy);
+            for (uint32_t k = cb; k < ce; ++k) {
+                if ((int)k == si || (int)k == sa || (int)k == sb) continue;
+                const gdt_d2 o = off64[k];
+                const double ex = gdt_disp64(ebx, o.x), ey = gdt_disp64(eby, o.y);
+                if (ex * ex + ey * ey < c.lim) { intruder = (int)k; break; }
+            }
+        }
+        const double cov = (double)r * bsmin + covb;
+        if (intruder < 0 && cov > 0. && cov * cov >= c.R2 * (1. + 4e-9)) { covered = true; break; }
+    }
+    return intruder;
+}
+
+__device__ inline void tri_flag(int* __restrict__ repair, int i, int a, int b, int intruder_caller)
+{
+    repair[i] = i;
+    repair[a] = a;
+    repair[b] = b;
+    if (intruder_caller >= 0) repair[intruder_caller] = intruder_caller;
+}
+
 __global__ __launch_bounds__(GDT_BLOCK) void k_test(const GdtSet* __restrict__ sets, int nsets,
                                                     const int* __restrict__ tris,
                                                     const uint32_t* __restrict__ ntri, int cap,
@@ -146,58 +227,16 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_test(const GdtSet* __restrict__ s
             (unsigned)b >= (unsigned)total_n)
             continue;   // malformed ring entry: the Euler check reports it
         const GdtSet S = sets[gdt_set_of(sets, nsets, i)];
-        const gdt_d2 v = xy[i], pa = xy[a], pb = xy[b];
-        double ax = pa.x - v.x, ay = pa.y - v.y, bx = pb.x - v.x, by = pb.y - v.y;
-        ax -= S.lx * rint(ax / S.lx);
-        ay -= S.ly * rint(ay / S.ly);
-        bx -= S.lx * rint(bx / S.lx);
-        by -= S.ly * rint(by / S.ly);
-        const double D = ax * by - ay * bx;
-        bool bad = !(D > 0.);
+        const GdtTriCircle c = tri_circle(S, xy, i, a, b);
+        bool bad = c.bad;
         int intruder = -1;
         if (!bad) {
-            const double aa = ax * ax + ay * ay, bb = bx * bx + by * by;
-            const double qx = (aa * by - bb * ay) / (2. * D);
-            const double qy = (bb * ax - aa * bx) / (2. * D);
-            const double R2 = qx * qx + qy * qy;
-            const double lim = R2 * (1. + 1e-9);
-            int qcx, qcy;
-            double oqx, oqy;
-            gdt_bin_point(S, v.x + qx, v.y + qy, qcx, qcy, oqx, oqy);
-            const double covb = fmin(fmin(oqx, S.bsx - oqx), fmin(oqy, S.bsy - oqy));
-            const double bsmin = fmin(S.bsx, S.bsy);
-            const int si = slot_of[i], sa = slot_of[a], sb = slot_of[b];
-            bool covered = false;
-            for (int r = 0; r <= S.rmax && intruder < 0; ++r) {
-                const int ncell = r ? 8 * r : 1;
-                for (int q = 0; q < ncell && intruder < 0; ++q) {
-                    int dx, dy;
-                    gdt_ring_cell(r, q, dx, dy);
-                    const int x = gdt_wrap(qcx + dx, S.xsize), y = gdt_wrap(qcy + dy, S.ysize);
-                    const int key = S.c0 + gdt_cell_key(S, x, y);
-                    const uint32_t cb = cstart[key], ce = cstart[key + 1];
-                    const double ebx = gdt_base64(dx, S.bsx, oqx), eby = gdt_base64(dy, S.bsy, oqy);
-                    for (uint32_t k = cb; k < ce; ++k) {
-                        if ((int)k == si || (int)k == sa || (int)k == sb) continue;
-                        const gdt_d2 o = off64[k];
-                        const double ex = gdt_disp64(ebx, o.x), ey = gdt_disp64(eby, o.y);
-                        if (ex * ex + ey * ey < lim) { intruder = (int)k; break; }
-                    }
-                }
-                const double cov = (double)r * bsmin + covb;
-                if (intruder < 0 && cov > 0. && cov * cov >= R2 * (1. + 4e-9)) { covered = true; break; }
-            }
+            bool covered;
+            intruder = tri_scan_global(S, xy, i, c, slot_of[i], slot_of[a], slot_of[b], cstart,
+                                       off64, covered);
             bad = intruder >= 0 || !covered;
         }
-        if (bad) {
-            repair[i] = i;
-            repair[a] = a;
-            repair[b] = b;
-            if (intruder >= 0) {
-                const int p = perm[intruder];
-                repair[p] = p;
-            }
-        }
+        if (bad) tri_flag(repair, i, a, b, intruder >= 0 ? perm[intruder] : -1);
     }
 }
 
@@ -214,7 +253,108 @@ hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, con
     return hipGetLastError();
 }
 
-// Flags in cell (slot) order: mark[s] = 1 when the point in slot s is flagged.
+// The same test with the tiled cell order, one block per tile: the triangles
+// are enumerated in slot order of their owner (toff over slots), so a tile's
+// triangles are one range; the block stages the tile's window (gdt_tile.h)
+// and scans, for each circle, the window cells of its conservative bounding
+// box in LDS.  A circle whose box leaves the window, or a window too full for
+// LDS, takes the global scan.
+__global__ __launch_bounds__(512) void k_test_tile(const GdtSet* __restrict__ sets,
+                                                   const GdtChunk* __restrict__ tiles,
+                                                   const int* __restrict__ tris,
+                                                   const uint32_t* __restrict__ toff, int cap,
+                                                   int total_n, const gdt_d2* __restrict__ xy,
+                                                   GdtRingIO io, const int* __restrict__ slot_of,
+                                                   int* __restrict__ repair)
+{
+    __shared__ uint32_t s_cs[GDT_WIN_CELLS + 1];
+    __shared__ int s_gstart[GDT_WIN_CELLS];
+    __shared__ gdt_d2 s_X64[GDT_WIN_MAXPTS];
+    __shared__ int s_orig[GDT_WIN_MAXPTS];
+    __shared__ uint32_t s_wsum[16];
+
+    const GdtChunk ch = tiles[blockIdx.x];
+    const GdtSet S = sets[ch.set];
+    const int k0 = S.c0 + ch.start * GDT_TILE_CELLS;
+    const int T0 = (int)io.cstart[k0], T1 = (int)io.cstart[k0 + GDT_TILE_CELLS];
+    const int R0 = min((int)toff[T0], cap), R1 = min((int)toff[T1], cap);
+    if (R0 >= R1) return;    // block-uniform
+    const GdtTileWin w = gdt_tile_window(S, ch.start);
+    GdtTileLds L;
+    L.cs = s_cs; L.gstart = s_gstart; L.X64 = s_X64; L.X32 = nullptr; L.orig = s_orig;
+    const bool fits = gdt_stage_window<false>(S, w, io, L, s_wsum);
+
+    for (int t = R0 + (int)threadIdx.x; t < R1; t += (int)blockDim.x) {
+        const int i = tris[3 * (size_t)t], a = tris[3 * (size_t)t + 1], b = tris[3 * (size_t)t + 2];
+        if ((unsigned)i >= (unsigned)total_n || (unsigned)a >= (unsigned)total_n ||
+            (unsigned)b >= (unsigned)total_n)
+            continue;   // malformed ring entry: the Euler check reports it
+        const GdtTriCircle c = tri_circle(S, xy, i, a, b);
+        if (c.bad) {
+            tri_flag(repair, i, a, b, -1);
+            continue;
+        }
+        const int si = slot_of[i];
+        bool done = false, bad = false;
+        int intruder = -1;
+        if (fits) {
+            // the owner's window cell and frame coordinate; the circle's box
+            const uint32_t pc = io.scell[si];
+            const int wx = (int)(pc & 0xffffu) - w.gx0, wy = (int)(pc >> 16) - w.gy0;
+            const int cv = wy * GDT_WIN + wx;
+            const gdt_d2 Xv = s_X64[s_cs[cv] + (si - s_gstart[cv])];
+            const double cx = Xv.x + c.qx, cy = Xv.y + c.qy;
+            const double R = sqrt(c.R2) * (1.0 + 1e-6);
+            const int x0 = (int)floor((cx - R) / S.bsx - 1e-3) + GDT_FRAME;
+            const int x1 = (int)floor((cx + R) / S.bsx + 1e-3) + GDT_FRAME;
+            const int y0 = (int)floor((cy - R) / S.bsy - 1e-3) + GDT_FRAME;
+            const int y1 = (int)floor((cy + R) / S.bsy + 1e-3) + GDT_FRAME;
+            if (x0 >= 0 && x1 < GDT_WIN && y0 >= 0 && y1 < GDT_WIN && x1 - x0 <= 2 * S.rmax &&
+                y1 - y0 <= 2 * S.rmax) {
+                for (int y = y0; y <= y1 && intruder < 0; ++y) {
+                    const uint32_t e = s_cs[y * GDT_WIN + x1 + 1];
+                    for (uint32_t k = s_cs[y * GDT_WIN + x0]; k < e; ++k) {
+                        const gdt_d2 X = s_X64[k];
+                        const double ex = X.x - cx, ey = X.y - cy;
+                        if (ex * ex + ey * ey < c.lim) {
+                            const int o = s_orig[k];
+                            if (o != i && o != a && o != b) { intruder = o; break; }
+                        }
+                    }
+                }
+                bad = intruder >= 0;
+                done = true;
+            }
+        }
+        if (!done) {
+            bool covered;
+            const int ks = tri_scan_global(S, xy, i, c, si, slot_of[a], slot_of[b], io.cstart,
+                                           io.off64, covered);
+            intruder = ks >= 0 ? io.perm[ks] : -1;
+            bad = ks >= 0 || !covered;
+        }
+        if (bad) tri_flag(repair, i, a, b, intruder);
+    }
+}
+
+hipError_t gdt_launch_test_tile(const GdtSet* d_sets, const GdtChunk* tiles, int ntiles,
+                                const int* tris, const uint32_t* toff, int cap, int total_n,
+                                const double* d_xy, const GdtCells& c, int* repair, hipStream_t s)
+{
+    if (ntiles <= 0 || cap <= 0) return hipSuccess;
+    GdtRingIO io;
+    io.cstart = c.cstart;
+    io.perm = c.perm;
+    io.off64 = c.off64;
+    io.off32 = c.off32;
+    io.scell = c.scell;
+    io.nbrs = nullptr;
+    io.stride = 0;
+    hipLaunchKernelGGL(k_test_tile, dim3(ntiles), dim3(512), 0, s, d_sets, tiles, tris, toff, cap,
+                       total_n, reinterpret_cast<const gdt_d2*>(d_xy), io, c.slot_of, repair);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(GDT_BLOCK) void k_mark(const int* __restrict__ repair,
                                                     const int* __restrict__ perm, int n,
                                                     uint32_t* __restrict__ mark)
