@@ -329,24 +329,31 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
     const bool wrap = st.ak >= 0;
     int bk = -1;
     double bx = 0., by = 0., bd2 = 0.;
-    Circ64 K = {0., 0., 0.};
     const double ax = st.ax, ay = st.ay, vx = st.vx, vy = st.vy;
     const int iv = st.iv, ak = st.ak;
     const int wx = st.c0 % GDT_WIN, wy = st.c0 / GDT_WIN;
+    // Both searches as one test, c taken when O(c) > Othr and P(c) < Pthr:
+    //   successor (a != 0): Othr = 0 (strictly left), P = in-circle value of
+    //     (v, a, best), Pthr = 0; before a best exists K = (-1,0,0), so that
+    //     P = -|c|^2 < 0 and the first left candidate is taken;
+    //   nearest neighbour (a = 0, so O = 0): Othr = -1, K = (1,0,0), so that
+    //     P = |c|^2, and Pthr = the best squared distance so far.
+    const double Othr = wrap ? 0. : -1.;
+    double Pthr = wrap ? 0. : 1e300;
+    Circ64 K;
+    K.k0 = wrap ? -1. : 1.;
+    K.kx = K.ky = 0.;
     auto load = [&](int k) { return L.X64[k]; };
     auto visit = [&](int k, const gdt_d2 X) {
         const double px = X.x - vx, py = X.y - vy;
         const double cc = gdt_dist2_64(px, py);
         const double O = gdt_orient64(ax, ay, px, py);
         const double P = gdt_incircle_cc64(K, px, py, cc);
-        // predicates combined without branches; one branch for a new best
-        const bool none = bk < 0;
-        const bool take_w = (O > 0.) & ((P < 0.) | none) & (k != ak);
-        const bool take_n = none | (cc < bd2);
-        const bool take = (k != iv) & (wrap ? take_w : take_n);
-        if (take) {
+        // one branch, taken only for a new best
+        if ((O > Othr) & (P < Pthr) & (k != iv) & (k != ak)) {
             bk = k; bx = px; by = py; bd2 = cc;
-            K = gdt_circ64(ax, ay, px, py);
+            if (wrap) K = gdt_circ64(ax, ay, px, py);
+            else Pthr = cc;
         }
     };
     auto visit_k = [&](int k) { visit(k, load(k)); };
@@ -413,21 +420,48 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
     const float cE = 2.f * (2.f * eps1 + 3.f * u);
     int bk = -1;
     float bx = 0.f, by = 0.f, bd2 = 0.f, bE = 0.f;
-    float K0 = 0.f, Kx = 0.f, Ky = 0.f, pab = 0.f, sab = 0.f, BP = 0.f;
+    float pab = 0.f, sab = 0.f, BP = 0.f;
+    // Both searches as one test on (O, P), as in the FP64 path (nearest
+    // neighbour: a = 0 so O = 0, K = (1,0,0) so P = |c|^2; successor before a
+    // best exists: K = (-1,0,0) so P = -|c|^2).  A candidate is certainly a new
+    // best when O > Oc and P < Pc, possibly one (the FP64 expression of the
+    // FP64 path decides) when O >= Op and P <= Pp, and certainly not otherwise.
+    float K0 = wrap ? -1.f : 1.f, Kx = 0.f, Ky = 0.f;
+    float Oc = -1.f, Op = -1.f, Pc = wrap ? 0.f : 3.0e38f, Pp = Pc;
     // Bounds that depend on mc, a bound on the max-norm of the displacements
     // of the region being scanned: (h+1) cells for offsets |dx|,|dy| <= h.
     float mc = 0.f, BO = 0.f, eabs = 0.f;
+    // nearest neighbour: |c|^2 + E(c) < bd2 - bE certain, |c|^2 - E(c) > bd2 + bE
+    // certainly not, with E(c) = cE |c|^2 + eabs.  As thresholds on |c|^2:
+    // Pc = (D - 8u S) / (1 + cE) with D = bd2 - bE - eabs, S = bd2 + bE + eabs
+    // (the rounding of D is below 2.01u S, the rest below u S, so Pc <= the
+    // exact D / (1 + cE)); Pp = S (1 + 8u) / (1 - cE) (four roundings of at
+    // most u each, so Pp > the exact S / (1 - cE)).  1 +- cE and 1 + 8u are
+    // exact in FP32.
+    auto set_nn = [&]() {
+        const float Sx = (bd2 + bE) + eabs;
+        Pc = (((bd2 - bE) - eabs) - (8.f * u) * Sx) / (1.f + cE);
+        Pp = (Sx * (1.f + 8.f * u)) / (1.f - cE);
+    };
     auto set_bp = [&]() {
         const float A0 = eP * sab * sab * sab + 1e-30f;
         const float A1 = cP * pab * sab + 3.f * eP * sab * sab;
         const float A2 = cP * pab + 3.f * eP * sab;
         BP = fmaf(mc, fmaf(mc, fmaf(mc, eP, A2), A1), A0);
+        Pc = -BP;
+        Pp = BP;
     };
     auto set_region = [&](int h) {
         mc = (float)(h + 1) * bsM * 1.0001f;
         BO = fmaf(4.f * (2.f * eps1 + 2.f * u) * ma + 4.f * eps0, mc, 4.f * eps0 * ma + 1e-30f);
         eabs = 8.f * eps0 * mc + 4.f * eps0 * eps0 + 1e-30f;
-        if (bk >= 0) set_bp();
+        if (wrap) {
+            Oc = BO;
+            Op = -BO;
+            if (bk >= 0) set_bp();
+        } else if (bk >= 0) {
+            set_nn();
+        }
     };
     set_region(GDT_SCAN0);
     const int wx = st.c0 % GDT_WIN, wy = st.c0 / GDT_WIN;
@@ -443,32 +477,21 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
         const float cc = fmaf(px, px, py * py);
         const float O = fmaf(ax, py, -(ay * px));
         const float P = fmaf(K0, cc, fmaf(Kx, px, Ky * py));
-        const float E = fmaf(cE, cc, eabs);
-        // predicates combined without branches; one branch when the
-        // candidate wins or a bound is not cleared
-        const bool none = bk < 0;
-        const bool okO = O > BO, decO = fabsf(O) > BO;
-        const bool inP = P < -BP, decP = fabsf(P) > BP;
-        const bool w_take = okO & (none | inP);
-        const bool w_und = (!decO) | (okO & !none & !decP);
-        const bool n_lt = cc + E < bd2 - bE, n_gt = cc - E > bd2 + bE;
-        const bool n_take = none | n_lt;
-        const bool n_und = !none & !n_lt & !n_gt;
-        const bool valid = (k != iv) & (!wrap | (k != ak));
-        bool take = valid & (wrap ? w_take : n_take);
-        const bool und = valid & (wrap ? w_und : n_und);
-        if (take | und) {
-            if (und) {
+        // one branch, taken only when the candidate may be a new best
+        if ((O >= Op) & (P <= Pp) & (k != iv) & (k != ak)) {
+            bool take = (O > Oc) & (P < Pc);
+            if (!take) {                      // undecided: the FP64 path's decision
                 ++nfb;
                 double c64x, c64y;
                 x64(k, c64x, c64y);
                 if (wrap) {
                     double a64x, a64y;
                     x64(ak, a64x, a64y);
-                    const bool left = decO ? okO : gdt_orient64(a64x, a64y, c64x, c64y) > 0.;
+                    const bool left = fabsf(O) > BO ? O > BO
+                                                    : gdt_orient64(a64x, a64y, c64x, c64y) > 0.;
                     if (!left) take = false;
-                    else if (none) take = true;
-                    else if (decP) take = inP;
+                    else if (bk < 0) take = true;
+                    else if (fabsf(P) > BP) take = P < -BP;
                     else {
                         double b64x, b64y;
                         x64(bk, b64x, b64y);
@@ -482,15 +505,20 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
                 }
             }
             if (take) {
-                bk = k; bx = px; by = py; bd2 = cc; bE = E;
-                const float aa = fmaf(ax, ax, ay * ay);
-                K0 = fmaf(ax, py, -(ay * px));
-                Kx = fmaf(cc, ay, -(aa * py));
-                Ky = fmaf(aa, px, -(cc * ax));
-                const float mb = GDT_FMAXF(fabsf(px), fabsf(py));
-                sab = ma + mb;
-                pab = ma * mb;
-                set_bp();
+                bk = k; bx = px; by = py; bd2 = cc;
+                bE = fmaf(cE, cc, eabs);
+                if (wrap) {
+                    const float aa = fmaf(ax, ax, ay * ay);
+                    K0 = fmaf(ax, py, -(ay * px));
+                    Kx = fmaf(cc, ay, -(aa * py));
+                    Ky = fmaf(aa, px, -(cc * ax));
+                    const float mb = GDT_FMAXF(fabsf(px), fabsf(py));
+                    sab = ma + mb;
+                    pab = ma * mb;
+                    set_bp();
+                } else {
+                    set_nn();
+                }
             }
         }
     };

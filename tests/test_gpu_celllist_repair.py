@@ -168,6 +168,31 @@ def test_circumcircle_known_answer():
     assert must <= set(np.nonzero(want >= 0)[0].tolist()) <= may
 
 
+def test_tiled_test_flags_within_known_answer():
+    """The per-tile LDS test inside testAndRepair (triangles in slot order)
+    flags between the vertices of the truly bad triangles and those plus the
+    points inside or on their circles, and the repaired rings equal the
+    oracle's triangulation of the moved points."""
+    L = 64.0
+    xy = inputs.uniform_points(4096, L, L, seed=1)
+    nb, c, dg = gpu_triangulate(xy, L, L, PREC_FP64)
+    tris, _ = _triangles(dg, nb, c)
+    i, a, b = tris[100]
+    k = [p for p in nb[a, :c[a]] if p not in (i, a, b)][0]
+    moved = xy.copy()
+    moved[k] = (xy[i] + xy[a] + xy[b]) / 3.0
+    bad, near = _bad_triangles(moved, L, tris)
+    must = set(tris[bad].reshape(-1).tolist())
+    may = must | {p for j in np.nonzero(bad)[0] for p in near[j]}
+    tri = torch.from_numpy(nb.reshape(-1).copy()).cuda()
+    num = torch.from_numpy(c.copy()).cuda()
+    dg.testAndRepairDelaunayTriangulation(torch.from_numpy(moved).cuda(), tri, num)
+    assert len(must) <= dg.last_repaired <= len(may)
+    on, oc = oracle.triangulate(moved, L, L, 1.0, 16)
+    got = tri.cpu().numpy().reshape(4096, dg.MaxSize)
+    assert rings.ring_hash(got, num.cpu().numpy()) == rings.ring_hash(on, oc)
+
+
 def test_euler_check_reports_corrupted_rings():
     L = 64.0
     xy = inputs.uniform_points(4096, L, L, seed=1)
