@@ -35,6 +35,8 @@
 #define GDT_WIN_MAXPTS 2048 // LDS entries per window (a fuller window defers its tile)
 #endif
 #define GDT_FRAME (GDT_WIN / 2)
+static_assert(GDT_SCAN0 <= GDT_SCAN && GDT_SCAN < GDT_HALO,
+              "the scan blocks and the row read ahead of them lie in the window");
 
 struct GdtTileWin {
     int tx, ty;          // tile coordinates
@@ -198,6 +200,37 @@ GDT_HD inline unsigned gdt_scan_block(const GdtTileLds& L, int c0, int h, Load& 
     return n;
 }
 
+// The block [-h,h]^2 around window cell c0 (1 <= h < GDT_HALO) as one flat
+// loop of n iterations, n counted first from the row ranges.  The range of the
+// row after the current one is read ahead, so a row switch -- which some lane
+// of a wave takes on most iterations -- is two moves and one LDS read, with no
+// wait.  Rows beyond the block are read ahead but never used; row h+1 still
+// lies in the window.  Returns n.
+template <class Load, class Fn>
+GDT_HD inline unsigned gdt_scan_rows(const GdtTileLds& L, int c0, int h, Load& load, Fn& fn)
+{
+    const int w = 2 * h + 1;
+    const uint32_t* p = L.cs + (c0 - h * GDT_WIN - h);      // first cell of the current row
+    unsigned n = 0;
+    for (int r = 0; r < w; ++r) n += p[r * GDT_WIN + w] - p[r * GDT_WIN];
+    uint32_t k = p[0], e = p[w];
+    uint32_t kn = p[GDT_WIN], en = p[GDT_WIN + w];
+    for (unsigned i = 0; i < n; ++i) {
+        if (k == e) {
+            do {            // the next non-empty row (one exists: i < n)
+                p += GDT_WIN;
+                k = kn;
+                e = en;
+                kn = p[GDT_WIN];
+                en = p[GDT_WIN + w];
+            } while (k == e);
+        }
+        fn((int)k, load((int)k));
+        ++k;
+    }
+    return n;
+}
+
 // Whether the cell block [dx0,dx1] x [dy0,dy1] around window cell c0 lies in
 // the staged window and within the grid's search radius (no cell twice).
 GDT_HD inline bool gdt_in_window(const GdtSet& S, int c0, int dx0, int dx1, int dy0, int dy1)
@@ -328,17 +361,24 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
 {
     const bool wrap = st.ak >= 0;
     int bk = -1;
-    double bx = 0., by = 0., bd2 = 0.;
     const double ax = st.ax, ay = st.ay, vx = st.vx, vy = st.vy;
-    const int iv = st.iv, ak = st.ak;
     const int wx = st.c0 % GDT_WIN, wy = st.c0 / GDT_WIN;
-    // Both searches as one test, c taken when O(c) > Othr and P(c) < Pthr:
-    //   successor (a != 0): Othr = 0 (strictly left), P = in-circle value of
-    //     (v, a, best), Pthr = 0; before a best exists K = (-1,0,0), so that
-    //     P = -|c|^2 < 0 and the first left candidate is taken;
-    //   nearest neighbour (a = 0, so O = 0): Othr = -1, K = (1,0,0), so that
-    //     P = |c|^2, and Pthr = the best squared distance so far.
+    // Both searches as one test, c taken when O(c) > Othr, P(c) < Pthr and
+    // c is not the excluded entry:
+    //   successor (a != 0): Othr = 0 (strictly left; v itself has O = 0),
+    //     P = in-circle value of (v, a, best), Pthr = 0, a excluded; before a
+    //     best exists K = (-1,0,0), so that P = -|c|^2 < 0 and the first left
+    //     candidate is taken;
+    //   nearest neighbour (a = 0, so O = 0): Othr = -1, v excluded,
+    //     K = (1,0,0), so that P = |c|^2, and Pthr = the best squared distance.
+    // A new best c sets K = circle (v, a, c) -- K0 = O(c) -- and Pthr = 0 in
+    // the successor search; with a = 0 the same expressions give K = (0,0,0),
+    // so K0 = O + nnf and Pthr = nnf |c|^2 (nnf = 1 for the nearest neighbour)
+    // serve both without a select.
+    const int ex = wrap ? st.ak : st.iv;
+    const double nnf = wrap ? 0. : 1.;
     const double Othr = wrap ? 0. : -1.;
+    const double aa = gdt_dist2_64(ax, ay);
     double Pthr = wrap ? 0. : 1e300;
     Circ64 K;
     K.k0 = wrap ? -1. : 1.;
@@ -350,15 +390,27 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
         const double O = gdt_orient64(ax, ay, px, py);
         const double P = gdt_incircle_cc64(K, px, py, cc);
         // one branch, taken only for a new best
-        if ((O > Othr) & (P < Pthr) & (k != iv) & (k != ak)) {
-            bk = k; bx = px; by = py; bd2 = cc;
-            if (wrap) K = gdt_circ64(ax, ay, px, py);
-            else Pthr = cc;
+        if ((O > Othr) & (P < Pthr) & (k != ex)) {
+            bk = k;
+            K.k0 = O + nnf;                        // gdt_circ64(a, c).k0, or 1
+            K.kx = fma(cc, ay, -(aa * py));        // gdt_circ64(a, c).kx, or 0
+            K.ky = fma(aa, px, -(cc * ax));        // gdt_circ64(a, c).ky, or 0
+            Pthr = nnf * cc;
         }
     };
     auto visit_k = [&](int k) { visit(k, load(k)); };
+    // the best's displacement and squared distance (the loop keeps only its entry)
+    double bx = 0., by = 0., bd2 = 0.;
+    auto best = [&]() {
+        if (bk >= 0) {
+            const gdt_d2 X = L.X64[bk];
+            bx = X.x - vx;
+            by = X.y - vy;
+            bd2 = gdt_dist2_64(bx, by);
+        }
+    };
     int hs = GDT_SCAN0;
-    npred += gdt_scan_block(L, st.c0, hs, load, visit);
+    npred += gdt_scan_rows(L, st.c0, hs, load, visit);
     if (bk < 0 && hs < GDT_SCAN) {
         npred += gdt_lds_block(L, wx, wy, -GDT_SCAN, GDT_SCAN, -GDT_SCAN, GDT_SCAN, hs, visit_k);
         hs = GDT_SCAN;
@@ -369,6 +421,7 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
     int dx0 = 0, dx1 = 0, dy0 = 0, dy1 = 0;
     // certification against the scanned block [-hs,hs]^2 (sets the box too)
     auto certify = [&]() {
+        best();
         const double cov = (double)hs * GDT_FMIN(S.bsx, S.bsy) + st.covb;
         ok = boxed = false;
         if (!wrap) {
@@ -402,6 +455,7 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
         ok = true;
     }
     if (!ok) return STEP_DEFER;
+    best();
     return gdt_tile_advance<double>(L, io, st, wrap, bk, bx, by);
 }
 
