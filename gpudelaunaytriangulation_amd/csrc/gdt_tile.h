@@ -473,40 +473,46 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
     const float ma = GDT_FMAXF(fabsf(ax), fabsf(ay));
     const float cE = 2.f * (2.f * eps1 + 3.f * u);
     int bk = -1;
-    float bx = 0.f, by = 0.f, bd2 = 0.f, bE = 0.f;
+    float bd2 = 0.f, bE = 0.f;
     float pab = 0.f, sab = 0.f, BP = 0.f;
     // Both searches as one test on (O, P), as in the FP64 path (nearest
     // neighbour: a = 0 so O = 0, K = (1,0,0) so P = |c|^2; successor before a
-    // best exists: K = (-1,0,0) so P = -|c|^2).  A candidate is certainly a new
-    // best when O > Oc and P < Pc, possibly one (the FP64 expression of the
-    // FP64 path decides) when O >= Op and P <= Pp, and certainly not otherwise.
+    // best exists: K = (-1,0,0) so P = -|c|^2; a new best sets K0 = O + nnf).
+    // A candidate is certainly a new best when O > Oc and P < Pc, possibly one
+    // (the FP64 expression of the FP64 path decides) when O >= Op and P <= Pp,
+    // and certainly not otherwise.
+    const int ex = wrap ? ak : iv;
+    const float nnf = wrap ? 0.f : 1.f;
+    const float aa = fmaf(ax, ax, ay * ay);
     float K0 = wrap ? -1.f : 1.f, Kx = 0.f, Ky = 0.f;
     float Oc = -1.f, Op = -1.f, Pc = wrap ? 0.f : 3.0e38f, Pp = Pc;
     // Bounds that depend on mc, a bound on the max-norm of the displacements
     // of the region being scanned: (h+1) cells for offsets |dx|,|dy| <= h.
-    float mc = 0.f, BO = 0.f, eabs = 0.f;
+    float mc = 0.f, cPmc = 0.f, BO = 0.f, eabs = 0.f;
     // nearest neighbour: |c|^2 + E(c) < bd2 - bE certain, |c|^2 - E(c) > bd2 + bE
     // certainly not, with E(c) = cE |c|^2 + eabs.  As thresholds on |c|^2:
-    // Pc = (D - 8u S) / (1 + cE) with D = bd2 - bE - eabs, S = bd2 + bE + eabs
-    // (the rounding of D is below 2.01u S, the rest below u S, so Pc <= the
-    // exact D / (1 + cE)); Pp = S (1 + 8u) / (1 - cE) (four roundings of at
-    // most u each, so Pp > the exact S / (1 - cE)).  1 +- cE and 1 + 8u are
-    // exact in FP32.
+    // Pc = (D - 8u S)(1 - cE) with D = bd2 - bE - eabs, S = bd2 + bE + eabs
+    // (the rounding of D is below 2.01u S, the rest below 2u S, and
+    // 1 - cE <= 1/(1 + cE), so Pc <= the exact D / (1 + cE) when positive, and
+    // negative otherwise); Pp = S (1 + 8u)(1 + 2 cE) (four roundings of at most
+    // u each, and 1 + 2 cE >= 1/(1 - cE), so Pp > the exact S / (1 - cE)).
+    const float cnn_c = 1.f - cE, cnn_p = (1.f + 8.f * u) * (1.f + 2.f * cE);
     auto set_nn = [&]() {
         const float Sx = (bd2 + bE) + eabs;
-        Pc = (((bd2 - bE) - eabs) - (8.f * u) * Sx) / (1.f + cE);
-        Pp = (Sx * (1.f + 8.f * u)) / (1.f - cE);
+        Pc = fmaf(-8.f * u, Sx, (bd2 - bE) - eabs) * cnn_c;
+        Pp = Sx * cnn_p;
     };
+    // in-circle bound of the current best b (sab = ma + mb, pab = ma mb):
+    //   BP = eP (sab + mc)^3 + cP pab mc (sab + mc)
     auto set_bp = [&]() {
-        const float A0 = eP * sab * sab * sab + 1e-30f;
-        const float A1 = cP * pab * sab + 3.f * eP * sab * sab;
-        const float A2 = cP * pab + 3.f * eP * sab;
-        BP = fmaf(mc, fmaf(mc, fmaf(mc, eP, A2), A1), A0);
+        const float t = sab + mc;
+        BP = fmaf(t, fmaf(cPmc, pab, (eP * t) * t), 1e-30f);
         Pc = -BP;
         Pp = BP;
     };
     auto set_region = [&](int h) {
         mc = (float)(h + 1) * bsM * 1.0001f;
+        cPmc = cP * mc;
         BO = fmaf(4.f * (2.f * eps1 + 2.f * u) * ma + 4.f * eps0, mc, 4.f * eps0 * ma + 1e-30f);
         eabs = 8.f * eps0 * mc + 4.f * eps0 * eps0 + 1e-30f;
         if (wrap) {
@@ -532,7 +538,7 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
         const float O = fmaf(ax, py, -(ay * px));
         const float P = fmaf(K0, cc, fmaf(Kx, px, Ky * py));
         // one branch, taken only when the candidate may be a new best
-        if ((O >= Op) & (P <= Pp) & (k != iv) & (k != ak)) {
+        if ((O >= Op) & (P <= Pp) & (k != ex)) {
             bool take = (O > Oc) & (P < Pc);
             if (!take) {                      // undecided: the FP64 path's decision
                 ++nfb;
@@ -559,13 +565,13 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
                 }
             }
             if (take) {
-                bk = k; bx = px; by = py; bd2 = cc;
+                bk = k;
+                bd2 = cc;
                 bE = fmaf(cE, cc, eabs);
+                K0 = O + nnf;                        // fmaf(ax, py, -(ay px)), or 1
+                Kx = fmaf(cc, ay, -(aa * py));       // 0 when a = 0
+                Ky = fmaf(aa, px, -(cc * ax));       // 0 when a = 0
                 if (wrap) {
-                    const float aa = fmaf(ax, ax, ay * ay);
-                    K0 = fmaf(ax, py, -(ay * px));
-                    Kx = fmaf(cc, ay, -(aa * py));
-                    Ky = fmaf(aa, px, -(cc * ax));
                     const float mb = GDT_FMAXF(fabsf(px), fabsf(py));
                     sab = ma + mb;
                     pab = ma * mb;
@@ -578,7 +584,7 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
     };
     auto visit_k = [&](int k) { visit(k, load(k)); };
     int hs = GDT_SCAN0;
-    npred += gdt_scan_block(L, st.c0, hs, load, visit);
+    npred += gdt_scan_rows(L, st.c0, hs, load, visit);
     if (bk < 0 && hs < GDT_SCAN) {
         set_region(GDT_SCAN);
         npred += gdt_lds_block(L, wx, wy, -GDT_SCAN, GDT_SCAN, -GDT_SCAN, GDT_SCAN, hs, visit_k);
@@ -639,7 +645,9 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
         ok = true;
     }
     if (!ok) return STEP_DEFER;
-    return gdt_tile_advance<float>(L, io, st, wrap, bk, bx, by);
+    // the best's displacement (the loop keeps only its entry)
+    const gdt_f2 B = L.X32[bk];
+    return gdt_tile_advance<float>(L, io, st, wrap, bk, B.x - vx, B.y - vy);
 }
 
 // Whole point, one step after another (host emulation and simple callers).
