@@ -204,8 +204,10 @@ GDT_HD inline unsigned gdt_scan_block(const GdtTileLds& L, int c0, int h, Load& 
 // loop of n iterations, n counted first from the row ranges.  The range of the
 // row after the current one is read ahead, so a row switch -- which some lane
 // of a wave takes on most iterations -- is two moves and one LDS read, with no
-// wait.  Rows beyond the block are read ahead but never used; row h+1 still
-// lies in the window.  Returns n.
+// wait; and the coordinates of the next entry are loaded, load(k), before the
+// current one is evaluated, fn(k, coords), so that the LDS latency overlaps
+// the predicate arithmetic.  Rows beyond the block are read ahead but never
+// used; row h+1 still lies in the window.  Returns n.
 template <class Load, class Fn>
 GDT_HD inline unsigned gdt_scan_rows(const GdtTileLds& L, int c0, int h, Load& load, Fn& fn)
 {
@@ -213,20 +215,36 @@ GDT_HD inline unsigned gdt_scan_rows(const GdtTileLds& L, int c0, int h, Load& l
     const uint32_t* p = L.cs + (c0 - h * GDT_WIN - h);      // first cell of the current row
     unsigned n = 0;
     for (int r = 0; r < w; ++r) n += p[r * GDT_WIN + w] - p[r * GDT_WIN];
+    if (n == 0) return 0;
     uint32_t k = p[0], e = p[w];
     uint32_t kn = p[GDT_WIN], en = p[GDT_WIN + w];
-    for (unsigned i = 0; i < n; ++i) {
-        if (k == e) {
-            do {            // the next non-empty row (one exists: i < n)
-                p += GDT_WIN;
-                k = kn;
-                e = en;
-                kn = p[GDT_WIN];
-                en = p[GDT_WIN + w];
-            } while (k == e);
-        }
-        fn((int)k, load((int)k));
-        ++k;
+    auto next_row = [&]() {     // the next non-empty row (one exists while entries remain)
+        do {
+            p += GDT_WIN;
+            k = kn;
+            e = en;
+            kn = p[GDT_WIN];
+            en = p[GDT_WIN + w];
+        } while (k == e);
+    };
+    if (k == e) next_row();
+    // two entries per trip, so that the two coordinate registers alternate
+    auto XA = load((int)k);
+    decltype(XA) XB;
+    for (unsigned i = 1;; ++i) {
+        uint32_t kc = k++;
+        bool more = i < n;
+        if (more && k == e) next_row();
+        XB = load((int)(more ? k : kc));
+        fn((int)kc, XA);
+        if (!more) break;
+        ++i;
+        kc = k++;
+        more = i < n;
+        if (more && k == e) next_row();
+        XA = load((int)(more ? k : kc));
+        fn((int)kc, XB);
+        if (!more) break;
     }
     return n;
 }
