@@ -346,3 +346,16 @@ GDT_HD inline GdtRingResult gdt_one_ring_f32(const GdtSet& S, int j, const GdtRi
     res.nfb = nfb;
     return res;
 }
+
+// Cell-offset bounding box of the circle through the origin, a and b, given
+// centre bounds [qxlo,qxhi] x [qylo,qyhi] and radius bound R, for a point whose
+// offset inside its cell is (ox,oy); `margin` is in cells.
+template <class T>
+GDT_HD inline void gdt_circle_cells(T qxlo, T qxhi, T qylo, T qyhi, T R, T ox, T oy, T bsx, T bsy,
+                                    T margin, int& dx0, int& dx1, int& dy0, int& dy1)
+{
+    dx0 = (int)floor((double)((ox + (qxlo - R)) / bsx - margin));
+    dx1 = (int)floor((double)((ox + (qxhi + R)) / bsx + margin));
+    dy0 = (int)floor((double)((oy + (qylo - R)) / bsy - margin));
+    dy1 = (int)floor((double)((oy + (qyhi + R)) / bsy + margin));
+}

@@ -258,19 +258,6 @@ GDT_HD inline bool gdt_in_window(const GdtSet& S, int c0, int dx0, int dx1, int 
            dx0 >= -S.rmax && dx1 <= S.rmax && dy0 >= -S.rmax && dy1 <= S.rmax;
 }
 
-// Cell-offset bounding box of the circle through the origin, a and b, given
-// centre bounds [qxlo,qxhi] x [qylo,qyhi] and radius bound R, for a point whose
-// offset inside its cell is (ox,oy); `margin` is in cells.
-template <class T>
-GDT_HD inline void gdt_circle_cells(T qxlo, T qxhi, T qylo, T qyhi, T R, T ox, T oy, T bsx, T bsy,
-                                    T margin, int& dx0, int& dx1, int& dy0, int& dy1)
-{
-    dx0 = (int)floor((double)((ox + (qxlo - R)) / bsx - margin));
-    dx1 = (int)floor((double)((ox + (qxhi + R)) / bsx + margin));
-    dy0 = (int)floor((double)((oy + (qylo - R)) / bsy - margin));
-    dy1 = (int)floor((double)((oy + (qyhi + R)) / bsy + margin));
-}
-
 // ---------------------------------------------------------------------------
 // Per point, one search step at a time (the kernel interleaves the points of
 // a tile across its lanes step by step).  A step scans the block of cells

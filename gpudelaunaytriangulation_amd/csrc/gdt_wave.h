@@ -2,12 +2,13 @@
 // (host/device routines), for the points the tile kernel hands back (circles
 // reaching past the staged window, windows too full for LDS).
 //
-// The search is the one of the direct routine (gdt_point.h): nearest
-// neighbour, then gift wrapping, each search visiting the Chebyshev rings of
-// cells r = 0, 1, ... around v's cell until its answer is certified.  Here the
-// cells of a ring are dealt to the lanes of a wavefront; each lane keeps a best
-// candidate, and the ring ends with a reduction of the lane bests to the wave's
-// best, which every lane then starts the next ring from.  Arithmetic is FP64
+// The search is the one of the tile routine (gdt_tile.h): nearest neighbour,
+// then gift wrapping, each search scanning the cells [-2,2]^2 around v's cell
+// and then the rest of its current circle's bounding box (or, while that is not
+// possible, further Chebyshev rings) until its answer is certified.  Here the
+// cells of a pass are dealt to the lanes of a wavefront; each lane keeps a best
+// candidate, and the pass ends with a reduction of the lane bests to the
+// wave's best, which every lane then starts the next pass from.  Arithmetic is FP64
 // with the displacement formula of the direct routine, so both precision paths
 // give these points the same row.
 //
@@ -92,14 +93,16 @@ GDT_HD inline GdtRingResult gdt_wave_ring(const GdtSet& S, int j, const GdtRingI
     // One search: nearest neighbour (wrap false) or successor of edge (v,a).
     auto search = [&](bool wrap, int a, double ax, double ay, GdtWaveBest& g) -> bool {
         g = gdt_wave_none();
-        for (int r = 0; r <= S.rmax; ++r) {
-            const int ncell = r ? 8 * r : 1;
+        // One pass over the cells of [x0,x1] x [y0,y1] outside [-hole,hole]^2
+        // (hole < 0: none), dealt to the lanes, then the wave reduction.
+        auto pass = [&](int x0, int x1, int y0, int y1, int hole) {
+            const int nx = x1 - x0 + 1, ncell = nx * (y1 - y0 + 1);
             auto lane_fn = [&](int lane, GdtWaveBest& m) {
                 Circ64 K = {0., 0., 0.};
                 if (wrap && m.k >= 0) K = gdt_circ64(ax, ay, m.x, m.y);
                 for (int q = lane; q < ncell; q += GDT_WAVE) {
-                    int dx, dy;
-                    gdt_ring_cell(r, q, dx, dy);
+                    const int dx = x0 + q % nx, dy = y0 + q / nx;
+                    if (dx >= -hole && dx <= hole && dy >= -hole && dy <= hole) continue;
                     const int x = gdt_wrap(cx + dx, S.xsize), y = gdt_wrap(cy + dy, S.ysize);
                     const int key = S.c0 + gdt_cell_key(S, x, y);
                     const uint32_t b = io.cstart[key], e = io.cstart[key + 1];
@@ -128,19 +131,53 @@ GDT_HD inline GdtRingResult gdt_wave_ring(const GdtSet& S, int j, const GdtRingI
                 }
             };
             g = w.ring(g, wrap, ax, ay, lane_fn);
-            if (g.k < 0) continue;
-            const double cov = (double)r * bsmin + covb;
-            if (!(cov > 0.)) continue;
-            if (!wrap) {
-                if (g.d2 <= cov * cov * (1.0 - 1e-12)) return true;
-                continue;
+        };
+        // The square [-hs,hs]^2 first (hs = 2), then either the rest of the
+        // bounding box of the current circle in one pass -- every later best's
+        // circle lies inside it (circles through v and a with centres left of
+        // v->a are nested along the bisector), so the search is then certified
+        // -- or, while there is no candidate or the box reaches past rmax, the
+        // next Chebyshev ring.
+        int hs = S.rmax < 2 ? S.rmax : 2;
+        pass(-hs, hs, -hs, hs, -1);
+        for (;;) {
+            if (g.k >= 0) {
+                const double cov = (double)hs * bsmin + covb;
+                int dx0 = 0, dx1 = 0, dy0 = 0, dy1 = 0;
+                bool boxed = false;
+                if (!wrap) {
+                    if (cov > 0. && g.d2 <= cov * cov * (1.0 - 1e-12)) return true;
+                    const double rad = sqrt(g.d2) * (1.0 + 1e-9);
+                    gdt_circle_cells<double>(0., 0., 0., 0., rad, ov.x, ov.y, S.bsx, S.bsy, 1e-3,
+                                             dx0, dx1, dy0, dy1);
+                    boxed = true;
+                } else {
+                    const Circ64 K = gdt_circ64(ax, ay, g.x, g.y);
+                    const double k0min = 1e-6 * (fabs(ax * g.y) + fabs(ay * g.x));
+                    if (K.k0 > k0min) {
+                        if (cov > 0. &&
+                            K.kx * K.kx + K.ky * K.ky <= cov * cov * (K.k0 * K.k0) * (1.0 - 1e-6))
+                            return true;
+                        const double qx = -K.kx / (2. * K.k0), qy = -K.ky / (2. * K.k0);
+                        const double rad = sqrt(qx * qx + qy * qy);
+                        const double m = 1e-6 * (fabs(qx) + fabs(qy) + rad) + 1e-9 * S.bsx;
+                        gdt_circle_cells<double>(qx - m, qx + m, qy - m, qy + m, rad, ov.x, ov.y,
+                                                 S.bsx, S.bsy, 1e-3, dx0, dx1, dy0, dy1);
+                        boxed = true;
+                    }
+                }
+                if (boxed) {
+                    if (dx0 >= -hs && dx1 <= hs && dy0 >= -hs && dy1 <= hs) return true;
+                    if (dx0 >= -S.rmax && dx1 <= S.rmax && dy0 >= -S.rmax && dy1 <= S.rmax) {
+                        pass(dx0, dx1, dy0, dy1, hs);
+                        return true;
+                    }
+                }
             }
-            const Circ64 K = gdt_circ64(ax, ay, g.x, g.y);
-            const double k0min = 1e-6 * (fabs(ax * g.y) + fabs(ay * g.x));
-            if (K.k0 > k0min && K.kx * K.kx + K.ky * K.ky <= cov * cov * (K.k0 * K.k0) * (1.0 - 1e-6))
-                return true;
+            if (hs >= S.rmax) return false;
+            ++hs;
+            pass(-hs, hs, -hs, hs, hs - 1);
         }
-        return false;
     };
 
     GdtWaveBest g;
