@@ -66,6 +66,8 @@ struct gdt_context {
     DBuf<int> perm, slot_of, worklist, repair, tris, defer;
     DBuf<gdt_d2> off64;
     DBuf<gdt_f2> off32;
+    DBuf<GdtKbRec> bkt;
+    DBuf<uint32_t> bmat, bscan;
     DBuf<int> status;
     DBuf<unsigned long long> counters, tpart;
     // Voronoi energy: its own set table (only p0, n and the box are used)
@@ -86,6 +88,7 @@ struct gdt_context {
         c.pkey = pkey.p; c.prank = prank.p; c.ccount = ccount.p; c.cstart = cstart.p;
         c.perm = perm.p; c.slot_of = slot_of.p; c.off64 = off64.p; c.off32 = off32.p;
         c.scell = scell.p; c.scan_tmp = scan_tmp.p;
+        c.bkt = bkt.p; c.bmat = bmat.p; c.bscan = bscan.p;
         return c;
     }
 };
@@ -166,6 +169,9 @@ static int configure(gdt_context* c, std::vector<GdtSet> sets)
     GDT_CHECK(c->ccount.reserve(K));
     GDT_CHECK(c->cstart.reserve(K + 1));
     GDT_CHECK(c->scan_tmp.reserve(gdt_scan_tmp_words(std::max(K, N + 1))));
+    GDT_CHECK(c->bkt.reserve(N));
+    GDT_CHECK(c->bmat.reserve(gdt_bucket_words(N, K)));
+    GDT_CHECK(c->bscan.reserve(gdt_bucket_words(N, K)));
     return GDT_OK;
 }
 
@@ -264,7 +270,8 @@ static int build_cells(gdt_context* c, const double* d_xy, hipStream_t s)
 {
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[0], s));
     GdtCells cl = c->cells();
-    GDT_CHECK(gdt_build_cells(c->d_sets.p, c->pch.p, c->npch, c->kch.p, c->nkch, c->total_n,
+    GDT_CHECK(gdt_build_cells(c->d_sets.p, (int)c->sets.size(), c->pch.p, c->npch, c->kch.p,
+                              c->nkch, c->total_n,
                               c->total_keys, d_xy, cl, s));
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
     c->cells_valid = true;
@@ -326,6 +333,7 @@ void gdt_destroy(gdt_context* c)
     c->off32.release(); c->status.release(); c->counters.release(); c->tpart.release();
     c->d_vsets.release(); c->ecell.release(); c->partial.release(); c->vslots.release();
     c->md.release();
+    c->bkt.release(); c->bmat.release(); c->bscan.release();
     if (c->h_md) (void)hipHostFree(c->h_md);
     if (c->md_stream) (void)hipStreamDestroy(c->md_stream);
     for (auto& e : c->ev)

@@ -2,14 +2,30 @@
 //
 // Replaces the reference's padded-bucket cell list (cellListGPU.cu:20-58 with
 // its host regrow loop, cellListGPU.cpp:404-460): no per-bucket capacity, no
-// host readback.  Steps, all on one stream:
-//   1. bin: cell key of every point, atomic arrival rank inside its cell;
-//   2. exclusive scan of the per-cell counts (wave64 shuffle scan, two levels);
-//   3. scatter the caller indices into cell order;
-//   4. per cell: sort its few indices ascending (the reference's CPU binning
-//      keeps index order inside a bucket, cellListGPU.cpp:306-326), then write
-//      each slot's FP64/FP32 offsets inside the cell and the inverse permutation.
+// host readback.  Slot order is cell-key order, and inside a cell ascending
+// caller index (the reference's CPU binning keeps index order inside a bucket,
+// cellListGPU.cpp:306-326).
+//
+// Two-level counting sort in four kernels, every pass reading or writing in
+// runs:
+//   1. k_kb_count: blocks of consecutive points histogram their key buckets
+//      (2^B consecutive cell keys, B >= 10) in LDS; one count per (bucket,
+//      block);
+//   2. k_kb_rowscan: per bucket, the exclusive scan of its block counts and
+//      the bucket total;
+//   3. k_kb_scatter: each block scans the bucket totals in LDS (block 0 also
+//      stores them), bins its points again and appends one 32-byte record
+//      (offset, caller index, key, packed cell) to its run of each bucket;
+//   4. k_kb_finalize: one block per bucket counting-sorts its points by cell
+//      in LDS, orders each cell by caller index, and writes cstart/ccount and
+//      the slot arrays (perm, slot_of, off64, off32, scell) of its key range
+//      in slot order; a bucket above KB_CAP points (dense clusters) takes the
+//      same steps through global memory.
+// More than 2^25 keys (beyond the LDS histograms) take the single-level path:
+// per-point atomic arrival ranks, scan, scatter, per-cell sort.
 #include "gdt_kernels.h"
+
+#include <algorithm>
 
 #define SCAN_ITEMS 8
 #define SCAN_TILE (GDT_BLOCK * SCAN_ITEMS)
@@ -231,13 +247,289 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_finalize(const GdtSet* __restrict
     }
 }
 
-hipError_t gdt_build_cells(const GdtSet* d_sets, const GdtChunk* pchunks, int npch,
-                           const GdtChunk* kchunks, int nkch, int total_n, int total_keys,
-                           const double* d_xy, GdtCells& c, hipStream_t s)
+// ---------------------------------------------------------------------------
+// Two-level build
+// ---------------------------------------------------------------------------
+#define KB_THREADS 1024
+#define KB_NB 256            // point blocks of passes 1 and 3 (at most)
+#define KB_MAXBUCKETS 8192   // LDS histogram bins of passes 1 and 3 (32 KB)
+#define KB_MAXSHIFT 12       // 2^B cells per bucket in LDS (pass 4, at most 32 KB)
+#define KB_CAP 4096          // points of a bucket sorted in LDS (4 per thread)
+
+struct KbPlan {
+    int B;                   // bucket = key >> B
+    int nkb;                 // buckets
+    int nb;                  // point blocks
+    int per;                 // points per block
+};
+
+static KbPlan kb_plan(size_t n, size_t k)
+{
+    KbPlan p;
+    p.B = 10;
+    while (((k + ((size_t)1 << p.B) - 1) >> p.B) > KB_MAXBUCKETS) ++p.B;
+    p.nkb = (int)((k + ((size_t)1 << p.B) - 1) >> p.B);
+    p.nb = (int)std::min<size_t>(KB_NB, (n + KB_THREADS - 1) / KB_THREADS);
+    if (p.nb < 1) p.nb = 1;
+    p.per = (int)((n + p.nb - 1) / p.nb);
+    return p;
+}
+
+// bscan layout: [nkb * nb] row-exclusive block offsets, [nkb] bucket totals,
+// [nkb + 1] bucket starts.
+size_t gdt_bucket_words(size_t n, size_t k)
+{
+    const KbPlan p = kb_plan(n, k);
+    return (size_t)p.nkb * p.nb + 2 * (size_t)p.nkb + 2;
+}
+
+// Cell of caller point i (its set found by binary search over the sets).
+__device__ inline void kb_bin(const GdtSet* __restrict__ sets, int nsets, const gdt_d2* __restrict__ xy,
+                              int i, uint32_t& key, uint32_t& packed, double& ox, double& oy)
+{
+    const GdtSet S = sets[gdt_set_of(sets, nsets, i)];
+    const gdt_d2 p = xy[i];
+    int cx, cy;
+    gdt_bin_point(S, p.x, p.y, cx, cy, ox, oy);
+    key = (uint32_t)(S.c0 + gdt_cell_key(S, cx, cy));
+    packed = (uint32_t)cx | ((uint32_t)cy << 16);
+}
+
+// Pass 1: bmat[bucket * nb + block] = points of the block in the bucket.
+__global__ __launch_bounds__(KB_THREADS) void k_kb_count(const GdtSet* __restrict__ sets, int nsets,
+                                                         const gdt_d2* __restrict__ xy, int n,
+                                                         KbPlan pl, uint32_t* __restrict__ bmat)
+{
+    extern __shared__ uint32_t hist[];
+    const int tid = threadIdx.x, b = blockIdx.x;
+    for (int q = tid; q < pl.nkb; q += KB_THREADS) hist[q] = 0u;
+    __syncthreads();
+    const int i0 = b * pl.per, i1 = min(n, i0 + pl.per);
+    for (int i = i0 + tid; i < i1; i += KB_THREADS) {
+        uint32_t key, packed;
+        double ox, oy;
+        kb_bin(sets, nsets, xy, i, key, packed, ox, oy);
+        atomicAdd(&hist[key >> pl.B], 1u);
+    }
+    __syncthreads();
+    for (int q = tid; q < pl.nkb; q += KB_THREADS) bmat[(size_t)q * pl.nb + b] = hist[q];
+}
+
+// Pass 2: per bucket (one block each, KB_NB threads), the exclusive scan of
+// its row of block counts, and the bucket total.
+__global__ __launch_bounds__(KB_NB) void k_kb_rowscan(KbPlan pl, const uint32_t* __restrict__ bmat,
+                                                      uint32_t* __restrict__ bscan)
+{
+    __shared__ uint32_t wsum[KB_NB / 64];
+    const int tid = threadIdx.x, bin = blockIdx.x;
+    const size_t row = (size_t)bin * pl.nb;
+    const uint32_t v = tid < pl.nb ? bmat[row + tid] : 0u;
+    const int lane = tid & 63, wid = tid >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < KB_NB / 64; ++w) {
+        const uint32_t t = wsum[w];
+        if (w < wid) woff += t;
+        tot += t;
+    }
+    if (tid < pl.nb) bscan[row + tid] = woff + x - v;
+    if (tid == 0) bscan[(size_t)pl.nkb * pl.nb + bin] = tot;
+}
+
+// Exclusive scan of a[0..m) in LDS by the block (m <= KB_THREADS * 8); a[m] = total.
+__device__ inline void kb_scan_lds(uint32_t* a, int m, uint32_t* wsum)
+{
+    const int tid = threadIdx.x;
+    const int ipt = (m + KB_THREADS - 1) / KB_THREADS;
+    const int b = tid * ipt, e = min(b + ipt, m);
+    uint32_t sum = 0;
+    for (int q = b; q < e; ++q) sum += a[q];
+    const int lane = tid & 63, wid = tid >> 6;
+    uint32_t x = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+    for (int w = 0; w < KB_THREADS / 64; ++w) {
+        const uint32_t t = wsum[w];
+        if (w < wid) woff += t;
+        tot += t;
+    }
+    uint32_t run = woff + x - sum;
+    for (int q = b; q < e; ++q) {
+        const uint32_t v = a[q];
+        a[q] = run;
+        run += v;
+    }
+    if (tid == 0) a[m] = tot;
+    __syncthreads();
+}
+
+// Pass 3: bucket starts (scan of the totals), then each point appended to its
+// (bucket, block) run.
+__global__ __launch_bounds__(KB_THREADS) void k_kb_scatter(const GdtSet* __restrict__ sets, int nsets,
+                                                           const gdt_d2* __restrict__ xy, int n,
+                                                           KbPlan pl, uint32_t* __restrict__ bscan,
+                                                           GdtKbRec* __restrict__ bkt)
+{
+    extern __shared__ uint32_t cursor[];                  // [nkb + 1]
+    __shared__ uint32_t wsum[KB_THREADS / 64];
+    const int tid = threadIdx.x, b = blockIdx.x;
+    const size_t m = (size_t)pl.nkb * pl.nb;
+    for (int q = tid; q < pl.nkb; q += KB_THREADS) cursor[q] = bscan[m + q];
+    __syncthreads();
+    kb_scan_lds(cursor, pl.nkb, wsum);
+    if (b == 0)
+        for (int q = tid; q <= pl.nkb; q += KB_THREADS) bscan[m + pl.nkb + q] = cursor[q];
+    for (int q = tid; q < pl.nkb; q += KB_THREADS) cursor[q] += bscan[(size_t)q * pl.nb + b];
+    __syncthreads();
+    const int i0 = b * pl.per, i1 = min(n, i0 + pl.per);
+    for (int i = i0 + tid; i < i1; i += KB_THREADS) {
+        GdtKbRec r;
+        kb_bin(sets, nsets, xy, i, r.key, r.cell, r.off.x, r.off.y);
+        r.idx = i;
+        r.pad = 0u;
+        bkt[atomicAdd(&cursor[r.key >> pl.B], 1u)] = r;
+    }
+}
+
+// cstart/ccount of the bucket's keys from the LDS scan cst[0..nk].
+__device__ inline void kb_write_starts(const uint32_t* cst, int nk, uint32_t k0, uint32_t base,
+                                       uint32_t* __restrict__ cstart, uint32_t* __restrict__ ccount)
+{
+    for (int q = threadIdx.x; q < nk; q += KB_THREADS) {
+        cstart[k0 + q] = base + cst[q];
+        ccount[k0 + q] = cst[q + 1] - cst[q];
+    }
+}
+
+// Slot arrays of slot `slot` from bucket record r.
+__device__ inline void kb_write_slot(GdtCells& c, uint32_t slot, const GdtKbRec& r)
+{
+    gdt_f2 o32;
+    o32.x = (float)r.off.x;
+    o32.y = (float)r.off.y;
+    c.perm[slot] = r.idx;
+    c.slot_of[r.idx] = (int)slot;
+    c.off64[slot] = r.off;
+    c.off32[slot] = o32;
+    c.scell[slot] = r.cell;
+}
+
+// Pass 4: one block per bucket.  Up to KB_CAP points are ranked in LDS (cell
+// by LDS counting sort, then caller index inside the cell) and the slot
+// arrays written in slot order, one slot per thread; a larger bucket goes
+// through global memory (perm holds bucket positions until the per-cell
+// insertion sort is done).
+__global__ __launch_bounds__(KB_THREADS) void k_kb_finalize(KbPlan pl, int total_keys, int total_n,
+                                                            const uint32_t* __restrict__ bscan,
+                                                            const GdtKbRec* __restrict__ bkt,
+                                                            GdtCells c)
+{
+    extern __shared__ uint32_t lds[];
+    uint32_t* cst = lds;                                  // [2^B + 1] cell starts
+    __shared__ uint32_t wsum[KB_THREADS / 64];
+    const int tid = threadIdx.x, bin = blockIdx.x;
+    const uint32_t* bstart = bscan + (size_t)pl.nkb * pl.nb + pl.nkb;
+    const uint32_t k0 = (uint32_t)bin << pl.B;
+    const int nk = min((int)(1 << pl.B), total_keys - (int)k0);
+    const uint32_t base = bstart[bin];
+    const int cnt = (int)(bstart[bin + 1] - base);
+    if (bin == pl.nkb - 1 && tid == 0) c.cstart[total_keys] = (uint32_t)total_n;
+    for (int q = tid; q <= nk; q += KB_THREADS) cst[q] = 0u;
+    __syncthreads();
+    if (cnt <= KB_CAP) {                                  // block-uniform
+        int* idxs = reinterpret_cast<int*>(lds + (1 << pl.B) + 1);   // [KB_CAP] by position
+        uint32_t* ord = lds + (1 << pl.B) + 1 + KB_CAP;   // [KB_CAP] bucket position by slot
+        constexpr int PER = KB_CAP / KB_THREADS;
+        uint32_t lc[PER], r0[PER];
+        int idx[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = tid + u * KB_THREADS;
+            lc[u] = 0u;
+            r0[u] = 0u;
+            idx[u] = 0;
+            if (q < cnt) {
+                lc[u] = bkt[base + q].key - k0;
+                idx[u] = bkt[base + q].idx;
+                r0[u] = atomicAdd(&cst[lc[u]], 1u);
+            }
+        }
+        __syncthreads();
+        kb_scan_lds(cst, nk, wsum);
+        kb_write_starts(cst, nk, k0, base, c.cstart, c.ccount);
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = tid + u * KB_THREADS;
+            if (q < cnt) idxs[cst[lc[u]] + r0[u]] = idx[u];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = tid + u * KB_THREADS;
+            if (q < cnt) {
+                // rank by caller index inside the cell
+                const uint32_t c0 = cst[lc[u]], c1 = cst[lc[u] + 1];
+                uint32_t r = 0;
+                for (uint32_t t = c0; t < c1; ++t) r += (uint32_t)(idxs[t] < idx[u]);
+                ord[c0 + r] = (uint32_t)q;
+            }
+        }
+        __syncthreads();
+        for (int t = tid; t < cnt; t += KB_THREADS) kb_write_slot(c, base + t, bkt[base + ord[t]]);
+        return;
+    }
+    uint32_t* cur = lds + (1 << pl.B) + 1;                // [2^B]
+    for (int q = tid; q < cnt; q += KB_THREADS) atomicAdd(&cst[bkt[base + q].key - k0], 1u);
+    __syncthreads();
+    kb_scan_lds(cst, nk, wsum);
+    kb_write_starts(cst, nk, k0, base, c.cstart, c.ccount);
+    for (int q = tid; q < nk; q += KB_THREADS) cur[q] = cst[q];
+    __syncthreads();
+    for (int q = tid; q < cnt; q += KB_THREADS) {
+        const uint32_t pos = atomicAdd(&cur[bkt[base + q].key - k0], 1u);
+        c.perm[base + pos] = (int)(base + q);
+    }
+    __syncthreads();
+    for (int q = tid; q < nk; q += KB_THREADS) {
+        const uint32_t b = base + cst[q], e = base + cst[q + 1];
+        for (uint32_t s = b + 1; s < e; ++s) {
+            const int v = c.perm[s];
+            const int vi = bkt[v].idx;
+            uint32_t t = s;
+            while (t > b && bkt[c.perm[t - 1]].idx > vi) {
+                c.perm[t] = c.perm[t - 1];
+                --t;
+            }
+            c.perm[t] = v;
+        }
+    }
+    __syncthreads();
+    for (int q = tid; q < cnt; q += KB_THREADS) {
+        const uint32_t slot = base + (uint32_t)q;
+        kb_write_slot(c, slot, bkt[c.perm[slot]]);
+    }
+}
+
+static hipError_t build_cells_single_level(const GdtSet* d_sets, const GdtChunk* pchunks, int npch,
+                                           const GdtChunk* kchunks, int nkch, int total_keys,
+                                           const gdt_d2* xy, GdtCells& c, hipStream_t s)
 {
     hipError_t e = gdt_fill_u32(c.ccount, 0u, (size_t)total_keys, s);
     if (e != hipSuccess) return e;
-    const gdt_d2* xy = reinterpret_cast<const gdt_d2*>(d_xy);
     hipLaunchKernelGGL(k_bin, dim3(npch), dim3(GDT_BLOCK), 0, s, d_sets, pchunks, xy, c.pkey,
                        c.prank, c.ccount);
     if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -248,7 +540,30 @@ hipError_t gdt_build_cells(const GdtSet* d_sets, const GdtChunk* pchunks, int np
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_finalize, dim3(nkch), dim3(GDT_BLOCK), 0, s, d_sets, kchunks, c.cstart,
                        xy, c.perm, c.slot_of, c.off64, c.off32, c.scell);
-    (void)total_n;
+    return hipGetLastError();
+}
+
+hipError_t gdt_build_cells(const GdtSet* d_sets, int nsets, const GdtChunk* pchunks, int npch,
+                           const GdtChunk* kchunks, int nkch, int total_n, int total_keys,
+                           const double* d_xy, GdtCells& c, hipStream_t s)
+{
+    const gdt_d2* xy = reinterpret_cast<const gdt_d2*>(d_xy);
+    const KbPlan pl = kb_plan((size_t)total_n, (size_t)total_keys);
+    if (pl.B > KB_MAXSHIFT)
+        return build_cells_single_level(d_sets, pchunks, npch, kchunks, nkch, total_keys, xy, c, s);
+    hipLaunchKernelGGL(k_kb_count, dim3(pl.nb), dim3(KB_THREADS), pl.nkb * sizeof(uint32_t), s,
+                       d_sets, nsets, xy, total_n, pl, c.bmat);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_kb_rowscan, dim3(pl.nkb), dim3(KB_NB), 0, s, pl, c.bmat, c.bscan);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_kb_scatter, dim3(pl.nb), dim3(KB_THREADS), (pl.nkb + 1) * sizeof(uint32_t),
+                       s, d_sets, nsets, xy, total_n, pl, c.bscan, c.bkt);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const size_t cells = ((size_t)1 << pl.B) + 1;
+    hipLaunchKernelGGL(k_kb_finalize, dim3(pl.nkb), dim3(KB_THREADS),
+                       (cells + 2 * KB_CAP) * sizeof(uint32_t), s, pl, total_keys, total_n, c.bscan,
+                       c.bkt, c);
     return hipGetLastError();
 }
 

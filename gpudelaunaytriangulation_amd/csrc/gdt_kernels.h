@@ -4,10 +4,19 @@
 #include <hip/hip_runtime.h>
 #include "gdt_point.h"
 
+// One point of the cell-list build in bucket order (one 32-byte store).
+struct GdtKbRec {
+    gdt_d2 off;         // offset inside the cell
+    int idx;            // caller index
+    uint32_t key;       // cell key
+    uint32_t cell;      // cx | cy << 16
+    uint32_t pad;
+};
+
 // Arrays of the cell list (cell order), owned by the context.
 struct GdtCells {
-    uint32_t* pkey;     // [N] cell key of each caller-order point
-    uint32_t* prank;    // [N] arrival rank inside its cell
+    uint32_t* pkey;     // [N] cell key of each caller-order point (single-level path)
+    uint32_t* prank;    // [N] arrival rank inside its cell (single-level path)
     uint32_t* ccount;   // [K] points per key
     uint32_t* cstart;   // [K+1] exclusive scan of ccount
     int* perm;          // [N] slot -> caller index
@@ -15,8 +24,14 @@ struct GdtCells {
     gdt_d2* off64;      // [N]
     gdt_f2* off32;      // [N]
     uint32_t* scell;    // [N]
-    uint32_t* scan_tmp; // scan scratch (gdt_scan_tmp_words(K) words)
+    uint32_t* scan_tmp; // scan scratch (gdt_scan_tmp_words(max(K, N+1)) words)
+    GdtKbRec* bkt;      // [N] the points in bucket order
+    uint32_t* bmat;     // [gdt_bucket_words(N, K)] per (bucket, block) counts
+    uint32_t* bscan;    // [gdt_bucket_words(N, K)] their exclusive scan
 };
+
+// Words of the (bucket, block) count table of the cell-list build.
+size_t gdt_bucket_words(size_t n, size_t k);
 
 size_t gdt_scan_tmp_words(size_t n);
 // out[0..n) = exclusive prefix sums of in, *total = sum (device pointer).
@@ -27,7 +42,7 @@ hipError_t gdt_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 hipError_t gdt_fill_i32(int* p, int v, size_t n, hipStream_t s);
 
 // Cell list over all sets.  pchunks: blocks of 256 points, kchunks: blocks of 256 keys.
-hipError_t gdt_build_cells(const GdtSet* d_sets, const GdtChunk* pchunks, int npch,
+hipError_t gdt_build_cells(const GdtSet* d_sets, int nsets, const GdtChunk* pchunks, int npch,
                            const GdtChunk* kchunks, int nkch, int total_n, int total_keys,
                            const double* d_xy, GdtCells& c, hipStream_t s);
 
