@@ -478,52 +478,53 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
     const float ma = GDT_FMAXF(fabsf(ax), fabsf(ay));
     const float cE = 2.f * (2.f * eps1 + 3.f * u);
     int bk = -1;
-    float bd2 = 0.f, bE = 0.f;
-    float pab = 0.f, sab = 0.f, BP = 0.f;
+    float bd2 = 0.f, mbb = 0.f, BP = 0.f;
     // Both searches as one test on (O, P), as in the FP64 path (nearest
     // neighbour: a = 0 so O = 0, K = (1,0,0) so P = |c|^2; successor before a
     // best exists: K = (-1,0,0) so P = -|c|^2; a new best sets K0 = O + nnf).
     // A candidate is certainly a new best when O > Oc and P < Pc, possibly one
     // (the FP64 expression of the FP64 path decides) when O >= Op and P <= Pp,
-    // and certainly not otherwise.
-    const int ex = wrap ? ak : iv;
+    // and certainly not otherwise.  v and a are excluded by entry: v's O = 0
+    // lies inside the orientation bound, so it would reach the FP64 decision.
     const float nnf = wrap ? 0.f : 1.f;
     const float aa = fmaf(ax, ax, ay * ay);
     float K0 = wrap ? -1.f : 1.f, Kx = 0.f, Ky = 0.f;
     float Oc = -1.f, Op = -1.f, Pc = wrap ? 0.f : 3.0e38f, Pp = Pc;
     // Bounds that depend on mc, a bound on the max-norm of the displacements
     // of the region being scanned: (h+1) cells for offsets |dx|,|dy| <= h.
-    float mc = 0.f, cPmc = 0.f, BO = 0.f, eabs = 0.f;
-    // nearest neighbour: |c|^2 + E(c) < bd2 - bE certain, |c|^2 - E(c) > bd2 + bE
-    // certainly not, with E(c) = cE |c|^2 + eabs.  As thresholds on |c|^2:
-    // Pc = (D - 8u S)(1 - cE) with D = bd2 - bE - eabs, S = bd2 + bE + eabs
-    // (the rounding of D is below 2.01u S, the rest below 2u S, and
-    // 1 - cE <= 1/(1 + cE), so Pc <= the exact D / (1 + cE) when positive, and
-    // negative otherwise); Pp = S (1 + 8u)(1 + 2 cE) (four roundings of at most
-    // u each, and 1 + 2 cE >= 1/(1 - cE), so Pp > the exact S / (1 - cE)).
-    const float cnn_c = 1.f - cE, cnn_p = (1.f + 8.f * u) * (1.f + 2.f * cE);
+    float mc = 0.f, tA = 0.f, qa = 0.f, BO = 0.f, eabs = 0.f, e2 = 0.f;
+    // nearest neighbour: |c|^2 + E(c) < bd2 - E(b) certain, |c|^2 - E(c) >
+    // bd2 + E(b) certainly not, with E(x) = cE |x|^2 + eabs (eabs of the
+    // current region, which bounds that of the region b was found in).  As
+    // thresholds on |c|^2, each one rounding away from the exact one:
+    //   Pc = bd2 (1 - 3cE) - e2 <= (bd2 (1 - cE) - 2 eabs) / (1 + cE),
+    //   Pp = bd2 (1 + 3cE) + e2 >= (bd2 (1 + cE) + 2 eabs) / (1 - cE),
+    // with e2 = 2.0001 eabs; 1 +- 3cE are exact in FP32 (cE = 26u).
+    const float nn_c = 1.f - 3.f * cE, nn_p = 1.f + 3.f * cE;
     auto set_nn = [&]() {
-        const float Sx = (bd2 + bE) + eabs;
-        Pc = fmaf(-8.f * u, Sx, (bd2 - bE) - eabs) * cnn_c;
-        Pp = Sx * cnn_p;
+        Pc = fmaf(bd2, nn_c, -e2);
+        Pp = fmaf(bd2, nn_p, e2);
     };
-    // in-circle bound of the current best b (sab = ma + mb, pab = ma mb):
-    //   BP = eP (sab + mc)^3 + cP pab mc (sab + mc)
-    auto set_bp = [&]() {
-        const float t = sab + mc;
-        BP = fmaf(t, fmaf(cPmc, pab, (eP * t) * t), 1e-30f);
+    // in-circle bound of the current best b, mb = |b|_inf (with sab = ma + mb,
+    // pab = ma mb):  BP = eP (sab + mc)^3 + cP pab mc (sab + mc)
+    //                   = t (eP t^2 + (cP ma mc) mb),  t = (ma + mc) + mb
+    auto set_bp = [&](float mb) {
+        const float t = tA + mb;
+        BP = fmaf(t, fmaf(qa, mb, (eP * t) * t), 1e-30f);
         Pc = -BP;
         Pp = BP;
     };
     auto set_region = [&](int h) {
         mc = (float)(h + 1) * bsM * 1.0001f;
-        cPmc = cP * mc;
+        tA = ma + mc;
+        qa = cP * mc * ma;
         BO = fmaf(4.f * (2.f * eps1 + 2.f * u) * ma + 4.f * eps0, mc, 4.f * eps0 * ma + 1e-30f);
         eabs = 8.f * eps0 * mc + 4.f * eps0 * eps0 + 1e-30f;
+        e2 = 2.0001f * eabs;
         if (wrap) {
             Oc = BO;
             Op = -BO;
-            if (bk >= 0) set_bp();
+            if (bk >= 0) set_bp(mbb);
         } else if (bk >= 0) {
             set_nn();
         }
@@ -543,7 +544,7 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
         const float O = fmaf(ax, py, -(ay * px));
         const float P = fmaf(K0, cc, fmaf(Kx, px, Ky * py));
         // one branch, taken only when the candidate may be a new best
-        if ((O >= Op) & (P <= Pp) & (k != ex)) {
+        if ((O >= Op) & (P <= Pp) & (k != iv) & (k != ak)) {
             bool take = (O > Oc) & (P < Pc);
             if (!take) {                      // undecided: the FP64 path's decision
                 ++nfb;
@@ -572,15 +573,12 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
             if (take) {
                 bk = k;
                 bd2 = cc;
-                bE = fmaf(cE, cc, eabs);
                 K0 = O + nnf;                        // fmaf(ax, py, -(ay px)), or 1
                 Kx = fmaf(cc, ay, -(aa * py));       // 0 when a = 0
                 Ky = fmaf(aa, px, -(cc * ax));       // 0 when a = 0
                 if (wrap) {
-                    const float mb = GDT_FMAXF(fabsf(px), fabsf(py));
-                    sab = ma + mb;
-                    pab = ma * mb;
-                    set_bp();
+                    mbb = GDT_FMAXF(fabsf(px), fabsf(py));
+                    set_bp(mbb);
                 } else {
                     set_nn();
                 }
@@ -607,6 +605,7 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
         ok = boxed = false;
         if (!wrap) {
             if (bk >= 0) {
+                const float bE = fmaf(cE, bd2, eabs);
                 ok = cov > 0.f && bd2 + bE <= cov * cov * 0.9999f;
                 const float rad = sqrtf(bd2 + bE) * 1.0001f;
                 gdt_circle_cells<float>(0.f, 0.f, 0.f, 0.f, rad, ox, oy, S.bsx32, S.bsy32, 1e-2f,
@@ -614,6 +613,7 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
                 boxed = true;
             }
         } else if (bk >= 0) {
+            const float sab = ma + mbb, pab = ma * mbb;
             const float EK0 = 4.f * (2.f * eps1 + 2.f * u) * pab + 4.f * eps0 * sab;
             const float EK = 4.f * (3.f * eps1 + 6.f * u) * pab * sab + 8.f * eps0 * sab * sab;
             const float Klo = K0 - EK0, Khi = K0 + EK0;

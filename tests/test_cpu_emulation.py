@@ -83,6 +83,11 @@ def test_emulated_kernels_match_oracle(emul, tmp_path, case):
             assert rings.ring_hash(nb, c) == ref, (prec, tiled, mode)
             if prec == 64:
                 assert nfb == 0
+            elif case == "uniform4096" and mode == 1:
+                # the LDS routine's filter decides all but a few per mille in
+                # FP32 (about 1.4e-3 of the predicates on uniform input); v
+                # and a are excluded by entry, not by their orientation
+                assert nfb < 0.005 * npred, (tiled, nfb, npred)
             outs[(prec, tiled, mode)] = (nb, c)
         # identical decisions: FP32-filtered rows equal FP64 rows byte for byte
         n64, c64 = outs[(64, tiled, mode)]
