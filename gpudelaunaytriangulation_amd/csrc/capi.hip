@@ -244,7 +244,7 @@ static void record_ring_stats(gdt_context* c)
     c->stats.fallbacks = c->h_counters[CT_FALLBACK];
     c->stats.neighbours = c->h_counters[CT_NBRSUM];
     c->stats.max_ring = c->h_status[ST_REQUIRED];
-    c->stats.deferred = c->h_status[ST_NDEFER];
+    c->stats.deferred = c->h_status[ST_NDEFER] + c->h_status[ST_NDEFER_TILE];
     if (!c->sets.empty()) {
         c->stats.xsize = c->sets[0].xsize;
         c->stats.ysize = c->sets[0].ysize;

@@ -33,7 +33,8 @@ enum {
     ST_ERRFLAGS = 1,     // bit 0: search radius exhausted, bit 1: ring did not close, bit 2: no neighbour
     ST_NFLAGGED = 2,     // number of points flagged by the circumcircle test
     ST_NTRI = 3,         // number of triangles enumerated
-    ST_NDEFER = 4,       // points the tile kernel deferred to the direct routine
+    ST_NDEFER = 4,       // points the tile kernel deferred to the wavefront kernel
+    ST_NDEFER_TILE = 5,  // deferred points a tile block finished itself with a wavefront
     ST_WORDS = 8
 };
 // ERR_DEFER is internal: the LDS routine hands the point to the direct routine.

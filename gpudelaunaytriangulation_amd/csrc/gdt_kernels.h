@@ -62,6 +62,8 @@ hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
 // blocks of 256 threads), the rest one lane each.
 #define GDT_WAVE_CAP 8192
 #define GDT_WAVE_BLOCKS 512
+// Deferred points a tile block finishes itself (the rest go to the list).
+#define GDT_TILE_DEFER 64
 
 // One-ring of every point, one block per GDT_TILE x GDT_TILE-cell tile (tiled
 // cell order).

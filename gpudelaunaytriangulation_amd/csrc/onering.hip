@@ -193,6 +193,8 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
     __shared__ int s_orig[GDT_WIN_MAXPTS];
     __shared__ uint32_t s_wsum[16];
     __shared__ int s_next;
+    __shared__ int s_dlist[GDT_TILE_DEFER];
+    __shared__ uint32_t s_dn;
 
     const GdtChunk ch = tiles[blockIdx.x];
     const GdtSet S = sets[ch.set];
@@ -218,7 +220,10 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
     // Lanes take points of the tile and advance them one search step per
     // iteration; a lane whose point is finished takes the next one, so a
     // wave's iterations track the total number of steps, not the longest ring.
-    if (tid == 0) s_next = Q0 + nt;
+    if (tid == 0) {
+        s_next = Q0 + nt;
+        s_dn = 0u;
+    }
     __syncthreads();
     using Real = typename std::conditional<PREC == 32, float, double>::type;
     GdtTileLane<Real> st;
@@ -237,8 +242,14 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
                 rc = gdt_tile_step_f64(S, L, io, st, npred);
             if (rc != STEP_RUN) {
                 if (rc == STEP_DEFER) {
-                    const uint32_t pos = atomicAdd(reinterpret_cast<unsigned int*>(&status[ST_NDEFER]), 1u);
-                    defer[pos] = j;
+                    // the block's own waves finish it after the tile (below)
+                    const uint32_t dpos = atomicAdd(&s_dn, 1u);
+                    if (dpos < GDT_TILE_DEFER) {
+                        s_dlist[dpos] = j;
+                    } else {
+                        const uint32_t pos = atomicAdd(reinterpret_cast<unsigned int*>(&status[ST_NDEFER]), 1u);
+                        defer[pos] = j;
+                    }
                 } else {
                     counts[L.orig[st.iv]] = st.count;
                     mx = max(mx, st.count);
@@ -251,6 +262,27 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
                     j = list ? list[q] : q;
                     gdt_tile_lane_init<Real>(w, L, j, io, S, st);
                 }
+            }
+        }
+    }
+    // Points whose circle left the window: one wavefront each, read from
+    // global memory (gdt_wave_ring), by this block's waves while other blocks
+    // still run their tiles.
+    __syncthreads();
+    const int ndl = min((int)s_dn, GDT_TILE_DEFER);
+    if (ndl > 0) {
+        if (tid == 0) atomicAdd(&status[ST_NDEFER_TILE], ndl);
+        GdtWaveDev wv;
+        wv.lane = tid & 63;
+        for (int t = tid >> 6; t < ndl; t += nt >> 6) {
+            const int jd = s_dlist[t];
+            const GdtRingResult r = gdt_wave_ring(S, jd, io, wv, wv.lane == 0);
+            npred += r.npred;
+            if (wv.lane == 0) {
+                counts[io.perm[jd]] = r.count;
+                mx = max(mx, r.count);
+                er |= r.err;
+                nsum += (unsigned)r.count;
             }
         }
     }
