@@ -80,6 +80,8 @@ struct gdt_context {
     hipStream_t md_stream = nullptr;
     int* h_status = nullptr;
     unsigned long long* h_counters = nullptr;
+    int* hd_status = nullptr;                    // device-side pointers of the two
+    unsigned long long* hd_counters = nullptr;   // pinned buffers above
     gdt_stats stats;
 
     GdtCells cells()
@@ -217,16 +219,14 @@ static int batch_sets(gdt_context* c, int nsets, const int* offsets, const doubl
 
 static int reset_status(gdt_context* c, hipStream_t s)
 {
-    GDT_CHECK(gdt_fill_i32(c->status.p, 0, ST_WORDS, s));
-    GDT_CHECK(gdt_fill_u32(reinterpret_cast<uint32_t*>(c->counters.p), 0u, 2 * CT_WORDS, s));
+    GDT_CHECK(gdt_status_reset(c->status.p, ST_WORDS, c->counters.p, CT_WORDS, s));
     return GDT_OK;
 }
 
 static int read_status(gdt_context* c, hipStream_t s)
 {
-    GDT_CHECK(hipMemcpyAsync(c->h_status, c->status.p, ST_WORDS * sizeof(int), hipMemcpyDeviceToHost, s));
-    GDT_CHECK(hipMemcpyAsync(c->h_counters, c->counters.p, CT_WORDS * sizeof(unsigned long long),
-                             hipMemcpyDeviceToHost, s));
+    GDT_CHECK(gdt_status_publish(c->status.p, ST_WORDS, c->counters.p, CT_WORDS, c->hd_status,
+                                 c->hd_counters, s));
     GDT_CHECK(hipStreamSynchronize(s));
     return GDT_OK;
 }
@@ -311,6 +311,8 @@ int gdt_create(gdt_context** out, int device)
     if (c->status.reserve(ST_WORDS) != hipSuccess || c->counters.reserve(CT_WORDS) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void**>(&c->h_status), ST_WORDS * sizeof(int), 0) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void**>(&c->h_counters), CT_WORDS * sizeof(unsigned long long), 0) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&c->hd_status), c->h_status, 0) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&c->hd_counters), c->h_counters, 0) != hipSuccess ||
         hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess ||
         hipEventCreate(&c->ev[2]) != hipSuccess || hipEventCreate(&c->ev[3]) != hipSuccess) {
         gdt_destroy(c);

@@ -44,6 +44,38 @@ __global__ void k_fill_i32(int* __restrict__ p, int v, size_t n)
         p[i] = v;
 }
 
+// Status words and counters: zeroed, and copied to pinned host memory, by one
+// small kernel each (one launch instead of two fills or two copies).
+__global__ void k_status_reset(int* __restrict__ st, int nst, unsigned long long* __restrict__ ct,
+                               int nct)
+{
+    const int t = threadIdx.x;
+    if (t < nst) st[t] = 0;
+    if (t < nct) ct[t] = 0ull;
+}
+
+__global__ void k_status_publish(const int* __restrict__ st, int nst,
+                                 const unsigned long long* __restrict__ ct, int nct,
+                                 int* __restrict__ hst, unsigned long long* __restrict__ hct)
+{
+    const int t = threadIdx.x;
+    if (t < nst) hst[t] = st[t];
+    if (t < nct) hct[t] = ct[t];
+}
+
+hipError_t gdt_status_reset(int* st, int nst, unsigned long long* ct, int nct, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(64), 0, s, st, nst, ct, nct);
+    return hipGetLastError();
+}
+
+hipError_t gdt_status_publish(const int* st, int nst, const unsigned long long* ct, int nct,
+                              int* hst, unsigned long long* hct, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_status_publish, dim3(1), dim3(64), 0, s, st, nst, ct, nct, hst, hct);
+    return hipGetLastError();
+}
+
 static int fill_grid(size_t n)
 {
     size_t b = (n + GDT_BLOCK - 1) / GDT_BLOCK;
