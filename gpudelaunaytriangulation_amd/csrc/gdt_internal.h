@@ -56,8 +56,11 @@ struct GdtSet {
     int pad0;
     double lx, ly;       // box
     double bsx, bsy;     // cell sides
+    double ibsx, ibsy;   // their reciprocals
     float bsx32, bsy32;
     float bsmax32, bsmin32;
+    float ibsx32, ibsy32;
+    float pad1, pad2;
 };
 
 struct GdtChunk { int set; int start; };   // a block's work: set id and first local item
@@ -187,8 +190,13 @@ inline GdtSet gdt_make_set(int p0, int n, int c0, double lx, double ly, double c
     s.lx = lx; s.ly = ly;
     s.bsx = lx / s.xsize;
     s.bsy = ly / s.ysize;
+    s.ibsx = 1.0 / s.bsx;
+    s.ibsy = 1.0 / s.bsy;
     s.bsx32 = (float)s.bsx;
     s.bsy32 = (float)s.bsy;
+    s.ibsx32 = (float)s.ibsx;
+    s.ibsy32 = (float)s.ibsy;
+    s.pad1 = s.pad2 = 0.f;
     s.bsmax32 = (float)(s.bsx > s.bsy ? s.bsx : s.bsy);
     s.bsmin32 = (float)(s.bsx < s.bsy ? s.bsx : s.bsy);
     return s;

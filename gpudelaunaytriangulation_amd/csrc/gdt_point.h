@@ -347,15 +347,58 @@ GDT_HD inline GdtRingResult gdt_one_ring_f32(const GdtSet& S, int j, const GdtRi
     return res;
 }
 
+// Reciprocal and square root for the certified boxes and the scan clipping:
+// a few ulp of error on the device (hardware approximation; one Newton step
+// for the FP64 reciprocal), far below the margins those boxes carry.  The
+// host build (CPU emulation) uses the correctly rounded operations.
+GDT_HD inline double gdt_rcp64(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+#else
+    return 1.0 / x;
+#endif
+}
+GDT_HD inline double gdt_sqrt64(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrt(x);
+#else
+    return std::sqrt(x);
+#endif
+}
+GDT_HD inline float gdt_rcp32(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.f / x;
+#endif
+}
+GDT_HD inline float gdt_sqrt32(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return std::sqrt(x);
+#endif
+}
+
 // Cell-offset bounding box of the circle through the origin, a and b, given
 // centre bounds [qxlo,qxhi] x [qylo,qyhi] and radius bound R, for a point whose
-// offset inside its cell is (ox,oy); `margin` is in cells.
-template <class T>
-GDT_HD inline void gdt_circle_cells(T qxlo, T qxhi, T qylo, T qyhi, T R, T ox, T oy, T bsx, T bsy,
-                                    T margin, int& dx0, int& dx1, int& dy0, int& dy1)
+// offset inside its cell is (ox,oy); ibsx, ibsy are the reciprocal cell sides
+// and `margin` is in cells.
+GDT_HD inline int gdt_cell_floor(double x)
 {
-    dx0 = (int)floor((double)((ox + (qxlo - R)) / bsx - margin));
-    dx1 = (int)floor((double)((ox + (qxhi + R)) / bsx + margin));
-    dy0 = (int)floor((double)((oy + (qylo - R)) / bsy - margin));
-    dy1 = (int)floor((double)((oy + (qyhi + R)) / bsy + margin));
+    return (int)floor(GDT_FMIN(GDT_FMAX(x, -1e9), 1e9));
+}
+template <class T>
+GDT_HD inline void gdt_circle_cells(T qxlo, T qxhi, T qylo, T qyhi, T R, T ox, T oy, T ibsx,
+                                    T ibsy, T margin, int& dx0, int& dx1, int& dy0, int& dy1)
+{
+    dx0 = gdt_cell_floor((double)((ox + (qxlo - R)) * ibsx - margin));
+    dx1 = gdt_cell_floor((double)((ox + (qxhi + R)) * ibsx + margin));
+    dy0 = gdt_cell_floor((double)((oy + (qylo - R)) * ibsy - margin));
+    dy1 = gdt_cell_floor((double)((oy + (qyhi + R)) * ibsy + margin));
 }

@@ -26,6 +26,9 @@
 #define GDT_SCAN0 2         // half-width (cells) of the block every search step scans first
 #endif
 #define GDT_SCAN 3          // half-width of the block scanned when the first finds nothing
+#ifndef GDT_SCAN_NN
+#define GDT_SCAN_NN 1       // half-width of the nearest-neighbour search's first block
+#endif
 #ifndef GDT_HALO
 #define GDT_HALO 4          // window halo (cells): reach of the circle-box extension
 #endif
@@ -35,8 +38,9 @@
 #define GDT_WIN_MAXPTS 2048 // LDS entries per window (a fuller window defers its tile)
 #endif
 #define GDT_FRAME (GDT_WIN / 2)
-static_assert(GDT_SCAN0 <= GDT_SCAN && GDT_SCAN < GDT_HALO,
-              "the scan blocks and the row read ahead of them lie in the window");
+static_assert(GDT_SCAN_NN <= GDT_SCAN0 && GDT_SCAN0 <= 2 && GDT_SCAN0 <= GDT_SCAN &&
+                  GDT_SCAN < GDT_HALO,
+              "first blocks fit a 5-row table; the scan blocks lie in the window");
 
 struct GdtTileWin {
     int tx, ty;          // tile coordinates
@@ -146,89 +150,90 @@ GDT_HD inline unsigned gdt_lds_block(const GdtTileLds& L, int wx, int wy, int x0
     }
 }
 
-// Cursor over the LDS entries of the block [-h,h]^2 around window cell c0:
-// its rows are contiguous LDS ranges, walked as one sequence.
-struct GdtBlockCursor {
-    int c, cend, h;
-    uint32_t k, e;
+// Columns of the scan block that may hold points strictly left of v->a.  A
+// point of cell (dx, dy) has displacement x in [dx bsx - ox, (dx+1) bsx - ox),
+// y likewise; it is left when ax y - ay x > 0, i.e. x < (ax/ay) y for ay > 0
+// and x > (ax/ay) y for ay < 0.  Over a row of cells the bound is linear in
+// dy, so row dy keeps the columns dx <= floor(s dy + c) (mode 1) or
+// dx >= floor(s dy + c) (mode 2), with a margin of m cells; mode 0 keeps whole
+// rows (nearest-neighbour search, or a nearly horizontal v->a, where the float
+// slope would be too coarse).
+struct GdtClip {
+    float s, c;
+    int mode;
 };
 
-GDT_HD inline GdtBlockCursor gdt_block_begin(const GdtTileLds& L, int c0, int h)
+GDT_HD inline GdtClip gdt_left_clip(bool wrap, float ax, float ay, float ox, float oy, float bsx,
+                                    float bsy, float m)
 {
-    GdtBlockCursor it;
-    it.h = h;
-    it.c = c0 - h * GDT_WIN;
-    it.cend = c0 + (h + 1) * GDT_WIN;
-    it.k = L.cs[it.c - h];
-    it.e = L.cs[it.c + h + 1];
-    return it;
-}
-
-// Next entry of the block, or -1 when it is exhausted.
-GDT_HD inline int gdt_block_next(const GdtTileLds& L, GdtBlockCursor& it)
-{
-    while (it.k == it.e) {
-        it.c += GDT_WIN;
-        if (it.c == it.cend) return -1;
-        it.k = L.cs[it.c - it.h];
-        it.e = L.cs[it.c + it.h + 1];
+    GdtClip cl;
+    cl.s = cl.c = 0.f;
+    cl.mode = 0;
+    if (!wrap || !(fabsf(ay) > 0.01f * fabsf(ax))) return cl;
+    const float r = ax * gdt_rcp32(ay);           // |r| < 100
+    const float span = r * bsy, ib = gdt_rcp32(bsx);
+    cl.s = span * ib;
+    if (ay > 0.f) {
+        cl.mode = 1;
+        cl.c = (fmaf(-r, oy, span > 0.f ? span : 0.f) + ox) * ib + m;
+    } else {
+        cl.mode = 2;
+        cl.c = (fmaf(-r, oy, span < 0.f ? span : 0.f) + ox) * ib - m;
     }
-    return (int)it.k++;
+    return cl;
 }
 
-// The block as one flat loop (a wave waits for its slowest lane once per step,
-// not once per row), software-pipelined: the coordinates of the next entry are
-// loaded, load(k), before the current one is evaluated, fn(k, coords), so the
-// LDS latency overlaps the predicate arithmetic.  Returns the entries visited.
-template <class Load, class Fn>
-GDT_HD inline unsigned gdt_scan_block(const GdtTileLds& L, int c0, int h, Load& load, Fn& fn)
+// Row table of the block [-h,h]^2 around window cell c0 (h <= 2), each row
+// clipped to the columns cl keeps: tbl[r] = first LDS entry | end << 16.
+// Returns the number of entries.
+GDT_HD inline unsigned gdt_block_rows(const GdtTileLds& L, int c0, int h, const GdtClip& cl,
+                                      uint32_t* tbl)
 {
-    GdtBlockCursor it = gdt_block_begin(L, c0, h);
-    int k = gdt_block_next(L, it);
-    if (k < 0) return 0u;
-    auto cur = load(k);
     unsigned n = 0;
-    for (;;) {
-        const int kn = gdt_block_next(L, it);
-        const auto nxt = load(kn >= 0 ? kn : k);
-        fn(k, cur);
-        ++n;
-        if (kn < 0) break;
-        k = kn;
-        cur = nxt;
+    for (int r = 0; r <= 2 * h; ++r) {
+        const int dy = r - h;
+        int lo = -h, hi = h;
+        if (cl.mode) {
+            float X = fmaf(cl.s, (float)dy, cl.c);
+            X = GDT_FMINF(GDT_FMAXF(X, -(float)h - 1.f), (float)h + 1.f);
+            const int bnd = (int)floorf(X);
+            if (cl.mode == 1) hi = bnd < hi ? bnd : hi;
+            else lo = bnd > lo ? bnd : lo;
+        }
+        const int rb = c0 + dy * GDT_WIN;
+        uint32_t k = 0, e = 0;
+        if (lo <= hi) {
+            k = L.cs[rb + lo];
+            e = L.cs[rb + hi + 1];
+        }
+        tbl[r] = k | (e << 16);
+        n += e - k;
     }
     return n;
 }
 
-// The block [-h,h]^2 around window cell c0 (1 <= h < GDT_HALO) as one flat
-// loop of n iterations, n counted first from the row ranges.  The range of the
-// row after the current one is read ahead, so a row switch -- which some lane
-// of a wave takes on most iterations -- is two moves and one LDS read, with no
-// wait; and the coordinates of the next entry are loaded, load(k), before the
-// current one is evaluated, fn(k, coords), so that the LDS latency overlaps
-// the predicate arithmetic.  Rows beyond the block are read ahead but never
-// used; row h+1 still lies in the window.  Returns n.
+// The n entries of a row table as one flat loop.  The next row's word is read
+// one row ahead, so a row switch -- which some lane of a wave takes on most
+// iterations -- is a few ALU operations and one LDS read, with no wait; two
+// entries go per trip, the next entry's coordinates loaded, load(k), before
+// the current one is evaluated, fn(k, coords).  The word after the last row
+// is read ahead but never used (the tables are laid out so that it exists).
 template <class Load, class Fn>
-GDT_HD inline unsigned gdt_scan_rows(const GdtTileLds& L, int c0, int h, Load& load, Fn& fn)
+GDT_HD inline void gdt_scan_tbl(const uint32_t* tbl, unsigned n, Load& load, Fn& fn)
 {
-    const int w = 2 * h + 1;
-    const uint32_t* p = L.cs + (c0 - h * GDT_WIN - h);      // first cell of the current row
-    unsigned n = 0;
-    for (int r = 0; r < w; ++r) n += p[r * GDT_WIN + w] - p[r * GDT_WIN];
-    if (n == 0) return 0;
-    uint32_t k = p[0], e = p[w];
-    uint32_t kn = p[GDT_WIN], en = p[GDT_WIN + w];
+    if (n == 0) return;
+    const uint32_t* p = tbl;
+    uint32_t k = p[0] & 0xffffu, e = p[0] >> 16;
+    uint32_t wn = p[1];
     auto next_row = [&]() {     // the next non-empty row (one exists while entries remain)
         do {
-            p += GDT_WIN;
-            k = kn;
-            e = en;
-            kn = p[GDT_WIN];
-            en = p[GDT_WIN + w];
+            ++p;
+            k = wn & 0xffffu;
+            e = wn >> 16;
+            wn = p[1];
         } while (k == e);
     };
     if (k == e) next_row();
-    // two entries per trip, so that the two coordinate registers alternate
     auto XA = load((int)k);
     decltype(XA) XB;
     for (unsigned i = 1;; ++i) {
@@ -246,7 +251,6 @@ GDT_HD inline unsigned gdt_scan_rows(const GdtTileLds& L, int c0, int h, Load& l
         fn((int)kc, XB);
         if (!more) break;
     }
-    return n;
 }
 
 // Whether the cell block [dx0,dx1] x [dy0,dy1] around window cell c0 lies in
@@ -290,6 +294,7 @@ struct GdtTileLane {
     Real ax, ay;      // displacement of a
     int count;        // ring entries emitted
     int* row;
+    uint32_t* tbl;    // the lane's row table: 5 words, and a sixth readable one after them
 };
 
 template <class Real>
@@ -414,14 +419,20 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
             bd2 = gdt_dist2_64(bx, by);
         }
     };
-    int hs = GDT_SCAN0;
-    npred += gdt_scan_rows(L, st.c0, hs, load, visit);
+    // nearest neighbour: the 3x3 block; successor: the 5x5 block, clipped to
+    // the columns that may hold points left of v->a
+    int hs = wrap ? GDT_SCAN0 : GDT_SCAN_NN;
+    const double ox = gdt_cell_offset(vx, wx, S.bsx), oy = gdt_cell_offset(vy, wy, S.bsy);
+    const GdtClip cl = gdt_left_clip(wrap, (float)ax, (float)ay, (float)ox, (float)oy,
+                                     (float)S.bsx, (float)S.bsy, 1e-3f);
+    const unsigned nb = gdt_block_rows(L, st.c0, hs, cl, st.tbl);
+    gdt_scan_tbl(st.tbl, nb, load, visit);
+    npred += nb;
     if (bk < 0 && hs < GDT_SCAN) {
         npred += gdt_lds_block(L, wx, wy, -GDT_SCAN, GDT_SCAN, -GDT_SCAN, GDT_SCAN, hs, visit_k);
         hs = GDT_SCAN;
     }
 
-    const double ox = gdt_cell_offset(vx, wx, S.bsx), oy = gdt_cell_offset(vy, wy, S.bsy);
     bool ok = false, boxed = false;
     int dx0 = 0, dx1 = 0, dy0 = 0, dy1 = 0;
     // certification against the scanned block [-hs,hs]^2 (sets the box too)
@@ -432,16 +443,17 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
         if (!wrap) {
             if (bk >= 0) {
                 ok = cov > 0. && bd2 <= cov * cov * (1.0 - 1e-12);
-                const double rad = sqrt(bd2) * (1.0 + 1e-9);
-                gdt_circle_cells<double>(0., 0., 0., 0., rad, ox, oy, S.bsx, S.bsy, 1e-3, dx0, dx1,
+                const double rad = gdt_sqrt64(bd2) * (1.0 + 1e-9);
+                gdt_circle_cells<double>(0., 0., 0., 0., rad, ox, oy, S.ibsx, S.ibsy, 1e-3, dx0, dx1,
                                          dy0, dy1);
                 boxed = true;
             }
         } else if (bk >= 0 && K.k0 > 1e-6 * (fabs(ax * by) + fabs(ay * bx))) {
-            const double qx = -K.kx / (2. * K.k0), qy = -K.ky / (2. * K.k0);
-            const double rad = sqrt(qx * qx + qy * qy);
+            const double i2k = gdt_rcp64(2. * K.k0);
+            const double qx = -K.kx * i2k, qy = -K.ky * i2k;
+            const double rad = gdt_sqrt64(qx * qx + qy * qy);
             const double m = 1e-6 * (fabs(qx) + fabs(qy) + rad) + 1e-9 * S.bsx;
-            gdt_circle_cells<double>(qx - m, qx + m, qy - m, qy + m, rad, ox, oy, S.bsx, S.bsy,
+            gdt_circle_cells<double>(qx - m, qx + m, qy - m, qy + m, rad, ox, oy, S.ibsx, S.ibsy,
                                      1e-3, dx0, dx1, dy0, dy1);
             ok = (dx0 >= -hs && dx1 <= hs && dy0 >= -hs && dy1 <= hs) ||
                  (cov > 0. && K.kx * K.kx + K.ky * K.ky <= cov * cov * (K.k0 * K.k0) * (1.0 - 1e-6));
@@ -529,7 +541,8 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
             set_nn();
         }
     };
-    set_region(GDT_SCAN0);
+    int hs = wrap ? GDT_SCAN0 : GDT_SCAN_NN;
+    set_region(hs);
     const int wx = st.c0 % GDT_WIN, wy = st.c0 / GDT_WIN;
     // FP64 displacement of LDS entry k: the FP64 path's value (fallbacks only)
     auto x64 = [&](int k, double& x, double& y) {
@@ -586,8 +599,17 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
         }
     };
     auto visit_k = [&](int k) { visit(k, load(k)); };
-    int hs = GDT_SCAN0;
-    npred += gdt_scan_rows(L, st.c0, hs, load, visit);
+    // nearest neighbour: the 3x3 block; successor: the 5x5 block, clipped to
+    // the columns that may hold points with O >= -BO (horizontal margin BO/|ay|)
+    const float ox = vx - (float)(wx - GDT_FRAME) * S.bsx32;
+    const float oy = vy - (float)(wy - GDT_FRAME) * S.bsy32;
+    {
+        const float m = wrap ? BO * gdt_rcp32(fabsf(ay) * S.bsx32) + 1e-3f : 0.f;
+        const GdtClip cl = gdt_left_clip(wrap, ax, ay, ox, oy, S.bsx32, S.bsy32, m);
+        const unsigned nb = gdt_block_rows(L, st.c0, hs, cl, st.tbl);
+        gdt_scan_tbl(st.tbl, nb, load, visit);
+        npred += nb;
+    }
     if (bk < 0 && hs < GDT_SCAN) {
         set_region(GDT_SCAN);
         npred += gdt_lds_block(L, wx, wy, -GDT_SCAN, GDT_SCAN, -GDT_SCAN, GDT_SCAN, hs, visit_k);
@@ -595,8 +617,6 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
     }
 
     const float bsm = S.bsmin32;
-    const float ox = vx - (float)(wx - GDT_FRAME) * S.bsx32;
-    const float oy = vy - (float)(wy - GDT_FRAME) * S.bsy32;
     bool ok = false, boxed = false;
     int dx0 = 0, dx1 = 0, dy0 = 0, dy1 = 0;
     // certification against the scanned block [-hs,hs]^2 (sets the box too)
@@ -607,8 +627,8 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
             if (bk >= 0) {
                 const float bE = fmaf(cE, bd2, eabs);
                 ok = cov > 0.f && bd2 + bE <= cov * cov * 0.9999f;
-                const float rad = sqrtf(bd2 + bE) * 1.0001f;
-                gdt_circle_cells<float>(0.f, 0.f, 0.f, 0.f, rad, ox, oy, S.bsx32, S.bsy32, 1e-2f,
+                const float rad = gdt_sqrt32(bd2 + bE) * 1.0001f;
+                gdt_circle_cells<float>(0.f, 0.f, 0.f, 0.f, rad, ox, oy, S.ibsx32, S.ibsy32, 1e-2f,
                                         dx0, dx1, dy0, dy1);
                 boxed = true;
             }
@@ -620,13 +640,14 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
             if (Klo > 0.0625f * K0) {
                 const float nxlo = -(Kx + EK), nxhi = -(Kx - EK);
                 const float nylo = -(Ky + EK), nyhi = -(Ky - EK);
-                const float qxlo = GDT_FMINF(nxlo / (2.f * Klo), nxlo / (2.f * Khi));
-                const float qxhi = GDT_FMAXF(nxhi / (2.f * Klo), nxhi / (2.f * Khi));
-                const float qylo = GDT_FMINF(nylo / (2.f * Klo), nylo / (2.f * Khi));
-                const float qyhi = GDT_FMAXF(nyhi / (2.f * Klo), nyhi / (2.f * Khi));
-                const float rad = sqrtf(GDT_FMAXF(qxlo * qxlo, qxhi * qxhi) +
-                                        GDT_FMAXF(qylo * qylo, qyhi * qyhi)) * 1.0001f;
-                gdt_circle_cells<float>(qxlo, qxhi, qylo, qyhi, rad, ox, oy, S.bsx32, S.bsy32,
+                const float il = gdt_rcp32(2.f * Klo), ih = gdt_rcp32(2.f * Khi);
+                const float qxlo = GDT_FMINF(nxlo * il, nxlo * ih);
+                const float qxhi = GDT_FMAXF(nxhi * il, nxhi * ih);
+                const float qylo = GDT_FMINF(nylo * il, nylo * ih);
+                const float qyhi = GDT_FMAXF(nyhi * il, nyhi * ih);
+                const float rad = gdt_sqrt32(GDT_FMAXF(qxlo * qxlo, qxhi * qxhi) +
+                                             GDT_FMAXF(qylo * qylo, qyhi * qyhi)) * 1.0001f;
+                gdt_circle_cells<float>(qxlo, qxhi, qylo, qyhi, rad, ox, oy, S.ibsx32, S.ibsy32,
                                         1e-2f, dx0, dx1, dy0, dy1);
                 const float gx = fabsf(Kx) + EK, gy = fabsf(Ky) + EK;
                 ok = (dx0 >= -hs && dx1 <= hs && dy0 >= -hs && dy1 <= hs) ||
@@ -662,14 +683,17 @@ GDT_HD inline GdtRingResult gdt_tile_ring(const GdtSet& S, const GdtTileWin& w, 
 {
     GdtRingResult res = {0, 0, 0u, 0u};
     int rc = STEP_RUN;
+    uint32_t tbl[6];
     if (PREC == 32) {
         GdtTileLane<float> st;
         gdt_tile_lane_init<float>(w, L, j, io, S, st);
+        st.tbl = tbl;
         while (rc == STEP_RUN) rc = gdt_tile_step_f32(S, L, io, st, res.npred, res.nfb);
         res.count = st.count;
     } else {
         GdtTileLane<double> st;
         gdt_tile_lane_init<double>(w, L, j, io, S, st);
+        st.tbl = tbl;
         while (rc == STEP_RUN) rc = gdt_tile_step_f64(S, L, io, st, res.npred);
         res.count = st.count;
     }

@@ -147,8 +147,8 @@ GDT_HD inline GdtRingResult gdt_wave_ring(const GdtSet& S, int j, const GdtRingI
                 bool boxed = false;
                 if (!wrap) {
                     if (cov > 0. && g.d2 <= cov * cov * (1.0 - 1e-12)) return true;
-                    const double rad = sqrt(g.d2) * (1.0 + 1e-9);
-                    gdt_circle_cells<double>(0., 0., 0., 0., rad, ov.x, ov.y, S.bsx, S.bsy, 1e-3,
+                    const double rad = gdt_sqrt64(g.d2) * (1.0 + 1e-9);
+                    gdt_circle_cells<double>(0., 0., 0., 0., rad, ov.x, ov.y, S.ibsx, S.ibsy, 1e-3,
                                              dx0, dx1, dy0, dy1);
                     boxed = true;
                 } else {
@@ -158,11 +158,12 @@ GDT_HD inline GdtRingResult gdt_wave_ring(const GdtSet& S, int j, const GdtRingI
                         if (cov > 0. &&
                             K.kx * K.kx + K.ky * K.ky <= cov * cov * (K.k0 * K.k0) * (1.0 - 1e-6))
                             return true;
-                        const double qx = -K.kx / (2. * K.k0), qy = -K.ky / (2. * K.k0);
-                        const double rad = sqrt(qx * qx + qy * qy);
+                        const double i2k = gdt_rcp64(2. * K.k0);
+                        const double qx = -K.kx * i2k, qy = -K.ky * i2k;
+                        const double rad = gdt_sqrt64(qx * qx + qy * qy);
                         const double m = 1e-6 * (fabs(qx) + fabs(qy) + rad) + 1e-9 * S.bsx;
                         gdt_circle_cells<double>(qx - m, qx + m, qy - m, qy + m, rad, ov.x, ov.y,
-                                                 S.bsx, S.bsy, 1e-3, dx0, dx1, dy0, dy1);
+                                                 S.ibsx, S.ibsy, 1e-3, dx0, dx1, dy0, dy1);
                         boxed = true;
                     }
                 }

@@ -195,6 +195,7 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
     __shared__ int s_next;
     __shared__ int s_dlist[GDT_TILE_DEFER];
     __shared__ uint32_t s_dn;
+    extern __shared__ uint32_t s_tbl[];     // [5 * blockDim + 1] the lanes' row tables
 
     const GdtChunk ch = tiles[blockIdx.x];
     const GdtSet S = sets[ch.set];
@@ -227,6 +228,7 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
     __syncthreads();
     using Real = typename std::conditional<PREC == 32, float, double>::type;
     GdtTileLane<Real> st;
+    st.tbl = s_tbl + 5 * tid;
     int q = Q0 + tid;
     bool have = q < Q1;
     int j = have ? (list ? list[q] : q) : 0;
@@ -328,11 +330,12 @@ hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nse
 {
     if (ntiles <= 0) return hipSuccess;
     if (block > 1024) block = 1024;
+    const size_t tbl = (5 * (size_t)block + 1) * sizeof(uint32_t);
     if (precision == 32)
-        hipLaunchKernelGGL(k_one_ring_tile<32>, dim3(ntiles), dim3(block), 0, s, d_sets, tiles,
+        hipLaunchKernelGGL(k_one_ring_tile<32>, dim3(ntiles), dim3(block), tbl, s, d_sets, tiles,
                            list, lscan, io, counts, status, counters, defer);
     else
-        hipLaunchKernelGGL(k_one_ring_tile<64>, dim3(ntiles), dim3(block), 0, s, d_sets, tiles,
+        hipLaunchKernelGGL(k_one_ring_tile<64>, dim3(ntiles), dim3(block), tbl, s, d_sets, tiles,
                            list, lscan, io, counts, status, counters, defer);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
