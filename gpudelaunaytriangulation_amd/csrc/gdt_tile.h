@@ -435,28 +435,26 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
 
     bool ok = false, boxed = false;
     int dx0 = 0, dx1 = 0, dy0 = 0, dy1 = 0;
-    // certification against the scanned block [-hs,hs]^2 (sets the box too)
+    // Certification against the scanned block [-hs,hs]^2 (sets the box too),
+    // one code path for both searches: the circle is centred at
+    // q = -(Kx,Ky)/(2 K0) -- the origin for the nearest neighbour, whose K is
+    // (1,0,0) -- with R^2 = |q|^2 (successor) or the best distance (nearest
+    // neighbour); certified when its box lies in the block, or when all of it
+    // is within cov of v: 2R <= cov (successor), R <= cov (nearest neighbour).
     auto certify = [&]() {
         best();
         const double cov = (double)hs * GDT_FMIN(S.bsx, S.bsy) + st.covb;
         ok = boxed = false;
-        if (!wrap) {
-            if (bk >= 0) {
-                ok = cov > 0. && bd2 <= cov * cov * (1.0 - 1e-12);
-                const double rad = gdt_sqrt64(bd2) * (1.0 + 1e-9);
-                gdt_circle_cells<double>(0., 0., 0., 0., rad, ox, oy, S.ibsx, S.ibsy, 1e-3, dx0, dx1,
-                                         dy0, dy1);
-                boxed = true;
-            }
-        } else if (bk >= 0 && K.k0 > 1e-6 * (fabs(ax * by) + fabs(ay * bx))) {
+        if (bk >= 0 && K.k0 > 1e-6 * (fabs(ax * by) + fabs(ay * bx))) {
             const double i2k = gdt_rcp64(2. * K.k0);
             const double qx = -K.kx * i2k, qy = -K.ky * i2k;
-            const double rad = gdt_sqrt64(qx * qx + qy * qy);
+            const double R2 = wrap ? qx * qx + qy * qy : bd2;
+            const double rad = gdt_sqrt64(R2) * (1.0 + 1e-9);
             const double m = 1e-6 * (fabs(qx) + fabs(qy) + rad) + 1e-9 * S.bsx;
             gdt_circle_cells<double>(qx - m, qx + m, qy - m, qy + m, rad, ox, oy, S.ibsx, S.ibsy,
                                      1e-3, dx0, dx1, dy0, dy1);
             ok = (dx0 >= -hs && dx1 <= hs && dy0 >= -hs && dy1 <= hs) ||
-                 (cov > 0. && K.kx * K.kx + K.ky * K.ky <= cov * cov * (K.k0 * K.k0) * (1.0 - 1e-6));
+                 (cov > 0. && (wrap ? 4. : 1.) * R2 <= cov * cov * (1.0 - 1e-6));
             boxed = true;
         }
     };
@@ -623,37 +621,34 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
     auto certify = [&]() {
         const float cov = (float)hs * bsm + st.covb - 9.5367431640625e-07f * (float)(hs + 2) * bsM;
         ok = boxed = false;
-        if (!wrap) {
-            if (bk >= 0) {
-                const float bE = fmaf(cE, bd2, eabs);
-                ok = cov > 0.f && bd2 + bE <= cov * cov * 0.9999f;
-                const float rad = gdt_sqrt32(bd2 + bE) * 1.0001f;
-                gdt_circle_cells<float>(0.f, 0.f, 0.f, 0.f, rad, ox, oy, S.ibsx32, S.ibsy32, 1e-2f,
-                                        dx0, dx1, dy0, dy1);
-                boxed = true;
-            }
-        } else if (bk >= 0) {
-            const float sab = ma + mbb, pab = ma * mbb;
-            const float EK0 = 4.f * (2.f * eps1 + 2.f * u) * pab + 4.f * eps0 * sab;
-            const float EK = 4.f * (3.f * eps1 + 6.f * u) * pab * sab + 8.f * eps0 * sab * sab;
-            const float Klo = K0 - EK0, Khi = K0 + EK0;
-            if (Klo > 0.0625f * K0) {
-                const float nxlo = -(Kx + EK), nxhi = -(Kx - EK);
-                const float nylo = -(Ky + EK), nyhi = -(Ky - EK);
-                const float il = gdt_rcp32(2.f * Klo), ih = gdt_rcp32(2.f * Khi);
-                const float qxlo = GDT_FMINF(nxlo * il, nxlo * ih);
-                const float qxhi = GDT_FMAXF(nxhi * il, nxhi * ih);
-                const float qylo = GDT_FMINF(nylo * il, nylo * ih);
-                const float qyhi = GDT_FMAXF(nyhi * il, nyhi * ih);
-                const float rad = gdt_sqrt32(GDT_FMAXF(qxlo * qxlo, qxhi * qxhi) +
-                                             GDT_FMAXF(qylo * qylo, qyhi * qyhi)) * 1.0001f;
-                gdt_circle_cells<float>(qxlo, qxhi, qylo, qyhi, rad, ox, oy, S.ibsx32, S.ibsy32,
-                                        1e-2f, dx0, dx1, dy0, dy1);
-                const float gx = fabsf(Kx) + EK, gy = fabsf(Ky) + EK;
-                ok = (dx0 >= -hs && dx1 <= hs && dy0 >= -hs && dy1 <= hs) ||
-                     (cov > 0.f && (gx * gx + gy * gy) * 1.001f <= (cov * cov) * (Klo * Klo));
-                boxed = true;
-            }
+        // one code path for both searches, as in the FP64 path: for the
+        // nearest neighbour a = 0 and K = (1,0,0) exactly, so the centre
+        // bounds are the origin and R^2 is the best distance plus its error
+        const float mb = wrap ? mbb : 0.f;
+        const float sab = ma + mb, pab = ma * mb;
+        const float EK0 = 4.f * (2.f * eps1 + 2.f * u) * pab + 4.f * eps0 * sab;
+        const float EK = 4.f * (3.f * eps1 + 6.f * u) * pab * sab + 8.f * eps0 * sab * sab;
+        const float Klo = K0 - EK0, Khi = K0 + EK0;
+        if (bk >= 0 && Klo > 0.0625f * K0) {
+            const float nxlo = -(Kx + EK), nxhi = -(Kx - EK);
+            const float nylo = -(Ky + EK), nyhi = -(Ky - EK);
+            const float il = gdt_rcp32(2.f * Klo), ih = gdt_rcp32(2.f * Khi);
+            const float qxlo = GDT_FMINF(nxlo * il, nxlo * ih);
+            const float qxhi = GDT_FMAXF(nxhi * il, nxhi * ih);
+            const float qylo = GDT_FMINF(nylo * il, nylo * ih);
+            const float qyhi = GDT_FMAXF(nyhi * il, nyhi * ih);
+            const float bE = fmaf(cE, bd2, eabs);
+            const float R2 = wrap ? GDT_FMAXF(qxlo * qxlo, qxhi * qxhi) +
+                                        GDT_FMAXF(qylo * qylo, qyhi * qyhi)
+                                  : bd2 + bE;
+            const float rad = gdt_sqrt32(R2) * 1.0001f;
+            gdt_circle_cells<float>(qxlo, qxhi, qylo, qyhi, rad, ox, oy, S.ibsx32, S.ibsy32,
+                                    1e-2f, dx0, dx1, dy0, dy1);
+            const float gx = fabsf(Kx) + EK, gy = fabsf(Ky) + EK;
+            const bool within = wrap ? (gx * gx + gy * gy) * 1.001f <= (cov * cov) * (Klo * Klo)
+                                     : R2 <= cov * cov * 0.9999f;
+            ok = (dx0 >= -hs && dx1 <= hs && dy0 >= -hs && dy1 <= hs) || (cov > 0.f && within);
+            boxed = true;
         }
     };
     certify();
