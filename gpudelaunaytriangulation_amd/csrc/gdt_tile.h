@@ -184,12 +184,13 @@ GDT_HD inline GdtClip gdt_left_clip(bool wrap, float ax, float ay, float ox, flo
 }
 
 // Row table of the block [-h,h]^2 around window cell c0 (h <= 2), each row
-// clipped to the columns cl keeps: tbl[r] = first LDS entry | end << 16.
-// Returns the number of entries.
+// clipped to the columns cl keeps: one word per non-empty row, first LDS entry
+// | end << 16.  Returns the number of entries.
 GDT_HD inline unsigned gdt_block_rows(const GdtTileLds& L, int c0, int h, const GdtClip& cl,
                                       uint32_t* tbl)
 {
     unsigned n = 0;
+    int ns = 0;
     for (int r = 0; r <= 2 * h; ++r) {
         const int dy = r - h;
         int lo = -h, hi = h;
@@ -201,13 +202,11 @@ GDT_HD inline unsigned gdt_block_rows(const GdtTileLds& L, int c0, int h, const 
             else lo = bnd > lo ? bnd : lo;
         }
         const int rb = c0 + dy * GDT_WIN;
-        uint32_t k = 0, e = 0;
         if (lo <= hi) {
-            k = L.cs[rb + lo];
-            e = L.cs[rb + hi + 1];
+            const uint32_t k = L.cs[rb + lo], e = L.cs[rb + hi + 1];
+            if (e > k) tbl[ns++] = k | (e << 16);
+            n += e - k;
         }
-        tbl[r] = k | (e << 16);
-        n += e - k;
     }
     return n;
 }
@@ -244,7 +243,7 @@ GDT_HD inline unsigned gdt_disk_rows(const GdtTileLds& L, int c0, int& dy, int d
         auto seg = [&](int a, int b) {
             if (a > b) return;
             const uint32_t k = L.cs[rb + a], e = L.cs[rb + b + 1];
-            tbl[ns++] = k | (e << 16);
+            if (e > k) tbl[ns++] = k | (e << 16);
             n += e - k;
         };
         if (dy >= -hs && dy <= hs) {
@@ -257,12 +256,13 @@ GDT_HD inline unsigned gdt_disk_rows(const GdtTileLds& L, int c0, int& dy, int d
     return n;
 }
 
-// The n entries of a row table as one flat loop.  The next row's word is read
-// one row ahead, so a row switch -- which some lane of a wave takes on most
-// iterations -- is a few ALU operations and one LDS read, with no wait; two
-// entries go per trip, the next entry's coordinates loaded, load(k), before
-// the current one is evaluated, fn(k, coords).  The word after the last row
-// is read ahead but never used (the tables are laid out so that it exists).
+// The n entries of a row table (non-empty rows only) as one flat loop.  The
+// row switch is branch-free: each half-trip reads the word after the current
+// row and takes it when the row ends, so every LDS access of the loop is
+// unconditional and the coordinates of the next entry, loaded, load(k),
+// before the current one is evaluated, fn(k, coords), are waited for one
+// half-trip later.  The word after the last row is read but never used (the
+// tables are laid out so that it exists).
 template <class Load, class Fn>
 GDT_HD inline void gdt_scan_tbl(const uint32_t* tbl, unsigned n, Load& load, Fn& fn)
 {
@@ -270,28 +270,26 @@ GDT_HD inline void gdt_scan_tbl(const uint32_t* tbl, unsigned n, Load& load, Fn&
     const uint32_t* p = tbl;
     uint32_t k = p[0] & 0xffffu, e = p[0] >> 16;
     uint32_t wn = p[1];
-    auto next_row = [&]() {     // the next non-empty row (one exists while entries remain)
-        do {
-            ++p;
-            k = wn & 0xffffu;
-            e = wn >> 16;
-            wn = p[1];
-        } while (k == e);
+    auto advance = [&](bool more) {
+        const bool sw = more & (k == e);
+        k = sw ? (wn & 0xffffu) : k;
+        e = sw ? (wn >> 16) : e;
+        p += sw ? 1 : 0;
+        wn = p[1];
     };
-    if (k == e) next_row();
     auto XA = load((int)k);
     decltype(XA) XB;
     for (unsigned i = 1;; ++i) {
         uint32_t kc = k++;
         bool more = i < n;
-        if (more && k == e) next_row();
+        advance(more);
         XB = load((int)(more ? k : kc));
         fn((int)kc, XA);
         if (!more) break;
         ++i;
         kc = k++;
         more = i < n;
-        if (more && k == e) next_row();
+        advance(more);
         XA = load((int)(more ? k : kc));
         fn((int)kc, XB);
         if (!more) break;
