@@ -58,6 +58,7 @@ struct gdt_context {
     std::vector<GdtSet> sets;
     int total_n = 0, total_keys = 0;
     bool cells_valid = false;
+    bool slots_valid = false;    // slot_of matches the cell list
     DBuf<GdtSet> d_sets;
     DBuf<GdtChunk> pch, kch, rch, tch;
     int npch = 0, nkch = 0, nrch = 0, rch_block = 0, ntch = 0;
@@ -266,15 +267,19 @@ static GdtRingIO ring_io(gdt_context* c, int* nbrs, int stride)
     return io;
 }
 
-static int build_cells(gdt_context* c, const double* d_xy, hipStream_t s)
+// The cell list of d_xy; slot_of (caller index -> slot) only when asked for
+// (the circumcircle test reads it, the global triangulation does not).
+static int build_cells(gdt_context* c, const double* d_xy, hipStream_t s, bool with_slot_of = true)
 {
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[0], s));
     GdtCells cl = c->cells();
+    if (!with_slot_of) cl.slot_of = nullptr;
     GDT_CHECK(gdt_build_cells(c->d_sets.p, (int)c->sets.size(), c->pch.p, c->npch, c->kch.p,
                               c->nkch, c->total_n,
                               c->total_keys, d_xy, cl, s));
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
     c->cells_valid = true;
+    c->slots_valid = with_slot_of;
     return GDT_OK;
 }
 
@@ -434,7 +439,7 @@ static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int*
 {
     int rc;
     if ((rc = ring_chunks(c)) != GDT_OK) return rc;
-    if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
+    if ((rc = build_cells(c, d_xy, s, false)) != GDT_OK) return rc;
     if ((rc = reset_status(c, s)) != GDT_OK) return rc;
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
     if (c->cfg.tiled) {
@@ -556,6 +561,10 @@ int gdt_test_circumcircles(gdt_context* c, const double* d_xy, int n, const int*
     hipStream_t s = (hipStream_t)stream;
     GDT_CHECK(gdt_fill_i32(d_repair, -1, (size_t)n, s));
     GDT_CHECK(gdt_fill_u32(reinterpret_cast<uint32_t*>(c->status.p + ST_NTRI), (uint32_t)ntri, 1, s));
+    if (!c->slots_valid) {
+        GDT_CHECK(gdt_launch_slot_of(c->perm.p, c->total_n, c->slot_of.p, s));
+        c->slots_valid = true;
+    }
     GdtCells cl = c->cells();
     GDT_CHECK(gdt_launch_test(c->d_sets.p, (int)c->sets.size(), d_tris,
                               reinterpret_cast<const uint32_t*>(c->status.p + ST_NTRI), ntri,

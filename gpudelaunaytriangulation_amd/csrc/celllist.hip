@@ -275,7 +275,7 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_finalize(const GdtSet* __restrict
         off64[s] = o64;
         off32[s] = o32;
         scell[s] = (uint32_t)cx | ((uint32_t)cy << 16);
-        slot_of[i] = (int)s;
+        if (slot_of) slot_of[i] = (int)s;
     }
 }
 
@@ -454,7 +454,7 @@ __device__ inline void kb_write_slot(GdtCells& c, uint32_t slot, const GdtKbRec&
     o32.x = (float)r.off.x;
     o32.y = (float)r.off.y;
     c.perm[slot] = r.idx;
-    c.slot_of[r.idx] = (int)slot;
+    if (c.slot_of) c.slot_of[r.idx] = (int)slot;
     c.off64[slot] = r.off;
     c.off32[slot] = o32;
     c.scell[slot] = r.cell;
@@ -596,6 +596,21 @@ hipError_t gdt_build_cells(const GdtSet* d_sets, int nsets, const GdtChunk* pchu
     hipLaunchKernelGGL(k_kb_finalize, dim3(pl.nkb), dim3(KB_THREADS),
                        (cells + 2 * KB_CAP) * sizeof(uint32_t), s, pl, total_keys, total_n, c.bscan,
                        c.bkt, c);
+    return hipGetLastError();
+}
+
+// slot_of from perm, for a cell list built without it.
+__global__ void k_slot_of(const int* __restrict__ perm, int n, int* __restrict__ slot_of)
+{
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x)
+        slot_of[perm[s]] = s;
+}
+
+hipError_t gdt_launch_slot_of(const int* perm, int n, int* slot_of, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_slot_of, dim3(fill_grid((size_t)n)), dim3(GDT_BLOCK), 0, s, perm, n,
+                       slot_of);
     return hipGetLastError();
 }
 

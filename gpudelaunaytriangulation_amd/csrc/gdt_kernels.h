@@ -20,7 +20,7 @@ struct GdtCells {
     uint32_t* ccount;   // [K] points per key
     uint32_t* cstart;   // [K+1] exclusive scan of ccount
     int* perm;          // [N] slot -> caller index
-    int* slot_of;       // [N] caller index -> slot
+    int* slot_of;       // [N] caller index -> slot (not written when null)
     gdt_d2* off64;      // [N]
     gdt_f2* off32;      // [N]
     uint32_t* scell;    // [N]
@@ -50,6 +50,9 @@ hipError_t gdt_fill_i32(int* p, int v, size_t n, hipStream_t s);
 hipError_t gdt_build_cells(const GdtSet* d_sets, int nsets, const GdtChunk* pchunks, int npch,
                            const GdtChunk* kchunks, int nkch, int total_n, int total_keys,
                            const double* d_xy, GdtCells& c, hipStream_t s);
+
+// slot_of[perm[s]] = s for a cell list built without slot_of.
+hipError_t gdt_launch_slot_of(const int* perm, int n, int* slot_of, hipStream_t s);
 
 hipError_t gdt_export_cells(const GdtSet* d_sets, const GdtSet& h_set, const GdtCells& c,
                             unsigned int* cell_sizes, int* idxs, int nmax, hipStream_t s);
