@@ -110,6 +110,26 @@ def test_valid_triangulation_flags_nothing():
     assert np.array_equal(tri.cpu().numpy().reshape(nb.shape), nb)
 
 
+def test_circumcircle_test_right_after_global_triangulation():
+    """The global triangulation builds its cell list without the caller ->
+    slot map; the circumcircle test on that cell list (no rebuild) fills the
+    map first and flags nothing for the valid triangulation."""
+    L = 64.0
+    xy = inputs.uniform_points(4096, L, L, seed=4)
+    nb, c, dg = gpu_triangulate(xy, L, L, PREC_FP64)
+    tris, _ = _triangles(dg, nb, c)
+    pts = torch.from_numpy(xy).cuda()
+    tt = torch.from_numpy(tris.reshape(-1).copy()).cuda()
+    rep = torch.zeros(4096, dtype=torch.int32, device="cuda")
+    nf = ctypes.c_int(-1)
+    s = torch.cuda.current_stream().cuda_stream
+    _lib.check(_lib.load().gdt_test_circumcircles(dg.ctx.handle, pts.data_ptr(), 4096,
+                                                  tt.data_ptr(), tris.shape[0], rep.data_ptr(),
+                                                  ctypes.byref(nf), s), "test")
+    assert nf.value == 0
+    assert (rep.cpu().numpy() < 0).all()
+
+
 def _bad_triangles(xy, L, tris):
     """Triangles whose circumcircle strictly contains another point (scipy
     periodic KD-tree; independent of both the oracle and the kernels)."""
