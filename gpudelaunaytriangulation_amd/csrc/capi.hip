@@ -83,6 +83,9 @@ struct gdt_context {
     unsigned long long* h_counters = nullptr;
     int* hd_status = nullptr;                    // device-side pointers of the two
     unsigned long long* hd_counters = nullptr;   // pinned buffers above
+    unsigned int* h_seq = nullptr;               // publish sequence number (pinned)
+    unsigned int* hd_seq = nullptr;
+    unsigned int pub_seq = 0;
     gdt_stats stats;
 
     GdtCells cells()
@@ -224,11 +227,25 @@ static int reset_status(gdt_context* c, hipStream_t s)
     return GDT_OK;
 }
 
+// Status words to the host: the publish kernel runs after every kernel of the
+// call (stream order) and stores a fresh sequence number after the words; the
+// host polls that number instead of sleeping in a stream synchronization, and
+// asks the stream every 4096 polls so that a failed stream ends the wait.
 static int read_status(gdt_context* c, hipStream_t s)
 {
+    const unsigned int seq = ++c->pub_seq;
     GDT_CHECK(gdt_status_publish(c->status.p, ST_WORDS, c->counters.p, CT_WORDS, c->hd_status,
-                                 c->hd_counters, s));
-    GDT_CHECK(hipStreamSynchronize(s));
+                                 c->hd_counters, c->hd_seq, seq, s));
+    for (unsigned long it = 1; __atomic_load_n(c->h_seq, __ATOMIC_ACQUIRE) != seq; ++it) {
+        if ((it & 4095u) == 0u) {
+            const hipError_t e = hipStreamQuery(s);
+            if (e == hipSuccess) {
+                if (__atomic_load_n(c->h_seq, __ATOMIC_ACQUIRE) != seq) return GDT_ERR_DEVICE;
+                break;
+            }
+            if (e != hipErrorNotReady) return GDT_ERR_DEVICE;
+        }
+    }
     return GDT_OK;
 }
 
@@ -318,11 +335,14 @@ int gdt_create(gdt_context** out, int device)
         hipHostMalloc(reinterpret_cast<void**>(&c->h_counters), CT_WORDS * sizeof(unsigned long long), 0) != hipSuccess ||
         hipHostGetDevicePointer(reinterpret_cast<void**>(&c->hd_status), c->h_status, 0) != hipSuccess ||
         hipHostGetDevicePointer(reinterpret_cast<void**>(&c->hd_counters), c->h_counters, 0) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&c->h_seq), sizeof(unsigned int), 0) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&c->hd_seq), c->h_seq, 0) != hipSuccess ||
         hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess ||
         hipEventCreate(&c->ev[2]) != hipSuccess || hipEventCreate(&c->ev[3]) != hipSuccess) {
         gdt_destroy(c);
         return GDT_ERR_DEVICE;
     }
+    __atomic_store_n(c->h_seq, 0u, __ATOMIC_RELEASE);
     *out = c;
     return GDT_OK;
 }
@@ -347,6 +367,7 @@ void gdt_destroy(gdt_context* c)
         if (e) (void)hipEventDestroy(e);
     if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
+    if (c->h_seq) (void)hipHostFree(c->h_seq);
     delete c;
 }
 

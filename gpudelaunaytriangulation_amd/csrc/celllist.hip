@@ -56,11 +56,15 @@ __global__ void k_status_reset(int* __restrict__ st, int nst, unsigned long long
 
 __global__ void k_status_publish(const int* __restrict__ st, int nst,
                                  const unsigned long long* __restrict__ ct, int nct,
-                                 int* __restrict__ hst, unsigned long long* __restrict__ hct)
+                                 int* __restrict__ hst, unsigned long long* __restrict__ hct,
+                                 unsigned int* __restrict__ hseq, unsigned int seq)
 {
     const int t = threadIdx.x;
     if (t < nst) hst[t] = st[t];
     if (t < nct) hct[t] = ct[t];
+    __threadfence_system();
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(hseq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 hipError_t gdt_status_reset(int* st, int nst, unsigned long long* ct, int nct, hipStream_t s)
@@ -70,9 +74,11 @@ hipError_t gdt_status_reset(int* st, int nst, unsigned long long* ct, int nct, h
 }
 
 hipError_t gdt_status_publish(const int* st, int nst, const unsigned long long* ct, int nct,
-                              int* hst, unsigned long long* hct, hipStream_t s)
+                              int* hst, unsigned long long* hct, unsigned int* hseq,
+                              unsigned int seq, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_status_publish, dim3(1), dim3(64), 0, s, st, nst, ct, nct, hst, hct);
+    hipLaunchKernelGGL(k_status_publish, dim3(1), dim3(64), 0, s, st, nst, ct, nct, hst, hct,
+                       hseq, seq);
     return hipGetLastError();
 }
 

@@ -41,9 +41,11 @@ hipError_t gdt_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* t
 hipError_t gdt_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 // Zero nst status words and nct counters (nst, nct <= 64).
 hipError_t gdt_status_reset(int* st, int nst, unsigned long long* ct, int nct, hipStream_t s);
-// Copy them to host memory hst/hct (device-visible pinned pointers).
+// Copy them to host memory hst/hct (device-visible pinned pointers), then
+// store seq to *hseq (system scope, after the copies are visible).
 hipError_t gdt_status_publish(const int* st, int nst, const unsigned long long* ct, int nct,
-                              int* hst, unsigned long long* hct, hipStream_t s);
+                              int* hst, unsigned long long* hct, unsigned int* hseq,
+                              unsigned int seq, hipStream_t s);
 hipError_t gdt_fill_i32(int* p, int v, size_t n, hipStream_t s);
 
 // Cell list over all sets.  pchunks: blocks of 256 points, kchunks: blocks of 256 keys.
