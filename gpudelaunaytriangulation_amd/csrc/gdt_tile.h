@@ -13,11 +13,12 @@
 // displacement is d = X(c) - X(v).
 //
 // The filtered path certifies against the FP64 displacement of the same
-// frame formula.  FP32 input error per coordinate (u = 2^-24):
-//   |X32 - X64| <= 3u|X| + 3u*bs   =>   |d32 - d64| <= 5u|d| + 6u(|Xv| + bs),
-// i.e. eps1 = 5u (relative) and eps0 = 7.5u(|Xv|_inf + bs) (absolute, with
-// margin).  The orientation, in-circle and distance bounds are the ones of
-// gdt_point.h written for general (eps1, eps0), doubled for slack.
+// frame formula.  Its FP32 frame coordinate is the FP64 one rounded to nearest,
+// so per coordinate (u = 2^-24)
+//   |X32 - X64| <= u|X64|   =>   |d32 - d64| <= u(|Xk| + |Xv|) + u|d| <= 2u|d| + 2u|Xv|,
+// i.e. eps1 = 2.5u (relative) and eps0 = 2.5u|Xv|_inf (absolute), with margin.
+// The orientation, in-circle and distance bounds are the ones of gdt_point.h
+// written for general (eps1, eps0), doubled for slack.
 #pragma once
 
 #include "gdt_point.h"
@@ -101,10 +102,9 @@ GDT_HD inline void gdt_tile_copy(const GdtSet& S, int c, int gstart, uint32_t at
         L.X64[at + k] = X;
         L.orig[at + k] = io.perm[s];
         if (F32) {
-            const gdt_f2 o32 = io.off32[s];
             gdt_f2 Y;
-            Y.x = gdt_frame32(fx, S.bsx32, o32.x);
-            Y.y = gdt_frame32(fy, S.bsy32, o32.y);
+            Y.x = (float)X.x;
+            Y.y = (float)X.y;
             L.X32[at + k] = Y;
         }
     }
@@ -535,8 +535,8 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
 {
     const float u = GDT_U32;
     const float bsM = S.bsmax32;
-    const float eps1 = 5.f * u;
-    const float eps0 = 7.5f * u * (GDT_FMAXF(fabsf(st.vx), fabsf(st.vy)) + bsM);
+    const float eps1 = 2.5f * u;
+    const float eps0 = 2.5f * u * (GDT_FMAXF(fabsf(st.vx), fabsf(st.vy)) + 1e-6f * bsM);
     const float cP = 8.f * (4.f * eps1 + 8.f * u), eP = 8.f * eps0;
     const bool wrap = st.ak >= 0;
     const float ax = st.ax, ay = st.ay, vx = st.vx, vy = st.vy;
@@ -565,7 +565,7 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
     // thresholds on |c|^2, each one rounding away from the exact one:
     //   Pc = bd2 (1 - 3cE) - e2 <= (bd2 (1 - cE) - 2 eabs) / (1 + cE),
     //   Pp = bd2 (1 + 3cE) + e2 >= (bd2 (1 + cE) + 2 eabs) / (1 - cE),
-    // with e2 = 2.0001 eabs; 1 +- 3cE are exact in FP32 (cE = 26u).
+    // with e2 = 2.0001 eabs; 1 +- 3cE are exact in FP32 (cE = 16u).
     const float nn_c = 1.f - 3.f * cE, nn_p = 1.f + 3.f * cE;
     auto set_nn = [&]() {
         Pc = fmaf(bd2, nn_c, -e2);
