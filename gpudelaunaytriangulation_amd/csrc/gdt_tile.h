@@ -215,15 +215,19 @@ GDT_HD inline unsigned gdt_block_rows(const GdtTileLds& L, int c0, int h, const 
 // meet the disk with centre in [qxlo,qxhi] x [qylo,qyhi] and radius R (cell
 // sides bsy and 1/ibsx, v's offset in its cell (ox,oy), margin in cells),
 // clipped to the columns cl keeps, without the scanned square [-hs,hs]^2, as
-// row segments in tbl (at most 4, two rows at a time).  Returns the entries.
+// row segments in tbl (at most 4, two rows at a time).  With `outer` (the
+// disk's column range lies within [-hs,hs]) the rows through the square have
+// nothing left and are skipped.  Returns the entries.
 template <class T>
-GDT_HD inline unsigned gdt_disk_rows(const GdtTileLds& L, int c0, int& dy, int dy1, int hs, T qxlo,
-                                     T qxhi, T qylo, T qyhi, T R, T ox, T oy, T bsy, T ibsx,
-                                     T margin, const GdtClip& cl, uint32_t* tbl)
+GDT_HD inline unsigned gdt_disk_rows(const GdtTileLds& L, int c0, int& dy, int dy1, int hs,
+                                     bool outer, T qxlo, T qxhi, T qylo, T qyhi, T R, T ox, T oy,
+                                     T bsy, T ibsx, T margin, const GdtClip& cl, uint32_t* tbl)
 {
     unsigned n = 0;
     int ns = 0;
-    for (int rows = 0; rows < 2 && dy <= dy1; ++rows, ++dy) {
+    for (int rows = 0; rows < 2; ++rows, ++dy) {
+        if (outer && dy >= -hs && dy <= hs) dy = hs + 1;
+        if (dy > dy1) break;
         const T y0 = (T)dy * bsy - oy, y1 = y0 + bsy;
         T d = y0 - qyhi;
         if (qylo - y1 > d) d = qylo - y1;
@@ -516,10 +520,11 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
     if (!ok && boxed && gdt_in_window(S, st.c0, dx0, dx1, dy0, dy1)) {
         // the rest of the circle: the cells of its rows that meet it, left of
         // v->a, outside the scanned square (every later best's circle lies inside)
+        const bool outer = dx0 >= -hs && dx1 <= hs;
         for (int dy = dy0; dy <= dy1;) {
-            const unsigned ne = gdt_disk_rows<double>(L, st.c0, dy, dy1, hs, cqx - cm, cqx + cm,
-                                                      cqy - cm, cqy + cm, crad, ox, oy, S.bsy,
-                                                      S.ibsx, 1e-3, cl, st.tbl);
+            const unsigned ne = gdt_disk_rows<double>(L, st.c0, dy, dy1, hs, outer, cqx - cm,
+                                                      cqx + cm, cqy - cm, cqy + cm, crad, ox, oy,
+                                                      S.bsy, S.ibsx, 1e-3, cl, st.tbl);
             gdt_scan_tbl(st.tbl, ne, load, visit);
             npred += ne;
         }
@@ -728,10 +733,11 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
         set_region(-h0 > h1 ? -h0 : h1);
         const GdtClip clx = gdt_left_clip(wrap, ax, ay, ox, oy, S.bsx32, S.bsy32,
                                           wrap ? BO * gdt_rcp32(fabsf(ay) * S.bsx32) + 1e-3f : 0.f);
+        const bool outer = dx0 >= -hs && dx1 <= hs;
         for (int dy = dy0; dy <= dy1;) {
-            const unsigned ne = gdt_disk_rows<float>(L, st.c0, dy, dy1, hs, cq[0], cq[1], cq[2],
-                                                     cq[3], crad, ox, oy, S.bsy32, S.ibsx32, 1e-2f,
-                                                     clx, st.tbl);
+            const unsigned ne = gdt_disk_rows<float>(L, st.c0, dy, dy1, hs, outer, cq[0], cq[1],
+                                                     cq[2], cq[3], crad, ox, oy, S.bsy32, S.ibsx32,
+                                                     1e-2f, clx, st.tbl);
             gdt_scan_tbl(st.tbl, ne, load, visit);
             npred += ne;
         }
