@@ -162,6 +162,7 @@ static int configure(gdt_context* c, std::vector<GdtSet> sets)
             for (int t = 0; t < c->sets[si].nkeys / GDT_TILE_CELLS; ++t) tch.push_back(GdtChunk{si, t});
     GDT_CHECK(upload(c->tch, tch));
     c->ntch = (int)tch.size();
+    GDT_CHECK(c->tpart.reserve(std::max(pch.size(), 2 * tch.size())));   // k_tri_count / k_ring_test_tile
     c->rch_block = 0;
     const size_t N = (size_t)c->total_n, K = (size_t)c->total_keys;
     GDT_CHECK(c->pkey.reserve(N));
@@ -550,6 +551,27 @@ static bool tiled_sets(const gdt_context* c)
     return !c->sets.empty() && c->sets[0].tiled && c->ntch > 0;
 }
 
+// Triangles of the current rings and their empty-circle test: with the tiled
+// cell order in one pass over the rings (no triangle array), else enumerate()
+// then test_triangles().  Leaves the triangle count in toff[n] and the sum of
+// counts in counters[CT_INSUM] either way.
+static int triangles_and_test(gdt_context* c, const double* d_xy, const int* d_nbrs,
+                              const int* d_counts, int stride, int cap, hipStream_t s)
+{
+    const int n = c->total_n;
+    if (!tiled_sets(c)) {
+        int rc;
+        if ((rc = enumerate(c, d_nbrs, d_counts, stride, c->tris.p, cap, false, s)) != GDT_OK) return rc;
+        return test_triangles(c, d_xy, false, cap, s);
+    }
+    GDT_CHECK(c->toff.reserve((size_t)n + 1));
+    GDT_CHECK(gdt_fill_i32(c->repair.p, -1, (size_t)n, s));
+    GDT_CHECK(gdt_launch_ring_test_tile(c->d_sets.p, c->tch.p, c->ntch, d_nbrs, d_counts, stride, n,
+                                       d_xy, c->cells(), c->repair.p, c->tpart.p, c->toff.p + n,
+                                       c->counters.p, c->status.p, s));
+    return GDT_OK;
+}
+
 int gdt_triangles(gdt_context* c, const int* d_nbrs, const int* d_counts, int n, int stride,
                   int* d_tris, int cap, int* ntri, long long* total_nbrs, void* stream)
 {
@@ -565,9 +587,9 @@ int gdt_triangles(gdt_context* c, const int* d_nbrs, const int* d_counts, int n,
     int rc = reset_status(c, s);
     if (rc != GDT_OK) return rc;
     if ((rc = enumerate(c, d_nbrs, d_counts, stride, d_tris, cap, false, s)) != GDT_OK) return rc;
-    unsigned int h_nt = 0;
-    GDT_CHECK(hipMemcpyAsync(&h_nt, c->toff.p + c->total_n, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+    GDT_CHECK(gdt_copy_u32(c->toff.p + c->total_n, reinterpret_cast<uint32_t*>(c->status.p + ST_NTRI), s));
     if ((rc = read_status(c, s)) != GDT_OK) return rc;
+    const unsigned int h_nt = (unsigned int)c->h_status[ST_NTRI];
     if (ntri) *ntri = (int)h_nt;
     if (total_nbrs) *total_nbrs = (long long)c->h_counters[CT_INSUM];
     return (int)h_nt > cap ? GDT_ERR_INVALID : GDT_OK;
@@ -661,18 +683,16 @@ static int test_and_repair_sets(gdt_context* c, const double* d_xy, int stride, 
     GDT_CHECK(c->repair.reserve((size_t)n));
     int rc;
     if ((rc = reset_status(c, s)) != GDT_OK) return rc;
-    const bool tiled = tiled_sets(c);
     // 1. cell list of the moved points (DelaunayGPU.cpp:580-586)
     if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
-    // 2. triangles of the current rings (getCircumcirclesGPU, DelaunayGPU.cpp:650-666)
-    if ((rc = enumerate(c, d_nbrs, d_counts, stride, c->tris.p, cap, tiled, s)) != GDT_OK) return rc;
-    // 3. empty-circle test (testTriangulation, DelaunayGPU.cpp:607-648)
-    if ((rc = test_triangles(c, d_xy, tiled, cap, s)) != GDT_OK) return rc;
+    // 2. triangles of the current rings (getCircumcirclesGPU, DelaunayGPU.cpp:650-666) and
+    // 3. their empty-circle test (testTriangulation, DelaunayGPU.cpp:607-648)
+    if ((rc = triangles_and_test(c, d_xy, d_nbrs, d_counts, stride, cap, s)) != GDT_OK) return rc;
     // 4. repair the flagged points (locallyRepairDelaunayTriangulation, DelaunayGPU.cpp:87-115)
     if ((rc = repair_flagged(c, c->repair.p, stride, d_nbrs, d_counts, precision, s)) != GDT_OK) return rc;
-    unsigned int h_nt = 0;
-    GDT_CHECK(hipMemcpyAsync(&h_nt, c->toff.p + n, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+    GDT_CHECK(gdt_copy_u32(c->toff.p + n, reinterpret_cast<uint32_t*>(c->status.p + ST_NTRI), s));
     if ((rc = read_status(c, s)) != GDT_OK) return rc;
+    const unsigned int h_nt = (unsigned int)c->h_status[ST_NTRI];
     const int nf = c->h_status[ST_NFLAGGED];
     if (nflagged) *nflagged = nf;
     c->stats.flagged = nf;
@@ -727,10 +747,8 @@ static int md_enqueue_step(gdt_context* c, double* d_xy, int stride, int* d_nbrs
     int rc;
     GDT_CHECK(gdt_launch_md_displace(c->d_sets.p, c->pch.p, c->npch, d_xy, sigma, seed, c->md.p, s));
     if ((rc = reset_status(c, s)) != GDT_OK) return rc;
-    const bool tiled = tiled_sets(c);
     if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
-    if ((rc = enumerate(c, d_nbrs, d_counts, stride, c->tris.p, cap, tiled, s)) != GDT_OK) return rc;
-    if ((rc = test_triangles(c, d_xy, tiled, cap, s)) != GDT_OK) return rc;
+    if ((rc = triangles_and_test(c, d_xy, d_nbrs, d_counts, stride, cap, s)) != GDT_OK) return rc;
     if ((rc = repair_flagged(c, c->repair.p, stride, d_nbrs, d_counts, precision, s)) != GDT_OK) return rc;
     GDT_CHECK(gdt_launch_md_step_end(c->status.p, c->counters.p, c->toff.p + n, n, c->md.p, s));
     return GDT_OK;

@@ -168,6 +168,12 @@ __global__ void k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restric
     if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
 }
 
+hipError_t gdt_copy_u32(const uint32_t* src, uint32_t* dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy_u32, dim3(1), dim3(64), 0, s, src, dst);
+    return hipGetLastError();
+}
+
 size_t gdt_scan_tmp_words(size_t n)
 {
     size_t words = 0;

@@ -39,6 +39,8 @@ hipError_t gdt_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* t
                         uint32_t* tmp, hipStream_t s);
 
 hipError_t gdt_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+// *dst = *src, on the stream (one word).
+hipError_t gdt_copy_u32(const uint32_t* src, uint32_t* dst, hipStream_t s);
 // Zero nst status words and nct counters (nst, nct <= 64).
 hipError_t gdt_status_reset(int* st, int nst, unsigned long long* ct, int nct, hipStream_t s);
 // Copy them to host memory hst/hct (device-visible pinned pointers), then
@@ -105,6 +107,16 @@ hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, con
 hipError_t gdt_launch_test_tile(const GdtSet* d_sets, const GdtChunk* tiles, int ntiles,
                                 const int* tris, const uint32_t* toff, int cap, int total_n,
                                 const double* d_xy, const GdtCells& c, int* repair, hipStream_t s);
+
+// Enumeration and empty-circle test in one pass over the rings, one block per
+// tile (tiled cell order): triangles owned by their smallest caller index,
+// flags as gdt_launch_test_tile, counters[CT_INSUM] += sum of counts, *ntri =
+// number of owned triangles; partials holds 2 * ntiles words.
+hipError_t gdt_launch_ring_test_tile(const GdtSet* d_sets, const GdtChunk* tiles, int ntiles,
+                                     const int* nbrs, const int* counts, int stride, int total_n,
+                                     const double* d_xy, const GdtCells& c, int* repair,
+                                     unsigned long long* partials, uint32_t* ntri,
+                                     unsigned long long* counters, int* status, hipStream_t s);
 
 // Slots of the flagged points, in slot order -> worklist; markscan (total_n+1
 // words) = exclusive scan of the flags with the count at markscan[total_n];
