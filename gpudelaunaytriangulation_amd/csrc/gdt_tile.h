@@ -183,11 +183,21 @@ GDT_HD inline GdtClip gdt_left_clip(bool wrap, float ax, float ay, float ox, flo
     return cl;
 }
 
+// Part sub of W equal parts of the LDS range [k, e) (W lanes sharing a point).
+GDT_HD inline void gdt_range_part(uint32_t& k, uint32_t& e, int sub, int W)
+{
+    if (W > 1) {
+        const uint32_t len = e - k;
+        e = k + len * (uint32_t)(sub + 1) / (uint32_t)W;
+        k = k + len * (uint32_t)sub / (uint32_t)W;
+    }
+}
+
 // Row table of the block [-h,h]^2 around window cell c0 (h <= 2), each row
-// clipped to the columns cl keeps: one word per non-empty row, first LDS entry
-// | end << 16.  Returns the number of entries.
+// clipped to the columns cl keeps (and cut to part sub of W): one word per
+// non-empty row, first LDS entry | end << 16.  Returns the number of entries.
 GDT_HD inline unsigned gdt_block_rows(const GdtTileLds& L, int c0, int h, const GdtClip& cl,
-                                      uint32_t* tbl)
+                                      uint32_t* tbl, int sub = 0, int W = 1)
 {
     unsigned n = 0;
     int ns = 0;
@@ -203,7 +213,8 @@ GDT_HD inline unsigned gdt_block_rows(const GdtTileLds& L, int c0, int h, const 
         }
         const int rb = c0 + dy * GDT_WIN;
         if (lo <= hi) {
-            const uint32_t k = L.cs[rb + lo], e = L.cs[rb + hi + 1];
+            uint32_t k = L.cs[rb + lo], e = L.cs[rb + hi + 1];
+            gdt_range_part(k, e, sub, W);
             if (e > k) tbl[ns++] = k | (e << 16);
             n += e - k;
         }
@@ -217,11 +228,13 @@ GDT_HD inline unsigned gdt_block_rows(const GdtTileLds& L, int c0, int h, const 
 // clipped to the columns cl keeps, without the scanned square [-hs,hs]^2, as
 // row segments in tbl (at most 4, two rows at a time).  With `outer` (the
 // disk's column range lies within [-hs,hs]) the rows through the square have
-// nothing left and are skipped.  Returns the entries.
+// nothing left and are skipped.  Segments are cut to part sub of W.  Returns
+// the entries.
 template <class T>
 GDT_HD inline unsigned gdt_disk_rows(const GdtTileLds& L, int c0, int& dy, int dy1, int hs,
                                      bool outer, T qxlo, T qxhi, T qylo, T qyhi, T R, T ox, T oy,
-                                     T bsy, T ibsx, T margin, const GdtClip& cl, uint32_t* tbl)
+                                     T bsy, T ibsx, T margin, const GdtClip& cl, uint32_t* tbl,
+                                     int sub = 0, int W = 1)
 {
     unsigned n = 0;
     int ns = 0;
@@ -246,7 +259,8 @@ GDT_HD inline unsigned gdt_disk_rows(const GdtTileLds& L, int c0, int& dy, int d
         const int rb = c0 + dy * GDT_WIN;
         auto seg = [&](int a, int b) {
             if (a > b) return;
-            const uint32_t k = L.cs[rb + a], e = L.cs[rb + b + 1];
+            uint32_t k = L.cs[rb + a], e = L.cs[rb + b + 1];
+            gdt_range_part(k, e, sub, W);
             if (e > k) tbl[ns++] = k | (e << 16);
             n += e - k;
         };
@@ -413,9 +427,14 @@ GDT_HD inline int gdt_tile_advance(const GdtTileLds& L, const GdtRingIO& io, Gdt
     return STEP_RUN;
 }
 
+// W lanes may share the point (W = 2, device only): each scans its part of
+// every row segment, and the pair then agrees on the better of its two bests,
+// so both end every step in the same state.
+template <int W = 1>
 GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const GdtRingIO& io,
-                                    GdtTileLane<double>& st, unsigned& npred)
+                                    GdtTileLane<double>& st, unsigned& npred, int sub = 0)
 {
+    static_assert(W == 1 || W == 2, "one or two lanes per point");
     const bool wrap = st.ak >= 0;
     int bk = -1;
     const double ax = st.ax, ay = st.ay, vx = st.vx, vy = st.vy;
@@ -456,6 +475,39 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
         }
     };
     auto visit_k = [&](int k) { visit(k, load(k)); };
+    // Pairs: the first lane keeps the partner's best when it lies strictly
+    // inside its own circle (nearest neighbour: strictly closer; ties to the
+    // lower entry), both lanes take that decision and recompute K and Pthr
+    // with the expressions of the visit, so their states are identical.
+    auto pair_reduce = [&]() {
+#if defined(__HIPCC__)
+        if constexpr (W > 1) {
+            const int pk = __shfl_xor(bk, 1, 64);
+            int win = bk;
+            if (pk >= 0 && pk != bk) {
+                if (bk < 0) {
+                    win = pk;
+                } else {
+                    const gdt_d2 X = L.X64[pk];
+                    const double px = X.x - vx, py = X.y - vy;
+                    const double P = gdt_incircle_cc64(K, px, py, gdt_dist2_64(px, py));
+                    if (P < Pthr || (P == Pthr && pk < bk)) win = pk;
+                }
+            }
+            win = __shfl(win, (int)(__lane_id() & ~1u), 64);
+            if (win != bk) {
+                bk = win;
+                const gdt_d2 X = L.X64[bk];
+                const double px = X.x - vx, py = X.y - vy;
+                const double cc = gdt_dist2_64(px, py);
+                K.k0 = gdt_orient64(ax, ay, px, py) + nnf;
+                K.kx = fma(cc, ay, -(aa * py));
+                K.ky = fma(aa, px, -(cc * ax));
+                Pthr = nnf * cc;
+            }
+        }
+#endif
+    };
     // the best's displacement and squared distance (the loop keeps only its entry)
     double bx = 0., by = 0., bd2 = 0.;
     auto best = [&]() {
@@ -472,9 +524,10 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
     const double ox = gdt_cell_offset(vx, wx, S.bsx), oy = gdt_cell_offset(vy, wy, S.bsy);
     const GdtClip cl = gdt_left_clip(wrap, (float)ax, (float)ay, (float)ox, (float)oy,
                                      (float)S.bsx, (float)S.bsy, 1e-3f);
-    const unsigned nb = gdt_block_rows(L, st.c0, hs, cl, st.tbl);
+    const unsigned nb = gdt_block_rows(L, st.c0, hs, cl, st.tbl, sub, W);
     gdt_scan_tbl(st.tbl, nb, load, visit);
     npred += nb;
+    pair_reduce();
     if (bk < 0 && hs < GDT_SCAN) {
         npred += gdt_lds_block(L, wx, wy, -GDT_SCAN, GDT_SCAN, -GDT_SCAN, GDT_SCAN, hs, visit_k);
         hs = GDT_SCAN;
@@ -524,10 +577,11 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
         for (int dy = dy0; dy <= dy1;) {
             const unsigned ne = gdt_disk_rows<double>(L, st.c0, dy, dy1, hs, outer, cqx - cm,
                                                       cqx + cm, cqy - cm, cqy + cm, crad, ox, oy,
-                                                      S.bsy, S.ibsx, 1e-3, cl, st.tbl);
+                                                      S.bsy, S.ibsx, 1e-3, cl, st.tbl, sub, W);
             gdt_scan_tbl(st.tbl, ne, load, visit);
             npred += ne;
         }
+        pair_reduce();
         ok = true;
     }
     if (!ok) return STEP_DEFER;
@@ -767,7 +821,7 @@ GDT_HD inline GdtRingResult gdt_tile_ring(const GdtSet& S, const GdtTileWin& w, 
         GdtTileLane<double> st;
         gdt_tile_lane_init<double>(w, L, j, io, S, st);
         st.tbl = tbl;
-        while (rc == STEP_RUN) rc = gdt_tile_step_f64(S, L, io, st, res.npred);
+        while (rc == STEP_RUN) rc = gdt_tile_step_f64<1>(S, L, io, st, res.npred);
         res.count = st.count;
     }
     if (rc == STEP_DEFER) res.err = ERR_DEFER;

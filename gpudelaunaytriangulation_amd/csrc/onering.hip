@@ -179,8 +179,10 @@ __global__ __launch_bounds__(256) void k_one_ring_wave(const GdtSet* __restrict_
 // staging.  The points of a window that does not fit the LDS budget, or of a
 // grid too small for the scan block, and points with a step whose circle
 // leaves the window, are appended to `defer` and handled by k_one_ring_wave
-// in the next launch.
-template <int PREC>
+// in the next launch.  With W = 2 (FP64, lists of points) two neighbouring
+// lanes share each point, each scanning half of every row (a list leaves most
+// lanes of a block without a point otherwise).
+template <int PREC, int W>
 __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict__ sets, const GdtChunk* __restrict__ tiles,
                      const int* __restrict__ list, const uint32_t* __restrict__ lscan,
                      GdtRingIO io, int* __restrict__ counts, int* __restrict__ status,
@@ -221,15 +223,17 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
     // Lanes take points of the tile and advance them one search step per
     // iteration; a lane whose point is finished takes the next one, so a
     // wave's iterations track the total number of steps, not the longest ring.
+    static_assert(W == 1 || (W == 2 && PREC == 64), "lane pairs: FP64 only");
     if (tid == 0) {
-        s_next = Q0 + nt;
+        s_next = Q0 + nt / W;
         s_dn = 0u;
     }
     __syncthreads();
     using Real = typename std::conditional<PREC == 32, float, double>::type;
     GdtTileLane<Real> st;
     st.tbl = s_tbl + 5 * tid;
-    int q = Q0 + tid;
+    const int sub = tid & (W - 1);      // the lane's part of its point's rows
+    int q = Q0 + tid / W;
     bool have = q < Q1;
     int j = have ? (list ? list[q] : q) : 0;
     if (have) gdt_tile_lane_init<Real>(w, L, j, io, S, st);
@@ -241,8 +245,9 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
             if constexpr (PREC == 32)
                 rc = gdt_tile_step_f32(S, L, io, st, npred, nfb);
             else
-                rc = gdt_tile_step_f64(S, L, io, st, npred);
-            if (rc != STEP_RUN) {
+                rc = gdt_tile_step_f64<W>(S, L, io, st, npred, sub);
+            // (both lanes of a pair end each step in the same state; the first reports)
+            if (rc != STEP_RUN && sub == 0) {
                 if (rc == STEP_DEFER) {
                     // the block's own waves finish it after the tile (below)
                     const uint32_t dpos = atomicAdd(&s_dn, 1u);
@@ -258,7 +263,15 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
                     nsum += (unsigned)st.count;
                     if (rc == STEP_NOCLOSE) er |= ERR_NOCLOSE;
                 }
-                q = atomicAdd(&s_next, 1);
+            }
+            if (rc != STEP_RUN) {
+                if constexpr (W > 1) {
+                    int qq = 0;
+                    if (sub == 0) qq = atomicAdd(&s_next, 1);
+                    q = __shfl(qq, (tid & 63) & ~(W - 1), 64);
+                } else {
+                    q = atomicAdd(&s_next, 1);
+                }
                 have = q < Q1;
                 if (have) {
                     j = list ? list[q] : q;
@@ -332,11 +345,14 @@ hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nse
     if (block > 1024) block = 1024;
     const size_t tbl = (5 * (size_t)block + 1) * sizeof(uint32_t);
     if (precision == 32)
-        hipLaunchKernelGGL(k_one_ring_tile<32>, dim3(ntiles), dim3(block), tbl, s, d_sets, tiles,
-                           list, lscan, io, counts, status, counters, defer);
+        hipLaunchKernelGGL((k_one_ring_tile<32, 1>), dim3(ntiles), dim3(block), tbl, s, d_sets,
+                           tiles, list, lscan, io, counts, status, counters, defer);
+    else if (list)
+        hipLaunchKernelGGL((k_one_ring_tile<64, 2>), dim3(ntiles), dim3(block), tbl, s, d_sets,
+                           tiles, list, lscan, io, counts, status, counters, defer);
     else
-        hipLaunchKernelGGL(k_one_ring_tile<64>, dim3(ntiles), dim3(block), tbl, s, d_sets, tiles,
-                           list, lscan, io, counts, status, counters, defer);
+        hipLaunchKernelGGL((k_one_ring_tile<64, 1>), dim3(ntiles), dim3(block), tbl, s, d_sets,
+                           tiles, list, lscan, io, counts, status, counters, defer);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (after_tile && (e = hipEventRecord(after_tile, s)) != hipSuccess) return e;
