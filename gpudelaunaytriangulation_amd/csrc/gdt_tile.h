@@ -427,14 +427,14 @@ GDT_HD inline int gdt_tile_advance(const GdtTileLds& L, const GdtRingIO& io, Gdt
     return STEP_RUN;
 }
 
-// W lanes may share the point (W = 2, device only): each scans its part of
-// every row segment, and the pair then agrees on the better of its two bests,
-// so both end every step in the same state.
+// W lanes may share the point (W = 2 or 4, device only): each scans its part
+// of every row segment, and the group then agrees on the best of their bests,
+// so all end every step in the same state.
 template <int W = 1>
 GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const GdtRingIO& io,
                                     GdtTileLane<double>& st, unsigned& npred, int sub = 0)
 {
-    static_assert(W == 1 || W == 2, "one or two lanes per point");
+    static_assert(W == 1 || W == 2 || W == 4, "1, 2 or 4 lanes per point");
     const bool wrap = st.ak >= 0;
     int bk = -1;
     const double ax = st.ax, ay = st.ay, vx = st.vx, vy = st.vy;
@@ -475,35 +475,39 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
         }
     };
     auto visit_k = [&](int k) { visit(k, load(k)); };
-    // Pairs: the first lane keeps the partner's best when it lies strictly
-    // inside its own circle (nearest neighbour: strictly closer; ties to the
-    // lower entry), both lanes take that decision and recompute K and Pthr
-    // with the expressions of the visit, so their states are identical.
+    // Lane groups, pairwise by lane distance o = 1, 2, ...: the lower lane of
+    // each pair keeps its partner's best when it lies strictly inside its own
+    // circle (nearest neighbour: strictly closer; ties to the lower entry),
+    // both take that decision and recompute K and Pthr with the expressions
+    // of the visit, so the group's states end identical.
     auto pair_reduce = [&]() {
 #if defined(__HIPCC__)
         if constexpr (W > 1) {
-            const int pk = __shfl_xor(bk, 1, 64);
-            int win = bk;
-            if (pk >= 0 && pk != bk) {
-                if (bk < 0) {
-                    win = pk;
-                } else {
-                    const gdt_d2 X = L.X64[pk];
-                    const double px = X.x - vx, py = X.y - vy;
-                    const double P = gdt_incircle_cc64(K, px, py, gdt_dist2_64(px, py));
-                    if (P < Pthr || (P == Pthr && pk < bk)) win = pk;
+#pragma unroll
+            for (int o = 1; o < W; o <<= 1) {
+                const int pk = __shfl_xor(bk, o, 64);
+                int win = bk;
+                if (pk >= 0 && pk != bk) {
+                    if (bk < 0) {
+                        win = pk;
+                    } else {
+                        const gdt_d2 X = L.X64[pk];
+                        const double px = X.x - vx, py = X.y - vy;
+                        const double P = gdt_incircle_cc64(K, px, py, gdt_dist2_64(px, py));
+                        if (P < Pthr || (P == Pthr && pk < bk)) win = pk;
+                    }
                 }
-            }
-            win = __shfl(win, (int)(__lane_id() & ~1u), 64);
-            if (win != bk) {
-                bk = win;
-                const gdt_d2 X = L.X64[bk];
-                const double px = X.x - vx, py = X.y - vy;
-                const double cc = gdt_dist2_64(px, py);
-                K.k0 = gdt_orient64(ax, ay, px, py) + nnf;
-                K.kx = fma(cc, ay, -(aa * py));
-                K.ky = fma(aa, px, -(cc * ax));
-                Pthr = nnf * cc;
+                win = __shfl(win, (int)(__lane_id() & ~(unsigned)(2 * o - 1)), 64);
+                if (win != bk) {
+                    bk = win;
+                    const gdt_d2 X = L.X64[bk];
+                    const double px = X.x - vx, py = X.y - vy;
+                    const double cc = gdt_dist2_64(px, py);
+                    K.k0 = gdt_orient64(ax, ay, px, py) + nnf;
+                    K.kx = fma(cc, ay, -(aa * py));
+                    K.ky = fma(aa, px, -(cc * ax));
+                    Pthr = nnf * cc;
+                }
             }
         }
 #endif

@@ -182,6 +182,10 @@ __global__ __launch_bounds__(256) void k_one_ring_wave(const GdtSet* __restrict_
 // in the next launch.  With W = 2 (FP64, lists of points) two neighbouring
 // lanes share each point, each scanning half of every row (a list leaves most
 // lanes of a block without a point otherwise).
+#ifndef GDT_LIST_LANES
+#define GDT_LIST_LANES 2    // lanes per point of the FP64 tile kernel over a list of points
+#endif
+
 template <int PREC, int W>
 __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict__ sets, const GdtChunk* __restrict__ tiles,
                      const int* __restrict__ list, const uint32_t* __restrict__ lscan,
@@ -223,7 +227,7 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
     // Lanes take points of the tile and advance them one search step per
     // iteration; a lane whose point is finished takes the next one, so a
     // wave's iterations track the total number of steps, not the longest ring.
-    static_assert(W == 1 || (W == 2 && PREC == 64), "lane pairs: FP64 only");
+    static_assert(W == 1 || PREC == 64, "lane groups: FP64 only");
     if (tid == 0) {
         s_next = Q0 + nt / W;
         s_dn = 0u;
@@ -348,7 +352,7 @@ hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nse
         hipLaunchKernelGGL((k_one_ring_tile<32, 1>), dim3(ntiles), dim3(block), tbl, s, d_sets,
                            tiles, list, lscan, io, counts, status, counters, defer);
     else if (list)
-        hipLaunchKernelGGL((k_one_ring_tile<64, 2>), dim3(ntiles), dim3(block), tbl, s, d_sets,
+        hipLaunchKernelGGL((k_one_ring_tile<64, GDT_LIST_LANES>), dim3(ntiles), dim3(block), tbl, s, d_sets,
                            tiles, list, lscan, io, counts, status, counters, defer);
     else
         hipLaunchKernelGGL((k_one_ring_tile<64, 1>), dim3(ntiles), dim3(block), tbl, s, d_sets,
