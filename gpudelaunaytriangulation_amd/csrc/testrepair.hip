@@ -142,23 +142,28 @@ struct GdtTriCircle {
     double R2, lim;      // squared radius and the conservative test bound
 };
 
-__device__ inline GdtTriCircle tri_circle(const GdtSet& S, const gdt_d2* __restrict__ xy, int i,
-                                          int a, int b)
+// Minimum images use the inverse box sides (ilx, ily): neighbours are far
+// closer than half a box, so the rounded quotient picks the same image.  The
+// centre uses the hardware reciprocal with one Newton step (relative error
+// ~1e-15, far inside the test's 1e-9 margin).
+__device__ inline GdtTriCircle tri_circle(const GdtSet& S, double ilx, double ily,
+                                          const gdt_d2* __restrict__ xy, int i, int a, int b)
 {
     GdtTriCircle c;
     const gdt_d2 v = xy[i], pa = xy[a], pb = xy[b];
     double ax = pa.x - v.x, ay = pa.y - v.y, bx = pb.x - v.x, by = pb.y - v.y;
-    ax -= S.lx * rint(ax / S.lx);
-    ay -= S.ly * rint(ay / S.ly);
-    bx -= S.lx * rint(bx / S.lx);
-    by -= S.ly * rint(by / S.ly);
+    ax -= S.lx * rint(ax * ilx);
+    ay -= S.ly * rint(ay * ily);
+    bx -= S.lx * rint(bx * ilx);
+    by -= S.ly * rint(by * ily);
     const double D = ax * by - ay * bx;
     c.bad = !(D > 0.);
     c.qx = c.qy = c.R2 = c.lim = 0.;
     if (!c.bad) {
         const double aa = ax * ax + ay * ay, bb = bx * bx + by * by;
-        c.qx = (aa * by - bb * ay) / (2. * D);
-        c.qy = (bb * ax - aa * bx) / (2. * D);
+        const double i2d = gdt_rcp64(2. * D);
+        c.qx = (aa * by - bb * ay) * i2d;
+        c.qy = (bb * ax - aa * bx) * i2d;
         c.R2 = c.qx * c.qx + c.qy * c.qy;
         c.lim = c.R2 * (1. + 1e-9);
     }
@@ -227,7 +232,7 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_test(const GdtSet* __restrict__ s
             (unsigned)b >= (unsigned)total_n)
             continue;   // malformed ring entry: the Euler check reports it
         const GdtSet S = sets[gdt_set_of(sets, nsets, i)];
-        const GdtTriCircle c = tri_circle(S, xy, i, a, b);
+        const GdtTriCircle c = tri_circle(S, 1.0 / S.lx, 1.0 / S.ly, xy, i, a, b);
         bool bad = c.bad;
         int intruder = -1;
         if (!bad) {
@@ -257,7 +262,8 @@ hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, con
 // window L holds (when `fits`): the window cells of the circle's conservative
 // bounding box are scanned in LDS, and a circle whose box leaves the window, or
 // a window too full for LDS, takes the global scan.  Flags the repair rows.
-__device__ inline void tri_test_in_tile(const GdtSet& S, const GdtTileWin& w, bool fits,
+__device__ inline void tri_test_in_tile(const GdtSet& S, double ilx, double ily,
+                                        const GdtTileWin& w, bool fits,
                                         const GdtTileLds& L, const gdt_d2* __restrict__ xy,
                                         const GdtRingIO& io, const int* __restrict__ slot_of,
                                         int i, int a, int b, int si, int* __restrict__ repair)
@@ -267,7 +273,7 @@ __device__ inline void tri_test_in_tile(const GdtSet& S, const GdtTileWin& w, bo
     const gdt_d2* s_X64 = L.X64;
     const int* s_orig = L.orig;
     {
-        const GdtTriCircle c = tri_circle(S, xy, i, a, b);
+        const GdtTriCircle c = tri_circle(S, ilx, ily, xy, i, a, b);
         if (c.bad) {
             tri_flag(repair, i, a, b, -1);
             return;
@@ -281,11 +287,11 @@ __device__ inline void tri_test_in_tile(const GdtSet& S, const GdtTileWin& w, bo
             const int cv = wy * GDT_WIN + wx;
             const gdt_d2 Xv = s_X64[s_cs[cv] + (si - s_gstart[cv])];
             const double cx = Xv.x + c.qx, cy = Xv.y + c.qy;
-            const double R = sqrt(c.R2) * (1.0 + 1e-6);
-            const int x0 = (int)floor((cx - R) / S.bsx - 1e-3) + GDT_FRAME;
-            const int x1 = (int)floor((cx + R) / S.bsx + 1e-3) + GDT_FRAME;
-            const int y0 = (int)floor((cy - R) / S.bsy - 1e-3) + GDT_FRAME;
-            const int y1 = (int)floor((cy + R) / S.bsy + 1e-3) + GDT_FRAME;
+            const double R = gdt_sqrt64(c.R2) * (1.0 + 1e-6);
+            const int x0 = (int)floor((cx - R) * S.ibsx - 1e-3) + GDT_FRAME;
+            const int x1 = (int)floor((cx + R) * S.ibsx + 1e-3) + GDT_FRAME;
+            const int y0 = (int)floor((cy - R) * S.ibsy - 1e-3) + GDT_FRAME;
+            const int y1 = (int)floor((cy + R) * S.ibsy + 1e-3) + GDT_FRAME;
             if (x0 >= 0 && x1 < GDT_WIN && y0 >= 0 && y1 < GDT_WIN && x1 - x0 <= 2 * S.rmax &&
                 y1 - y0 <= 2 * S.rmax) {
                 for (int y = y0; y <= y1 && intruder < 0; ++y) {
@@ -334,6 +340,7 @@ __global__ __launch_bounds__(512) void k_test_tile(const GdtSet* __restrict__ se
 
     const GdtChunk ch = tiles[blockIdx.x];
     const GdtSet S = sets[ch.set];
+    const double ilx = 1.0 / S.lx, ily = 1.0 / S.ly;
     const int k0 = S.c0 + ch.start * GDT_TILE_CELLS;
     const int T0 = (int)io.cstart[k0], T1 = (int)io.cstart[k0 + GDT_TILE_CELLS];
     const int R0 = min((int)toff[T0], cap), R1 = min((int)toff[T1], cap);
@@ -348,7 +355,7 @@ __global__ __launch_bounds__(512) void k_test_tile(const GdtSet* __restrict__ se
         if ((unsigned)i >= (unsigned)total_n || (unsigned)a >= (unsigned)total_n ||
             (unsigned)b >= (unsigned)total_n)
             continue;   // malformed ring entry: the Euler check reports it
-        tri_test_in_tile(S, w, fits, L, xy, io, slot_of, i, a, b, slot_of[i], repair);
+        tri_test_in_tile(S, ilx, ily, w, fits, L, xy, io, slot_of, i, a, b, slot_of[i], repair);
     }
 }
 
@@ -384,6 +391,7 @@ __global__ __launch_bounds__(512) void k_ring_test_tile(const GdtSet* __restrict
     const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
     const GdtChunk ch = tiles[blockIdx.x];
     const GdtSet S = sets[ch.set];
+    const double ilx = 1.0 / S.lx, ily = 1.0 / S.ly;
     const int k0 = S.c0 + ch.start * GDT_TILE_CELLS;
     const int T0 = (int)io.cstart[k0], T1 = (int)io.cstart[k0 + GDT_TILE_CELLS];
     if (T0 >= T1) {     // block-uniform: an empty tile
@@ -436,7 +444,7 @@ __global__ __launch_bounds__(512) void k_ring_test_tile(const GdtSet* __restrict
             const int nl = min((int)s_tn, GDT_RT_LCAP);
             for (int t = tid; t < nl; t += nt) {
                 const int4 e = s_tri[t];
-                tri_test_in_tile(S, w, fits, L, xy, io, slot_of, e.x, e.y, e.z, e.w, repair);
+                tri_test_in_tile(S, ilx, ily, w, fits, L, xy, io, slot_of, e.x, e.y, e.z, e.w, repair);
             }
             // also the barrier before the list is rewritten
             if (!__syncthreads_or(pending)) break;
