@@ -49,6 +49,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 BIN = os.path.join(PKG, "bin")
 DROPIN_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_check.cpp")
 DROPIN_BIN = os.path.join(BIN, "dropin_check")
+GPUARRAY_SRC = os.path.join(ROOT, "tests", "cpp", "gpuarray_check.cpp")
+GPUARRAY_BIN = os.path.join(BIN, "gpuarray_check")
 
 
 def cxx_dropin_cmd(src: str, out: str) -> list[str]:
@@ -58,15 +60,14 @@ def cxx_dropin_cmd(src: str, out: str) -> list[str]:
             "-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath,{ROCM}/lib", "-o", out]
 
 
-def build_dropin_check() -> str:
-    """The C++ drop-in check program (run by the GPU tests)."""
+def build_dropin_check(src: str = DROPIN_SRC, out: str = DROPIN_BIN) -> str:
+    """A C++ drop-in check program (run by the GPU tests)."""
     os.makedirs(BIN, exist_ok=True)
-    r = subprocess.run(cxx_dropin_cmd(DROPIN_SRC, DROPIN_BIN + ".tmp"), capture_output=True,
-                       text=True)
+    r = subprocess.run(cxx_dropin_cmd(src, out + ".tmp"), capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"C++ drop-in check failed to build:\n{r.stderr}")
-    os.replace(DROPIN_BIN + ".tmp", DROPIN_BIN)
-    return DROPIN_BIN
+        raise RuntimeError(f"C++ drop-in check {os.path.basename(src)} failed to build:\n{r.stderr}")
+    os.replace(out + ".tmp", out)
+    return out
 
 
 def build(verbose: bool = False) -> str:
@@ -81,8 +82,9 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
-    if os.path.exists(DROPIN_SRC):
-        build_dropin_check()
+    for src, out in ((DROPIN_SRC, DROPIN_BIN), (GPUARRAY_SRC, GPUARRAY_BIN)):
+        if os.path.exists(src):
+            build_dropin_check(src, out)
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -4,8 +4,9 @@
 // Class surface kept from the reference so that callers compile unchanged:
 //   Index2D                       indexer.h:32-62
 //   periodicBoundaries, PeriodicBoxPtr   periodicBoundaries.h:25-73, 191
-//   GPUArray<T>, ArrayHandle<T>   gpuarray.h:26-62, 305-328, 429-590,
-//                                 983-1127 (host/device mirror), 1316-1335 (resize)
+//   GPUArray<T>, ArrayHandle<T>,  gpuarray.h:26-62, 305-380, 429-590,
+//   ArrayHandleAsync<T>           983-1127 (host/device mirror), 1316-1373 (1-D and
+//                                 2-D resize), mapped arrays
 //   cellListGPU                   cellListGPU.h:17-121
 //   DelaunayGPU                   DelaunayGPU.h:18-58
 // Errors: a failing HIP call or library status throws std::runtime_error (the
@@ -103,14 +104,36 @@ struct access_location { enum Enum { host, device }; };
 struct access_mode { enum Enum { read, readwrite, overwrite }; };
 struct data_location { enum Enum { host, device, hostdevice }; };
 
-//! Host/device mirrored array.  Data live where they were last written; an
-//! acquire copies only when the requested side is stale and the mode is not
-//! overwrite (gpuarray.h:983-1127).  One handle at a time (gpuarray.h:991).
+//! Host/device mirrored array (gpuarray.h:429-590, 983-1127).  Data live where
+//! they were last written; an acquire copies only when the requested side is
+//! stale and the mode is not overwrite.  One handle at a time (gpuarray.h:991).
+//! 2-D arrays pad each row to the reference's pitch, width + 16 - width % 16
+//! (gpuarray.h:673-677).  Mapped arrays live in pinned host memory that the
+//! device addresses directly (hipHostMalloc with hipHostMallocMapped, not the
+//! reference's registration of existing memory): acquires copy nothing, and a
+//! synchronous host acquire of device data synchronizes the device
+//! (gpuarray.h:936-940).  Asynchronous acquires (ArrayHandleAsync) copy on the
+//! null stream.
 template <class T>
 class GPUArray {
 public:
     GPUArray() = default;
-    explicit GPUArray(unsigned int n) { allocate(n); }
+    explicit GPUArray(unsigned int num_elements) { allocate(num_elements, num_elements, num_elements ? 1u : 0u); }
+    GPUArray(unsigned int width, unsigned int height)
+    {
+        const unsigned int p = pitch_of(width);
+        allocate(p * height, p, height);
+    }
+    GPUArray(unsigned int num_elements, bool mapped) : m_mapped(mapped)
+    {
+        allocate(num_elements, num_elements, num_elements ? 1u : 0u);
+    }
+    explicit GPUArray(bool mapped) : m_mapped(mapped) {}
+    GPUArray(unsigned int width, unsigned int height, bool mapped) : m_mapped(mapped)
+    {
+        const unsigned int p = pitch_of(width);
+        allocate(p * height, p, height);
+    }
     GPUArray(const GPUArray& o) { copy_from(o); }
     GPUArray& operator=(const GPUArray& o)
     {
@@ -129,61 +152,71 @@ public:
     ~GPUArray() { deallocate(); }
 
     unsigned int getNumElements() const { return m_n; }
+    //! Row width in elements including padding (1-D: the element count).
+    unsigned int getPitch() const { return m_pitch; }
+    //! Rows (1-D: 1).
+    unsigned int getHeight() const { return m_height; }
     bool isNull() const { return m_h == nullptr; }
+    bool isAcquired() const { return m_acquired; }
 
     void swap(GPUArray& o) noexcept
     {
         std::swap(m_n, o.m_n);
+        std::swap(m_pitch, o.m_pitch);
+        std::swap(m_height, o.m_height);
         std::swap(m_h, o.m_h);
         std::swap(m_d, o.m_d);
         std::swap(m_loc, o.m_loc);
         std::swap(m_acquired, o.m_acquired);
+        std::swap(m_mapped, o.m_mapped);
     }
 
-    //! Resize, keeping the first min(old, new) elements (gpuarray.h:1316-1335).
+    //! Resize, keeping the first min(old, new) elements of both copies; new
+    //! elements are zero (gpuarray.h:1316-1335).
     void resize(unsigned int n)
     {
         if (m_acquired) throw std::logic_error("GPUArray::resize while acquired");
-        if (n == m_n) return;
-        GPUArray<T> t(n);
-        const size_t keep = (size_t)std::min(n, m_n) * sizeof(T);
-        if (keep) {
-            if (m_loc == data_location::device) {
-                gdt_detail::hip_check(hipMemcpy(t.m_d, m_d, keep, hipMemcpyDeviceToDevice),
-                                      "GPUArray::resize");
-                t.m_loc = data_location::device;
-            } else {
-                std::memcpy(t.m_h, m_h, keep);
-                if (m_loc == data_location::host) {
-                    t.m_loc = data_location::host;
-                } else {
-                    gdt_detail::hip_check(hipMemcpy(t.m_d, m_d, keep, hipMemcpyDeviceToDevice),
-                                          "GPUArray::resize");
-                    t.m_loc = data_location::hostdevice;
-                }
-            }
-        }
+        if (n == m_n && m_height <= 1) return;
+        GPUArray<T> t(n, m_mapped);
+        copy_rows(t, std::min(n, m_n), 1, 0, 0);
+        t.m_loc = m_loc;
+        swap(t);
+    }
+    //! Resize a 2-D array, keeping the overlapping rows and columns of both
+    //! copies; new elements are zero (gpuarray.h:1344-1373).
+    void resize(unsigned int width, unsigned int height)
+    {
+        if (m_acquired) throw std::logic_error("GPUArray::resize while acquired");
+        GPUArray<T> t(width, height, m_mapped);
+        if (!isNull())
+            copy_rows(t, std::min(m_pitch, t.m_pitch), std::min(m_height, height), m_pitch,
+                      t.m_pitch);
+        t.m_loc = isNull() ? t.m_loc : m_loc;
         swap(t);
     }
 
-    T* acquire(access_location::Enum where, access_mode::Enum mode)
+    //! Zero both copies from element `first` on (gpuarray.h:908-924).
+    void memclear(unsigned int first = 0)
+    {
+        if (isNull() || first >= m_n) return;
+        const size_t bytes = (size_t)(m_n - first) * sizeof(T);
+        std::memset(m_h + first, 0, bytes);
+        if (!m_mapped)
+            gdt_detail::hip_check(hipMemset(m_d + first, 0, bytes), "GPUArray::memclear");
+    }
+
+    T* acquire(access_location::Enum where, access_mode::Enum mode, bool async = false)
     {
         if (m_acquired) throw std::logic_error("GPUArray already acquired");
         if (isNull()) return nullptr;
         m_acquired = true;
-        const size_t bytes = (size_t)m_n * sizeof(T);
         if (where == access_location::host) {
-            if (m_loc == data_location::device && mode != access_mode::overwrite)
-                gdt_detail::hip_check(hipMemcpy(m_h, m_d, bytes, hipMemcpyDeviceToHost),
-                                      "GPUArray device->host");
-            const bool both = mode == access_mode::read &&
-                              m_loc != data_location::host;
+            if (m_loc == data_location::device && mode != access_mode::overwrite) to_host(async);
+            const bool both = mode == access_mode::read && m_loc != data_location::host;
             m_loc = both ? data_location::hostdevice : data_location::host;
             return m_h;
         }
-        if (m_loc == data_location::host && mode != access_mode::overwrite)
-            gdt_detail::hip_check(hipMemcpy(m_d, m_h, bytes, hipMemcpyHostToDevice),
-                                  "GPUArray host->device");
+        if (m_loc == data_location::host && mode != access_mode::overwrite) to_device(async);
         const bool both = mode == access_mode::read && m_loc != data_location::device;
         m_loc = both ? data_location::hostdevice : data_location::device;
         return m_d;
@@ -191,46 +224,93 @@ public:
     void release() { m_acquired = false; }
 
 private:
-    void allocate(unsigned int n)
+    static unsigned int pitch_of(unsigned int width) { return width + (16u - (width & 15u)); }
+
+    void allocate(unsigned int n, unsigned int pitch, unsigned int height)
     {
         m_n = n;
+        m_pitch = pitch;
+        m_height = height;
+        m_loc = data_location::hostdevice;
         if (!n) return;
         const size_t bytes = (size_t)n * sizeof(T);
+        if (m_mapped) {
+            gdt_detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m_h), bytes,
+                                                hipHostMallocMapped),
+                                  "GPUArray mapped allocation");
+            gdt_detail::hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&m_d), m_h, 0),
+                                  "GPUArray mapped device pointer");
+            std::memset(m_h, 0, bytes);
+            return;
+        }
         gdt_detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m_h), bytes, 0),
                               "GPUArray host allocation");
         gdt_detail::hip_check(hipMalloc(reinterpret_cast<void**>(&m_d), bytes),
                               "GPUArray device allocation");
         std::memset(m_h, 0, bytes);
         gdt_detail::hip_check(hipMemset(m_d, 0, bytes), "GPUArray device clear");
-        m_loc = data_location::hostdevice;
     }
     void deallocate()
     {
         if (m_h) (void)hipHostFree(m_h);
-        if (m_d) (void)hipFree(m_d);
+        if (m_d && !m_mapped) (void)hipFree(m_d);
         m_h = nullptr;
         m_d = nullptr;
-        m_n = 0;
+        m_n = m_pitch = m_height = 0;
         m_acquired = false;
         m_loc = data_location::hostdevice;
     }
+    void to_host(bool async)
+    {
+        const size_t bytes = (size_t)m_n * sizeof(T);
+        if (m_mapped) {
+            if (!async) gdt_detail::hip_check(hipDeviceSynchronize(), "GPUArray mapped sync");
+            return;
+        }
+        gdt_detail::hip_check(async ? hipMemcpyAsync(m_h, m_d, bytes, hipMemcpyDeviceToHost, nullptr)
+                                    : hipMemcpy(m_h, m_d, bytes, hipMemcpyDeviceToHost),
+                              "GPUArray device->host");
+    }
+    void to_device(bool async)
+    {
+        if (m_mapped) return;
+        const size_t bytes = (size_t)m_n * sizeof(T);
+        gdt_detail::hip_check(async ? hipMemcpyAsync(m_d, m_h, bytes, hipMemcpyHostToDevice, nullptr)
+                                    : hipMemcpy(m_d, m_h, bytes, hipMemcpyHostToDevice),
+                              "GPUArray host->device");
+    }
+    //! rows x cols elements of both copies into t (row pitches sp, tp; 0: one row)
+    void copy_rows(GPUArray& t, unsigned int cols, unsigned int rows, unsigned int sp,
+                   unsigned int tp) const
+    {
+        if (!cols || !rows || isNull() || t.isNull()) return;
+        for (unsigned int r = 0; r < rows; ++r)
+            std::memcpy(t.m_h + (size_t)r * tp, m_h + (size_t)r * sp, (size_t)cols * sizeof(T));
+        if (m_mapped || t.m_mapped) return;
+        gdt_detail::hip_check(hipMemcpy2D(t.m_d, (size_t)std::max(tp, cols) * sizeof(T), m_d,
+                                          (size_t)std::max(sp, cols) * sizeof(T),
+                                          (size_t)cols * sizeof(T), rows, hipMemcpyDeviceToDevice),
+                              "GPUArray resize");
+    }
     void copy_from(const GPUArray& o)
     {
-        allocate(o.m_n);
+        m_mapped = o.m_mapped;
+        allocate(o.m_n, o.m_pitch, o.m_height);
         if (!m_n) return;
         const size_t bytes = (size_t)m_n * sizeof(T);
-        if (o.m_loc != data_location::device) std::memcpy(m_h, o.m_h, bytes);
-        if (o.m_loc != data_location::host)
+        if (o.m_loc != data_location::device || m_mapped) std::memcpy(m_h, o.m_h, bytes);
+        if (o.m_loc != data_location::host && !m_mapped)
             gdt_detail::hip_check(hipMemcpy(m_d, o.m_d, bytes, hipMemcpyDeviceToDevice),
                                   "GPUArray copy");
         m_loc = o.m_loc;
     }
 
-    unsigned int m_n = 0;
+    unsigned int m_n = 0, m_pitch = 0, m_height = 0;
     T* m_h = nullptr;
     T* m_d = nullptr;
     data_location::Enum m_loc = data_location::hostdevice;
     bool m_acquired = false;
+    bool m_mapped = false;
 };
 
 //! Scoped access to a GPUArray (gpuarray.h:305-328).
@@ -250,6 +330,30 @@ public:
     ~ArrayHandle() { m_arr.release(); }
     ArrayHandle(const ArrayHandle&) = delete;
     ArrayHandle& operator=(const ArrayHandle&) = delete;
+    T* const data;
+
+private:
+    GPUArray<T>& m_arr;
+};
+
+//! The same with asynchronous copies on the null stream (gpuarray.h:356-380):
+//! host data acquired this way are valid after the caller synchronizes.
+template <class T>
+class ArrayHandleAsync {
+public:
+    ArrayHandleAsync(GPUArray<T>& a, access_location::Enum where = access_location::host,
+                     access_mode::Enum mode = access_mode::readwrite)
+        : data(a.acquire(where, mode, true)), m_arr(a)
+    {
+    }
+    ArrayHandleAsync(const GPUArray<T>& a, access_location::Enum where = access_location::host,
+                     access_mode::Enum mode = access_mode::read)
+        : ArrayHandleAsync(const_cast<GPUArray<T>&>(a), where, mode)
+    {
+    }
+    ~ArrayHandleAsync() { m_arr.release(); }
+    ArrayHandleAsync(const ArrayHandleAsync&) = delete;
+    ArrayHandleAsync& operator=(const ArrayHandleAsync&) = delete;
     T* const data;
 
 private:

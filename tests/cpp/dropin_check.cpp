@@ -41,7 +41,7 @@ int main(int argc, char** argv)
     std::vector<double> xy, moved;
     if (!read_points(argv[1], n, xy) || !read_points(argv[5], n, moved)) return 3;
     try {
-        GPUArray<double2> points(n);
+        GPUArray<double2> points((unsigned int)n);
         {
             ArrayHandle<double2> h(points, access_location::host, access_mode::overwrite);
             for (int i = 0; i < n; ++i) h.data[i] = make_double2(xy[2 * i], xy[2 * i + 1]);

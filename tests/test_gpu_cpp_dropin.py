@@ -103,3 +103,15 @@ def test_cpp_dropin_classes(tmp_path, prec):
     got = out["voro"]
     mask = np.arange(out["ms"][1])[None, :] < out["c1"][:, None]
     assert np.allclose(got[mask], ref[mask], rtol=1e-12, atol=1e-12)
+
+
+def test_gpuarray_mirror_matches_reference_semantics():
+    """GPUArray/ArrayHandle/ArrayHandleAsync (tests/cpp/gpuarray_check.cpp):
+    location state machine, 1-D and 2-D resizes of both copies with the
+    reference's pitch, copies, memclear, mapped arrays, asynchronous handles
+    and the one-handle rule, each checked against known values."""
+    from gpudelaunaytriangulation_amd.build import GPUARRAY_BIN
+    print("[cpp gpuarray] run", flush=True)
+    r = subprocess.run([GPUARRAY_BIN], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "all passed" in r.stdout

@@ -41,6 +41,7 @@ def test_cpp_dropin_header_compiles(tmp_path):
     links against the library with a plain host compiler."""
     import subprocess
     from gpudelaunaytriangulation_amd import build as B
-    out = str(tmp_path / "dropin_check")
-    r = subprocess.run(B.cxx_dropin_cmd(B.DROPIN_SRC, out), capture_output=True, text=True)
-    assert r.returncode == 0, r.stderr
+    for src in (B.DROPIN_SRC, B.GPUARRAY_SRC):
+        out = str(tmp_path / os.path.basename(src)[:-4])
+        r = subprocess.run(B.cxx_dropin_cmd(src, out), capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
