@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--precisions", default="fp32f,fp64", help="comma list of fp32f,fp64")
     ap.add_argument("--md-steps", type=int, default=1000,
                     help="steps of the 256-set Brownian test-and-repair loop (0: skip)")
+    ap.add_argument("--md-precision", default="fp64", choices=("fp64", "fp32f"),
+                    help="predicates of the Brownian loop's repairs")
     ap.add_argument("--no-voronoi", action="store_true", help="skip the Voronoi energy timing")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the bench line, run a plain device copy of MIB MiB three "
@@ -94,16 +96,17 @@ def time_path(dg, pts, tri, num, steps, warmup, torch):
             dg.stats())
 
 
-def bench_md(steps, torch):
+def bench_md(steps, torch, precision="fp64"):
     """256 sets, N log-uniform in [1024, 65536], box sqrt(N): one batched
     triangulation, then `steps` hipGraph-replayed steps of sigma = 0.01
     displacement -> test -> repair."""
     import numpy as np
-    from gpudelaunaytriangulation_amd import inputs
+    from gpudelaunaytriangulation_amd import PREC_FP32F, PREC_FP64, inputs
     from gpudelaunaytriangulation_amd.delaunay import BatchedDelaunayGPU
     sizes = inputs.batch_sizes(256)
     pts, boxes = inputs.batch_points(sizes)
     bd = BatchedDelaunayGPU(sizes, [(L, L) for L in boxes], 16, 1.0)
+    bd.setPrecision(PREC_FP32F if precision == "fp32f" else PREC_FP64)
     X = torch.from_numpy(np.concatenate(pts)).cuda()
     tri = torch.zeros(bd.N * bd.MaxSize, dtype=torch.int32, device="cuda")
     num = torch.zeros(bd.N, dtype=torch.int32, device="cuda")
@@ -119,7 +122,7 @@ def bench_md(steps, torch):
     rep = bd.md_run(X, tri, num, steps, 0.01, seed=99, step0=5, use_graph=True, check_every=100)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
-    return {"workload": f"256 sets, {bd.N} points, sigma 0.01, hipGraph replay",
+    return {"workload": f"256 sets, {bd.N} points, sigma 0.01, hipGraph replay, {precision}",
             "points": bd.N, "batch_triangulation_sets_per_s": 256 / t_tri,
             "batch_triangulation_points_per_s": bd.N / t_tri, "steps": rep["steps"],
             "steps_per_s": rep["steps"] / t, "sets_steps_per_s": 256 * rep["steps"] / t,
@@ -268,7 +271,7 @@ def main():
     }
     if args.md_steps > 0:
         try:
-            line["md_c4"] = bench_md(args.md_steps, torch)
+            line["md_c4"] = bench_md(args.md_steps, torch, args.md_precision)
         except Exception as e:  # reported, never hides the headline number
             line["md_c4"] = {"error": repr(e)}
         print(f"[bench] md: {line['md_c4']}", file=sys.stderr, flush=True)

@@ -593,9 +593,13 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
     return gdt_tile_advance<double>(L, io, st, wrap, bk, bx, by);
 }
 
+// W = 2: two neighbouring lanes share the point, as in the FP64 step.
+template <int W = 1>
 GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const GdtRingIO& io,
-                                    GdtTileLane<float>& st, unsigned& npred, unsigned& nfb)
+                                    GdtTileLane<float>& st, unsigned& npred, unsigned& nfb,
+                                    int sub = 0)
 {
+    static_assert(W == 1 || W == 2, "one or two lanes per point");
     const float u = GDT_U32;
     const float bsM = S.bsmax32;
     const float eps1 = 2.5f * u;
@@ -716,6 +720,28 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
         }
     };
     auto visit_k = [&](int k) { visit(k, load(k)); };
+    // Pairs: the lower lane visits its partner's best (the filtered test, with
+    // the FP64 decision when undecided, as for any candidate); the upper lane
+    // then restarts from no best and visits the winner, which sets its state
+    // with the same expressions, so both end identical.
+    auto pair_reduce = [&]() {
+#if defined(__HIPCC__)
+        if constexpr (W > 1) {
+            const int pk = __shfl_xor(bk, 1, 64);
+            if (sub == 0 && pk >= 0 && pk != bk) visit(pk, load(pk));
+            const int win = __shfl(bk, (int)(__lane_id() & ~1u), 64);
+            if (sub != 0 && win != bk) {
+                bk = -1;
+                bd2 = mbb = BP = 0.f;
+                K0 = wrap ? -1.f : 1.f;
+                Kx = Ky = 0.f;
+                Pc = wrap ? 0.f : 3.0e38f;
+                Pp = Pc;
+                if (win >= 0) visit(win, load(win));
+            }
+        }
+#endif
+    };
     // nearest neighbour: the 3x3 block; successor: the 5x5 block, clipped to
     // the columns that may hold points with O >= -BO (horizontal margin BO/|ay|)
     const float ox = vx - (float)(wx - GDT_FRAME) * S.bsx32;
@@ -723,9 +749,10 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
     {
         const float clm = wrap ? BO * gdt_rcp32(fabsf(ay) * S.bsx32) + 1e-3f : 0.f;
         const GdtClip cl = gdt_left_clip(wrap, ax, ay, ox, oy, S.bsx32, S.bsy32, clm);
-        const unsigned nb = gdt_block_rows(L, st.c0, hs, cl, st.tbl);
+        const unsigned nb = gdt_block_rows(L, st.c0, hs, cl, st.tbl, sub, W);
         gdt_scan_tbl(st.tbl, nb, load, visit);
         npred += nb;
+        pair_reduce();
     }
     if (bk < 0 && hs < GDT_SCAN) {
         set_region(GDT_SCAN);
@@ -795,10 +822,11 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
         for (int dy = dy0; dy <= dy1;) {
             const unsigned ne = gdt_disk_rows<float>(L, st.c0, dy, dy1, hs, outer, cq[0], cq[1],
                                                      cq[2], cq[3], crad, ox, oy, S.bsy32, S.ibsx32,
-                                                     1e-2f, clx, st.tbl);
+                                                     1e-2f, clx, st.tbl, sub, W);
             gdt_scan_tbl(st.tbl, ne, load, visit);
             npred += ne;
         }
+        pair_reduce();
         ok = true;
     }
     if (!ok) return STEP_DEFER;

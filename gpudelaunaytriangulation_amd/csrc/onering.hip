@@ -179,11 +179,11 @@ __global__ __launch_bounds__(256) void k_one_ring_wave(const GdtSet* __restrict_
 // staging.  The points of a window that does not fit the LDS budget, or of a
 // grid too small for the scan block, and points with a step whose circle
 // leaves the window, are appended to `defer` and handled by k_one_ring_wave
-// in the next launch.  With W = 2 (FP64, lists of points) two neighbouring
-// lanes share each point, each scanning half of every row (a list leaves most
-// lanes of a block without a point otherwise).
+// in the next launch.  With W = 2 (lists of points) two neighbouring lanes
+// share each point, each scanning half of every row (a list leaves most lanes
+// of a block without a point otherwise).
 #ifndef GDT_LIST_LANES
-#define GDT_LIST_LANES 2    // lanes per point of the FP64 tile kernel over a list of points
+#define GDT_LIST_LANES 2    // lanes per point of the tile kernel over a list of points
 #endif
 
 template <int PREC, int W>
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
     // Lanes take points of the tile and advance them one search step per
     // iteration; a lane whose point is finished takes the next one, so a
     // wave's iterations track the total number of steps, not the longest ring.
-    static_assert(W == 1 || PREC == 64, "lane groups: FP64 only");
+    static_assert(W == 1 || W == 2, "one or two lanes per point");
     if (tid == 0) {
         s_next = Q0 + nt / W;
         s_dn = 0u;
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
         if (have) {
             int rc;
             if constexpr (PREC == 32)
-                rc = gdt_tile_step_f32(S, L, io, st, npred, nfb);
+                rc = gdt_tile_step_f32<W>(S, L, io, st, npred, nfb, sub);
             else
                 rc = gdt_tile_step_f64<W>(S, L, io, st, npred, sub);
             // (both lanes of a pair end each step in the same state; the first reports)
@@ -348,7 +348,10 @@ hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nse
     if (ntiles <= 0) return hipSuccess;
     if (block > 1024) block = 1024;
     const size_t tbl = (5 * (size_t)block + 1) * sizeof(uint32_t);
-    if (precision == 32)
+    if (precision == 32 && list)
+        hipLaunchKernelGGL((k_one_ring_tile<32, GDT_LIST_LANES>), dim3(ntiles), dim3(block), tbl,
+                           s, d_sets, tiles, list, lscan, io, counts, status, counters, defer);
+    else if (precision == 32)
         hipLaunchKernelGGL((k_one_ring_tile<32, 1>), dim3(ntiles), dim3(block), tbl, s, d_sets,
                            tiles, list, lscan, io, counts, status, counters, defer);
     else if (list)
