@@ -265,7 +265,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("k_one_ring_tile<32>") if n == N_POINTS else None,
                      "traffic_source": "profiles/k_one_ring_tile_uniform1M_hbm_traffic.md",
-                     "kernel": "k_one_ring_tile<32>", "algorithmic_bytes_per_launch": ring_bytes},
+                     "kernel": "k_one_ring_tile<32, 1>", "algorithmic_bytes_per_launch": ring_bytes},
         "triangulation_roofline_frac": TRI_BYTES(n, n) / (f32["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "cpu_baseline": cpu,
     }
