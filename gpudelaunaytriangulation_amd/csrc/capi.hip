@@ -58,13 +58,12 @@ struct gdt_context {
     std::vector<GdtSet> sets;
     int total_n = 0, total_keys = 0;
     bool cells_valid = false;
-    bool slots_valid = false;    // slot_of matches the cell list
     DBuf<GdtSet> d_sets;
     DBuf<GdtChunk> pch, kch, rch, tch;
     int npch = 0, nkch = 0, nrch = 0, rch_block = 0, ntch = 0;
 
     DBuf<uint32_t> pkey, prank, ccount, cstart, scell, scan_tmp, tcount, toff, mark, markscan;
-    DBuf<int> perm, slot_of, worklist, repair, tris, defer;
+    DBuf<int> perm, worklist, repair, tris, defer;
     DBuf<gdt_d2> off64;
     DBuf<gdt_f2> off32;
     DBuf<GdtKbRec> bkt;
@@ -92,7 +91,7 @@ struct gdt_context {
     {
         GdtCells c;
         c.pkey = pkey.p; c.prank = prank.p; c.ccount = ccount.p; c.cstart = cstart.p;
-        c.perm = perm.p; c.slot_of = slot_of.p; c.off64 = off64.p; c.off32 = off32.p;
+        c.perm = perm.p; c.slot_of = nullptr; c.off64 = off64.p; c.off32 = off32.p;
         c.scell = scell.p; c.scan_tmp = scan_tmp.p;
         c.bkt = bkt.p; c.bmat = bmat.p; c.bscan = bscan.p;
         return c;
@@ -168,7 +167,6 @@ static int configure(gdt_context* c, std::vector<GdtSet> sets)
     GDT_CHECK(c->pkey.reserve(N));
     GDT_CHECK(c->prank.reserve(N));
     GDT_CHECK(c->perm.reserve(N));
-    GDT_CHECK(c->slot_of.reserve(N));
     GDT_CHECK(c->off64.reserve(N));
     GDT_CHECK(c->off32.reserve(N));
     GDT_CHECK(c->scell.reserve(N));
@@ -285,19 +283,15 @@ static GdtRingIO ring_io(gdt_context* c, int* nbrs, int stride)
     return io;
 }
 
-// The cell list of d_xy; slot_of (caller index -> slot) only when asked for
-// (the circumcircle test reads it, the global triangulation does not).
-static int build_cells(gdt_context* c, const double* d_xy, hipStream_t s, bool with_slot_of = true)
+// The cell list of d_xy (no caller index -> slot map: nothing reads one).
+static int build_cells(gdt_context* c, const double* d_xy, hipStream_t s)
 {
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[0], s));
     GdtCells cl = c->cells();
-    if (!with_slot_of) cl.slot_of = nullptr;
     GDT_CHECK(gdt_build_cells(c->d_sets.p, (int)c->sets.size(), c->pch.p, c->npch, c->kch.p,
-                              c->nkch, c->total_n,
-                              c->total_keys, d_xy, cl, s));
+                              c->nkch, c->total_n, c->total_keys, d_xy, cl, s));
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
     c->cells_valid = true;
-    c->slots_valid = with_slot_of;
     return GDT_OK;
 }
 
@@ -356,7 +350,7 @@ void gdt_destroy(gdt_context* c)
     c->d_sets.release(); c->pch.release(); c->kch.release(); c->rch.release(); c->tch.release();
     c->pkey.release(); c->prank.release(); c->ccount.release(); c->cstart.release();
     c->scell.release(); c->scan_tmp.release(); c->tcount.release(); c->toff.release();
-    c->mark.release(); c->markscan.release(); c->perm.release(); c->slot_of.release();
+    c->mark.release(); c->markscan.release(); c->perm.release();
     c->worklist.release(); c->repair.release(); c->tris.release(); c->defer.release(); c->off64.release();
     c->off32.release(); c->status.release(); c->counters.release(); c->tpart.release();
     c->d_vsets.release(); c->ecell.release(); c->partial.release(); c->vslots.release();
@@ -461,7 +455,7 @@ static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int*
 {
     int rc;
     if ((rc = ring_chunks(c)) != GDT_OK) return rc;
-    if ((rc = build_cells(c, d_xy, s, false)) != GDT_OK) return rc;
+    if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
     if ((rc = reset_status(c, s)) != GDT_OK) return rc;
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
     if (c->cfg.tiled) {
@@ -530,19 +524,13 @@ static int enumerate(gdt_context* c, const int* d_nbrs, const int* d_counts, int
     return GDT_OK;
 }
 
-// Empty-circle test of the triangles enumerate() wrote (slot order when the
-// cell order is tiled: then one LDS block per tile).
-static int test_triangles(gdt_context* c, const double* d_xy, bool tiled, int cap, hipStream_t s)
+// Empty-circle test of the triangles enumerate() wrote.
+static int test_triangles(gdt_context* c, const double* d_xy, int cap, hipStream_t s)
 {
     const int n = c->total_n;
     GDT_CHECK(gdt_fill_i32(c->repair.p, -1, (size_t)n, s));
-    GdtCells cl = c->cells();
-    if (tiled)
-        GDT_CHECK(gdt_launch_test_tile(c->d_sets.p, c->tch.p, c->ntch, c->tris.p, c->toff.p, cap, n,
-                                       d_xy, cl, c->repair.p, s));
-    else
-        GDT_CHECK(gdt_launch_test(c->d_sets.p, (int)c->sets.size(), c->tris.p, c->toff.p + n, cap,
-                                  n, d_xy, cl, c->repair.p, s));
+    GDT_CHECK(gdt_launch_test(c->d_sets.p, (int)c->sets.size(), c->tris.p, c->toff.p + n, cap,
+                              n, d_xy, c->cells(), c->repair.p, s));
     return GDT_OK;
 }
 
@@ -562,7 +550,7 @@ static int triangles_and_test(gdt_context* c, const double* d_xy, const int* d_n
     if (!tiled_sets(c)) {
         int rc;
         if ((rc = enumerate(c, d_nbrs, d_counts, stride, c->tris.p, cap, false, s)) != GDT_OK) return rc;
-        return test_triangles(c, d_xy, false, cap, s);
+        return test_triangles(c, d_xy, cap, s);
     }
     GDT_CHECK(c->toff.reserve((size_t)n + 1));
     GDT_CHECK(gdt_fill_i32(c->repair.p, -1, (size_t)n, s));
@@ -604,10 +592,6 @@ int gdt_test_circumcircles(gdt_context* c, const double* d_xy, int n, const int*
     hipStream_t s = (hipStream_t)stream;
     GDT_CHECK(gdt_fill_i32(d_repair, -1, (size_t)n, s));
     GDT_CHECK(gdt_fill_u32(reinterpret_cast<uint32_t*>(c->status.p + ST_NTRI), (uint32_t)ntri, 1, s));
-    if (!c->slots_valid) {
-        GDT_CHECK(gdt_launch_slot_of(c->perm.p, c->total_n, c->slot_of.p, s));
-        c->slots_valid = true;
-    }
     GdtCells cl = c->cells();
     GDT_CHECK(gdt_launch_test(c->d_sets.p, (int)c->sets.size(), d_tris,
                               reinterpret_cast<const uint32_t*>(c->status.p + ST_NTRI), ntri,

@@ -611,21 +611,6 @@ hipError_t gdt_build_cells(const GdtSet* d_sets, int nsets, const GdtChunk* pchu
     return hipGetLastError();
 }
 
-// slot_of from perm, for a cell list built without it.
-__global__ void k_slot_of(const int* __restrict__ perm, int n, int* __restrict__ slot_of)
-{
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x)
-        slot_of[perm[s]] = s;
-}
-
-hipError_t gdt_launch_slot_of(const int* perm, int n, int* slot_of, hipStream_t s)
-{
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_slot_of, dim3(fill_grid((size_t)n)), dim3(GDT_BLOCK), 0, s, perm, n,
-                       slot_of);
-    return hipGetLastError();
-}
-
 // Reference layout: cell_sizes[cy*xsize+cx], idxs[cell*nmax + k] (zero padded).
 __global__ __launch_bounds__(GDT_BLOCK) void k_export(const GdtSet* __restrict__ sets,
                                                       const uint32_t* __restrict__ cstart,

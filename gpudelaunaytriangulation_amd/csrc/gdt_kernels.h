@@ -55,8 +55,6 @@ hipError_t gdt_build_cells(const GdtSet* d_sets, int nsets, const GdtChunk* pchu
                            const GdtChunk* kchunks, int nkch, int total_n, int total_keys,
                            const double* d_xy, GdtCells& c, hipStream_t s);
 
-// slot_of[perm[s]] = s for a cell list built without slot_of.
-hipError_t gdt_launch_slot_of(const int* perm, int n, int* slot_of, hipStream_t s);
 
 hipError_t gdt_export_cells(const GdtSet* d_sets, const GdtSet& h_set, const GdtCells& c,
                             unsigned int* cell_sizes, int* idxs, int nmax, hipStream_t s);
@@ -102,15 +100,10 @@ hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, con
                            int max_tri, int total_n, const double* d_xy, const GdtCells& c, int* repair,
                            hipStream_t s);
 
-// The same test, one block per tile, for triangles enumerated in slot order
-// (toff over slots); repair must be -1 on entry.
-hipError_t gdt_launch_test_tile(const GdtSet* d_sets, const GdtChunk* tiles, int ntiles,
-                                const int* tris, const uint32_t* toff, int cap, int total_n,
-                                const double* d_xy, const GdtCells& c, int* repair, hipStream_t s);
 
 // Enumeration and empty-circle test in one pass over the rings, one block per
 // tile (tiled cell order): triangles owned by their smallest caller index,
-// flags as gdt_launch_test_tile, counters[CT_INSUM] += sum of counts, *ntri =
+// flags as gdt_launch_test, counters[CT_INSUM] += sum of counts, *ntri =
 // number of owned triangles; partials holds 2 * ntiles words.
 hipError_t gdt_launch_ring_test_tile(const GdtSet* d_sets, const GdtChunk* tiles, int ntiles,
                                      const int* nbrs, const int* counts, int stride, int total_n,

@@ -170,11 +170,13 @@ __device__ inline GdtTriCircle tri_circle(const GdtSet& S, double ilx, double il
     return c;
 }
 
-// Global-memory scan of one circle; returns the intruder's slot (-1 none) and
-// whether the circle was covered.
+// Global-memory scan of the circle of triangle (i, a, b); returns the
+// intruder's caller index (-1 none; the vertices, which lie on the circle, are
+// told apart by caller index) and whether the circle was covered.
 __device__ inline int tri_scan_global(const GdtSet& S, const gdt_d2* __restrict__ xy, int i,
-                                      const GdtTriCircle& c, int si, int sa, int sb,
+                                      const GdtTriCircle& c, int a, int b,
                                       const uint32_t* __restrict__ cstart,
+                                      const int* __restrict__ perm,
                                       const gdt_d2* __restrict__ off64, bool& covered)
 {
     const gdt_d2 v = xy[i];
@@ -195,10 +197,12 @@ __device__ inline int tri_scan_global(const GdtSet& S, const gdt_d2* __restrict_
             const uint32_t cb = cstart[key], ce = cstart[key + 1];
             const double ebx = gdt_base64(dx, S.bsx, oqx), eby = gdt_base64(dy, S.bsy, oqy);
             for (uint32_t k = cb; k < ce; ++k) {
-                if ((int)k == si || (int)k == sa || (int)k == sb) continue;
                 const gdt_d2 o = off64[k];
                 const double ex = gdt_disp64(ebx, o.x), ey = gdt_disp64(eby, o.y);
-                if (ex * ex + ey * ey < c.lim) { intruder = (int)k; break; }
+                if (ex * ex + ey * ey < c.lim) {
+                    const int p = perm[k];
+                    if (p != i && p != a && p != b) { intruder = p; break; }
+                }
             }
         }
         const double cov = (double)r * bsmin + covb;
@@ -220,7 +224,6 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_test(const GdtSet* __restrict__ s
                                                     const uint32_t* __restrict__ ntri, int cap,
                                                     int total_n, const gdt_d2* __restrict__ xy,
                                                     const uint32_t* __restrict__ cstart,
-                                                    const int* __restrict__ slot_of,
                                                     const int* __restrict__ perm,
                                                     const gdt_d2* __restrict__ off64,
                                                     int* __restrict__ repair)
@@ -237,11 +240,10 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_test(const GdtSet* __restrict__ s
         int intruder = -1;
         if (!bad) {
             bool covered;
-            intruder = tri_scan_global(S, xy, i, c, slot_of[i], slot_of[a], slot_of[b], cstart,
-                                       off64, covered);
+            intruder = tri_scan_global(S, xy, i, c, a, b, cstart, perm, off64, covered);
             bad = intruder >= 0 || !covered;
         }
-        if (bad) tri_flag(repair, i, a, b, intruder >= 0 ? perm[intruder] : -1);
+        if (bad) tri_flag(repair, i, a, b, intruder);
     }
 }
 
@@ -253,7 +255,7 @@ hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, con
     int grid = (max_tri + GDT_BLOCK - 1) / GDT_BLOCK;
     if (grid > 8192) grid = 8192;
     hipLaunchKernelGGL(k_test, dim3(grid), dim3(GDT_BLOCK), 0, s, d_sets, nsets, tris, ntri,
-                       max_tri, total_n, reinterpret_cast<const gdt_d2*>(d_xy), c.cstart, c.slot_of, c.perm, c.off64,
+                       max_tri, total_n, reinterpret_cast<const gdt_d2*>(d_xy), c.cstart, c.perm, c.off64,
                        repair);
     return hipGetLastError();
 }
@@ -265,8 +267,8 @@ hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, con
 __device__ inline void tri_test_in_tile(const GdtSet& S, double ilx, double ily,
                                         const GdtTileWin& w, bool fits,
                                         const GdtTileLds& L, const gdt_d2* __restrict__ xy,
-                                        const GdtRingIO& io, const int* __restrict__ slot_of,
-                                        int i, int a, int b, int si, int* __restrict__ repair)
+                                        const GdtRingIO& io, int i, int a, int b, int si,
+                                        int* __restrict__ repair)
 {
     const uint32_t* s_cs = L.cs;
     const int* s_gstart = L.gstart;
@@ -311,51 +313,10 @@ __device__ inline void tri_test_in_tile(const GdtSet& S, double ilx, double ily,
         }
         if (!done) {
             bool covered;
-            const int ks = tri_scan_global(S, xy, i, c, si, slot_of[a], slot_of[b], io.cstart,
-                                           io.off64, covered);
-            intruder = ks >= 0 ? io.perm[ks] : -1;
-            bad = ks >= 0 || !covered;
+            intruder = tri_scan_global(S, xy, i, c, a, b, io.cstart, io.perm, io.off64, covered);
+            bad = intruder >= 0 || !covered;
         }
         if (bad) tri_flag(repair, i, a, b, intruder);
-    }
-}
-
-// The same test with the tiled cell order, one block per tile: the triangles
-// are enumerated in slot order of their owner (toff over slots), so a tile's
-// triangles are one range; the block stages the tile's window (gdt_tile.h)
-// and tests them there (tri_test_in_tile).
-__global__ __launch_bounds__(512) void k_test_tile(const GdtSet* __restrict__ sets,
-                                                   const GdtChunk* __restrict__ tiles,
-                                                   const int* __restrict__ tris,
-                                                   const uint32_t* __restrict__ toff, int cap,
-                                                   int total_n, const gdt_d2* __restrict__ xy,
-                                                   GdtRingIO io, const int* __restrict__ slot_of,
-                                                   int* __restrict__ repair)
-{
-    __shared__ uint32_t s_cs[GDT_WIN_CELLS + 1];
-    __shared__ int s_gstart[GDT_WIN_CELLS];
-    __shared__ gdt_d2 s_X64[GDT_WIN_MAXPTS];
-    __shared__ int s_orig[GDT_WIN_MAXPTS];
-    __shared__ uint32_t s_wsum[16];
-
-    const GdtChunk ch = tiles[blockIdx.x];
-    const GdtSet S = sets[ch.set];
-    const double ilx = 1.0 / S.lx, ily = 1.0 / S.ly;
-    const int k0 = S.c0 + ch.start * GDT_TILE_CELLS;
-    const int T0 = (int)io.cstart[k0], T1 = (int)io.cstart[k0 + GDT_TILE_CELLS];
-    const int R0 = min((int)toff[T0], cap), R1 = min((int)toff[T1], cap);
-    if (R0 >= R1) return;    // block-uniform
-    const GdtTileWin w = gdt_tile_window(S, ch.start);
-    GdtTileLds L;
-    L.cs = s_cs; L.gstart = s_gstart; L.X64 = s_X64; L.X32 = nullptr; L.orig = s_orig;
-    const bool fits = gdt_stage_window<false>(S, w, io, L, s_wsum);
-
-    for (int t = R0 + (int)threadIdx.x; t < R1; t += (int)blockDim.x) {
-        const int i = tris[3 * (size_t)t], a = tris[3 * (size_t)t + 1], b = tris[3 * (size_t)t + 2];
-        if ((unsigned)i >= (unsigned)total_n || (unsigned)a >= (unsigned)total_n ||
-            (unsigned)b >= (unsigned)total_n)
-            continue;   // malformed ring entry: the Euler check reports it
-        tri_test_in_tile(S, ilx, ily, w, fits, L, xy, io, slot_of, i, a, b, slot_of[i], repair);
     }
 }
 
@@ -374,7 +335,6 @@ __global__ __launch_bounds__(512) void k_ring_test_tile(const GdtSet* __restrict
                                                         const int* __restrict__ counts, int stride,
                                                         int total_n, const gdt_d2* __restrict__ xy,
                                                         GdtRingIO io,
-                                                        const int* __restrict__ slot_of,
                                                         int* __restrict__ repair,
                                                         unsigned long long* __restrict__ partials,
                                                         int* __restrict__ status)
@@ -444,7 +404,7 @@ __global__ __launch_bounds__(512) void k_ring_test_tile(const GdtSet* __restrict
             const int nl = min((int)s_tn, GDT_RT_LCAP);
             for (int t = tid; t < nl; t += nt) {
                 const int4 e = s_tri[t];
-                tri_test_in_tile(S, ilx, ily, w, fits, L, xy, io, slot_of, e.x, e.y, e.z, e.w, repair);
+                tri_test_in_tile(S, ilx, ily, w, fits, L, xy, io, e.x, e.y, e.z, e.w, repair);
             }
             // also the barrier before the list is rewritten
             if (!__syncthreads_or(pending)) break;
@@ -519,7 +479,7 @@ hipError_t gdt_launch_ring_test_tile(const GdtSet* d_sets, const GdtChunk* tiles
     io.stride = 0;
     hipLaunchKernelGGL(k_ring_test_tile, dim3(ntiles), dim3(512), 0, s, d_sets, tiles, nbrs,
                        counts, stride, total_n, reinterpret_cast<const gdt_d2*>(d_xy), io,
-                       c.slot_of, repair, partials, status);
+                       repair, partials, status);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sum_ring_partials, dim3(1), dim3(256), 0, s, partials, ntiles, counters,
@@ -527,23 +487,6 @@ hipError_t gdt_launch_ring_test_tile(const GdtSet* d_sets, const GdtChunk* tiles
     return hipGetLastError();
 }
 
-hipError_t gdt_launch_test_tile(const GdtSet* d_sets, const GdtChunk* tiles, int ntiles,
-                                const int* tris, const uint32_t* toff, int cap, int total_n,
-                                const double* d_xy, const GdtCells& c, int* repair, hipStream_t s)
-{
-    if (ntiles <= 0 || cap <= 0) return hipSuccess;
-    GdtRingIO io;
-    io.cstart = c.cstart;
-    io.perm = c.perm;
-    io.off64 = c.off64;
-    io.off32 = c.off32;
-    io.scell = c.scell;
-    io.nbrs = nullptr;
-    io.stride = 0;
-    hipLaunchKernelGGL(k_test_tile, dim3(ntiles), dim3(512), 0, s, d_sets, tiles, tris, toff, cap,
-                       total_n, reinterpret_cast<const gdt_d2*>(d_xy), io, c.slot_of, repair);
-    return hipGetLastError();
-}
 
 __global__ __launch_bounds__(GDT_BLOCK) void k_mark(const int* __restrict__ repair,
                                                     const int* __restrict__ perm, int n,
