@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=N_POINTS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--block", type=int, default=None, help="threads per block (one-ring kernel)")
     ap.add_argument("--tiled", type=int, default=None, help="1: 32x32-cell tiles (LDS kernel), 2: the same with Morton order inside, 0: row-major")
     ap.add_argument("--cell-scale", type=float, default=None, help="cell size multiplier")
@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--md-precision", default="fp64", choices=("fp64", "fp32f"),
                     help="predicates of the Brownian loop's repairs")
     ap.add_argument("--no-voronoi", action="store_true", help="skip the Voronoi energy timing")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the 4,096-point case (GPU and CPU oracle timings)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the bench line, run a plain device copy of MIB MiB three "
                          "times (reference kernel for scaling rocprofv3 FETCH/WRITE_SIZE)")
@@ -94,6 +96,49 @@ def time_path(dg, pts, tri, num, steps, warmup, torch):
     ms = ev0.elapsed_time(ev1) / steps
     return (ms, wall, sum(ring_ms) / steps, sum(cell_ms) / steps, sum(defer_ms) / steps,
             dg.stats())
+
+
+def cpu_model():
+    """The host CPU's model name (first /proc/cpuinfo entry), or 'unknown'."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def bench_c1(torch, cpu_runs, with_cpu):
+    """4,096 uniform points in a box of side 64 (the reference's CPU-runnable
+    case): GPU points/s per precision (HIP events, 20 calls after 3) and,
+    unless skipped, the CPU oracle on the same points (median of cpu_runs)."""
+    from gpudelaunaytriangulation_amd import PREC_FP32F, PREC_FP64, inputs
+    from gpudelaunaytriangulation_amd.delaunay import DelaunayGPU, PeriodicBoundaries
+    n, L = 4096, 64.0
+    xy = inputs.uniform_points(n, L, L, seed=1)
+    pts = torch.from_numpy(xy).cuda()
+    res = {"workload": "4096 uniform points, box 64, cell 1.0"}
+    for name, prec in (("fp32f", PREC_FP32F), ("fp64", PREC_FP64)):
+        dg = DelaunayGPU(n, 16, 1.0, PeriodicBoundaries(L, L))
+        dg.setPrecision(prec)
+        tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device="cuda")
+        num = torch.zeros(n, dtype=torch.int32, device="cuda")
+        ms = time_path(dg, pts, tri, num, 20, 3, torch)[0]
+        res[f"gpu_{name}_points_per_s"] = n / (ms * 1e-3)
+    if with_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        oracle.triangulate(xy, L, L, 1.0, 16, nthreads=threads)
+        times = []
+        for _ in range(cpu_runs):
+            t0 = time.perf_counter()
+            oracle.triangulate(xy, L, L, 1.0, 16, nthreads=threads)
+            times.append(time.perf_counter() - t0)
+        res["cpu_points_per_s"] = n / sorted(times)[len(times) // 2]
+        res["cpu_cores"] = threads
+    return res
 
 
 def bench_md(steps, torch, precision="fp64"):
@@ -231,7 +276,7 @@ def main():
         cpu = {"value": n / t, "unit": "points/s", "cores": threads, "kind": "port",
                "sample": f"the same {n} points, full triangulation incl. cell list, "
                          f"median of {args.cpu_runs} runs after 1 warm-up, "
-                         f"{threads} OpenMP threads (oracle/gdt_oracle.c)"}
+                         f"{threads} OpenMP threads (oracle/gdt_oracle.c) on {cpu_model()}"}
 
     line = {
         "metric": "points/sec triangulated, 1M uniform: FP32-filtered vs FP64 GPU, topology exact",
@@ -269,6 +314,12 @@ def main():
         "triangulation_roofline_frac": TRI_BYTES(n, n) / (f32["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "cpu_baseline": cpu,
     }
+    if not args.no_c1:
+        try:
+            line["c1_4096"] = bench_c1(torch, args.cpu_runs, not args.no_cpu_baseline)
+        except Exception as e:  # reported, never hides the headline number
+            line["c1_4096"] = {"error": repr(e)}
+        print(f"[bench] c1: {line['c1_4096']}", file=sys.stderr, flush=True)
     if args.md_steps > 0:
         try:
             line["md_c4"] = bench_md(args.md_steps, torch, args.md_precision)
