@@ -52,6 +52,7 @@ struct DevGuard {
 
 struct gdt_context {
     int device = 0;
+    int num_cu = 256;                    // compute units of the device
     gdt_config cfg = {512, 1, 1.0, 0};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
@@ -333,7 +334,8 @@ int gdt_create(gdt_context** out, int device)
         hipHostMalloc(reinterpret_cast<void**>(&c->h_seq), sizeof(unsigned int), 0) != hipSuccess ||
         hipHostGetDevicePointer(reinterpret_cast<void**>(&c->hd_seq), c->h_seq, 0) != hipSuccess ||
         hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess ||
-        hipEventCreate(&c->ev[2]) != hipSuccess || hipEventCreate(&c->ev[3]) != hipSuccess) {
+        hipEventCreate(&c->ev[2]) != hipSuccess || hipEventCreate(&c->ev[3]) != hipSuccess ||
+        hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
         gdt_destroy(c);
         return GDT_ERR_DEVICE;
     }
@@ -459,8 +461,12 @@ static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int*
     if ((rc = reset_status(c, s)) != GDT_OK) return rc;
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
     if (c->cfg.tiled) {
+        // A grid of at most one tile per CU runs one wave of blocks: each lane
+        // then takes a single point (1024 threads for about 1024 points), so
+        // the block ends after its longest ring instead of after two.
+        const int block = c->ntch <= c->num_cu ? std::max(c->cfg.block, 1024) : c->cfg.block;
         GDT_CHECK(gdt_launch_one_ring_tile(precision, c->d_sets.p, (int)c->sets.size(), c->tch.p,
-                                           c->ntch, c->cfg.block, c->total_n, nullptr, nullptr,
+                                           c->ntch, block, c->total_n, nullptr, nullptr,
                                            ring_io(c, d_nbrs, stride), d_counts, c->status.p,
                                            c->counters.p, c->defer.p,
                                            c->cfg.profile ? c->ev[2] : nullptr, s));
