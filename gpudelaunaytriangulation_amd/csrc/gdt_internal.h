@@ -38,7 +38,9 @@ enum {
     ST_WORDS = 8
 };
 // ERR_DEFER is internal: the LDS routine hands the point to the direct routine.
-enum { ERR_SEARCH = 1, ERR_NOCLOSE = 2, ERR_NONEIGHBOUR = 4, ERR_DEFER = 8 };
+// ERR_TIE is internal: the wavefront routine built without the tie rule met
+// an exact tie and hands the point to the one built with it.
+enum { ERR_SEARCH = 1, ERR_NOCLOSE = 2, ERR_NONEIGHBOUR = 4, ERR_DEFER = 8, ERR_TIE = 16 };
 // 64-bit counters (gdt_counters[...]).
 // CT_NBRSUM: sum of ring lengths written; CT_INSUM: sum of input counts seen by
 // the triangle enumeration (Euler check).
@@ -243,6 +245,33 @@ GDT_HD inline double gdt_incircle_cc64(const Circ64& k, double cx, double cy, do
 GDT_HD inline double gdt_incircle64(const Circ64& k, double cx, double cy)
 {
     return gdt_incircle_cc64(k, cx, cy, gdt_dist2_64(cx, cy));
+}
+
+// Exact ties (degenerate input: a candidate exactly on the circle, two points
+// at exactly the same distance) are decided by a symbolic perturbation, the
+// same one in every search of every point, so that the rings of a lattice form
+// one triangulation: each point's lifted coordinate |p|^2 is raised by
+// eps^(1 + caller index), the lowest caller index most.  Nearest neighbour:
+// of two points at the same distance the lower caller index wins.  In-circle:
+// c, exactly on the circle through v, a and b (displacements from v; b and c
+// left of v->a), counts as inside when the most raised of the four points is
+// b (b and c lie on the same side of line va), or v, a and c lies on that
+// point's side of the line through the two others (raising one point's lift
+// tilts the plane through the three about the line through the other two),
+// and as outside when it is c.  p* are caller indices.
+GDT_HD inline int gdt_imin(int x, int y) { return x < y ? x : y; }
+GDT_HD inline bool gdt_nn_tie(int pc, int pb) { return pc < pb; }
+GDT_HD inline bool gdt_incircle_tie(int pv, int pa, int pb, int pc, double ax, double ay,
+                                    double bx, double by, double cx, double cy)
+{
+    const int m = gdt_imin(gdt_imin(pv, pa), gdt_imin(pb, pc));
+    if (m == pc) return false;
+    if (m == pb) return true;
+    if (m == pa)   // c on a's side of line v b
+        return (gdt_orient64(bx, by, ax, ay) > 0.) == (gdt_orient64(bx, by, cx, cy) > 0.);
+    // v dominant: c on v's side of line a b (v is the origin)
+    const double ux = bx - ax, uy = by - ay;
+    return (gdt_orient64(ux, uy, -ax, -ay) > 0.) == (gdt_orient64(ux, uy, cx - ax, cy - ay) > 0.);
 }
 
 // ---------------------------------------------------------------------------

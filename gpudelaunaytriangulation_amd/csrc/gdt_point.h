@@ -123,7 +123,9 @@ GDT_HD inline GdtRingResult gdt_one_ring_f64(const GdtSet& S, int j, const GdtRi
             const double px = gdt_disp64(bx, o.x), py = gdt_disp64(by, o.y);
             const double d2 = gdt_dist2_64(px, py);
             ++npred;
-            if (nn < 0 || d2 < nd2) { nn = (int)k; nd2 = d2; nx = px; ny = py; }
+            if (nn < 0 || d2 < nd2 || (d2 == nd2 && gdt_nn_tie(io.perm[k], io.perm[nn]))) {
+                nn = (int)k; nd2 = d2; nx = px; ny = py;
+            }
         }
     };
     auto nn_done = [&](int r) {
@@ -157,7 +159,10 @@ GDT_HD inline GdtRingResult gdt_one_ring_f64(const GdtSet& S, int j, const GdtRi
                 if (!(gdt_orient64(ax, ay, px, py) > 0.)) continue;
                 if (best >= 0) {
                     ++npred;
-                    if (!(gdt_incircle64(K, px, py) < 0.)) continue;
+                    const double P = gdt_incircle64(K, px, py);
+                    if (!(P < 0.) && !(P == 0. && gdt_incircle_tie(io.perm[j], io.perm[a], io.perm[best],
+                                                                    io.perm[k], ax, ay, bx_, by_, px, py)))
+                        continue;
                 }
                 best = (int)k; bx_ = px; by_ = py;
                 K = gdt_circ64(ax, ay, px, py);
@@ -236,7 +241,8 @@ GDT_HD inline GdtRingResult gdt_one_ring_f32(const GdtSet& S, int j, const GdtRi
                 double cx64, cy64, nx64, ny64;
                 gdt_disp64_of(S, ov, io.off64, (int)k, gdt_pack_dxy(dx, dy), cx64, cy64);
                 gdt_disp64_of(S, ov, io.off64, nn, ndxy, nx64, ny64);
-                take = gdt_dist2_64(cx64, cy64) < gdt_dist2_64(nx64, ny64);
+                const double dc = gdt_dist2_64(cx64, cy64), dn = gdt_dist2_64(nx64, ny64);
+                take = dc < dn || (dc == dn && gdt_nn_tie(io.perm[k], io.perm[nn]));
             }
             if (take) { nn = (int)k; ndxy = gdt_pack_dxy(dx, dy); nd2 = d2; nE = E; nx = px; ny = py; }
         }
@@ -318,7 +324,11 @@ GDT_HD inline GdtRingResult gdt_one_ring_f32(const GdtSet& S, int j, const GdtRi
                         gdt_disp64_of(S, ov, io.off64, best, bdxy, b64x, b64y);
                         gdt_disp64_of(S, ov, io.off64, (int)k, dxy, c64x, c64y);
                         const Circ64 K64 = gdt_circ64(a64x, a64y, b64x, b64y);
-                        inside = gdt_incircle64(K64, c64x, c64y) < 0.;
+                        const double P64 = gdt_incircle64(K64, c64x, c64y);
+                        inside = P64 < 0. ||
+                                 (P64 == 0. && gdt_incircle_tie(io.perm[j], io.perm[a], io.perm[best],
+                                                                io.perm[k], a64x, a64y, b64x, b64y,
+                                                                c64x, c64y));
                     }
                     if (!inside) continue;
                 }

@@ -412,6 +412,21 @@ GDT_HD inline double gdt_cell_offset(double X, int wx, double bs)
 
 // Result of one step: 0 running, 1 ring closed, 2 defer, 3 non-generic (no close)
 enum { STEP_RUN = 0, STEP_DONE = 1, STEP_DEFER = 2, STEP_NOCLOSE = 3 };
+// Best entry of an FP64 tile search that ties exactly with the best circle.
+#define GDT_TIE (-2)
+
+// b of any of the W lanes that share a point (all W lanes must call it).
+template <int W>
+GDT_HD inline bool gdt_group_any(bool b)
+{
+#if defined(__HIPCC__)
+    if constexpr (W > 1) {
+#pragma unroll
+        for (int o = 1; o < W; o <<= 1) b |= __shfl_xor((int)b, o, 64) != 0;
+    }
+#endif
+    return b;
+}
 
 // Emit the step's neighbour and advance the lane (shared tail of both paths).
 template <class Real>
@@ -460,26 +475,34 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
     K.k0 = wrap ? -1. : 1.;
     K.kx = K.ky = 0.;
     auto load = [&](int k) { return L.X64[k]; };
+    // An exact tie with the current best (P == Pthr: degenerate input) sets
+    // bk = GDT_TIE and Pthr = -1e300, so nothing more is taken and the step
+    // hands the point to the wavefront routine, which decides ties by the
+    // symbolic perturbation of gdt_incircle_tie / gdt_nn_tie.  No other state:
+    // the test is the branch's own comparison.
     auto visit = [&](int k, const gdt_d2 X) {
         const double px = X.x - vx, py = X.y - vy;
         const double cc = gdt_dist2_64(px, py);
         const double O = gdt_orient64(ax, ay, px, py);
         const double P = gdt_incircle_cc64(K, px, py, cc);
-        // one branch, taken only for a new best
-        if ((O > Othr) & (P < Pthr) & (k != ex)) {
-            bk = k;
+        // one branch, taken only for a new best (or an exact tie)
+        if ((O > Othr) & (P <= Pthr) & (k != ex)) {
+            const bool t = P == Pthr;
+            bk = t ? GDT_TIE : k;
             K.k0 = O + nnf;                        // gdt_circ64(a, c).k0, or 1
             K.kx = fma(cc, ay, -(aa * py));        // gdt_circ64(a, c).kx, or 0
             K.ky = fma(aa, px, -(cc * ax));        // gdt_circ64(a, c).ky, or 0
-            Pthr = nnf * cc;
+            Pthr = t ? -1e300 : nnf * cc;
         }
     };
     auto visit_k = [&](int k) { visit(k, load(k)); };
     // Lane groups, pairwise by lane distance o = 1, 2, ...: the lower lane of
     // each pair keeps its partner's best when it lies strictly inside its own
-    // circle (nearest neighbour: strictly closer; ties to the lower entry),
-    // both take that decision and recompute K and Pthr with the expressions
-    // of the visit, so the group's states end identical.
+    // circle (nearest neighbour: strictly closer), both take that decision and
+    // recompute K and Pthr with the expressions of the visit, so the group's
+    // states end identical.  A tie in either lane, or between them, makes the
+    // group's best GDT_TIE for the rest of the step (Pthr = -1e300 takes
+    // nothing more), so the point is deferred.
     auto pair_reduce = [&]() {
 #if defined(__HIPCC__)
         if constexpr (W > 1) {
@@ -487,18 +510,24 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
             for (int o = 1; o < W; o <<= 1) {
                 const int pk = __shfl_xor(bk, o, 64);
                 int win = bk;
-                if (pk >= 0 && pk != bk) {
+                if (bk == GDT_TIE || pk == GDT_TIE) {
+                    win = GDT_TIE;
+                } else if (pk >= 0 && pk != bk) {
                     if (bk < 0) {
                         win = pk;
                     } else {
                         const gdt_d2 X = L.X64[pk];
                         const double px = X.x - vx, py = X.y - vy;
                         const double P = gdt_incircle_cc64(K, px, py, gdt_dist2_64(px, py));
-                        if (P < Pthr || (P == Pthr && pk < bk)) win = pk;
+                        if (P < Pthr) win = pk;
+                        else if (P == Pthr) win = GDT_TIE;
                     }
                 }
                 win = __shfl(win, (int)(__lane_id() & ~(unsigned)(2 * o - 1)), 64);
-                if (win != bk) {
+                if (win == GDT_TIE) {
+                    bk = GDT_TIE;
+                    Pthr = -1e300;
+                } else if (win != bk) {
                     bk = win;
                     const gdt_d2 X = L.X64[bk];
                     const double px = X.x - vx, py = X.y - vy;
@@ -588,7 +617,7 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
         pair_reduce();
         ok = true;
     }
-    if (!ok) return STEP_DEFER;
+    if (!ok) return STEP_DEFER;     // (a best of GDT_TIE is never certified)
     best();
     return gdt_tile_advance<double>(L, io, st, wrap, bk, bx, by);
 }
@@ -672,6 +701,9 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
         y = X.y - V.y;
     };
     auto load = [&](int k) { return L.X32[k]; };
+    // an exact tie met in the FP64 decision (the FP64 path's GDT_TIE): the
+    // point is deferred at the end of the step
+    bool tie = false;
     auto visit = [&](int k, const gdt_f2 Y) {
         const float px = Y.x - vx, py = Y.y - vy;
         const float cc = fmaf(px, px, py * py);
@@ -696,12 +728,16 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
                         double b64x, b64y;
                         x64(bk, b64x, b64y);
                         const Circ64 K64 = gdt_circ64(a64x, a64y, b64x, b64y);
-                        take = gdt_incircle64(K64, c64x, c64y) < 0.;
+                        const double P64 = gdt_incircle64(K64, c64x, c64y);
+                        take = P64 < 0.;
+                        tie |= P64 == 0.;
                     }
                 } else {
                     double b64x, b64y;
                     x64(bk, b64x, b64y);
-                    take = gdt_dist2_64(c64x, c64y) < gdt_dist2_64(b64x, b64y);
+                    const double dc = gdt_dist2_64(c64x, c64y), db = gdt_dist2_64(b64x, b64y);
+                    take = dc < db;
+                    tie |= dc == db;
                 }
             }
             if (take) {
@@ -829,7 +865,7 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
         pair_reduce();
         ok = true;
     }
-    if (!ok) return STEP_DEFER;
+    if (!ok || gdt_group_any<W>(tie)) return STEP_DEFER;
     // the best's displacement (the loop keeps only its entry)
     const gdt_f2 B = L.X32[bk];
     return gdt_tile_advance<float>(L, io, st, wrap, bk, B.x - vx, B.y - vy);

@@ -20,7 +20,8 @@
 //
 // The routine is written against a wave policy W with
 //     GdtWaveBest W::ring(const GdtWaveBest& start, bool wrap, double ax,
-//                         double ay, Fn& lane_fn)
+//                         double ay, GdtTieIds& t, Fn& lane_fn)
+// (template argument TIES: whether exact ties are decided or only reported)
 // which runs lane_fn(lane, best) on every lane (each starting from `start`)
 // and returns the reduction of the lane bests, identical on every lane: a
 // device policy with cross-lane shuffles, and a host policy that runs the
@@ -45,23 +46,51 @@ GDT_HD inline GdtWaveBest gdt_wave_none()
     return b;
 }
 
-// Whether candidate c beats the current best b.
-GDT_HD inline bool gdt_wave_better(bool wrap, double ax, double ay, const GdtWaveBest& b,
-                                   const GdtWaveBest& c)
+// What exact ties need: slot -> caller index, and the slots of v and a (a -1
+// in the nearest-neighbour search).  Caller indices are read only for a tie.
+// `tie` records that one was met by a routine built without the tie rule.
+struct GdtTieIds {
+    const int* perm;
+    int v, a;
+    bool tie;
+};
+
+// Whether candidate c beats the current best b.  TIES: exact ties by
+// gdt_nn_tie / gdt_incircle_tie of the caller indices (a total order);
+// otherwise a tie keeps b and is recorded in t.tie (the kernel that inlines
+// this variant keeps its registers for its own loop).
+template <bool TIES>
+GDT_HD inline bool gdt_wave_better(bool wrap, double ax, double ay, GdtTieIds& t,
+                                   const GdtWaveBest& b, const GdtWaveBest& c)
 {
     if (c.k < 0) return false;
     if (b.k < 0) return true;
-    if (!wrap) return c.d2 < b.d2 || (c.d2 == b.d2 && c.k < b.k);
+    if (!wrap) {
+        if (!(c.d2 == b.d2)) return c.d2 < b.d2;
+        if (!TIES) {
+            t.tie = true;
+            return false;
+        }
+        return gdt_nn_tie(t.perm[c.k], t.perm[b.k]);
+    }
     const Circ64 K = gdt_circ64(ax, ay, b.x, b.y);
-    return gdt_incircle64(K, c.x, c.y) < 0.;
+    const double P = gdt_incircle64(K, c.x, c.y);
+    if (!(P == 0.)) return P < 0.;
+    if (!TIES) {
+        t.tie = true;
+        return false;
+    }
+    return gdt_incircle_tie(t.perm[t.v], t.perm[t.a], t.perm[b.k], t.perm[c.k], ax, ay, b.x, b.y,
+                            c.x, c.y);
 }
 
 // Reduction tree of the lane bests (the device policy's shuffle-down order):
 // after the pass with offset `off`, lane i < off holds the better of lanes i
 // and i + off.
 struct GdtWaveHost {
-    template <class Fn>
-    GdtWaveBest ring(const GdtWaveBest& start, bool wrap, double ax, double ay, Fn& lane_fn)
+    template <bool TIES, class Fn>
+    GdtWaveBest ring(const GdtWaveBest& start, bool wrap, double ax, double ay, GdtTieIds& t,
+                     Fn& lane_fn)
     {
         GdtWaveBest lb[GDT_WAVE];
         for (int l = 0; l < GDT_WAVE; ++l) {
@@ -70,14 +99,16 @@ struct GdtWaveHost {
         }
         for (int off = GDT_WAVE / 2; off > 0; off >>= 1)
             for (int i = 0; i < off; ++i)
-                if (gdt_wave_better(wrap, ax, ay, lb[i], lb[i + off])) lb[i] = lb[i + off];
+                if (gdt_wave_better<TIES>(wrap, ax, ay, t, lb[i], lb[i + off])) lb[i] = lb[i + off];
         return lb[0];
     }
 };
 
 // One point's ring.  `lead` is true on the one lane that writes the row.
-// npred counts the predicates this lane evaluated.
-template <class W>
+// npred counts the predicates this lane evaluated.  Built without the tie rule
+// (TIES false), a point that meets an exact tie returns ERR_TIE (its row
+// partly written) for the caller to hand to the variant with it.
+template <bool TIES = true, class W>
 GDT_HD inline GdtRingResult gdt_wave_ring(const GdtSet& S, int j, const GdtRingIO& io, W& w,
                                           bool lead)
 {
@@ -90,8 +121,13 @@ GDT_HD inline GdtRingResult gdt_wave_ring(const GdtSet& S, int j, const GdtRingI
     const size_t row = (size_t)io.perm[j] * (size_t)io.stride;
     unsigned npred = 0;
 
+    GdtTieIds t;
+    t.perm = io.perm;
+    t.v = j;
+    t.tie = false;
     // One search: nearest neighbour (wrap false) or successor of edge (v,a).
     auto search = [&](bool wrap, int a, double ax, double ay, GdtWaveBest& g) -> bool {
+        t.a = a;
         g = gdt_wave_none();
         // One pass over the cells of [x0,x1] x [y0,y1] outside [-hole,hole]^2
         // (hole < 0: none), dealt to the lanes, then the wave reduction.
@@ -117,20 +153,31 @@ GDT_HD inline GdtRingResult gdt_wave_ring(const GdtSet& S, int j, const GdtRingI
                         c.d2 = gdt_dist2_64(c.x, c.y);
                         ++npred;
                         if (!wrap) {
-                            if (gdt_wave_better(false, 0., 0., m, c)) m = c;
+                            if (gdt_wave_better<TIES>(false, 0., 0., t, m, c)) m = c;
                             continue;
                         }
                         if (!(gdt_orient64(ax, ay, c.x, c.y) > 0.)) continue;
                         if (m.k >= 0) {
                             ++npred;
-                            if (!(gdt_incircle64(K, c.x, c.y) < 0.)) continue;
+                            const double P = gdt_incircle64(K, c.x, c.y);
+                            if (!(P <= 0.)) continue;
+                            if (P == 0.) {
+                                if constexpr (!TIES) {
+                                    t.tie = true;
+                                    continue;
+                                } else if (!gdt_incircle_tie(io.perm[j], io.perm[a], io.perm[m.k],
+                                                             io.perm[k], ax, ay, m.x, m.y, c.x,
+                                                             c.y)) {
+                                    continue;
+                                }
+                            }
                         }
                         m = c;
                         K = gdt_circ64(ax, ay, c.x, c.y);
                     }
                 }
             };
-            g = w.ring(g, wrap, ax, ay, lane_fn);
+            g = w.template ring<TIES>(g, wrap, ax, ay, t, lane_fn);
         };
         // The square [-hs,hs]^2 first (hs = 2), then either the rest of the
         // bounding box of the current circle in one pass -- every later best's
@@ -198,12 +245,14 @@ GDT_HD inline GdtRingResult gdt_wave_ring(const GdtSet& S, int j, const GdtRingI
             res.err |= g.k < 0 ? ERR_NONEIGHBOUR : ERR_SEARCH;
             break;
         }
+        if (!TIES && t.tie) break;      // handed on (the flag is the same on every lane)
         if (g.k == nn) break;
         a = g.k;
         ax = g.x;
         ay = g.y;
     }
     res.count = count;
+    if (!TIES && t.tie) res.err |= ERR_TIE;
     res.npred = npred;
     return res;
 }

@@ -106,8 +106,9 @@ __global__ __launch_bounds__(1024) void k_one_ring_list(const GdtSet* __restrict
 // then a shuffle-down tree leaves the best at lane 0, which is broadcast.
 struct GdtWaveDev {
     int lane;
-    template <class Fn>
-    __device__ GdtWaveBest ring(const GdtWaveBest& start, bool wrap, double ax, double ay, Fn& fn)
+    template <bool TIES, class Fn>
+    __device__ GdtWaveBest ring(const GdtWaveBest& start, bool wrap, double ax, double ay,
+                                GdtTieIds& t, Fn& fn)
     {
         GdtWaveBest m = start;
         fn(lane, m);
@@ -118,13 +119,14 @@ struct GdtWaveDev {
             p.x = __shfl_down(m.x, off, 64);
             p.y = __shfl_down(m.y, off, 64);
             p.d2 = __shfl_down(m.d2, off, 64);
-            if (lane < off && gdt_wave_better(wrap, ax, ay, m, p)) m = p;
+            if (lane < off && gdt_wave_better<TIES>(wrap, ax, ay, t, m, p)) m = p;
         }
         GdtWaveBest g;
         g.k = __shfl(m.k, 0, 64);
         g.x = __shfl(m.x, 0, 64);
         g.y = __shfl(m.y, 0, 64);
         g.d2 = __shfl(m.d2, 0, 64);
+        if constexpr (!TIES) t.tie = __ballot(t.tie) != 0;
         return g;
     }
 };
@@ -295,9 +297,16 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
         wv.lane = tid & 63;
         for (int t = tid >> 6; t < ndl; t += nt >> 6) {
             const int jd = s_dlist[t];
-            const GdtRingResult r = gdt_wave_ring(S, jd, io, wv, wv.lane == 0);
+            // (built without the tie rule, which would cost this kernel's main
+            // loop registers: a point with an exact tie goes to k_one_ring_wave)
+            const GdtRingResult r = gdt_wave_ring<false>(S, jd, io, wv, wv.lane == 0);
             npred += r.npred;
-            if (wv.lane == 0) {
+            if (r.err & ERR_TIE) {
+                if (wv.lane == 0) {
+                    const uint32_t pos = atomicAdd(reinterpret_cast<unsigned int*>(&status[ST_NDEFER]), 1u);
+                    defer[pos] = jd;
+                }
+            } else if (wv.lane == 0) {
                 counts[io.perm[jd]] = r.count;
                 mx = max(mx, r.count);
                 er |= r.err;

@@ -26,6 +26,12 @@
  *   - ring overflow is always detected and reports the FINAL ring size
  *     (reference: DelaunayGPU.cu:874-879 reports MaxSize+1 and only on GPU);
  *   - cells are clamped to [0,xsize-1] x [0,ysize-1].
+ *
+ * Degenerate input (four cocircular points, equidistant neighbours) is
+ * outside the reference's contract (README.md:55) and outside this oracle's:
+ * the device routines decide exact ties by a symbolic perturbation, and the
+ * tests check those cases against exact integer arithmetic
+ * (tests/exact_delaunay.py), not against this code.
  */
 #ifndef GDT_ORACLE_H
 #define GDT_ORACLE_H

@@ -96,7 +96,10 @@ int main(int argc, char** argv)
                 if (!fits || (r.err & ERR_DEFER)) {
                     const unsigned np = fits ? r.npred : 0u, nf = fits ? r.nfb : 0u;
                     ++ndefer;
-                    r = gdt_wave_ring(S, j, io, wave, true);
+                    // the block's own pass has no tie rule; a tie goes on to
+                    // the wavefront kernel, which has it
+                    r = gdt_wave_ring<false>(S, j, io, wave, true);
+                    if (r.err & ERR_TIE) r = gdt_wave_ring<true>(S, j, io, wave, true);
                     r.npred += np;
                     r.nfb += nf;
                 }

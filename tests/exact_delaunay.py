@@ -73,3 +73,90 @@ def rings(P, lx_cells, ly_cells, D=3.0):
         assert len(ring) == len(succ)
         out.append(ring)
     return out
+
+
+def _sos_sign(d0, coef, ids):
+    """Sign of d0 + sum_j coef[j] eps^(1 + ids[j]) as eps -> 0+ (0 only when
+    every term vanishes): the term of the smallest caller index dominates."""
+    if d0 != 0:
+        return 1 if d0 > 0 else -1
+    for j in np.argsort(ids, kind="stable"):
+        if coef[j] != 0:
+            return 1 if coef[j] > 0 else -1
+    return 0
+
+
+def perturbed_violations(P, lx_cells, ly_cells, nbrs, counts, D=3.0):
+    """Triangles of the rings that are not Delaunay for the symbolically
+    perturbed points (lifted coordinate |p|^2 + eps^(1 + caller index)).
+
+    Every point v with ring a_0 .. a_{c-1} gives triangles (v, a_m, a_m+1);
+    each must be counter-clockwise and have no other point inside or on its
+    circle after the perturbation.  The 3x3 in-circle determinant is linear in
+    its lifted column: with rows p-d, q-d, r-d (coordinates relative to d) it
+    gains (e_p - e_d) C_p + (e_q - e_d) C_q + (e_r - e_d) C_r, C_x the
+    cofactors of the lifted column, whose sum is orient(p, q, r).  For perturbed
+    points in general position the Delaunay triangulation is unique, so zero
+    violations together with consistent rings pin the triangulation.  Box of
+    at least 2 D + 1 in units of the spacing (one image of each point within D).
+    """
+    n = P.shape[0]
+    Lx, Ly = int(round(lx_cells * SCALE)), int(round(ly_cells * SCALE))
+    Dm = int(D * SCALE)
+    bad = 0
+    for v in range(n):
+        d = P - P[v]
+        d[:, 0] -= Lx * np.round(d[:, 0] / Lx).astype(np.int64)
+        d[:, 1] -= Ly * np.round(d[:, 1] / Ly).astype(np.int64)
+        near = np.nonzero((np.abs(d[:, 0]) <= Dm) & (np.abs(d[:, 1]) <= Dm))[0]
+        near = near[near != v]
+        C = d[near]
+        cv = int(counts[v])
+        for m in range(cv):
+            a, b = int(nbrs[v, m]), int(nbrs[v, (m + 1) % cv])
+            A, B = d[a], d[b]
+            if A[0] * B[1] - A[1] * B[0] <= 0:
+                bad += 1
+                continue
+            keep = (near != a) & (near != b)
+            K, ik = C[keep], near[keep]
+            z = np.zeros_like(K)
+            d0 = _incircle(z, np.broadcast_to(A, K.shape), np.broadcast_to(B, K.shape), K)
+            inside = d0 > 0
+            for t in np.nonzero(d0 == 0)[0]:
+                p, q, r = -K[t], A - K[t], B - K[t]
+                cp = int(q[0] * r[1] - q[1] * r[0])
+                cq = -int(p[0] * r[1] - p[1] * r[0])
+                cr = int(p[0] * q[1] - p[1] * q[0])
+                s = _sos_sign(0, [cp, cq, cr, -(cp + cq + cr)], [v, a, b, int(ik[t])])
+                inside[t] = s >= 0
+            bad += int(inside.sum())
+    return bad
+
+
+def ring_inconsistencies(nbrs, counts):
+    """Directed ring edges whose triangle is missing from a vertex's ring:
+    triangle (v, a, b) counter-clockwise means b is followed by v around a."""
+    n = len(counts)
+    pos = [dict() for _ in range(n)]
+    for i in range(n):
+        for m in range(int(counts[i])):
+            pos[i][int(nbrs[i, m])] = m
+    bad = 0
+    for i in range(n):
+        ci = int(counts[i])
+        for m in range(ci):
+            a, b = int(nbrs[i, m]), int(nbrs[i, (m + 1) % ci])
+            if b not in pos[a] or i not in pos[a] or (pos[a][b] + 1) % int(counts[a]) != pos[a][i]:
+                bad += 1
+    return bad
+
+
+def square_lattice(nx, ny, spacing=1.0):
+    """Exact square lattice of the given spacing (a multiple of 2^-9 cells),
+    points at the centres of its squares, integer coordinates in units of
+    2^-10; periodic box nx * spacing by ny * spacing."""
+    h = int(round(spacing * SCALE))
+    assert h == spacing * SCALE and h % 2 == 0
+    X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    return np.stack([X.reshape(-1) * h + h // 2, Y.reshape(-1) * h + h // 2], axis=1).astype(np.int64)

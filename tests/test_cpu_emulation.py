@@ -95,3 +95,56 @@ def test_emulated_kernels_match_oracle(emul, tmp_path, case):
         assert np.array_equal(c64, c32)
         for i in range(len(c64)):
             assert np.array_equal(n64[i, :c64[i]], n32[i, :c64[i]])
+
+
+def _flip(nb, c, v, m):
+    """Rings after flipping edge (v, b = ring_v[m]) of triangles (v, a, b) and
+    (v, b, d) to edge (a, d)."""
+    R = [list(nb[i, :c[i]]) for i in range(len(c))]
+    cv = len(R[v])
+    a, b, d = R[v][(m - 1) % cv], R[v][m], R[v][(m + 1) % cv]
+    R[v].remove(b)
+    R[b].remove(v)
+    R[a].insert(R[a].index(v), d)       # around a: b, d, v
+    R[d].insert(R[d].index(b), a)       # around d: v, a, b
+    out = np.full_like(nb, -1)
+    cnt = np.array([len(r) for r in R], dtype=c.dtype)
+    for i, r in enumerate(R):
+        out[i, :len(r)] = r
+    return out, cnt
+
+
+# exact square lattices: every square's four points are cocircular
+TIES = {"square16": (16, 16, 1.0), "rect12x20": (12, 20, 1.0), "half16": (16, 16, 0.5)}
+
+
+@pytest.mark.parametrize("case", sorted(TIES))
+def test_emulated_kernels_exact_ties(emul, tmp_path, case):
+    """Exact ties (cocircular squares, equidistant neighbours) are decided by
+    the symbolic perturbation |p|^2 + eps^(1 + caller index): every path and
+    precision gives the same rings, and they are the Delaunay triangulation of
+    the perturbed points (checked with exact integer determinants and the
+    perturbation's leading term, not with the kernels' tie code)."""
+    from exact_delaunay import SCALE, perturbed_violations, ring_inconsistencies, square_lattice
+    nx, ny, sp = TIES[case]
+    P = square_lattice(nx, ny, sp)
+    xy, lx, ly = P / SCALE, nx * sp, ny * sp
+    ref = None
+    for tiled, mode in ((1, 0), (0, 0), (1, 1), (2, 1), (1, 2), (0, 2)):
+        for prec in (64, 32):
+            nb, c, _, _, err = run(emul, str(tmp_path), xy, lx, ly, tiled, prec, mode=mode)
+            assert err == 0
+            if ref is None:
+                ref = rings.ring_hash(nb, c)
+                assert int(c.sum()) == 6 * len(c)
+                assert ring_inconsistencies(nb, c) == 0
+                assert perturbed_violations(P, lx, ly, nb, c) == 0
+                # the other diagonal of a square: still a triangulation with
+                # empty (unperturbed) circles, but not the perturbed one's
+                # a diagonal: neither the same row nor the same column
+                v, m = next((v, m) for v in range(len(c)) for m in range(c[v])
+                            if (P[nb[v, m]] != P[v]).all())
+                fn, fc = _flip(nb, c, v, m)
+                assert ring_inconsistencies(fn, fc) == 0
+                assert perturbed_violations(P, lx, ly, fn, fc) > 0
+            assert rings.ring_hash(nb, c) == ref, (tiled, mode, prec)
