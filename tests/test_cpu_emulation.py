@@ -148,3 +148,46 @@ def test_emulated_kernels_exact_ties(emul, tmp_path, case):
                 assert ring_inconsistencies(fn, fc) == 0
                 assert perturbed_violations(P, lx, ly, fn, fc) > 0
             assert rings.ring_hash(nb, c) == ref, (tiled, mode, prec)
+
+
+def test_tie_rule_matches_perturbed_determinant(tmp_path):
+    """gdt_incircle_tie against the leading term of the perturbed in-circle
+    determinant (exact integers, tests/exact_delaunay.py), for every ordered
+    choice of v, a, b, c among the 12 integer points of the circle
+    x^2 + y^2 = 25 with b and c left of v->a, and all 24 ways to rank their
+    caller indices: all four branches of the rule (which point is raised
+    most) meet every configuration of the circle."""
+    from itertools import permutations
+
+    from exact_delaunay import _sos_sign
+    exe = str(tmp_path / "tie_check")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", f"-I{CSRC}",
+                    os.path.join(ROOT, "tests", "cpu_emul", "tie_check.cpp"), "-o", exe], check=True)
+    circ = [(x, y) for x in range(-5, 6) for y in range(-5, 6) if x * x + y * y == 25]
+    assert len(circ) == 12
+    lines = []
+    for v in circ:
+        others = [(x - v[0], y - v[1]) for (x, y) in circ if (x, y) != v]
+        for A, B, C in permutations(others, 3):
+            if A[0] * B[1] - A[1] * B[0] <= 0 or A[0] * C[1] - A[1] * C[0] <= 0:
+                continue
+            p, q, r = (-C[0], -C[1]), (A[0] - C[0], A[1] - C[1]), (B[0] - C[0], B[1] - C[1])
+            cp = q[0] * r[1] - q[1] * r[0]
+            cq = -(p[0] * r[1] - p[1] * r[0])
+            cr = p[0] * q[1] - p[1] * q[0]
+            for pv, pa, pb, pc in permutations(range(4)):
+                lines.append((pv, pa, pb, pc, A, B, C, cp, cq, cr))
+    # determinant rows v - c, a - c, b - c (cofactors cp, cq, cr of the lifted
+    # column); the tested point c carries -(cp + cq + cr); > 0: c inside
+    exp = []
+    for (pv, pa, pb, pc, A, B, C, cp, cq, cr) in lines:
+        s = _sos_sign(0, [cp, cq, cr, -(cp + cq + cr)], [pv, pa, pb, pc])
+        assert s != 0
+        exp.append(1 if s > 0 else 0)
+    inp = "".join(f"ic {pv} {pa} {pb} {pc} {A[0]} {A[1]} {B[0]} {B[1]} {C[0]} {C[1]}\n"
+                  for (pv, pa, pb, pc, A, B, C, *_r) in lines)
+    out = subprocess.run([exe], input=inp, capture_output=True, text=True, check=True).stdout.split()
+    got = [int(x) for x in out]
+    assert len(got) == len(exp) > 1000
+    bad = [i for i in range(len(exp)) if got[i] != exp[i]]
+    assert not bad, (len(bad), lines[bad[0]][:7])
