@@ -51,8 +51,8 @@ struct GdtTileWin {
 struct GdtTileLds {
     uint32_t* cs;        // [GDT_WIN_CELLS+1] first LDS entry of each window cell
     int* gstart;         // [GDT_WIN_CELLS] first global slot of each window cell
-    gdt_d2* X64;         // [GDT_WIN_MAXPTS] FP64 frame coordinates
-    gdt_f2* X32;         // [GDT_WIN_MAXPTS] FP32 frame coordinates (filtered path)
+    gdt_d2* X64;         // [GDT_WIN_MAXPTS + 1] FP64 frame coordinates
+    gdt_f2* X32;         // [GDT_WIN_MAXPTS + 1] FP32 frame coordinates (filtered path)
     int* orig;           // [GDT_WIN_MAXPTS] caller index
 };
 
@@ -280,35 +280,41 @@ GDT_HD inline unsigned gdt_disk_rows(const GdtTileLds& L, int c0, int& dy, int d
 // unconditional and the coordinates of the next entry, loaded, load(k),
 // before the current one is evaluated, fn(k, coords), are waited for one
 // half-trip later.  The word after the last row is read but never used (the
-// tables are laid out so that it exists).
+// tables are laid out so that it exists), and so is the entry after the last
+// row's end, read as the last prefetch: its index is a cell end, at most the
+// window's point count, so the coordinate arrays hold GDT_WIN_MAXPTS + 1
+// entries.
 template <class Load, class Fn>
 GDT_HD inline void gdt_scan_tbl(const uint32_t* tbl, unsigned n, Load& load, Fn& fn)
 {
     if (n == 0) return;
     const uint32_t* p = tbl;
-    uint32_t k = p[0] & 0xffffu, e = p[0] >> 16;
+    // the current entry k and its segment's end e stay packed as in the table
+    // (k in the low half, e in the high half; k < e <= 2^16, so ++ke steps k)
+    uint32_t ke = p[0];
     uint32_t wn = p[1];
     auto advance = [&](bool more) {
-        const bool sw = more & (k == e);
-        k = sw ? (wn & 0xffffu) : k;
-        e = sw ? (wn >> 16) : e;
+        const bool sw = more & ((ke & 0xffffu) == (ke >> 16));
+        ke = sw ? wn : ke;
         p += sw ? 1 : 0;
         wn = p[1];
     };
-    auto XA = load((int)k);
+    auto XA = load((int)(ke & 0xffffu));
     decltype(XA) XB;
     for (unsigned i = 1;; ++i) {
-        uint32_t kc = k++;
+        uint32_t kc = ke & 0xffffu;
+        ++ke;
         bool more = i < n;
         advance(more);
-        XB = load((int)(more ? k : kc));
+        XB = load((int)(ke & 0xffffu));
         fn((int)kc, XA);
         if (!more) break;
         ++i;
-        kc = k++;
+        kc = ke & 0xffffu;
+        ++ke;
         more = i < n;
         advance(more);
-        XA = load((int)(more ? k : kc));
+        XA = load((int)(ke & 0xffffu));
         fn((int)kc, XB);
         if (!more) break;
     }

@@ -196,8 +196,8 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
 {
     __shared__ uint32_t s_cs[GDT_WIN_CELLS + 1];
     __shared__ int s_gstart[GDT_WIN_CELLS];
-    __shared__ gdt_d2 s_X64[GDT_WIN_MAXPTS];
-    __shared__ gdt_f2 s_X32[PREC == 32 ? GDT_WIN_MAXPTS : 1];
+    __shared__ gdt_d2 s_X64[GDT_WIN_MAXPTS + 1];     // (+1: read by the scan's last prefetch)
+    __shared__ gdt_f2 s_X32[PREC == 32 ? GDT_WIN_MAXPTS + 1 : 1];
     __shared__ int s_orig[GDT_WIN_MAXPTS];
     __shared__ uint32_t s_wsum[16];
     __shared__ int s_next;

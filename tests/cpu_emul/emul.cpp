@@ -71,8 +71,8 @@ int main(int argc, char** argv)
         // one emulated block per tile: stage counts, scan, copy, then every point
         std::vector<uint32_t> cs(GDT_WIN_CELLS + 1);
         std::vector<int> gstart(GDT_WIN_CELLS), orig(GDT_WIN_MAXPTS);
-        std::vector<gdt_d2> X64(GDT_WIN_MAXPTS);
-        std::vector<gdt_f2> X32(GDT_WIN_MAXPTS);
+        std::vector<gdt_d2> X64(GDT_WIN_MAXPTS + 1);
+        std::vector<gdt_f2> X32(GDT_WIN_MAXPTS + 1);
         GdtTileLds L = {cs.data(), gstart.data(), X64.data(), X32.data(), orig.data()};
         const int ntiles = S.nkeys / GDT_TILE_CELLS;
         for (int t = 0; t < ntiles; ++t) {
