@@ -81,21 +81,23 @@ def time_path(dg, pts, tri, num, steps, warmup, torch):
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ring_ms, cell_ms, defer_ms = [], [], []
+    # the library sums its own HIP-event kernel times over the timed calls;
+    # the loop itself makes no other call
+    s0 = dg.stats()
     t0 = time.perf_counter()
     ev0.record(stream)
     for _ in range(steps):
         dg.globalDelaunayTriangulation(pts, tri, num)
-        st = dg.stats()
-        ring_ms.append(st["ms_ring"])
-        cell_ms.append(st["ms_cells"])
-        defer_ms.append(st["ms_defer"])
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = ev0.elapsed_time(ev1) / steps
-    return (ms, wall, sum(ring_ms) / steps, sum(cell_ms) / steps, sum(defer_ms) / steps,
-            dg.stats())
+    s1 = dg.stats()
+    calls = s1["profiled_calls"] - s0["profiled_calls"]       # steps, or 0 without profile = 1
+    assert calls in (0, steps)
+    per = {k: (s1[k + "_total"] - s0[k + "_total"]) / max(calls, 1)
+           for k in ("ms_ring", "ms_cells", "ms_defer")}
+    return ms, wall, per["ms_ring"], per["ms_cells"], per["ms_defer"], s1
 
 
 def cpu_model():

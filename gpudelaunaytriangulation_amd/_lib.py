@@ -57,6 +57,8 @@ class Stats(ctypes.Structure):
         ("xsize", ctypes.c_int), ("ysize", ctypes.c_int), ("bsx", ctypes.c_double),
         ("bsy", ctypes.c_double), ("ms_cells", ctypes.c_float), ("ms_ring", ctypes.c_float),
         ("deferred", ctypes.c_int), ("ms_defer", ctypes.c_float),
+        ("profiled_calls", ctypes.c_ulonglong), ("ms_cells_total", ctypes.c_double),
+        ("ms_ring_total", ctypes.c_double), ("ms_defer_total", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:

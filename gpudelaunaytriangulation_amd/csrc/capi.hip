@@ -486,6 +486,10 @@ static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int*
         c->stats.ms_cells = a;
         c->stats.ms_ring = b;
         c->stats.ms_defer = d;
+        ++c->stats.profiled_calls;
+        c->stats.ms_cells_total += a;
+        c->stats.ms_ring_total += b;
+        c->stats.ms_defer_total += d;
     }
     return rc;
 }

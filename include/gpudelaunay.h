@@ -69,6 +69,11 @@ typedef struct {
     float ms_ring;                   /* last one-ring kernel (profile = 1), ms */
     int deferred;                    /* points the LDS tile kernel handed back */
     float ms_defer;                  /* kernels of the handed-back points (profile = 1), ms */
+    /* running totals over every global triangulation timed with profile = 1
+     * since the context was created (read twice and subtract to average a
+     * loop without reading the statistics inside it) */
+    unsigned long long profiled_calls;
+    double ms_cells_total, ms_ring_total, ms_defer_total;
 } gdt_stats;
 
 /* ---- context ---------------------------------------------------------- */
