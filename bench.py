@@ -148,7 +148,7 @@ def bench_md(steps, torch, precision="fp64"):
     triangulation, then `steps` hipGraph-replayed steps of sigma = 0.01
     displacement -> test -> repair."""
     import numpy as np
-    from gpudelaunaytriangulation_amd import PREC_FP32F, PREC_FP64, inputs
+    from gpudelaunaytriangulation_amd import PREC_FP32F, PREC_FP64, inputs, rings
     from gpudelaunaytriangulation_amd.delaunay import BatchedDelaunayGPU
     sizes = inputs.batch_sizes(256)
     pts, boxes = inputs.batch_points(sizes)
@@ -163,18 +163,31 @@ def bench_md(steps, torch, precision="fp64"):
     bd.globalDelaunayTriangulation(X, tri, num)
     torch.cuda.synchronize()
     t_tri = time.perf_counter() - t0
+    max_ring0 = int(num.max().item())
     bd.md_run(X, tri, num, 5, 0.01, seed=99, step0=0, use_graph=True, check_every=5)   # warm-up
-    torch.cuda.synchronize()
+    torch.cuda.synchronize()      # (the warm-up also widens the rows to the longest ring + 4)
     t0 = time.perf_counter()
     rep = bd.md_run(X, tri, num, steps, 0.01, seed=99, step0=5, use_graph=True, check_every=100)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
+    # the final rings against a fresh global triangulation of the final positions
+    fresh = BatchedDelaunayGPU(sizes, [(L, L) for L in boxes], bd.MaxSize, 1.0)
+    tri2 = torch.zeros(bd.N * fresh.MaxSize, dtype=torch.int32, device="cuda")
+    num2 = torch.zeros(bd.N, dtype=torch.int32, device="cuda")
+    fresh.globalDelaunayTriangulation(X, tri2, num2)
+    torch.cuda.synchronize()
+    final_match = (rings.ring_hash(tri.cpu().numpy().reshape(bd.N, bd.MaxSize), num.cpu().numpy())
+                   == rings.ring_hash(tri2.cpu().numpy().reshape(bd.N, fresh.MaxSize),
+                                      num2.cpu().numpy()))
     return {"workload": f"256 sets, {bd.N} points, sigma 0.01, hipGraph replay, {precision}",
             "points": bd.N, "batch_triangulation_sets_per_s": 256 / t_tri,
             "batch_triangulation_points_per_s": bd.N / t_tri, "steps": rep["steps"],
             "steps_per_s": rep["steps"] / t, "sets_steps_per_s": 256 * rep["steps"] / t,
             "repaired_fraction": rep["flagged_total"] / (rep["steps"] * bd.N),
+            "max_ring_start": max_ring0, "max_size": bd.MaxSize,
             "recoveries": rep["recoveries"], "recovered": rep["recovered"],
+            "skipped_replays_after_halt": rep["skipped_replays"],
+            "final_rings_match": bool(final_match),
             "step_roofline_frac": (48 + 32 + 96 * rep["flagged_total"] / (rep["steps"] * bd.N))
             * bd.N * rep["steps"] / t / 1e9 / HBM_PEAK_GBS}
 

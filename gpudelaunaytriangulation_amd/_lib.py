@@ -68,7 +68,7 @@ class Stats(ctypes.Structure):
 class MdReport(ctypes.Structure):
     _fields_ = [("steps_done", ctypes.c_int), ("first_bad_step", ctypes.c_int),
                 ("required", ctypes.c_int), ("max_flagged", ctypes.c_int),
-                ("flagged_total", ctypes.c_longlong)]
+                ("skipped_steps", ctypes.c_int), ("flagged_total", ctypes.c_longlong)]
 
 
 _lib = None

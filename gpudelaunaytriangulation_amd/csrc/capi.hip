@@ -87,6 +87,9 @@ struct gdt_context {
     unsigned int* hd_seq = nullptr;
     unsigned int pub_seq = 0;
     gdt_stats stats;
+    // While a Brownian step is enqueued: the device word that halts the step's
+    // kernels (&md->halted), else null.
+    const int* halt = nullptr;
 
     GdtCells cells()
     {
@@ -95,6 +98,7 @@ struct gdt_context {
         c.perm = perm.p; c.slot_of = nullptr; c.off64 = off64.p; c.off32 = off32.p;
         c.scell = scell.p; c.scan_tmp = scan_tmp.p;
         c.bkt = bkt.p; c.bmat = bmat.p; c.bscan = bscan.p;
+        c.halt = halt;
         return c;
     }
 };
@@ -530,7 +534,7 @@ static int enumerate(gdt_context* c, const int* d_nbrs, const int* d_counts, int
     GDT_CHECK(gdt_launch_triangles(c->d_sets.p, c->pch.p, c->npch, c->total_n,
                                    slot_order ? c->perm.p : nullptr, d_nbrs, d_counts, stride,
                                    c->tcount.p, c->toff.p, tris, cap, c->scan_tmp.p, c->tpart.p,
-                                   c->counters.p, c->status.p, s));
+                                   c->counters.p, c->status.p, c->halt, s));
     return GDT_OK;
 }
 
@@ -612,7 +616,7 @@ int gdt_test_circumcircles(gdt_context* c, const double* d_xy, int n, const int*
         GDT_CHECK(c->worklist.reserve((size_t)n));
         GDT_CHECK(gdt_launch_compact(d_repair, c->perm.p, n, c->mark.p, c->markscan.p,
                                      c->worklist.p, reinterpret_cast<uint32_t*>(c->status.p + ST_NFLAGGED),
-                                     c->scan_tmp.p, s));
+                                     c->scan_tmp.p, nullptr, s));
         int h = 0;
         GDT_CHECK(hipMemcpyAsync(&h, c->status.p + ST_NFLAGGED, sizeof(int), hipMemcpyDeviceToHost, s));
         GDT_CHECK(hipStreamSynchronize(s));
@@ -635,7 +639,7 @@ static int repair_flagged(gdt_context* c, const int* d_repair, int stride, int* 
     GDT_CHECK(c->worklist.reserve(n));
     uint32_t* worklen = reinterpret_cast<uint32_t*>(c->status.p + ST_NFLAGGED);
     GDT_CHECK(gdt_launch_compact(d_repair, c->perm.p, c->total_n, c->mark.p, c->markscan.p,
-                                 c->worklist.p, worklen, c->scan_tmp.p, s));
+                                 c->worklist.p, worklen, c->scan_tmp.p, c->halt, s));
     if (tiled_sets(c)) {
         GDT_CHECK(gdt_launch_one_ring_tile(precision, c->d_sets.p, (int)c->sets.size(), c->tch.p,
                                            c->ntch, c->cfg.block, c->total_n, c->worklist.p,
@@ -733,9 +737,12 @@ int gdt_test_and_repair_batch(gdt_context* c, const double* d_xy, int nsets, con
 }  // extern "C"
 
 // One step on stream s, device only (capturable): every buffer it touches is
-// reserved beforehand by gdt_md_run.
-static int md_enqueue_step(gdt_context* c, double* d_xy, int stride, int* d_nbrs, int* d_counts,
-                           int precision, double sigma, unsigned long long seed, hipStream_t s)
+// reserved beforehand by gdt_md_run.  Every kernel of the step that reads or
+// writes the positions, the cell list, the rings or the loop state watches
+// md->halted (set by the end of a failed step).
+static int md_enqueue_step_halting(gdt_context* c, double* d_xy, int stride, int* d_nbrs,
+                                   int* d_counts, int precision, double sigma,
+                                   unsigned long long seed, hipStream_t s)
 {
     const int n = c->total_n, cap = 2 * n + 64;
     int rc;
@@ -744,8 +751,18 @@ static int md_enqueue_step(gdt_context* c, double* d_xy, int stride, int* d_nbrs
     if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
     if ((rc = triangles_and_test(c, d_xy, d_nbrs, d_counts, stride, cap, s)) != GDT_OK) return rc;
     if ((rc = repair_flagged(c, c->repair.p, stride, d_nbrs, d_counts, precision, s)) != GDT_OK) return rc;
-    GDT_CHECK(gdt_launch_md_step_end(c->status.p, c->counters.p, c->toff.p + n, n, c->md.p, s));
+    GDT_CHECK(gdt_launch_md_step_end(c->status.p, c->counters.p, c->toff.p + n, n, stride, c->md.p, s));
     return GDT_OK;
+}
+
+static int md_enqueue_step(gdt_context* c, double* d_xy, int stride, int* d_nbrs, int* d_counts,
+                           int precision, double sigma, unsigned long long seed, hipStream_t s)
+{
+    c->halt = &c->md.p->halted;
+    const int rc = md_enqueue_step_halting(c, d_xy, stride, d_nbrs, d_counts, precision, sigma,
+                                           seed, s);
+    c->halt = nullptr;
+    return rc;
 }
 
 // Restores the profile flag (events are not recorded inside the loop).
@@ -820,9 +837,7 @@ int gdt_md_run(gdt_context* c, double* d_xy, int nsets, const int* offsets, cons
         if (he == hipSuccess)
             he = hipMemcpyAsync(c->h_md, c->md.p, sizeof(GdtMdState), hipMemcpyDeviceToHost, s);
         if (he == hipSuccess) he = hipStreamSynchronize(s);
-        if (he == hipSuccess && rc == GDT_OK &&
-            (c->h_md->errflags || c->h_md->euler || c->h_md->required > stride))
-            break;
+        if (he == hipSuccess && rc == GDT_OK && c->h_md->halted) break;
     }
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -833,8 +848,12 @@ int gdt_md_run(gdt_context* c, double* d_xy, int nsets, const int* offsets, cons
     rep->first_bad_step = m.first_bad;
     rep->required = m.required;
     rep->max_flagged = m.max_flagged;
+    rep->skipped_steps = done - rep->steps_done;
     rep->flagged_total = (long long)m.flagged_total;
     c->stats.flagged = m.max_flagged;
+    // a halted loop left the cell list of its last built step; the caller
+    // re-triangulates, which rebuilds it
+    if (m.halted) c->cells_valid = false;
     // a ring longer than the stride truncates its row, which the next step
     // then reports as a malformed triangulation: the overflow is the cause
     if (m.required > stride) return GDT_OVERFLOW;

@@ -214,8 +214,10 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_bin(const GdtSet* __restrict__ se
                                                    const gdt_d2* __restrict__ xy,
                                                    uint32_t* __restrict__ pkey,
                                                    uint32_t* __restrict__ prank,
-                                                   uint32_t* __restrict__ ccount)
+                                                   uint32_t* __restrict__ ccount,
+                                                   const int* __restrict__ halt)
 {
+    if (halt && *halt) return;
     const GdtChunk ch = chunks[blockIdx.x];
     const GdtSet S = sets[ch.set];
     const int li = ch.start + (int)threadIdx.x;
@@ -235,8 +237,10 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_scatter(const GdtSet* __restrict_
                                                        const uint32_t* __restrict__ pkey,
                                                        const uint32_t* __restrict__ prank,
                                                        const uint32_t* __restrict__ cstart,
-                                                       int* __restrict__ perm)
+                                                       int* __restrict__ perm,
+                                                       const int* __restrict__ halt)
 {
+    if (halt && *halt) return;
     const GdtChunk ch = chunks[blockIdx.x];
     const GdtSet S = sets[ch.set];
     const int li = ch.start + (int)threadIdx.x;
@@ -253,8 +257,10 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_finalize(const GdtSet* __restrict
                                                         int* __restrict__ slot_of,
                                                         gdt_d2* __restrict__ off64,
                                                         gdt_f2* __restrict__ off32,
-                                                        uint32_t* __restrict__ scell)
+                                                        uint32_t* __restrict__ scell,
+                                                        const int* __restrict__ halt)
 {
+    if (halt && *halt) return;
     const GdtChunk ch = chunks[blockIdx.x];
     const GdtSet S = sets[ch.set];
     const int lk = ch.start + (int)threadIdx.x;
@@ -342,8 +348,10 @@ __device__ inline void kb_bin(const GdtSet* __restrict__ sets, int nsets, const 
 // Pass 1: bmat[bucket * nb + block] = points of the block in the bucket.
 __global__ __launch_bounds__(KB_THREADS) void k_kb_count(const GdtSet* __restrict__ sets, int nsets,
                                                          const gdt_d2* __restrict__ xy, int n,
-                                                         KbPlan pl, uint32_t* __restrict__ bmat)
+                                                         KbPlan pl, uint32_t* __restrict__ bmat,
+                                                         const int* __restrict__ halt)
 {
+    if (halt && *halt) return;
     extern __shared__ uint32_t hist[];
     const int tid = threadIdx.x, b = blockIdx.x;
     for (int q = tid; q < pl.nkb; q += KB_THREADS) hist[q] = 0u;
@@ -426,8 +434,10 @@ __device__ inline void kb_scan_lds(uint32_t* a, int m, uint32_t* wsum)
 __global__ __launch_bounds__(KB_THREADS) void k_kb_scatter(const GdtSet* __restrict__ sets, int nsets,
                                                            const gdt_d2* __restrict__ xy, int n,
                                                            KbPlan pl, uint32_t* __restrict__ bscan,
-                                                           GdtKbRec* __restrict__ bkt)
+                                                           GdtKbRec* __restrict__ bkt,
+                                                           const int* __restrict__ halt)
 {
+    if (halt && *halt) return;
     extern __shared__ uint32_t cursor[];                  // [nkb + 1]
     __shared__ uint32_t wsum[KB_THREADS / 64];
     const int tid = threadIdx.x, b = blockIdx.x;
@@ -482,6 +492,7 @@ __global__ __launch_bounds__(KB_THREADS) void k_kb_finalize(KbPlan pl, int total
                                                             const GdtKbRec* __restrict__ bkt,
                                                             GdtCells c)
 {
+    if (c.halt && *c.halt) return;
     extern __shared__ uint32_t lds[];
     uint32_t* cst = lds;                                  // [2^B + 1] cell starts
     __shared__ uint32_t wsum[KB_THREADS / 64];
@@ -575,15 +586,15 @@ static hipError_t build_cells_single_level(const GdtSet* d_sets, const GdtChunk*
     hipError_t e = gdt_fill_u32(c.ccount, 0u, (size_t)total_keys, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_bin, dim3(npch), dim3(GDT_BLOCK), 0, s, d_sets, pchunks, xy, c.pkey,
-                       c.prank, c.ccount);
+                       c.prank, c.ccount, c.halt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     e = gdt_scan_u32(c.ccount, c.cstart, (size_t)total_keys, c.cstart + total_keys, c.scan_tmp, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_scatter, dim3(npch), dim3(GDT_BLOCK), 0, s, d_sets, pchunks, c.pkey,
-                       c.prank, c.cstart, c.perm);
+                       c.prank, c.cstart, c.perm, c.halt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_finalize, dim3(nkch), dim3(GDT_BLOCK), 0, s, d_sets, kchunks, c.cstart,
-                       xy, c.perm, c.slot_of, c.off64, c.off32, c.scell);
+                       xy, c.perm, c.slot_of, c.off64, c.off32, c.scell, c.halt);
     return hipGetLastError();
 }
 
@@ -596,13 +607,13 @@ hipError_t gdt_build_cells(const GdtSet* d_sets, int nsets, const GdtChunk* pchu
     if (pl.B > KB_MAXSHIFT)
         return build_cells_single_level(d_sets, pchunks, npch, kchunks, nkch, total_keys, xy, c, s);
     hipLaunchKernelGGL(k_kb_count, dim3(pl.nb), dim3(KB_THREADS), pl.nkb * sizeof(uint32_t), s,
-                       d_sets, nsets, xy, total_n, pl, c.bmat);
+                       d_sets, nsets, xy, total_n, pl, c.bmat, c.halt);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_kb_rowscan, dim3(pl.nkb), dim3(KB_NB), 0, s, pl, c.bmat, c.bscan);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_kb_scatter, dim3(pl.nb), dim3(KB_THREADS), (pl.nkb + 1) * sizeof(uint32_t),
-                       s, d_sets, nsets, xy, total_n, pl, c.bscan, c.bkt);
+                       s, d_sets, nsets, xy, total_n, pl, c.bscan, c.bkt, c.halt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const size_t cells = ((size_t)1 << pl.B) + 1;
     hipLaunchKernelGGL(k_kb_finalize, dim3(pl.nkb), dim3(KB_THREADS),

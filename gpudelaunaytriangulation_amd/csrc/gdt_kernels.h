@@ -28,6 +28,8 @@ struct GdtCells {
     GdtKbRec* bkt;      // [N] the points in bucket order
     uint32_t* bmat;     // [gdt_bucket_words(N, K)] per (bucket, block) counts
     uint32_t* bscan;    // [gdt_bucket_words(N, K)] their exclusive scan
+    const int* halt;    // device word: when non-null and set, the kernels of the
+                        // build (and of the tests that take these arrays) exit at once
 };
 
 // Words of the (bucket, block) count table of the cell-list build.
@@ -93,7 +95,8 @@ hipError_t gdt_launch_triangles(const GdtSet* d_sets, const GdtChunk* pchunks, i
                                 int total_n, const int* perm, const int* nbrs, const int* counts,
                                 int stride, uint32_t* tcount, uint32_t* toff, int* tris, int cap,
                                 uint32_t* scan_tmp, unsigned long long* partials,
-                                unsigned long long* counters, int* status, hipStream_t s);
+                                unsigned long long* counters, int* status, const int* halt,
+                                hipStream_t s);
 
 // Empty-circle test of *ntri triangles; repair must be -1 on entry.
 hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, const uint32_t* ntri,
@@ -113,10 +116,11 @@ hipError_t gdt_launch_ring_test_tile(const GdtSet* d_sets, const GdtChunk* tiles
 
 // Slots of the flagged points, in slot order -> worklist; markscan (total_n+1
 // words) = exclusive scan of the flags with the count at markscan[total_n];
-// count -> *worklen (device).
+// count -> *worklen (device).  With *halt set (halt non-null) no point is
+// marked, so the repair that reads the list does nothing.
 hipError_t gdt_launch_compact(const int* repair, const int* perm, int total_n, uint32_t* mark,
                               uint32_t* markscan, int* worklist, uint32_t* worklen,
-                              uint32_t* scan_tmp, hipStream_t s);
+                              uint32_t* scan_tmp, const int* halt, hipStream_t s);
 
 // Voronoi energy (per-set sums in energy[nsets]; partial holds
 // nsets * GDT_SUM_PARTS doubles, ecell total_n) and its adjoint (G holds
@@ -137,13 +141,19 @@ hipError_t gdt_launch_voronoi_adjoint(int precision, const GdtSet* d_sets, int n
                                       double* grad, hipStream_t s);
 
 // Brownian test-and-repair loop: state kept on the device across steps.
+// The first step that fails (an error flag, a ring longer than the row
+// stride, or an input that is not a triangulation) sets `halted`; every kernel
+// of every later step then exits at once, so no step runs on truncated rows
+// and the positions stay those of the failing step (its displacement done).
 struct GdtMdState {
     int step;                         // step index of the next displacement
     int errflags;                     // OR of the steps' ERR_* flags
     int euler;                        // 1 if a step started from a non-triangulation
     int required;                     // longest ring seen
-    int first_bad;                    // first step with an error, -1 none
+    int first_bad;                    // the failing step, -1 none
     int max_flagged;                  // most points repaired in one step
+    int halted;                       // 1 from the failing step's end on
+    int pad;
     unsigned long long flagged_total; // points repaired, summed over steps
 };
 hipError_t gdt_launch_md_init(GdtMdState* md, int step0, hipStream_t s);
@@ -151,4 +161,5 @@ hipError_t gdt_launch_md_displace(const GdtSet* d_sets, const GdtChunk* pchunks,
                                   double* xy, double sigma, unsigned long long seed,
                                   const GdtMdState* md, hipStream_t s);
 hipError_t gdt_launch_md_step_end(const int* status, const unsigned long long* counters,
-                                  const uint32_t* ntri, int total_n, GdtMdState* md, hipStream_t s);
+                                  const uint32_t* ntri, int total_n, int stride, GdtMdState* md,
+                                  hipStream_t s);

@@ -4,7 +4,11 @@
 // noise of gdt_rng.h, wrapped into its box) -> triangles of the current rings
 // -> cell list -> empty-circle test -> compaction -> repair -> bookkeeping
 // (k_md_step_end).  Nothing in a step reads back to the host, so a step is
-// captured once in a hipGraph and replayed.  The reference moves the points in
+// captured once in a hipGraph and replayed.  The first step that fails sets
+// md->halted at its end, and every kernel of the later replays exits at once
+// (this one, the cell list, the ring test, and the compaction, which then
+// marks no point, so the repair has nothing to do): the positions keep the
+// failing step's displacement and no step runs on truncated rows.  The reference moves the points in
 // its driver (triangulation.cpp:349-415) and calls
 // testAndRepairDelaunayTriangulation (DelaunayGPU.cpp:567-602) once per step,
 // with host synchronisations inside.
@@ -17,6 +21,7 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_md_displace(const GdtSet* __restr
                                                            unsigned long long seed,
                                                            const GdtMdState* __restrict__ md)
 {
+    if (md->halted) return;
     const GdtChunk ch = chunks[blockIdx.x];
     const GdtSet S = sets[ch.set];
     const int li = ch.start + (int)threadIdx.x;
@@ -31,17 +36,24 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_md_displace(const GdtSet* __restr
 
 // End of a step (one thread): fold the step's status into the sticky words,
 // check the Euler counts of the triangulation the step started from, and
-// advance the step counter.
+// advance the step counter.  A step that found an error, a repaired ring
+// longer than the row stride (its row is truncated) or an input that is not a
+// triangulation halts the loop; a halted replay changes nothing.
 __global__ void k_md_step_end(const int* __restrict__ status,
                               const unsigned long long* __restrict__ counters,
-                              const uint32_t* __restrict__ ntri, int total_n,
+                              const uint32_t* __restrict__ ntri, int total_n, int stride,
                               GdtMdState* __restrict__ md)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (md->halted) return;
     const int err = status[ST_ERRFLAGS];
     const bool euler = *ntri != 2u * (uint32_t)total_n ||
                        counters[CT_INSUM] != 6ULL * (unsigned long long)total_n;
-    if ((err || euler) && md->first_bad < 0) md->first_bad = md->step;
+    const bool over = status[ST_REQUIRED] > stride;
+    if (err || euler || over) {
+        md->first_bad = md->step;
+        md->halted = 1;
+    }
     md->errflags |= err;
     if (euler) md->euler = 1;
     md->required = max(md->required, status[ST_REQUIRED]);
@@ -60,6 +72,8 @@ __global__ void k_md_init(GdtMdState* __restrict__ md, int step0)
     md->required = 0;
     md->first_bad = -1;
     md->max_flagged = 0;
+    md->halted = 0;
+    md->pad = 0;
     md->flagged_total = 0ULL;
 }
 
@@ -79,8 +93,10 @@ hipError_t gdt_launch_md_displace(const GdtSet* d_sets, const GdtChunk* pchunks,
 }
 
 hipError_t gdt_launch_md_step_end(const int* status, const unsigned long long* counters,
-                                  const uint32_t* ntri, int total_n, GdtMdState* md, hipStream_t s)
+                                  const uint32_t* ntri, int total_n, int stride, GdtMdState* md,
+                                  hipStream_t s)
 {
-    hipLaunchKernelGGL(k_md_step_end, dim3(1), dim3(64), 0, s, status, counters, ntri, total_n, md);
+    hipLaunchKernelGGL(k_md_step_end, dim3(1), dim3(64), 0, s, status, counters, ntri, total_n,
+                       stride, md);
     return hipGetLastError();
 }

@@ -19,8 +19,10 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_tri_count(const GdtSet* __restric
                                                          const int* __restrict__ counts, int stride,
                                                          uint32_t* __restrict__ tcount,
                                                          unsigned long long* __restrict__ partials,
-                                                         int* __restrict__ status)
+                                                         int* __restrict__ status,
+                                                         const int* __restrict__ halt)
 {
+    if (halt && *halt) return;
     __shared__ unsigned long long s_tot[GDT_BLOCK / 64];
     const GdtChunk ch = chunks[blockIdx.x];
     const GdtSet S = sets[ch.set];
@@ -83,8 +85,10 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_tri_write(const GdtSet* __restric
                                                          const int* __restrict__ nbrs,
                                                          const int* __restrict__ counts, int stride,
                                                          const uint32_t* __restrict__ toff,
-                                                         int* __restrict__ tris, int cap)
+                                                         int* __restrict__ tris, int cap,
+                                                         const int* __restrict__ halt)
 {
+    if (halt && *halt) return;
     const GdtChunk ch = chunks[blockIdx.x];
     const GdtSet S = sets[ch.set];
     const int li = ch.start + (int)threadIdx.x;
@@ -115,17 +119,18 @@ hipError_t gdt_launch_triangles(const GdtSet* d_sets, const GdtChunk* pchunks, i
                                 int total_n, const int* perm, const int* nbrs, const int* counts,
                                 int stride, uint32_t* tcount, uint32_t* toff, int* tris, int cap,
                                 uint32_t* scan_tmp, unsigned long long* partials,
-                                unsigned long long* counters, int* status, hipStream_t s)
+                                unsigned long long* counters, int* status, const int* halt,
+                                hipStream_t s)
 {
     hipLaunchKernelGGL(k_tri_count, dim3(npch), dim3(GDT_BLOCK), 0, s, d_sets, pchunks, perm, nbrs,
-                       counts, stride, tcount, partials, status);
+                       counts, stride, tcount, partials, status, halt);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partials, npch, counters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = gdt_scan_u32(tcount, toff, (size_t)total_n, toff + total_n, scan_tmp, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_tri_write, dim3(npch), dim3(GDT_BLOCK), 0, s, d_sets, pchunks, perm, nbrs,
-                       counts, stride, toff, tris, cap);
+                       counts, stride, toff, tris, cap, halt);
     return hipGetLastError();
 }
 
@@ -226,8 +231,10 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_test(const GdtSet* __restrict__ s
                                                     const uint32_t* __restrict__ cstart,
                                                     const int* __restrict__ perm,
                                                     const gdt_d2* __restrict__ off64,
-                                                    int* __restrict__ repair)
+                                                    int* __restrict__ repair,
+                                                    const int* __restrict__ halt)
 {
+    if (halt && *halt) return;
     const int nt = min((int)*ntri, cap);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
         const int i = tris[3 * (size_t)t], a = tris[3 * (size_t)t + 1], b = tris[3 * (size_t)t + 2];
@@ -256,7 +263,7 @@ hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, con
     if (grid > 8192) grid = 8192;
     hipLaunchKernelGGL(k_test, dim3(grid), dim3(GDT_BLOCK), 0, s, d_sets, nsets, tris, ntri,
                        max_tri, total_n, reinterpret_cast<const gdt_d2*>(d_xy), c.cstart, c.perm, c.off64,
-                       repair);
+                       repair, c.halt);
     return hipGetLastError();
 }
 
@@ -337,8 +344,10 @@ __global__ __launch_bounds__(512) void k_ring_test_tile(const GdtSet* __restrict
                                                         GdtRingIO io,
                                                         int* __restrict__ repair,
                                                         unsigned long long* __restrict__ partials,
-                                                        int* __restrict__ status)
+                                                        int* __restrict__ status,
+                                                        const int* __restrict__ halt)
 {
+    if (halt && *halt) return;
     __shared__ uint32_t s_cs[GDT_WIN_CELLS + 1];
     __shared__ int s_gstart[GDT_WIN_CELLS];
     __shared__ gdt_d2 s_X64[GDT_WIN_MAXPTS];
@@ -479,7 +488,7 @@ hipError_t gdt_launch_ring_test_tile(const GdtSet* d_sets, const GdtChunk* tiles
     io.stride = 0;
     hipLaunchKernelGGL(k_ring_test_tile, dim3(ntiles), dim3(512), 0, s, d_sets, tiles, nbrs,
                        counts, stride, total_n, reinterpret_cast<const gdt_d2*>(d_xy), io,
-                       repair, partials, status);
+                       repair, partials, status, c.halt);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sum_ring_partials, dim3(1), dim3(256), 0, s, partials, ntiles, counters,
@@ -490,10 +499,12 @@ hipError_t gdt_launch_ring_test_tile(const GdtSet* d_sets, const GdtChunk* tiles
 
 __global__ __launch_bounds__(GDT_BLOCK) void k_mark(const int* __restrict__ repair,
                                                     const int* __restrict__ perm, int n,
-                                                    uint32_t* __restrict__ mark)
+                                                    uint32_t* __restrict__ mark,
+                                                    const int* __restrict__ halt)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) mark[s] = repair[perm[s]] >= 0 ? 1u : 0u;
+    const bool off = halt && *halt;
+    if (s < n) mark[s] = !off && repair[perm[s]] >= 0 ? 1u : 0u;
 }
 
 // Flagged slots, in slot order, to worklist[pos[s]]; *worklen = pos[n].
@@ -509,10 +520,10 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_compact(const uint32_t* __restric
 
 hipError_t gdt_launch_compact(const int* repair, const int* perm, int total_n, uint32_t* mark,
                               uint32_t* markscan, int* worklist, uint32_t* worklen,
-                              uint32_t* scan_tmp, hipStream_t s)
+                              uint32_t* scan_tmp, const int* halt, hipStream_t s)
 {
     const int g = (total_n + GDT_BLOCK - 1) / GDT_BLOCK;
-    hipLaunchKernelGGL(k_mark, dim3(g), dim3(GDT_BLOCK), 0, s, repair, perm, total_n, mark);
+    hipLaunchKernelGGL(k_mark, dim3(g), dim3(GDT_BLOCK), 0, s, repair, perm, total_n, mark, halt);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = gdt_scan_u32(mark, markscan, (size_t)total_n, markscan + total_n, scan_tmp, s);

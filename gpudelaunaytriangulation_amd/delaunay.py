@@ -388,7 +388,7 @@ class DelaunayGPU:
         _lib.check(st, "testAndRepairDelaunayTriangulation")
 
 
-MD_HEADROOM = 2   # extra row slots after an overflow inside the Brownian loop
+MD_HEADROOM = 4   # row slots beyond the longest ring, given before and after the Brownian loop runs
 
 
 class BatchedDelaunayGPU:
@@ -475,23 +475,44 @@ class BatchedDelaunayGPU:
             return
         _lib.check(st, "testAndRepairDelaunayTriangulation (batch)")
 
+    def restride(self, GPUTriangulation, new_max: int) -> None:
+        """Widen the rows of a valid ring table to ``new_max`` slots in
+        place (row i moves to ``i*new_max``), as the reference's resize of
+        its triangulation array does (DelaunayGPU.cpp:111, 130)."""
+        new_max = int(new_max)
+        if new_max <= self.MaxSize:
+            return
+        rows = GPUTriangulation.view(self.N, self.MaxSize).clone()
+        GPUTriangulation.resize_(self.N * new_max)
+        GPUTriangulation.view(self.N, new_max)[:, :self.MaxSize].copy_(rows)
+        self.MaxSize = new_max
+
     # --- Brownian test-and-repair loop -----------------------------------
     def md_run(self, points, GPUTriangulation, cellNeighborNum, nsteps: int, sigma: float = 0.01,
                seed: int = 1, step0: int = 0, use_graph: bool = True, check_every: int = 100,
-               stream=None) -> dict:
+               stream=None, headroom: int = MD_HEADROOM) -> dict:
         """``nsteps`` steps of: move every point by sigma*N(0,1) per coordinate
         (counter-based noise keyed by (seed, step, set, point)), then
         test-and-repair, all on the device (one hipGraph replay per step when
-        ``use_graph``).  The rings must triangulate ``points`` on entry.  A
-        step that overflows ``MaxSize`` or fails is recovered at the next
-        check by a global triangulation of the current positions (the
-        positions do not depend on the rings), then the loop continues.
-        Returns the totals (steps, repaired points, recoveries)."""
+        ``use_graph``).  The rings must triangulate ``points`` on entry.
+
+        Before the loop the rows are widened to the longest ring plus
+        ``headroom`` slots (0: as they are).  The first step that fails (a
+        repaired ring longer than ``MaxSize``, an error) halts the loop on
+        the device at its end: no later step runs, the positions keep that
+        step's displacement, and the host completes the step with a global
+        triangulation of the current positions (after widening the rows by
+        ``headroom`` beyond the longest ring on an overflow), then goes on
+        from the next step.  Returns the totals (steps, repaired points,
+        recoveries, and for each recovery the failing step)."""
         self._check(points, GPUTriangulation, cellNeighborNum)
         if GPUTriangulation.numel() != self.N * self.MaxSize:
             raise ValueError("GPUTriangulation must hold N*MaxSize entries")
         L = _lib.load()
-        out = dict(steps=0, flagged_total=0, max_flagged=0, recoveries=0, recovered=[])
+        if headroom > 0:
+            self.restride(GPUTriangulation, int(cellNeighborNum.max().item()) + int(headroom))
+        out = dict(steps=0, flagged_total=0, max_flagged=0, recoveries=0, recovered=[],
+                   skipped_replays=0)
         step, left = int(step0), int(nsteps)
         while left > 0:
             rep = _lib.MdReport()
@@ -505,16 +526,24 @@ class BatchedDelaunayGPU:
             out["steps"] += rep.steps_done
             out["flagged_total"] += rep.flagged_total
             out["max_flagged"] = max(out["max_flagged"], rep.max_flagged)
+            out["skipped_replays"] += rep.skipped_steps
             if st == _lib.GDT_OK:
                 break
             if st not in (_lib.GDT_OVERFLOW, _lib.GDT_ERR_NONGENERIC, _lib.GDT_ERR_EULER):
                 _lib.check(st, "md_run")
+            if rep.first_bad_step != step - 1:
+                raise RuntimeError(f"md_run: the loop halted at step {rep.first_bad_step} but "
+                                   f"the positions are at step {step}")
             if st == _lib.GDT_OVERFLOW:
-                # rings keep changing as the points move: leave headroom
-                self.MaxSize = max(self.MaxSize, int(rep.required)) + MD_HEADROOM
+                # rings keep changing as the points move: leave headroom (the
+                # truncated rows are rebuilt below, so no row needs moving)
+                self.MaxSize = max(self.MaxSize, int(rep.required)) + max(int(headroom), 1)
+                GPUTriangulation.resize_(self.N * self.MaxSize)
             out["recoveries"] += 1
-            out["recovered"].append(dict(status=_lib.status_string(st), first_bad_step=
-                                         rep.first_bad_step, at_step=step, required=rep.required))
+            out["recovered"].append(dict(status=_lib.status_string(st), failed_step=
+                                         rep.first_bad_step, required=rep.required,
+                                         max_size=self.MaxSize))
+            # completes the failing step: the triangulation of its positions
             self.globalDelaunayTriangulation(points, GPUTriangulation, cellNeighborNum, stream)
         self.last_repaired = out["max_flagged"]
         return out

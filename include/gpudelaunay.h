@@ -169,20 +169,27 @@ int gdt_test_and_repair_batch(gdt_context* ctx, const double* d_xy, int nsets, c
  * moves every point by sigma * N(0,1) per coordinate (counter-based noise of
  * (seed, step, set, point); wrapped into its box), then runs test-and-repair
  * (triangles -> cell list -> empty-circle test -> repair) entirely on the
- * device.  use_graph = 1 captures one step in a hipGraph and replays it.  The
- * device state is read back every check_every steps; the run stops at the
- * first check that finds an error (GDT_OVERFLOW with report.required, which
- * takes precedence because a truncated row also fails the next step's Euler
- * check; GDT_ERR_NONGENERIC; GDT_ERR_EULER): positions are then `steps_done`
- * steps ahead and the rings must be rebuilt by gdt_triangulate_batch.  Replaces the
+ * device.  use_graph = 1 captures one step in a hipGraph and replays it.
+ *
+ * The first step that fails -- a repaired ring longer than `stride` (its row
+ * is truncated), an error flag, or input rings that are not a triangulation
+ * -- halts the loop ON THE DEVICE at the end of that step: every kernel of
+ * every later step exits at once, so no step ever runs on truncated rows.
+ * The failing step's displacement is applied and counted in steps_done
+ * (first_bad_step = step0 + steps_done - 1); its rings are invalid and must
+ * be rebuilt by gdt_triangulate_batch of the current positions, which
+ * completes that step.  The host reads the device state every check_every
+ * steps and returns at the first check after the halt (GDT_OVERFLOW with
+ * report.required; GDT_ERR_NONGENERIC; GDT_ERR_EULER).  Replaces the
  * reference driver's loop of host displacement + testAndRepair per step
  * (triangulation.cpp:349-415; DelaunayGPU.cpp:567-602).  A null stream runs on
  * a stream of the context (synchronised with the device first). */
 typedef struct {
-    int steps_done;         /* steps applied to the positions */
-    int first_bad_step;     /* first step whose status had an error, -1 none */
+    int steps_done;         /* steps whose displacement was applied */
+    int first_bad_step;     /* the step that halted the loop, -1 none */
     int required;           /* longest ring seen */
     int max_flagged;        /* most points repaired in one step */
+    int skipped_steps;      /* replays enqueued after the halt, which did nothing */
     long long flagged_total;/* points repaired, summed over the steps */
 } gdt_md_report;
 

@@ -6,9 +6,15 @@ hipGraph replays:
   triangulation of those positions (canonical, per set);
 * the graph run (K replays of one captured step, with readbacks between
   checks) equals the eager run for K = 1, 3 and 12;
-* on the 256-set workload (about 3.97 M points, sigma = 0.01), 100 replayed
+* on the 256-set workload (about 3.92 M points, sigma = 0.01), 100 replayed
   steps end with rings equal to a fresh global triangulation of the final
-  positions."""
+  positions, and the full 1,000-step run ends with every set's rings equal to
+  the oracle's triangulation of its final positions (the positions of the
+  four smallest sets equal to the numpy restatement);
+* a step whose repaired ring outgrows the row stride halts the loop on the
+  device: no later step moves a point or touches a row, and the host's
+  recovery (global triangulation of the positions, wider rows) continues
+  from the next step, so the run still equals the restatement."""
 import numpy as np
 import pytest
 
@@ -112,9 +118,11 @@ def test_c4_hundred_steps_equal_fresh_triangulation():
 
 def test_overflow_inside_the_loop_is_recovered_exactly():
     """Rows sized to the starting triangulation (guess 4, grown only to the
-    initial longest ring) overflow once large moves lengthen some rings; the
-    loop reports the overflow, grows MaxSize, re-triangulates globally and
-    goes on; the final positions and rings still equal the restatement."""
+    initial longest ring, no headroom) overflow once large moves lengthen
+    some rings; the device halts the loop at the end of the failing step
+    (the positions are exactly that step's, no later step ran), the host
+    grows MaxSize, re-triangulates globally and goes on; the final positions
+    and rings still equal the restatement."""
     sizes, pts, boxes = small_batch()
     sigma, K = 0.25, 12
     bd = BatchedDelaunayGPU(sizes, [(L, L) for L in boxes], 4, 1.0)
@@ -123,11 +131,12 @@ def test_overflow_inside_the_loop_is_recovered_exactly():
     num = torch.zeros(bd.N, dtype=torch.int32, device="cuda")
     bd.globalDelaunayTriangulation(X, tri, num)
     ms0 = bd.MaxSize
-    rep = bd.md_run(X, tri, num, K, sigma, SEED, use_graph=True, check_every=1)
+    rep = bd.md_run(X, tri, num, K, sigma, SEED, use_graph=True, check_every=1, headroom=0)
     torch.cuda.synchronize()
     assert rep["steps"] == K
     assert rep["recoveries"] >= 1 and bd.MaxSize > ms0
     assert all(r["status"] == "ring longer than the row stride" for r in rep["recovered"])
+    assert rep["skipped_replays"] == 0      # checked after every step: nothing to skip
     ref_pos, ref_rings = [], []
     for s, (xy, L) in enumerate(zip(pts, boxes)):
         for t in range(K):
@@ -138,3 +147,62 @@ def test_overflow_inside_the_loop_is_recovered_exactly():
     got = X.cpu().numpy()
     assert np.array_equal(got.view(np.uint64), np.concatenate(ref_pos).view(np.uint64))
     assert same_sets(canonical_sets(bd, tri, num), ref_rings)
+
+
+def test_halted_loop_runs_no_step_after_the_failure():
+    """With checks only every 6 steps, the replays after an overflow inside
+    a check interval are enqueued but must change nothing: the positions
+    returned with the failure are exactly those of the failing step (the
+    restatement's, step for step) and the recovery continues from there."""
+    sizes, pts, boxes = small_batch()
+    sigma, K = 0.25, 12
+    bd = BatchedDelaunayGPU(sizes, [(L, L) for L in boxes], 4, 1.0)
+    X = torch.from_numpy(np.concatenate(pts)).cuda()
+    tri = torch.zeros(bd.N * bd.MaxSize, dtype=torch.int32, device="cuda")
+    num = torch.zeros(bd.N, dtype=torch.int32, device="cuda")
+    bd.globalDelaunayTriangulation(X, tri, num)
+    rep = bd.md_run(X, tri, num, K, sigma, SEED, use_graph=True, check_every=6, headroom=0)
+    torch.cuda.synchronize()
+    assert rep["steps"] == K and rep["recoveries"] >= 1
+    assert rep["skipped_replays"] >= 1, "no replay followed a failure inside a check interval"
+    ref_pos = []
+    for s, (xy, L) in enumerate(zip(pts, boxes)):
+        for t in range(K):
+            xy = inputs.md_step(xy, L, L, sigma, SEED, t, s)
+        ref_pos.append(xy)
+    got = X.cpu().numpy()
+    assert np.array_equal(got.view(np.uint64), np.concatenate(ref_pos).view(np.uint64))
+    ref = []
+    for xy, L in zip(ref_pos, boxes):
+        nb, c = oracle.triangulate(xy, L, L, 1.0, 24)
+        ref.append((c, rings.canonical(nb, c)))
+    assert same_sets(canonical_sets(bd, tri, num), ref)
+
+
+def test_c4_thousand_steps_equal_oracle():
+    """The full C4 run: 256 sets, 1,000 hipGraph-replayed steps of sigma =
+    0.01 from the batched triangulation, rows widened to the longest ring + 4
+    before the loop.  Every set's final rings equal the oracle's
+    triangulation of its final positions, and the four smallest sets'
+    positions equal the numpy restatement of the noise, step for step."""
+    sizes = inputs.batch_sizes(256)
+    pts, boxes = inputs.batch_points(sizes)
+    bd, X, tri, num = start(sizes, pts, boxes)
+    K = 1000
+    rep = bd.md_run(X, tri, num, K, SIGMA, SEED, use_graph=True, check_every=100)
+    torch.cuda.synchronize()
+    assert rep["steps"] == K
+    got = X.cpu().numpy()
+    canon = canonical_sets(bd, tri, num)
+    for s in np.argsort(sizes, kind="stable")[:4]:
+        xy, L = pts[s], boxes[s]
+        for t in range(K):
+            xy = inputs.md_step(xy, L, L, SIGMA, SEED, t, int(s))
+        a, b = bd.offsets[s], bd.offsets[s + 1]
+        assert np.array_equal(got[a:b].view(np.uint64), xy.view(np.uint64)), f"set {s}"
+    for s in range(bd.nsets):
+        a, b = bd.offsets[s], bd.offsets[s + 1]
+        L = boxes[s]
+        nb, c = oracle.triangulate(np.ascontiguousarray(got[a:b]), L, L, 1.0, 24)
+        assert np.array_equal(canon[s][0], c), f"set {s}: counts differ from the oracle"
+        assert np.array_equal(canon[s][1], rings.canonical(nb, c)), f"set {s}: rings differ"
