@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--no-voronoi", action="store_true", help="skip the Voronoi energy timing")
     ap.add_argument("--no-c1", action="store_true",
                     help="skip the 4,096-point case (GPU and CPU oracle timings)")
+    ap.add_argument("--no-c3b", action="store_true",
+                    help="skip the jittered-lattice case (2^20 points, both precisions)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the bench line, run a plain device copy of MIB MiB three "
                          "times (reference kernel for scaling rocprofv3 FETCH/WRITE_SIZE)")
@@ -140,6 +142,44 @@ def bench_c1(torch, cpu_runs, with_cpu):
             times.append(time.perf_counter() - t0)
         res["cpu_points_per_s"] = n / sorted(times)[len(times) // 2]
         res["cpu_cores"] = threads
+    return res
+
+
+def bench_c3b(torch, steps, warmup):
+    """The jittered triangular lattice: 1024 x 1024 points, spacing 1, rows
+    sqrt(3)/2 apart, Gaussian jitter sigma = 1e-5 (near-cocircular quads),
+    box 1024 x 886.81, cell size 1: points/s and fallback fraction per
+    precision, rows identical across precisions and equal to the committed
+    oracle hash."""
+    from gpudelaunaytriangulation_amd import PREC_FP32F, PREC_FP64, inputs, rings
+    from gpudelaunaytriangulation_amd.delaunay import DelaunayGPU, PeriodicBoundaries
+    lx, ly = inputs.lattice_box(1024, 1024)
+    xy = inputs.lattice_points(1024, 1024)
+    n = xy.shape[0]
+    pts = torch.from_numpy(xy).cuda()
+    res = {"workload": f"jittered triangular lattice 1024x1024 (sigma 1e-5), box {lx:g}x{ly:.6f}, "
+                       f"cell 1.0"}
+    hashes = {}
+    for name, prec in (("fp32f", PREC_FP32F), ("fp64", PREC_FP64)):
+        dg = DelaunayGPU(n, 16, 1.0, PeriodicBoundaries(lx, ly))
+        dg.setPrecision(prec)
+        dg.ctx.set_config(profile=1)
+        tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device="cuda")
+        num = torch.zeros(n, dtype=torch.int32, device="cuda")
+        ms, _, ring_ms, _, _, st = time_path(dg, pts, tri, num, steps, warmup, torch)
+        hashes[name] = rings.ring_hash(tri.cpu().numpy().reshape(n, dg.MaxSize), num.cpu().numpy())
+        res[f"{name}_points_per_s"] = n / (ms * 1e-3)
+        res[f"{name}_ms_per_step"] = ms
+        res[f"{name}_one_ring_ms"] = ring_ms
+        res[f"{name}_deferred_points"] = st["deferred"]
+        if name == "fp32f":
+            res["fallback_fraction"] = st["fallbacks"] / max(1, st["predicates"])
+            res["predicates_per_point"] = st["predicates"] / n
+    res["topology_identical_fp32_fp64"] = hashes["fp32f"] == hashes["fp64"]
+    gpath = os.path.join(ROOT, "tests", "golden", "golden.json")
+    with open(gpath) as fh:
+        res["matches_golden_oracle_hash"] = (json.load(fh)["workloads"]["c3b"]["ring_sha256"]
+                                             == hashes["fp32f"])
     return res
 
 
@@ -335,6 +375,12 @@ def main():
         except Exception as e:  # reported, never hides the headline number
             line["c1_4096"] = {"error": repr(e)}
         print(f"[bench] c1: {line['c1_4096']}", file=sys.stderr, flush=True)
+    if not args.no_c3b:
+        try:
+            line["c3b"] = bench_c3b(torch, args.steps, args.warmup)
+        except Exception as e:  # reported, never hides the headline number
+            line["c3b"] = {"error": repr(e)}
+        print(f"[bench] c3b: {line['c3b']}", file=sys.stderr, flush=True)
     if args.md_steps > 0:
         try:
             line["md_c4"] = bench_md(args.md_steps, torch, args.md_precision)

@@ -1,8 +1,10 @@
 """ctypes binding of libgpudelaunay.so (the C ABI in include/gpudelaunay.h).
 
 The library is built in-tree by ``gpudelaunaytriangulation_amd.build``.  There
-is no CPU fallback: if the shared library is missing or cannot be loaded,
-every entry point raises ``LibraryNotBuilt``.
+is no fallback of any kind: if the shared library is missing or cannot be
+loaded, every entry point raises ``LibraryNotBuilt``.  The library's CPU mode
+(``gdt_host_*``, the reference's ``setGPUcompute(false)``) runs only when a
+caller selects it; a device call never turns into a host one.
 """
 from __future__ import annotations
 
@@ -32,6 +34,8 @@ EXPORTS = (
     "gdt_test_and_repair", "gdt_voronoi_forward", "gdt_voronoi_adjoint",
     "gdt_voronoi_forward_batch", "gdt_voronoi_adjoint_batch", "gdt_triangulate_batch",
     "gdt_test_and_repair_batch", "gdt_md_run", "gdt_voronoi_vertices",
+    "gdt_host_celllist", "gdt_host_triangulate", "gdt_host_test_and_repair",
+    "gdt_host_voronoi_vertices",
 )
 
 
@@ -127,6 +131,10 @@ def load():
     _sig(L, "gdt_voronoi_vertices", [_vp, _vp, _i, _d, _d, _vp, _vp, _i, _vp, _vp])
     _sig(L, "gdt_md_run", [_vp, _vp, _i, _ip, ctypes.POINTER(_d), _d, _i, _vp, _vp, _i, _d,
                            ctypes.c_ulonglong, _i, _i, _i, _i, ctypes.POINTER(MdReport), _vp])
+    _sig(L, "gdt_host_celllist", [_vp, _i, _d, _d, _d, _vp, _vp, _i, _ip, _ip, _ip])
+    _sig(L, "gdt_host_triangulate", [_vp, _i, _d, _d, _d, _i, _vp, _vp, _i, _i, _ip])
+    _sig(L, "gdt_host_test_and_repair", [_vp, _i, _d, _d, _d, _i, _vp, _vp, _i, _i, _ip, _ip])
+    _sig(L, "gdt_host_voronoi_vertices", [_vp, _i, _d, _d, _vp, _vp, _i, _vp, _i])
     _lib = L
     return L
 

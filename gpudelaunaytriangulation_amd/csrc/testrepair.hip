@@ -5,6 +5,7 @@
 // appends triangles through one global atomic counter (order varies run to
 // run), by a count + exclusive scan + write (deterministic order), and
 // gpu_test_circumcircles_kernel (DelaunayGPU.cu:290-313).
+#include "gdt_circle.h"
 #include "gdt_kernels.h"
 #include "gdt_stage.h"
 
@@ -134,95 +135,8 @@ hipError_t gdt_launch_triangles(const GdtSet* d_sets, const GdtChunk* pchunks, i
     return hipGetLastError();
 }
 
-// Empty-circle test.  The circle through the triangle (circumcentre q, radius
-// R, both FP64 relative to the first vertex) is scanned by Chebyshev rings of
-// cells around q's cell until the covered square contains the whole circle.
-// The test is conservative: a point within relative 1e-9 of the circle counts
-// as inside, and a circle that cannot be covered is reported, so no invalid
-// triangle escapes (a flagged valid one is merely re-triangulated to itself).
-
-struct GdtTriCircle {
-    bool bad;            // not counter-clockwise: invalid whatever else holds
-    double qx, qy;       // circumcentre relative to the first vertex
-    double R2, lim;      // squared radius and the conservative test bound
-};
-
-// Minimum images use the inverse box sides (ilx, ily): neighbours are far
-// closer than half a box, so the rounded quotient picks the same image.  The
-// centre uses the hardware reciprocal with one Newton step (relative error
-// ~1e-15, far inside the test's 1e-9 margin).
-__device__ inline GdtTriCircle tri_circle(const GdtSet& S, double ilx, double ily,
-                                          const gdt_d2* __restrict__ xy, int i, int a, int b)
-{
-    GdtTriCircle c;
-    const gdt_d2 v = xy[i], pa = xy[a], pb = xy[b];
-    double ax = pa.x - v.x, ay = pa.y - v.y, bx = pb.x - v.x, by = pb.y - v.y;
-    ax -= S.lx * rint(ax * ilx);
-    ay -= S.ly * rint(ay * ily);
-    bx -= S.lx * rint(bx * ilx);
-    by -= S.ly * rint(by * ily);
-    const double D = ax * by - ay * bx;
-    c.bad = !(D > 0.);
-    c.qx = c.qy = c.R2 = c.lim = 0.;
-    if (!c.bad) {
-        const double aa = ax * ax + ay * ay, bb = bx * bx + by * by;
-        const double i2d = gdt_rcp64(2. * D);
-        c.qx = (aa * by - bb * ay) * i2d;
-        c.qy = (bb * ax - aa * bx) * i2d;
-        c.R2 = c.qx * c.qx + c.qy * c.qy;
-        c.lim = c.R2 * (1. + 1e-9);
-    }
-    return c;
-}
-
-// Global-memory scan of the circle of triangle (i, a, b); returns the
-// intruder's caller index (-1 none; the vertices, which lie on the circle, are
-// told apart by caller index) and whether the circle was covered.
-__device__ inline int tri_scan_global(const GdtSet& S, const gdt_d2* __restrict__ xy, int i,
-                                      const GdtTriCircle& c, int a, int b,
-                                      const uint32_t* __restrict__ cstart,
-                                      const int* __restrict__ perm,
-                                      const gdt_d2* __restrict__ off64, bool& covered)
-{
-    const gdt_d2 v = xy[i];
-    int qcx, qcy;
-    double oqx, oqy;
-    gdt_bin_point(S, v.x + c.qx, v.y + c.qy, qcx, qcy, oqx, oqy);
-    const double covb = fmin(fmin(oqx, S.bsx - oqx), fmin(oqy, S.bsy - oqy));
-    const double bsmin = fmin(S.bsx, S.bsy);
-    int intruder = -1;
-    covered = false;
-    for (int r = 0; r <= S.rmax && intruder < 0; ++r) {
-        const int ncell = r ? 8 * r : 1;
-        for (int q = 0; q < ncell && intruder < 0; ++q) {
-            int dx, dy;
-            gdt_ring_cell(r, q, dx, dy);
-            const int x = gdt_wrap(qcx + dx, S.xsize), y = gdt_wrap(qcy + dy, S.ysize);
-            const int key = S.c0 + gdt_cell_key(S, x, y);
-            const uint32_t cb = cstart[key], ce = cstart[key + 1];
-            const double ebx = gdt_base64(dx, S.bsx, oqx), eby = gdt_base64(dy, S.bsy, oqy);
-            for (uint32_t k = cb; k < ce; ++k) {
-                const gdt_d2 o = off64[k];
-                const double ex = gdt_disp64(ebx, o.x), ey = gdt_disp64(eby, o.y);
-                if (ex * ex + ey * ey < c.lim) {
-                    const int p = perm[k];
-                    if (p != i && p != a && p != b) { intruder = p; break; }
-                }
-            }
-        }
-        const double cov = (double)r * bsmin + covb;
-        if (intruder < 0 && cov > 0. && cov * cov >= c.R2 * (1. + 4e-9)) { covered = true; break; }
-    }
-    return intruder;
-}
-
-__device__ inline void tri_flag(int* __restrict__ repair, int i, int a, int b, int intruder_caller)
-{
-    repair[i] = i;
-    repair[a] = a;
-    repair[b] = b;
-    if (intruder_caller >= 0) repair[intruder_caller] = intruder_caller;
-}
+// The circle of a triangle and its scan over the cell list (tri_circle,
+// tri_scan_global, tri_flag) are shared with the CPU mode: gdt_circle.h.
 
 __global__ __launch_bounds__(GDT_BLOCK) void k_test(const GdtSet* __restrict__ sets, int nsets,
                                                     const int* __restrict__ tris,

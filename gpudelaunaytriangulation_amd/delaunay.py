@@ -2,9 +2,10 @@
 
 Names, arguments and error behaviour follow the reference classes
 ``DelaunayGPU`` (src/model/DelaunayGPU.h:18-130), ``cellListGPU``
-(src/utilities/cellListGPU.h:17-121) and ``periodicBoundaries``
-(src/utilities/periodicBoundaries.h:25-73).  Arrays are torch tensors on a
-HIP device, standing in for the reference's ``GPUArray``:
+(src/utilities/cellListGPU.h:17-121), ``periodicBoundaries``
+(src/utilities/periodicBoundaries.h:25-73) and ``multiProfiler``
+(src/utilities/multiProfiler.h:9-75).  Arrays are torch tensors on a HIP
+device, standing in for the reference's ``GPUArray``:
 
 * points            float64 [N, 2]        (GPUArray<double2>)
 * GPUTriangulation  int32   [N*MaxSize]   (GPUArray<int>, read as [N][MaxSize])
@@ -14,13 +15,18 @@ As in the reference, the triangulation call may grow ``MaxSize`` and resize
 the caller's ``GPUTriangulation`` in place (DelaunayGPU.cpp:130, 154).
 Unlike the reference, ring overflow is always detected, whatever
 ``setSafetyMode`` says (the reference silently truncates when it is off,
-DelaunayGPU.cu:874-879).  There is no CPU mode: ``setGPUcompute(False)``
-raises, because the reference's OpenMP path is not part of this library.
+DelaunayGPU.cu:874-879).
+
+CPU mode (``setGPUcompute(False)``, with ``setOMPthreads``) runs the library's
+own per-point routines on the host (``gdt_host_*``), on CPU tensors, as the
+reference's OpenMP branches do; it never stands in for a device call (a
+device call with CPU tensors raises).
 """
 from __future__ import annotations
 
 import ctypes
 import math
+import time
 
 import numpy as np
 
@@ -84,6 +90,66 @@ class PeriodicBoundaries:
 PeriodicBoxPtr = PeriodicBoundaries
 
 
+class MultiProfiler:
+    """Named wall-clock timers (multiProfiler, multiProfiler.h:9-75): ``start``
+    and ``end`` bracket a region, ``timing`` is its mean duration in seconds
+    over the completed calls, ``print`` lists every region."""
+
+    def __init__(self):
+        self.numberOfProfiles = 0
+        self.processMap: dict[str, int] = {}
+        self.startTimes: list[float] = []
+        self.endTimes: list[float] = []
+        self.functionCalls: list[int] = []
+        self.timeTaken: list[float] = []
+
+    def addName(self, name: str) -> None:
+        if name not in self.processMap:
+            self.processMap[name] = self.numberOfProfiles
+            self.startTimes.append(0.0)
+            self.endTimes.append(0.0)
+            self.functionCalls.append(0)
+            self.timeTaken.append(0.0)
+            self.numberOfProfiles += 1
+
+    def start(self, name: str) -> None:
+        self.addName(name)
+        self.startTimes[self.processMap[name]] = time.perf_counter()
+
+    def end(self, name: str) -> None:
+        i = self.processMap[name]
+        self.endTimes[i] = time.perf_counter()
+        self.timeTaken[i] += self.endTimes[i] - self.startTimes[i]
+        self.functionCalls[i] += 1
+
+    def timing(self, name: str) -> float:
+        i = self.processMap.get(name)
+        if i is None or self.functionCalls[i] == 0:
+            return 0.0
+        return self.timeTaken[i] / self.functionCalls[i]
+
+    def print(self) -> None:
+        for name in sorted(self.processMap):
+            print(f"process {name} averaged\t {self.timing(name)} \t over "
+                  f"{self.functionCalls[self.processMap[name]]} function calls")
+
+
+multiProfiler = MultiProfiler
+
+
+def _host_ptr(t, dtype, name: str) -> int:
+    """Address of a contiguous CPU tensor (CPU mode's stand-in for a host handle)."""
+    if torch is None or not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor")
+    if t.device.type != "cpu":
+        raise ValueError(f"{name} must be a CPU tensor in CPU mode (setGPUcompute(False))")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return int(t.data_ptr())
+
+
 class Context:
     """Owns one gdt_context (device scratch buffers) on a HIP device."""
 
@@ -139,10 +205,12 @@ class CellListGPU:
     The list is a counting sort (cell-ordered arrays); ``cell_sizes`` and the
     padded ``idxs`` of the reference layout are produced on request."""
 
-    def __init__(self, box: PeriodicBoundaries | None = None, context: Context | None = None,
-                 device: int = 0):
+    def __init__(self, box: PeriodicBoundaries | None = None, context=None, device: int = 0):
         self.Box = box if box is not None else PeriodicBoundaries(1.0)
-        self.ctx = context if context is not None else Context(device)
+        # a Context, or a callable giving one; made on first device use
+        self._ctx_src = context
+        self._ctx = context if isinstance(context, Context) else None
+        self.device = int(device)
         self.Np = 0
         self.a = 1.0
         self.xsize = self.ysize = 0
@@ -150,6 +218,12 @@ class CellListGPU:
         self.Nmax = 0
         self._cell_sizes = None
         self._idxs = None
+
+    @property
+    def ctx(self) -> Context:
+        if self._ctx is None:
+            self._ctx = self._ctx_src() if callable(self._ctx_src) else Context(self.device)
+        return self._ctx
 
     def setNp(self, nn: int) -> None:
         self.Np = int(nn)
@@ -222,9 +296,29 @@ class CellListGPU:
         self.Nmax = max(self.Nmax, mo.value)
         self._cell_sizes = self._idxs = None
 
-    # The reference's CPU binning (cellListGPU.cpp:289-330) yields the same
-    # buckets in the same order; here both names run the device build.
-    compute = computeGPU
+    def compute(self, points, stream=None) -> None:
+        """cellListGPU::compute(points) (cellListGPU.cpp:289-330): CPU tensors
+        are binned on the host (same buckets, index order inside each), device
+        tensors by the device build."""
+        if isinstance(points, torch.Tensor) and points.is_cuda:
+            self.computeGPU(points, stream)
+            return
+        L = _lib.load()
+        n = int(points.shape[0])
+        x11, _, _, x22 = self.Box.getBoxDims()
+        ptr = _host_ptr(points, torch.float64, "points")
+        xs, ys, mo = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(L.gdt_host_celllist(ptr, n, x11, x22, self.a, None, None, 0, ctypes.byref(xs),
+                                       ctypes.byref(ys), ctypes.byref(mo)), "gdt_host_celllist")
+        self.Np = n
+        self.xsize, self.ysize, self.boxsize = xs.value, ys.value, x11 / xs.value
+        self.Nmax = max(self.Nmax, mo.value)
+        nc, nmax = self.xsize * self.ysize, max(1, self.Nmax)
+        self._cell_sizes = torch.zeros(nc, dtype=torch.int32)
+        self._idxs = torch.zeros(nc * nmax, dtype=torch.int32)
+        _lib.check(L.gdt_host_celllist(ptr, n, x11, x22, self.a, self._cell_sizes.data_ptr(),
+                                       self._idxs.data_ptr(), nmax, ctypes.byref(xs),
+                                       ctypes.byref(ys), ctypes.byref(mo)), "gdt_host_celllist")
 
     def _export(self):
         L = _lib.load()
@@ -263,8 +357,9 @@ class DelaunayGPU:
 
     def __init__(self, N: int | None = None, maximumNeighborsGuess: int = 16,
                  cellSize: float = 1.0, bx: PeriodicBoundaries | None = None, device: int = 0):
-        self.ctx = Context(device)
+        self._ctx = None               # the device context, made on first device use
         self.device = int(device)
+        self.prof = MultiProfiler()
         self.safetyMode = False
         self.GPUcompute = True
         self.OMPThreadsNum = 1
@@ -273,20 +368,28 @@ class DelaunayGPU:
         self.MaxSize = 4
         self.cellsize = 1.10           # reference default (DelaunayGPU.cpp:7)
         self.Box = PeriodicBoundaries(1.0)
-        self.cList = CellListGPU(self.Box, self.ctx)
+        self.cList = CellListGPU(self.Box, lambda: self.ctx)
         self.last_repaired = 0
         if N is not None:
             self.initialize(N, maximumNeighborsGuess, cellSize, bx)
 
+    @property
+    def ctx(self) -> Context:
+        if self._ctx is None:
+            self._ctx = Context(self.device)
+        return self._ctx
+
     # --- set-up (DelaunayGPU.cpp:16-62) -----------------------------------
     def initialize(self, N: int, maximumNeighborsGuess: int, cellSize: float,
                    bx: PeriodicBoundaries | None) -> None:
+        self.prof.start("initialization")
         self.Ncells = int(N)
         self.MaxSize = max(4, int(maximumNeighborsGuess))
         self.cellsize = float(cellSize)
         if bx is not None:
             self.setBox(bx)
         self._init_cell_list()
+        self.prof.end("initialization")
 
     def _init_cell_list(self) -> None:
         self.cList.setNp(self.Ncells)
@@ -301,7 +404,10 @@ class DelaunayGPU:
         self._init_cell_list()
 
     def updateList(self, points) -> None:
-        self.cList.computeGPU(points)
+        if self.GPUcompute:
+            self.cList.computeGPU(points)
+        else:
+            self.cList.compute(points)
 
     def setBox(self, bx: PeriodicBoundaries) -> None:
         self.Box = bx
@@ -311,9 +417,8 @@ class DelaunayGPU:
         self.safetyMode = bool(sm)
 
     def setGPUcompute(self, flag: bool) -> None:
-        if not flag:
-            raise NotImplementedError("CPU mode is not part of this library (GPU only)")
-        self.GPUcompute = True
+        """False: CPU mode, on CPU tensors, over setOMPthreads threads."""
+        self.GPUcompute = bool(flag)
 
     def setOMPthreads(self, num: int) -> None:
         self.OMPThreadsNum = int(num)
@@ -339,6 +444,17 @@ class DelaunayGPU:
             raise ValueError("cellNeighborNum must have N entries")
         return n
 
+    def _check_host_args(self, points, tri, num):
+        n = int(points.shape[0])
+        if points.dim() != 2 or points.shape[1] != 2:
+            raise ValueError("points must be [N, 2]")
+        _host_ptr(points, torch.float64, "points")
+        _host_ptr(tri, torch.int32, "GPUTriangulation")
+        _host_ptr(num, torch.int32, "cellNeighborNum")
+        if num.numel() != n:
+            raise ValueError("cellNeighborNum must have N entries")
+        return n
+
     def globalDelaunayTriangulation(self, points, GPUTriangulation, cellNeighborNum,
                                     stream=None) -> None:
         """DelaunayGPU::globalDelaunayTriangulation (DelaunayGPU.cpp:118-157)."""
@@ -350,35 +466,58 @@ class DelaunayGPU:
             self.Ncells = n
             GPUTriangulation.resize_(n * self.MaxSize)
             self._init_cell_list()
-        self._check_args(points, GPUTriangulation, cellNeighborNum)
+        self.prof.start("globalTriangulation")
+        if self.GPUcompute:
+            self._check_args(points, GPUTriangulation, cellNeighborNum)
+        else:
+            self._check_host_args(points, GPUTriangulation, cellNeighborNum)
         L = _lib.load()
         x11, _, _, x22 = self.Box.getBoxDims()
         while True:
             req = ctypes.c_int(0)
-            st = L.gdt_triangulate(self.ctx.handle, points.data_ptr(), n, x11, x22, self.cellsize,
-                                   self.MaxSize, GPUTriangulation.data_ptr(),
-                                   cellNeighborNum.data_ptr(), self.precision, ctypes.byref(req),
-                                   _stream_ptr(stream))
+            if self.GPUcompute:
+                st = L.gdt_triangulate(self.ctx.handle, points.data_ptr(), n, x11, x22,
+                                       self.cellsize, self.MaxSize, GPUTriangulation.data_ptr(),
+                                       cellNeighborNum.data_ptr(), self.precision,
+                                       ctypes.byref(req), _stream_ptr(stream))
+            else:
+                st = L.gdt_host_triangulate(points.data_ptr(), n, x11, x22, self.cellsize,
+                                            self.MaxSize, GPUTriangulation.data_ptr(),
+                                            cellNeighborNum.data_ptr(), self.precision,
+                                            self.OMPThreadsNum, ctypes.byref(req))
             if st == _lib.GDT_OVERFLOW:
                 self.resize(req.value)
                 GPUTriangulation.resize_(n * self.MaxSize)
                 continue
+            self.prof.end("globalTriangulation")
             _lib.check(st, "globalDelaunayTriangulation")
             return
 
     def testAndRepairDelaunayTriangulation(self, points, GPUTriangulation, cellNeighborNum,
                                            stream=None) -> None:
         """DelaunayGPU::testAndRepairDelaunayTriangulation (DelaunayGPU.cpp:567-602)."""
-        n = self._check_args(points, GPUTriangulation, cellNeighborNum)
+        if self.GPUcompute:
+            n = self._check_args(points, GPUTriangulation, cellNeighborNum)
+        else:
+            n = self._check_host_args(points, GPUTriangulation, cellNeighborNum)
         if GPUTriangulation.numel() != n * self.MaxSize:
             raise ValueError("GPUTriangulation must hold N*MaxSize entries")
         L = _lib.load()
         x11, _, _, x22 = self.Box.getBoxDims()
         nf, req = ctypes.c_int(0), ctypes.c_int(0)
-        st = L.gdt_test_and_repair(self.ctx.handle, points.data_ptr(), n, x11, x22, self.cellsize,
-                                   self.MaxSize, GPUTriangulation.data_ptr(),
-                                   cellNeighborNum.data_ptr(), self.precision, ctypes.byref(nf),
-                                   ctypes.byref(req), _stream_ptr(stream))
+        self.prof.start("testAndRepair")
+        if self.GPUcompute:
+            st = L.gdt_test_and_repair(self.ctx.handle, points.data_ptr(), n, x11, x22,
+                                       self.cellsize, self.MaxSize, GPUTriangulation.data_ptr(),
+                                       cellNeighborNum.data_ptr(), self.precision,
+                                       ctypes.byref(nf), ctypes.byref(req), _stream_ptr(stream))
+        else:
+            st = L.gdt_host_test_and_repair(points.data_ptr(), n, x11, x22, self.cellsize,
+                                            self.MaxSize, GPUTriangulation.data_ptr(),
+                                            cellNeighborNum.data_ptr(), self.precision,
+                                            self.OMPThreadsNum, ctypes.byref(nf),
+                                            ctypes.byref(req))
+        self.prof.end("testAndRepair")
         self.last_repaired = nf.value
         if st == _lib.GDT_OVERFLOW:
             # a repaired ring outgrew the row stride: re-triangulate globally

@@ -237,6 +237,43 @@ int gdt_voronoi_adjoint_batch(gdt_context* ctx, const double* d_xy, int nsets, c
                               const double* d_area, const double* d_perim, const double* d_gscale,
                               double* d_grad, void* stream);
 
+/* ---- CPU mode (DelaunayGPU::setGPUcompute(false)) ----------------------
+ *
+ * The same per-point routines as the kernels, compiled for the host and run
+ * over `nthreads` OpenMP threads (<= 0: the OpenMP default; the reference's
+ * setOMPthreads).  Host pointers, no context and no HIP call; the layouts,
+ * statuses and rings are those of the device calls.  The cell list uses
+ * row-major cells (the reference's own order). */
+
+/* Cell list on the host: *xsize, *ysize and *max_occupancy always; with
+ * cell_sizes non-null also cell_sizes[xsize*ysize] and idxs[cell*nmax + k]
+ * in the reference layout (GDT_OVERFLOW when nmax < *max_occupancy).
+ * Replaces cellListGPU::compute(points) (cellListGPU.cpp:289-330). */
+int gdt_host_celllist(const double* xy, int n, double lx, double ly, double cell_size,
+                      unsigned int* cell_sizes, int* idxs, int nmax, int* xsize, int* ysize,
+                      int* max_occupancy);
+
+/* Global triangulation on the host (statuses as gdt_triangulate).
+ * Replaces Voronoi_Calc_CPU + get_neighbors_CPU (DelaunayGPU.cpp:159-343;
+ * the OpenMP branches of gpu_voronoi_calc / gpu_get_neighbors,
+ * DelaunayGPU.cu:1116-1317). */
+int gdt_host_triangulate(const double* xy, int n, double lx, double ly, double cell_size,
+                         int stride, int* nbrs, int* counts, int precision, int nthreads,
+                         int* required);
+
+/* Test and repair on the host (statuses as gdt_test_and_repair).
+ * Replaces testAndRepairDelaunayTriangulation with GPUcompute == false
+ * (DelaunayGPU.cpp:567-706; DelaunayGPU.cu:1346-1409 CPU branch). */
+int gdt_host_test_and_repair(const double* xy, int n, double lx, double ly, double cell_size,
+                             int stride, int* nbrs, int* counts, int precision, int nthreads,
+                             int* nflagged, int* required);
+
+/* Voronoi vertices relative to each point, vertices[2*(i*stride + m)] (the
+ * reference's GPUVoroCur, filled by get_oneRing_function,
+ * DelaunayGPU.cu:921-922); rows that are not rings give NaN. */
+int gdt_host_voronoi_vertices(const double* xy, int n, double lx, double ly, const int* nbrs,
+                              const int* counts, int stride, double* vertices, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,17 +9,24 @@
 //                                 2-D resize), mapped arrays
 //   cellListGPU                   cellListGPU.h:17-121
 //   DelaunayGPU                   DelaunayGPU.h:18-58
+//   multiProfiler                 multiProfiler.h:9-75
 // Errors: a failing HIP call or library status throws std::runtime_error (the
-// reference prints and throws std::exception, std_include.h:61-72).  There is
-// no CPU mode: setGPUcompute(false) throws.  Only rectangular boxes.
+// reference prints and throws std::exception, std_include.h:61-72).
+// setGPUcompute(false) selects the library's CPU mode (gdt_host_*: the same
+// per-point routines on the host over setOMPthreads OpenMP threads), which
+// reads and writes the arrays through host handles, as the reference's CPU
+// branches do.  Only rectangular boxes.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
 #include <hip/hip_vector_types.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <iostream>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -42,6 +49,60 @@ inline void status_check(int st, const char* what)
 }
 
 }  // namespace gdt_detail
+
+// ---------------------------------------------------------------------------
+//! Named wall-clock timers with the reference's interface (multiProfiler.h:9-75):
+//! start/end bracket a region, timing() is its mean duration in seconds over
+//! the completed calls, print() lists every region.
+class multiProfiler {
+public:
+    using clock = std::chrono::high_resolution_clock;
+    multiProfiler() = default;
+
+    void addName(const std::string& name)
+    {
+        if (processMap.count(name)) return;
+        processMap.emplace(name, numberOfProfiles);
+        startTimes.push_back(clock::now());
+        endTimes.push_back(clock::now());
+        functionCalls.push_back(0);
+        timeTaken.push_back(0.);
+        ++numberOfProfiles;
+    }
+    void start(const std::string& name)
+    {
+        addName(name);
+        startTimes[(size_t)processMap[name]] = clock::now();
+    }
+    void end(const std::string& name)
+    {
+        auto it = processMap.find(name);
+        if (it == processMap.end()) return;
+        const size_t k = (size_t)it->second;
+        endTimes[k] = clock::now();
+        timeTaken[k] += std::chrono::duration<double>(endTimes[k] - startTimes[k]).count();
+        functionCalls[k] += 1;
+    }
+    double timing(const std::string& name) const
+    {
+        auto it = processMap.find(name);
+        if (it == processMap.end() || functionCalls[(size_t)it->second] == 0) return 0.;
+        return timeTaken[(size_t)it->second] / functionCalls[(size_t)it->second];
+    }
+    void print() const
+    {
+        for (const auto& kv : processMap)
+            std::cout << "process " << kv.first << " averaged\t " << timing(kv.first) << " \t over "
+                      << functionCalls[(size_t)kv.second] << " function calls" << std::endl;
+    }
+
+    int numberOfProfiles = 0;
+    std::vector<clock::time_point> startTimes;
+    std::vector<clock::time_point> endTimes;
+    std::vector<int> functionCalls;
+    std::vector<double> timeTaken;
+    std::map<std::string, int> processMap;
+};
 
 // ---------------------------------------------------------------------------
 //! Row-major 2D indexer: (i, j) -> j*w + i (indexer.h:32-62).
@@ -474,10 +535,36 @@ public:
                                  "gdt_celllist_export");
         gdt_detail::hip_check(hipDeviceSynchronize(), "cellListGPU::computeGPU");
     }
-    //! The reference's CPU binning gives the same buckets in the same order.
-    void compute(GPUArray<double2>& points) { computeGPU(points); }
+    //! Bin the points on the host (cellListGPU::compute, cellListGPU.cpp:289-330):
+    //! the same buckets in the same order as computeGPU.
+    void compute(GPUArray<double2>& points)
+    {
+        const int n = (int)points.getNumElements();
+        if (xsize == 0) setGridSize(m_a);
+        ArrayHandle<double2> p(points, access_location::host, access_mode::read);
+        const double* xy = reinterpret_cast<const double*>(p.data);
+        int xs = 0, ys = 0, mo = 0;
+        gdt_detail::status_check(gdt_host_celllist(xy, n, Box->x11, Box->x22, m_a, nullptr, nullptr,
+                                                   0, &xs, &ys, &mo),
+                                 "cellListGPU::compute");
+        Np = n;
+        xsize = xs;
+        ysize = ys;
+        boxsize = Box->x11 / xs;
+        totalCells = xs * ys;
+        Nmax = std::max(Nmax, mo);
+        cell_indexer = Index2D(xsize, ysize);
+        cell_list_indexer = Index2D(Nmax, totalCells);
+        if ((int)cell_sizes.getNumElements() != totalCells) cell_sizes.resize(totalCells);
+        if ((int)idxs.getNumElements() != totalCells * Nmax) idxs.resize(totalCells * Nmax);
+        ArrayHandle<unsigned int> cs(cell_sizes, access_location::host, access_mode::overwrite);
+        ArrayHandle<int> ix(idxs, access_location::host, access_mode::overwrite);
+        gdt_detail::status_check(gdt_host_celllist(xy, n, Box->x11, Box->x22, m_a, cs.data, ix.data,
+                                                   Nmax, &xs, &ys, &mo),
+                                 "cellListGPU::compute");
+    }
     void computeGPU() { computeGPU(particles); }
-    void compute() { computeGPU(particles); }
+    void compute() { compute(particles); }
 
     const GPUArray<unsigned int>& getCellSizeArray() const { return cell_sizes; }
     const GPUArray<int>& getIdxArray() const { return idxs; }
@@ -509,12 +596,14 @@ public:
 
     void initialize(int N, int maximumNeighborsGuess, double cellSize, PeriodicBoxPtr bx)
     {
+        prof.start("initialization");
         Ncells = N;
         MaxSize = std::max(4, maximumNeighborsGuess);
         cellsize = cellSize;
         setBox(std::move(bx));
         cList.setNp(N);
         cList.setGridSize(cellsize);
+        prof.end("initialization");
     }
     void resize(const int nmax) { MaxSize = nmax; }
     void setCellListSize(double csize)
@@ -522,17 +611,19 @@ public:
         cellsize = csize;
         cList.setGridSize(cellsize);
     }
-    void updateList(GPUArray<double2>& points) { cList.computeGPU(points); }
+    void updateList(GPUArray<double2>& points)
+    {
+        if (GPUcompute) cList.computeGPU(points);
+        else cList.compute(points);
+    }
     void setBox(PeriodicBoxPtr bx)
     {
         Box = bx ? std::move(bx) : std::make_shared<periodicBoundaries>();
         cList.setBox(Box);
     }
     void setSafetyMode(bool sm) { safetyMode = sm; }
-    void setGPUcompute(bool flag)
-    {
-        if (!flag) throw std::invalid_argument("DelaunayGPU: CPU mode is not part of this library");
-    }
+    //! false: CPU mode (the library's per-point routines on the host, OpenMP)
+    void setGPUcompute(bool flag) { GPUcompute = flag; }
     void setOMPthreads(unsigned int num) { OMPThreadsNum = num; }
     //! 64: all-FP64 predicates; 32: FP32 with a certified bound + FP64 fallback
     void setPrecision(int p)
@@ -552,22 +643,28 @@ public:
         if (n == 0) return;
         Ncells = n;
         if ((int)cellNeighborNum.getNumElements() != n) cellNeighborNum.resize(n);
+        prof.start("globalTriangulation");
+        const access_location::Enum where = GPUcompute ? access_location::device : access_location::host;
         for (;;) {
             if ((int)GPUTriangulation.getNumElements() != n * MaxSize)
                 GPUTriangulation.resize(n * MaxSize);
             int req = 0, st;
             {
-                ArrayHandle<double2> p(points, access_location::device, access_mode::read);
-                ArrayHandle<int> t(GPUTriangulation, access_location::device, access_mode::overwrite);
-                ArrayHandle<int> c(cellNeighborNum, access_location::device, access_mode::overwrite);
-                st = gdt_triangulate(m_ctx->get(), reinterpret_cast<const double*>(p.data), n,
-                                     Box->x11, Box->x22, cellsize, MaxSize, t.data, c.data,
-                                     precision, &req, nullptr);
+                ArrayHandle<double2> p(points, where, access_mode::read);
+                ArrayHandle<int> t(GPUTriangulation, where, access_mode::overwrite);
+                ArrayHandle<int> c(cellNeighborNum, where, access_mode::overwrite);
+                const double* xy = reinterpret_cast<const double*>(p.data);
+                st = GPUcompute
+                         ? gdt_triangulate(m_ctx->get(), xy, n, Box->x11, Box->x22, cellsize, MaxSize,
+                                           t.data, c.data, precision, &req, nullptr)
+                         : gdt_host_triangulate(xy, n, Box->x11, Box->x22, cellsize, MaxSize, t.data,
+                                                c.data, precision, (int)OMPThreadsNum, &req);
             }
             if (st == GDT_OVERFLOW) {
                 MaxSize = req;
                 continue;
             }
+            prof.end("globalTriangulation");
             gdt_detail::status_check(st, "DelaunayGPU::globalDelaunayTriangulation");
             break;
         }
@@ -583,14 +680,22 @@ public:
         if ((int)GPUTriangulation.getNumElements() != n * MaxSize)
             throw std::invalid_argument("GPUTriangulation must hold N*MaxSize entries");
         int nf = 0, req = 0, st;
+        prof.start("testAndRepair");
         {
-            ArrayHandle<double2> p(points, access_location::device, access_mode::read);
-            ArrayHandle<int> t(GPUTriangulation, access_location::device, access_mode::readwrite);
-            ArrayHandle<int> c(cellNeighborNum, access_location::device, access_mode::readwrite);
-            st = gdt_test_and_repair(m_ctx->get(), reinterpret_cast<const double*>(p.data), n,
-                                     Box->x11, Box->x22, cellsize, MaxSize, t.data, c.data,
-                                     precision, &nf, &req, nullptr);
+            const access_location::Enum where =
+                GPUcompute ? access_location::device : access_location::host;
+            ArrayHandle<double2> p(points, where, access_mode::read);
+            ArrayHandle<int> t(GPUTriangulation, where, access_mode::readwrite);
+            ArrayHandle<int> c(cellNeighborNum, where, access_mode::readwrite);
+            const double* xy = reinterpret_cast<const double*>(p.data);
+            st = GPUcompute
+                     ? gdt_test_and_repair(m_ctx->get(), xy, n, Box->x11, Box->x22, cellsize,
+                                           MaxSize, t.data, c.data, precision, &nf, &req, nullptr)
+                     : gdt_host_test_and_repair(xy, n, Box->x11, Box->x22, cellsize, MaxSize,
+                                                t.data, c.data, precision, (int)OMPThreadsNum, &nf,
+                                                &req);
         }
+        prof.end("testAndRepair");
         lastRepaired = nf;
         if (st == GDT_OVERFLOW) {
             MaxSize = req;
@@ -608,6 +713,8 @@ public:
         return s;
     }
 
+    //! wall-clock timers of the calls (initialization, globalTriangulation, testAndRepair)
+    multiProfiler prof;
     PeriodicBoxPtr Box;
     int MaxSize = 4;
     //! Voronoi vertex m of point i relative to it at [i*MaxSize + m]
@@ -620,14 +727,19 @@ protected:
         if (!voronoiVertices) return;
         const int n = (int)points.getNumElements();
         if ((int)GPUVoroCur.getNumElements() != n * MaxSize) GPUVoroCur.resize(n * MaxSize);
-        ArrayHandle<double2> p(points, access_location::device, access_mode::read);
-        ArrayHandle<int> t(tri, access_location::device, access_mode::read);
-        ArrayHandle<int> c(num, access_location::device, access_mode::read);
-        ArrayHandle<double2> v(GPUVoroCur, access_location::device, access_mode::overwrite);
-        gdt_detail::status_check(gdt_voronoi_vertices(m_ctx->get(), reinterpret_cast<const double*>(p.data), n,
-                                                      Box->x11, Box->x22, t.data, c.data, MaxSize,
-                                                      reinterpret_cast<double*>(v.data), nullptr),
-                                 "gdt_voronoi_vertices");
+        const access_location::Enum where = GPUcompute ? access_location::device : access_location::host;
+        ArrayHandle<double2> p(points, where, access_mode::read);
+        ArrayHandle<int> t(tri, where, access_mode::read);
+        ArrayHandle<int> c(num, where, access_mode::read);
+        ArrayHandle<double2> v(GPUVoroCur, where, access_mode::overwrite);
+        const double* xy = reinterpret_cast<const double*>(p.data);
+        double* out = reinterpret_cast<double*>(v.data);
+        const int st = GPUcompute ? gdt_voronoi_vertices(m_ctx->get(), xy, n, Box->x11, Box->x22,
+                                                         t.data, c.data, MaxSize, out, nullptr)
+                                  : gdt_host_voronoi_vertices(xy, n, Box->x11, Box->x22, t.data,
+                                                              c.data, MaxSize, out,
+                                                              (int)OMPThreadsNum);
+        gdt_detail::status_check(st, "GPUVoroCur");
     }
 
     std::shared_ptr<gdtContext> m_ctx;
@@ -635,6 +747,7 @@ protected:
     int Ncells = 0;
     double cellsize = 1.10;     // reference default (DelaunayGPU.cpp:7)
     bool safetyMode = false;
+    bool GPUcompute = true;
     unsigned int OMPThreadsNum = 1;
     int precision = GDT_PREC_FP64;
     bool voronoiVertices = true;

@@ -3,7 +3,10 @@
 // GPUArray<double2>, triangulate, read the rings through ArrayHandle, move
 // the points on the host, test-and-repair; cellListGPU binning; GPUVoroCur.
 //
-// Usage: dropin_check <points.bin> <n> <lx> <ly> <moved.bin> <precision> <out.bin>
+// Usage: dropin_check <points.bin> <n> <lx> <ly> <moved.bin> <precision> <out.bin> [cpu]
+// With `cpu`, DelaunayGPU runs in CPU mode (setGPUcompute(false), 4 OpenMP
+// threads) and the cell list is binned by cellListGPU::compute on the host.
+// Either way the profiler (del.prof) must have timed both calls.
 // out.bin: int32 MaxSize0, int32 rings0[n][MaxSize0], int32 counts0[n],
 //          int32 MaxSize1, int32 rings1[n][MaxSize1], int32 counts1[n],
 //          int32 lastRepaired, int32 xsize, int32 ysize, int32 Nmax,
@@ -13,6 +16,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 static bool read_points(const char* path, int n, std::vector<double>& v)
@@ -34,7 +38,8 @@ static void dump(FILE* f, GPUArray<T>& a)
 
 int main(int argc, char** argv)
 {
-    if (argc != 8) { std::fprintf(stderr, "bad args\n"); return 2; }
+    if (argc != 8 && argc != 9) { std::fprintf(stderr, "bad args\n"); return 2; }
+    const bool cpu = argc == 9 && std::string(argv[8]) == "cpu";
     const int n = std::atoi(argv[2]);
     const double lx = std::atof(argv[3]), ly = std::atof(argv[4]);
     const int prec = std::atoi(argv[6]);
@@ -50,6 +55,10 @@ int main(int argc, char** argv)
         // a small guess on purpose: the call grows MaxSize and the arrays
         DelaunayGPU del(n, 4, 1.0, box);
         del.setPrecision(prec);
+        if (cpu) {
+            del.setGPUcompute(false);
+            del.setOMPthreads(4);
+        }
         GPUArray<int> tri, num;
         del.globalDelaunayTriangulation(points, tri, num);
         FILE* f = std::fopen(argv[7], "wb");
@@ -60,7 +69,7 @@ int main(int argc, char** argv)
         dump(f, num);
 
         // move on the host (the reference driver moves the points through a
-        // host handle as well), then test and repair on the device
+        // host handle as well), then test and repair
         {
             ArrayHandle<double2> h(points, access_location::host, access_mode::readwrite);
             for (int i = 0; i < n; ++i) h.data[i] = make_double2(moved[2 * i], moved[2 * i + 1]);
@@ -76,7 +85,8 @@ int main(int argc, char** argv)
         cl.setBox(box);
         cl.setNp(n);
         cl.setGridSize(1.0);
-        cl.computeGPU(points);
+        if (cpu) cl.compute(points);
+        else cl.computeGPU(points);
         int xs = cl.getXsize(), ys = cl.getYsize(), nmax = cl.getNmax();
         std::fwrite(&xs, 4, 1, f);
         std::fwrite(&ys, 4, 1, f);
@@ -85,6 +95,11 @@ int main(int argc, char** argv)
         dump(f, cl.idxs);
         dump(f, del.GPUVoroCur);
         std::fclose(f);
+        if (!(del.prof.timing("globalTriangulation") > 0.) || !(del.prof.timing("testAndRepair") > 0.)) {
+            std::fprintf(stderr, "error: the profiler did not time the calls\n");
+            return 5;
+        }
+        del.prof.print();
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;

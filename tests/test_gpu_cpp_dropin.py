@@ -5,7 +5,8 @@ rings after a global triangulation that starts from MaxSize = 4 (so the call
 must grow it), rings after moving the points on the host and calling
 testAndRepair, the exported cell list (closed-form binning, index order in
 each cell) and GPUVoroCur (circumcentres of consecutive ring neighbours,
-relative 1e-12)."""
+relative 1e-12); in device mode and in CPU mode (setGPUcompute(false),
+setOMPthreads(4), cellListGPU::compute), with del.prof timing every call."""
 import os
 import subprocess
 
@@ -65,18 +66,24 @@ def circumcentres(xy, nb, c):
     return out
 
 
+@pytest.mark.parametrize("mode", ["gpu", "cpu"])
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32F])
-def test_cpp_dropin_classes(tmp_path, prec):
+def test_cpp_dropin_classes(tmp_path, prec, mode):
+    """mode "cpu": DelaunayGPU with setGPUcompute(false) and 4 OpenMP threads
+    and cellListGPU::compute on the host, the same checks."""
     assert os.path.exists(DROPIN_BIN), "C++ drop-in check not built (run __graft_entry__.build())"
     xy = inputs.uniform_points(N, L, L, seed=1)
     moved = inputs.wrap_into_box(xy + 0.02 * inputs.gaussian(31, 2 * N).reshape(-1, 2), L, L)
     p0, p1, po = (str(tmp_path / f) for f in ("p.bin", "m.bin", "o.bin"))
     xy.tofile(p0)
     moved.tofile(p1)
-    print(f"[cpp dropin] precision {prec}", flush=True)
-    r = subprocess.run([DROPIN_BIN, p0, str(N), repr(L), repr(L), p1, str(prec), po],
-                       capture_output=True, text=True, timeout=120)
+    print(f"[cpp dropin] precision {prec}, {mode}", flush=True)
+    args = [DROPIN_BIN, p0, str(N), repr(L), repr(L), p1, str(prec), po]
+    r = subprocess.run(args + (["cpu"] if mode == "cpu" else []), capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stderr
+    for region in ("initialization", "globalTriangulation", "testAndRepair"):
+        assert f"process {region} averaged" in r.stdout       # del.prof.print()
     out = parse(po, N)
 
     nb, c = oracle.triangulate(xy, L, L, 1.0, 16)
