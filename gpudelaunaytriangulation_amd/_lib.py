@@ -22,6 +22,7 @@ GDT_ERR_NONGENERIC = -3
 GDT_ERR_EULER = -4
 GDT_ERR_SEARCH = -5
 GDT_ERR_NOMEM = -6
+GDT_ERR_SCRATCH = -7
 
 PREC_FP64 = 64
 PREC_FP32F = 32
@@ -35,7 +36,7 @@ EXPORTS = (
     "gdt_voronoi_forward_batch", "gdt_voronoi_adjoint_batch", "gdt_triangulate_batch",
     "gdt_test_and_repair_batch", "gdt_md_run", "gdt_voronoi_vertices",
     "gdt_host_celllist", "gdt_host_triangulate", "gdt_host_test_and_repair",
-    "gdt_host_voronoi_vertices",
+    "gdt_host_voronoi_vertices", "gdt_triangulate_noalloc",
 )
 
 
@@ -97,6 +98,15 @@ def load():
         raise LibraryNotBuilt(
             f"{LIB_PATH} not found: run `python -m gpudelaunaytriangulation_amd.build` "
             "(there is no CPU fallback)")
+    # torch's HIP libraries need the unversioned libamdhip64.so from their own
+    # directory: loaded after this library (which binds libamdhip64.so.7), they
+    # would bring a second HIP runtime into the process, and this library's
+    # runtime would find no device.  Loaded first, torch's runtime is the one
+    # this library binds to.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     try:
         L = ctypes.CDLL(LIB_PATH)
     except OSError as e:
@@ -131,6 +141,7 @@ def load():
     _sig(L, "gdt_voronoi_vertices", [_vp, _vp, _i, _d, _d, _vp, _vp, _i, _vp, _vp])
     _sig(L, "gdt_md_run", [_vp, _vp, _i, _ip, ctypes.POINTER(_d), _d, _i, _vp, _vp, _i, _d,
                            ctypes.c_ulonglong, _i, _i, _i, _i, ctypes.POINTER(MdReport), _vp])
+    _sig(L, "gdt_triangulate_noalloc", [_vp, _i, _d, _d, _d, _i, _vp, _vp, _i, _ip, _vp])
     _sig(L, "gdt_host_celllist", [_vp, _i, _d, _d, _d, _vp, _vp, _i, _ip, _ip, _ip])
     _sig(L, "gdt_host_triangulate", [_vp, _i, _d, _d, _d, _i, _vp, _vp, _i, _i, _ip])
     _sig(L, "gdt_host_test_and_repair", [_vp, _i, _d, _d, _d, _i, _vp, _vp, _i, _i, _ip, _ip])
