@@ -23,7 +23,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GDT_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["celllist.hip", "onering.hip", "testrepair.hip", "voronoi.hip", "mdstep.hip",
-           "capi.hip"]
+           "capi.hip", "noalloc.hip"]
 # CPU mode: the same per-point routines compiled for the host (g++, OpenMP)
 HOST_SOURCES = ["hostpath.cpp"]
 

@@ -314,6 +314,7 @@ const char* gdt_status_string(int st)
         case GDT_ERR_EULER: return "not a triangulation of the torus (Euler check)";
         case GDT_ERR_SEARCH: return "search radius exhausted (box too small)";
         case GDT_ERR_NOMEM: return "out of memory";
+        case GDT_ERR_SCRATCH: return "input too large for the fixed scratch of gdt_triangulate_noalloc";
         default: return "unknown status";
     }
 }

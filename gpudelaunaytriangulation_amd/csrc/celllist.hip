@@ -333,6 +333,11 @@ size_t gdt_bucket_words(size_t n, size_t k)
     return (size_t)p.nkb * p.nb + 2 * (size_t)p.nkb + 2;
 }
 
+bool gdt_cells_two_level(size_t n, size_t k)
+{
+    return kb_plan(n, k).B <= KB_MAXSHIFT;
+}
+
 // Cell of caller point i (its set found by binary search over the sets).
 __device__ inline void kb_bin(const GdtSet* __restrict__ sets, int nsets, const gdt_d2* __restrict__ xy,
                               int i, uint32_t& key, uint32_t& packed, double& ox, double& oy)

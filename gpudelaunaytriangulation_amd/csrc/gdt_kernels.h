@@ -34,6 +34,9 @@ struct GdtCells {
 
 // Words of the (bucket, block) count table of the cell-list build.
 size_t gdt_bucket_words(size_t n, size_t k);
+// True when n points over k keys take the two-level build, which needs only
+// bkt, bmat, bscan and the slot arrays (no pkey, prank or scan scratch).
+bool gdt_cells_two_level(size_t n, size_t k);
 
 size_t gdt_scan_tmp_words(size_t n);
 // out[0..n) = exclusive prefix sums of in, *total = sum (device pointer).

@@ -36,7 +36,8 @@ typedef enum {
     GDT_ERR_NONGENERIC = -3,   /* a ring did not close / no neighbour: degenerate input */
     GDT_ERR_EULER = -4,        /* sum(counts) != 6n or triangles != 2n */
     GDT_ERR_SEARCH = -5,       /* search radius exhausted (box too small for the point set) */
-    GDT_ERR_NOMEM = -6
+    GDT_ERR_NOMEM = -6,
+    GDT_ERR_SCRATCH = -7       /* input too large for gdt_triangulate_noalloc's fixed scratch */
 } gdt_status;
 
 typedef enum {
@@ -117,6 +118,20 @@ int gdt_celllist_export(gdt_context* ctx, unsigned int* d_cell_sizes, int* d_idx
 int gdt_triangulate(gdt_context* ctx, const double* d_xy, int n, double lx, double ly,
                     double cell_size, int stride, int* d_nbrs, int* d_counts, int precision,
                     int* required, void* stream);
+
+/* The same global triangulation without any allocation, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc); it has no counterpart in the
+ * reference.  No context: every scratch buffer lives in one fixed 96 MiB
+ * device array of the library (each GPU holds its own copy from the library's
+ * first use on it until the process ends), enough for 2^20 points at density
+ * 1 with cell size 1 (68 bytes per point, 8 per cell key and the cell-list
+ * bucket tables); larger inputs give GDT_ERR_SCRATCH.  Default configuration
+ * (32x32-cell tiles), the current device, statuses as gdt_triangulate.  Calls
+ * that overlap on one GPU share that array and race: serialise them. */
+int gdt_triangulate_noalloc(const double* d_xy, int n, double lx, double ly, double cell_size,
+                            int stride, int* d_nbrs, int* d_counts, int precision, int* required,
+                            void* stream);
 
 /* ---- test and repair (DelaunayGPU::testAndRepairDelaunayTriangulation) */
 
