@@ -25,6 +25,16 @@
 #endif
 
 #define GDT_BLOCK 256
+
+// Audit of the FP32 filter (host test builds only): every decision the filter
+// takes from its FP32 value alone is reported as a claimed sign (+1 / -1) of
+// an exact predicate on the FP64 displacements the FP64 path would use (a, b,
+// c relative to v):  ORIENT  a x c;  INCIRCLE  K0|c|^2 + Kx cx + Ky cy of the
+// circle (v, a, b);  DIST  |c|^2 - |b|^2.  Compiled out everywhere else.
+enum { GDT_AUD_ORIENT = 0, GDT_AUD_INCIRCLE = 1, GDT_AUD_DIST = 2 };
+#ifndef GDT_FILTER_AUDIT
+#define GDT_FILTER_AUDIT(kind, sign, ax, ay, bx, by, cx, cy) ((void)0)
+#endif
 #define GDT_HARD_CAP 256      // ring walk cap: a longer ring means non-generic input
 
 // Device status word layout (gdt_status[...]).
@@ -280,4 +290,6 @@ GDT_HD inline bool gdt_incircle_tie(int pv, int pa, int pb, int pc, double ax, d
 // offsets, of dcx*bs and of the two sums); all bounds below include a factor 2
 // of slack for the rounding of the bound itself.
 // ---------------------------------------------------------------------------
+#ifndef GDT_U32     // (a host test shrinks it to show that a too-tight bound is caught)
 #define GDT_U32 5.9604644775390625e-08f
+#endif

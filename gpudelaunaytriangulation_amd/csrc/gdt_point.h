@@ -233,10 +233,26 @@ GDT_HD inline GdtRingResult gdt_one_ring_f32(const GdtSet& S, int j, const GdtRi
             const float E = fmaf(26.f * u, d2, eabs);
             ++npred;
             bool take;
+#ifdef GDT_AUDIT_FILTER
+            auto audit_nn = [&](int sign) {
+                double cx64, cy64, nx64, ny64;
+                gdt_disp64_of(S, ov, io.off64, (int)k, gdt_pack_dxy(dx, dy), cx64, cy64);
+                gdt_disp64_of(S, ov, io.off64, nn, ndxy, nx64, ny64);
+                GDT_FILTER_AUDIT(GDT_AUD_DIST, sign, 0., 0., nx64, ny64, cx64, cy64);
+            };
+#endif
             if (nn < 0) take = true;
-            else if (d2 + E < nd2 - nE) take = true;
-            else if (d2 - E > nd2 + nE) take = false;
-            else {
+            else if (d2 + E < nd2 - nE) {
+                take = true;
+#ifdef GDT_AUDIT_FILTER
+                audit_nn(-1);
+#endif
+            } else if (d2 - E > nd2 + nE) {
+                take = false;
+#ifdef GDT_AUDIT_FILTER
+                audit_nn(1);
+#endif
+            } else {
                 ++nfb;
                 double cx64, cy64, nx64, ny64;
                 gdt_disp64_of(S, ov, io.off64, (int)k, gdt_pack_dxy(dx, dy), cx64, cy64);
@@ -299,6 +315,17 @@ GDT_HD inline GdtRingResult gdt_one_ring_f32(const GdtSet& S, int j, const GdtRi
                 const float px = bx + o.x, py = by + o.y;
                 const float O = fmaf(ax, py, -(ay * px));
                 ++npred;
+#ifdef GDT_AUDIT_FILTER
+                auto audit_sc = [&](int kind, int sign) {
+                    double a64x, a64y, b64x = 0., b64y = 0., c64x, c64y;
+                    gdt_disp64_of(S, ov, io.off64, a, adxy, a64x, a64y);
+                    if (best >= 0) gdt_disp64_of(S, ov, io.off64, best, bdxy, b64x, b64y);
+                    gdt_disp64_of(S, ov, io.off64, (int)k, dxy, c64x, c64y);
+                    GDT_FILTER_AUDIT(kind, sign, a64x, a64y, b64x, b64y, c64x, c64y);
+                };
+                if (O > BO) audit_sc(GDT_AUD_ORIENT, 1);
+                if (O < -BO) audit_sc(GDT_AUD_ORIENT, -1);
+#endif
                 bool left;
                 if (O > BO) left = true;
                 else if (O < -BO) left = false;
@@ -315,6 +342,10 @@ GDT_HD inline GdtRingResult gdt_one_ring_f32(const GdtSet& S, int j, const GdtRi
                     const float P = fmaf(K0, cc, fmaf(Kx, px, Ky * py));
                     ++npred;
                     bool inside;
+#ifdef GDT_AUDIT_FILTER
+                    if (P < -BP) audit_sc(GDT_AUD_INCIRCLE, -1);
+                    if (P > BP) audit_sc(GDT_AUD_INCIRCLE, 1);
+#endif
                     if (P < -BP) inside = true;
                     else if (P > BP) inside = false;
                     else {

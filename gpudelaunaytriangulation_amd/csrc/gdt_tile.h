@@ -710,6 +710,27 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
     // an exact tie met in the FP64 decision (the FP64 path's GDT_TIE): the
     // point is deferred at the end of the step
     bool tie = false;
+#ifdef GDT_AUDIT_FILTER
+    auto audit = [&](int kind, int sign, int k) {
+        double a64x = 0., a64y = 0., b64x = 0., b64y = 0., c64x, c64y;
+        if (ak >= 0) x64(ak, a64x, a64y);
+        if (bk >= 0) x64(bk, b64x, b64y);
+        x64(k, c64x, c64y);
+        GDT_FILTER_AUDIT(kind, sign, a64x, a64y, b64x, b64y, c64x, c64y);
+    };
+    // the claims of an FP32-only decision: the orientation (successor), and
+    // against a best the in-circle value (successor) or the distance order
+    auto audit_fp32 = [&](int k, float O, float P, bool taken) {
+        if (taken) {
+            if (wrap) audit(GDT_AUD_ORIENT, 1, k);
+            if (bk >= 0) audit(wrap ? GDT_AUD_INCIRCLE : GDT_AUD_DIST, -1, k);
+        } else if (wrap && O < Op) {
+            audit(GDT_AUD_ORIENT, -1, k);
+        } else if (bk >= 0 && P > Pp) {
+            audit(wrap ? GDT_AUD_INCIRCLE : GDT_AUD_DIST, 1, k);
+        }
+    };
+#endif
     auto visit = [&](int k, const gdt_f2 Y) {
         const float px = Y.x - vx, py = Y.y - vy;
         const float cc = fmaf(px, px, py * py);
@@ -718,6 +739,9 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
         // one branch, taken only when the candidate may be a new best
         if ((O >= Op) & (P <= Pp) & (k != iv) & (k != ak)) {
             bool take = (O > Oc) & (P < Pc);
+#ifdef GDT_AUDIT_FILTER
+            if (take) audit_fp32(k, O, P, true);
+#endif
             if (!take) {                      // undecided: the FP64 path's decision
                 ++nfb;
                 double c64x, c64y;
@@ -725,6 +749,12 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
                 if (wrap) {
                     double a64x, a64y;
                     x64(ak, a64x, a64y);
+#ifdef GDT_AUDIT_FILTER
+                    if (fabsf(O) > BO) audit(GDT_AUD_ORIENT, O > BO ? 1 : -1, k);
+                    if ((fabsf(O) > BO ? O > BO : gdt_orient64(a64x, a64y, c64x, c64y) > 0.) &&
+                        bk >= 0 && fabsf(P) > BP)
+                        audit(GDT_AUD_INCIRCLE, P < -BP ? -1 : 1, k);
+#endif
                     const bool left = fabsf(O) > BO ? O > BO
                                                     : gdt_orient64(a64x, a64y, c64x, c64y) > 0.;
                     if (!left) take = false;
@@ -760,6 +790,11 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
                 }
             }
         }
+#ifdef GDT_AUDIT_FILTER
+        else if ((k != iv) & (k != ak)) {
+            audit_fp32(k, O, P, false);
+        }
+#endif
     };
     auto visit_k = [&](int k) { visit(k, load(k)); };
     // Pairs: the lower lane visits its partner's best (the filtered test, with

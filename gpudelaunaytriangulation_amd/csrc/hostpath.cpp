@@ -5,12 +5,15 @@
 // DelaunayGPU.cu:1036-1409 and DelaunayGPU.cpp:159-343, 567-706, with
 // cellListGPU::compute binning on the host, cellListGPU.cpp:289-330).  This
 // file does the same with this library's own routines: the per-point one-ring
-// (gdt_point.h), the triangle circle test (gdt_circle.h) and the Voronoi
-// vertices (gdt_vor.h) are the code the kernels run, compiled here for the
+// (gdt_point.h, and for points whose search reaches the grid's search radius
+// the wavefront routine of gdt_wave.h), the triangle circle test
+// (gdt_circle.h) and the Voronoi vertices (gdt_vor.h) are the code the
+// kernels run, compiled here for the
 // host (g++, -ffp-contract=off like the kernels) and called from OpenMP loops.
 // Host pointers only; no HIP call is made.
 #include "gdt_circle.h"
 #include "gdt_vor.h"
+#include "gdt_wave.h"
 #include "gpudelaunay.h"
 
 #include <omp.h>
@@ -108,7 +111,10 @@ int status_of(int err, int required, int stride)
 }
 
 // Rings of the slots `slots` (all slots when null), the kernels' per-point
-// routine in the requested precision; returns the OR of the error bits and the
+// routine in the requested precision; a point whose search reaches the grid's
+// search radius (boxes of a few cells) is redone by the routine the device
+// path hands such points to (gdt_wave.h, its 64 lanes emulated), as the
+// device's deferred points are.  Returns the OR of the error bits and the
 // longest ring in *maxc.
 int rings_of(const HostCells& h, const int* slots, int nslots, int* nbrs, int* counts, int stride,
              int precision, int nt, int* maxc)
@@ -118,8 +124,12 @@ int rings_of(const HostCells& h, const int* slots, int nslots, int* nbrs, int* c
 #pragma omp parallel for num_threads(nt) schedule(dynamic, 256) reduction(max : mx) reduction(| : err)
     for (int t = 0; t < nslots; ++t) {
         const int j = slots ? slots[t] : t;
-        const GdtRingResult r = precision == GDT_PREC_FP32F ? gdt_one_ring_f32(h.S, j, io)
-                                                            : gdt_one_ring_f64(h.S, j, io);
+        GdtRingResult r = precision == GDT_PREC_FP32F ? gdt_one_ring_f32(h.S, j, io)
+                                                      : gdt_one_ring_f64(h.S, j, io);
+        if (r.err & ERR_SEARCH) {
+            GdtWaveHost w;
+            r = gdt_wave_ring<true>(h.S, j, io, w, true);
+        }
         counts[h.perm[(size_t)j]] = r.count;
         mx = std::max(mx, r.count);
         err |= r.err;

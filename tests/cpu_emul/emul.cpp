@@ -5,8 +5,21 @@
 // wavefront routine) or the wavefront routine (gdt_wave.h, 64 emulated lanes)
 // for every point.
 // Usage: emul <points.bin> <n> <lx> <ly> <cell_size> <tiled> <precision 64|32> <stride>
-//             <out.bin> [mode 0=direct 1=tile 2=wave]
+//             <out.bin> [mode 0=direct 1=tile 2=wave] [audit.bin]
 // out.bin: int32 counts[n], int32 nbrs[n*stride], uint64 npred, uint64 nfb, int32 errflags
+// Built with -DGDT_AUDIT_FILTER, every decision the FP32 filter takes from its
+// FP32 value alone goes to audit.bin: int32 kind, int32 claimed sign, then the
+// FP64 displacements ax, ay, bx, by, cx, cy (GDT_FILTER_AUDIT in gdt_internal.h).
+#include <vector>
+#ifdef GDT_AUDIT_FILTER
+struct AuditRec {
+    int kind, sign;
+    double ax, ay, bx, by, cx, cy;
+};
+static std::vector<AuditRec> g_audit;
+#define GDT_FILTER_AUDIT(kind, sign, ax, ay, bx, by, cx, cy) \
+    g_audit.push_back(AuditRec{(kind), (sign), (ax), (ay), (bx), (by), (cx), (cy)})
+#endif
 #include "gdt_tile.h"
 #include "gdt_wave.h"
 
@@ -16,11 +29,11 @@
 
 int main(int argc, char** argv)
 {
-    if (argc != 10 && argc != 11) { std::fprintf(stderr, "bad args\n"); return 2; }
+    if (argc < 10 || argc > 12) { std::fprintf(stderr, "bad args\n"); return 2; }
     const int n = std::atoi(argv[2]);
     const double lx = std::atof(argv[3]), ly = std::atof(argv[4]), a = std::atof(argv[5]);
     const int tiled = std::atoi(argv[6]), prec = std::atoi(argv[7]), stride = std::atoi(argv[8]);
-    const int mode = argc == 11 ? std::atoi(argv[10]) : 0;
+    const int mode = argc >= 11 ? std::atoi(argv[10]) : 0;
     if (mode == 1 && !tiled) { std::fprintf(stderr, "tile mode needs tiled order\n"); return 2; }
     std::vector<double> xy(2 * (size_t)n);
     FILE* f = std::fopen(argv[1], "rb");
@@ -116,5 +129,13 @@ int main(int argc, char** argv)
     std::fwrite(&nfb, 8, 1, f);
     std::fwrite(&err, 4, 1, f);
     std::fclose(f);
+#ifdef GDT_AUDIT_FILTER
+    if (argc == 12) {
+        f = std::fopen(argv[11], "wb");
+        if (!f) return 4;
+        if (!g_audit.empty()) std::fwrite(g_audit.data(), sizeof(AuditRec), g_audit.size(), f);
+        std::fclose(f);
+    }
+#endif
     return 0;
 }

@@ -85,6 +85,19 @@ def test_host_triangulation_c2_golden_hash():
     assert rings.ring_hash(nb, cnt) == g["ring_sha256"]
 
 
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32F])
+def test_host_grid_smaller_than_the_halo(prec):
+    """6x6 cells: the per-point search reaches the grid's search radius and
+    the wavefront routine (the device's deferred path) finishes those points;
+    the rings equal scipy's Qhull."""
+    from qhull_periodic import periodic_rings
+    xy = inputs.uniform_points(40, 6.0, 6.0, seed=43)
+    qn, qc = periodic_rings(xy, 6.0, 6.0)
+    st, nb, cnt, _ = host_tri(xy, 6.0, 6.0, prec=prec)
+    assert st == _lib.GDT_OK, _lib.status_string(st)
+    assert rings.ring_hash(nb, cnt) == rings.ring_hash(qn, qc)
+
+
 def test_host_overflow_reports_required():
     xy = inputs.uniform_points(4096, 64.0, 64.0, seed=1)
     st, _, cnt, req = host_tri(xy, 64.0, 64.0, stride=5)

@@ -144,14 +144,12 @@ def test_clustered_points_take_the_deferred_path(prec):
 @pytest.mark.parametrize("prec", PRECS)
 def test_grid_smaller_than_the_halo(prec):
     """6x6 cells (search radius 2 < the 3-cell scan block): every point is
-    deferred (wavefront kernel); the rings equal scipy's Qhull (the reference
-    path itself keeps a virtual corner in such small boxes)."""
+    deferred to the wavefront kernel, which triangulates them; the rings equal
+    scipy's Qhull (the reference path itself keeps a virtual corner in such
+    small boxes, and the oracle reproduces that)."""
     from qhull_periodic import periodic_rings
     xy = inputs.uniform_points(40, 6.0, 6.0, seed=43)
     qn, qc = periodic_rings(xy, 6.0, 6.0)
-    try:
-        nb, c, _ = gpu_triangulate(xy, 6.0, 6.0, prec)
-    except Exception as e:  # the box may be too small for the search radius
-        assert "search radius" in str(e)
-        return
+    nb, c, dg = gpu_triangulate(xy, 6.0, 6.0, prec)
+    assert dg.stats()["deferred"] == 40
     assert rings.ring_hash(nb, c) == rings.ring_hash(qn, qc)
