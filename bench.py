@@ -44,6 +44,12 @@ def parse():
     ap.add_argument("--block", type=int, default=None, help="threads per block (one-ring kernel)")
     ap.add_argument("--tiled", type=int, default=None, help="1: 32x32-cell tiles (LDS kernel), 2: the same with Morton order inside, 0: row-major")
     ap.add_argument("--cell-scale", type=float, default=None, help="cell size multiplier")
+    ap.add_argument("--tile-shift", type=int, default=None,
+                    help="LDS tile edge 2^k cells (4 or 5; default: automatic)")
+    ap.add_argument("--lanes", type=int, default=None,
+                    help="lanes per point of the tile kernel (1 or 2; default: automatic)")
+    ap.add_argument("--no-small-n", action="store_true",
+                    help="skip the 16,384 / 65,536 / 262,144-point timings")
     ap.add_argument("--precisions", default="fp32f,fp64", help="comma list of fp32f,fp64")
     ap.add_argument("--md-steps", type=int, default=1000,
                     help="steps of the 256-set Brownian test-and-repair loop (0: skip)")
@@ -183,6 +189,31 @@ def bench_c3b(torch, steps, warmup):
     return res
 
 
+def bench_small_n(torch, sizes=(16384, 65536, 262144)):
+    """Uniform points at density 1 (box sqrt(N)), default (automatic)
+    configuration: points/s per precision over 20 calls after 3 (HIP events),
+    with the tile edge and lanes per point the library chose."""
+    from gpudelaunaytriangulation_amd import PREC_FP32F, PREC_FP64, inputs
+    from gpudelaunaytriangulation_amd.delaunay import DelaunayGPU, PeriodicBoundaries
+    import numpy as np
+    out = {}
+    for n in sizes:
+        L = float(np.sqrt(n))
+        pts = torch.from_numpy(inputs.uniform_points(n, L, L, seed=2)).cuda()
+        row = {}
+        for name, prec in (("fp32f", PREC_FP32F), ("fp64", PREC_FP64)):
+            dg = DelaunayGPU(n, 16, 1.0, PeriodicBoundaries(L, L))
+            dg.setPrecision(prec)
+            tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device="cuda")
+            num = torch.zeros(n, dtype=torch.int32, device="cuda")
+            ms, _, _, _, _, st = time_path(dg, pts, tri, num, 20, 3, torch)
+            row[f"{name}_points_per_s"] = n / (ms * 1e-3)
+            row["tile_cells"] = 1 << st["tile_shift"]
+            row["lanes_per_point"] = st["lanes"]
+        out[str(n)] = row
+    return out
+
+
 def bench_md(steps, torch, precision="fp64"):
     """256 sets, N log-uniform in [1024, 65536], box sqrt(N): one batched
     triangulation, then `steps` hipGraph-replayed steps of sigma = 0.01
@@ -204,6 +235,10 @@ def bench_md(steps, torch, precision="fp64"):
     torch.cuda.synchronize()
     t_tri = time.perf_counter() - t0
     max_ring0 = int(num.max().item())
+    # the loop's configuration (tile edge, lanes per repaired point, block),
+    # tuned on 20 replayed steps of the same start (autotune.tune_md)
+    from gpudelaunaytriangulation_amd import autotune
+    tuned = autotune.tune_md(bd, X, tri, num, steps=20, seed=7)
     bd.md_run(X, tri, num, 5, 0.01, seed=99, step0=0, use_graph=True, check_every=5)   # warm-up
     torch.cuda.synchronize()      # (the warm-up also widens the rows to the longest ring + 4)
     t0 = time.perf_counter()
@@ -228,6 +263,9 @@ def bench_md(steps, torch, precision="fp64"):
             "recoveries": rep["recoveries"], "recovered": rep["recovered"],
             "skipped_replays_after_halt": rep["skipped_replays"],
             "final_rings_match": bool(final_match),
+            "tuned_config": {k: tuned[k] for k in ("block", "tile_shift", "repair_lanes")},
+            "tuner_table_ms_per_step": [(r["block"], r["tile_shift"], r["repair_lanes"],
+                                         round(r["ms_per_step"], 4)) for r in tuned["table"]],
             "step_roofline_frac": (48 + 32 + 96 * rep["flagged_total"] / (rep["steps"] * bd.N))
             * bd.N * rep["steps"] / t / 1e9 / HBM_PEAK_GBS}
 
@@ -288,7 +326,8 @@ def main():
             continue
         dg = DelaunayGPU(n, 16, 1.0, PeriodicBoundaries(L, L))
         dg.setPrecision(prec)
-        dg.ctx.set_config(profile=1, block=args.block, tiled=args.tiled, cell_scale=args.cell_scale)
+        dg.ctx.set_config(profile=1, block=args.block, tiled=args.tiled, cell_scale=args.cell_scale,
+                          tile_shift=args.tile_shift, lanes=args.lanes)
         tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device="cuda")
         num = torch.zeros(n, dtype=torch.int32, device="cuda")
         ms, wall, ring_ms, cell_ms, defer_ms, st = time_path(dg, pts, tri, num, args.steps,
@@ -348,7 +387,8 @@ def main():
         "data": "synthetic: SplitMix64 uniform points, seed 2",
         "config": {"workload": f"uniform periodic {n} points, box {L:g}x{L:g}, cell 1.0, "
                                f"FP32-filtered global triangulation",
-                   "n": n, "box": L, "cell_size": 1.0, "block": dg.ctx.config().block},
+                   "n": n, "box": L, "cell_size": 1.0, "block": dg.ctx.config().block,
+                   "tile_cells": 1 << st["tile_shift"], "lanes_per_point": st["lanes"]},
         "fp64_points_per_s": n / (f64["ms"] * 1e-3),
         "fp64_ms_per_step": f64["ms"],
         "fp32_over_fp64": f64["ms"] / f32["ms"],
@@ -375,6 +415,12 @@ def main():
         except Exception as e:  # reported, never hides the headline number
             line["c1_4096"] = {"error": repr(e)}
         print(f"[bench] c1: {line['c1_4096']}", file=sys.stderr, flush=True)
+    if not args.no_small_n:
+        try:
+            line["small_n"] = bench_small_n(torch)
+        except Exception as e:  # reported, never hides the headline number
+            line["small_n"] = {"error": repr(e)}
+        print(f"[bench] small_n: {line['small_n']}", file=sys.stderr, flush=True)
     if not args.no_c3b:
         try:
             line["c3b"] = bench_c3b(torch, args.steps, args.warmup)

@@ -52,7 +52,8 @@ class GdtError(RuntimeError):
 
 class Config(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int), ("tiled", ctypes.c_int), ("cell_scale", ctypes.c_double),
-                ("profile", ctypes.c_int)]
+                ("profile", ctypes.c_int), ("tile_shift", ctypes.c_int), ("lanes", ctypes.c_int),
+                ("repair_lanes", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -64,6 +65,7 @@ class Stats(ctypes.Structure):
         ("deferred", ctypes.c_int), ("ms_defer", ctypes.c_float),
         ("profiled_calls", ctypes.c_ulonglong), ("ms_cells_total", ctypes.c_double),
         ("ms_ring_total", ctypes.c_double), ("ms_defer_total", ctypes.c_double),
+        ("tile_shift", ctypes.c_int), ("lanes", ctypes.c_int),
     ]
 
     def as_dict(self) -> dict:

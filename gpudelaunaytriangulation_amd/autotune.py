@@ -3,11 +3,15 @@
 Tunes what ``gdt_set_config`` exposes: threads per block of the one-ring
 kernels (the reference fixes 128, DelaunayGPU.cu:16), the cell-size scale of
 the internal grid (the reference takes the caller's cell size,
-triangulation.cpp:172) and the cell order (32x32-cell LDS tiles or
-row-major).  Every candidate is timed on the caller's own points with HIP
-events; the rings of every candidate must be identical (canonical hash) —
-the output never depends on the configuration — and the fastest is applied
-to the context.  Results are cached per (N, box, precision, device) for the
+triangulation.cpp:172), the cell order (LDS tiles or row-major), the LDS
+tile's edge (16x16 or 32x32 cells: the window every block stages) and the
+lanes that share one point (one or two, for the global triangulation and for
+repairs).  Every candidate is timed on the caller's own points with HIP
+events; the rings of every candidate must be identical (canonical hash) --
+the output never depends on the configuration -- and the fastest is applied
+to the context.  ``tune`` times global triangulations; ``tune_md`` times the
+Brownian test-and-repair loop of a batch (hipGraph replays), where the
+repairs' knobs matter.  Results are cached per workload and device for the
 life of the process.
 """
 from __future__ import annotations
@@ -19,19 +23,47 @@ import torch
 from . import _lib, rings
 from .delaunay import DelaunayGPU
 
-DEFAULT_BLOCKS = (128, 256, 320, 512)
-DEFAULT_SCALES = (0.8, 0.9, 1.0, 1.15)
+DEFAULT_BLOCKS = (256, 512)
+DEFAULT_SCALES = (0.9, 1.0, 1.15)
+KNOBS = ("block", "tiled", "cell_scale", "tile_shift", "lanes", "repair_lanes")
 
 _cache: dict = {}
 
 
-def candidates(blocks=DEFAULT_BLOCKS, scales=DEFAULT_SCALES, row_major=True):
-    """(block, tiled, cell_scale) triples: every block x scale in tile order,
-    plus the row-major order at the reference's cell size."""
-    out = [(b, 1, s) for b, s in itertools.product(blocks, scales)]
+def _as_config(c) -> dict:
+    """A candidate as a knob dict: a dict, or a (block, tiled, cell_scale
+    [, tile_shift [, lanes [, repair_lanes]]]) tuple."""
+    if isinstance(c, dict):
+        return dict(c)
+    return dict(zip(KNOBS, c))
+
+
+def candidates(blocks=DEFAULT_BLOCKS, scales=DEFAULT_SCALES, tile_shifts=(4, 5), lanes=(1, 2),
+               row_major=True):
+    """Global-triangulation candidates: every block x scale x tile edge x
+    lanes per point in tile order, plus the row-major order at the caller's
+    cell size."""
+    out = [dict(block=b, tiled=1, cell_scale=s, tile_shift=t, lanes=w)
+           for b, s, t, w in itertools.product(blocks, scales, tile_shifts, lanes)]
     if row_major:
-        out += [(b, 0, 1.0) for b in blocks]
+        out += [dict(block=b, tiled=0, cell_scale=1.0) for b in blocks]
     return out
+
+
+def md_candidates(blocks=DEFAULT_BLOCKS, tile_shifts=(4, 5), repair_lanes=(1, 2)):
+    """Brownian-loop candidates: block x tile edge x lanes per repaired point."""
+    return [dict(block=b, tiled=1, tile_shift=t, repair_lanes=r)
+            for b, t, r in itertools.product(blocks, tile_shifts, repair_lanes)]
+
+
+def _apply(ctx, cfg: dict, profile=None) -> None:
+    ctx.set_config(profile=profile, **{k: cfg[k] for k in KNOBS if k in cfg})
+
+
+def _restore(ctx, saved) -> None:
+    ctx.set_config(block=saved.block, tiled=saved.tiled, cell_scale=saved.cell_scale,
+                   profile=saved.profile, tile_shift=saved.tile_shift, lanes=saved.lanes,
+                   repair_lanes=saved.repair_lanes)
 
 
 def _time(dg, pts, tri, num, reps):
@@ -48,24 +80,24 @@ def _time(dg, pts, tri, num, reps):
 
 def tune(dg: DelaunayGPU, points, configs=None, reps: int = 5, check: bool = True) -> dict:
     """Time every configuration on ``points`` with ``dg``'s box, cell size and
-    precision; apply and return the fastest ({block, tiled, cell_scale, ms,
-    table}).  Raises if two configurations give different rings."""
+    precision; apply and return the fastest (its knobs, ``ms``, ``table``,
+    ``ring_hash``).  Raises if two configurations give different rings."""
     n = int(points.shape[0])
     x11, _, _, x22 = dg.Box.getBoxDims()
     dev = torch.cuda.get_device_name(points.device)
-    key = (n, x11, x22, dg.cellsize, dg.precision, dev)
+    key = ("tri", n, x11, x22, dg.cellsize, dg.precision, dev)
     if key in _cache:
         best = _cache[key]
-        dg.ctx.set_config(block=best["block"], tiled=best["tiled"], cell_scale=best["cell_scale"])
+        _apply(dg.ctx, best)
         return best
-    configs = candidates() if configs is None else list(configs)
+    configs = [_as_config(c) for c in (candidates() if configs is None else configs)]
     saved = dg.ctx.config()
     tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device=points.device)
     num = torch.zeros(n, dtype=torch.int32, device=points.device)
     table, ref_hash = [], None
     try:
-        for block, tiled, scale in configs:
-            dg.ctx.set_config(block=block, tiled=tiled, cell_scale=scale, profile=0)
+        for cfg in configs:
+            _apply(dg.ctx, cfg, profile=0)
             ms = _time(dg, points, tri, num, reps)
             if check:
                 nb = tri.cpu().numpy().reshape(n, dg.MaxSize)
@@ -73,15 +105,65 @@ def tune(dg: DelaunayGPU, points, configs=None, reps: int = 5, check: bool = Tru
                 if ref_hash is None:
                     ref_hash = h
                 elif h != ref_hash:
-                    raise RuntimeError(f"configuration {(block, tiled, scale)} changed the rings")
-            table.append(dict(block=block, tiled=tiled, cell_scale=scale, ms=ms))
+                    raise RuntimeError(f"configuration {cfg} changed the rings")
+            table.append(dict(cfg, ms=ms))
     finally:
-        dg.ctx.set_config(block=saved.block, tiled=saved.tiled, cell_scale=saved.cell_scale,
-                          profile=saved.profile)
+        _restore(dg.ctx, saved)
     best = min(table, key=lambda r: r["ms"])
     best = dict(best, table=table, ring_hash=ref_hash)
     _cache[key] = best
-    dg.ctx.set_config(block=best["block"], tiled=best["tiled"], cell_scale=best["cell_scale"])
+    _apply(dg.ctx, best)
+    return best
+
+
+def tune_md(bd, points, tri, num, configs=None, steps: int = 20, sigma: float = 0.01,
+            seed: int = 1, check: bool = True) -> dict:
+    """Tune the Brownian loop of the batch ``bd`` (a BatchedDelaunayGPU whose
+    rings ``tri``/``num`` triangulate ``points``): every configuration runs
+    ``steps`` hipGraph-replayed steps from the same start (a copy of the
+    positions and rings), timed with HIP events; the final positions and
+    rings must agree across configurations.  The caller's arrays are left as
+    they were; the fastest configuration is applied and returned (its knobs,
+    ``ms_per_step``, ``table``)."""
+    dev = torch.cuda.get_device_name(points.device)
+    key = ("md", tuple(bd.offsets), tuple(bd.boxes), bd.precision, steps, dev)
+    if key in _cache:
+        best = _cache[key]
+        _apply(bd.ctx, best)
+        return best
+    configs = [_as_config(c) for c in (md_candidates() if configs is None else configs)]
+    saved = bd.ctx.config()
+    ms0 = bd.MaxSize
+    table, ref = [], None
+    try:
+        for cfg in configs:
+            _apply(bd.ctx, cfg, profile=0)
+            bd.MaxSize = ms0
+            X, T, C = points.clone(), tri.clone(), num.clone()
+            bd.md_run(X, T, C, 2, sigma, seed=seed + 1, step0=0)       # warm-up: capture, widen
+            bd.globalDelaunayTriangulation(X.copy_(points), T, C)      # back to the start
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            rep = bd.md_run(X, T, C, steps, sigma, seed=seed, step0=0, check_every=steps)
+            e1.record()
+            torch.cuda.synchronize()
+            if check:
+                nb = T.cpu().numpy().reshape(bd.N, bd.MaxSize)
+                state = (X.cpu().numpy().tobytes(), rings.ring_hash(nb, C.cpu().numpy()))
+                if ref is None:
+                    ref = state
+                elif state != ref:
+                    raise RuntimeError(f"configuration {cfg} changed the Brownian run")
+            table.append(dict(cfg, ms_per_step=e0.elapsed_time(e1) / steps,
+                              recoveries=rep["recoveries"]))
+    finally:
+        _restore(bd.ctx, saved)
+        bd.MaxSize = ms0
+    best = min(table, key=lambda r: r["ms_per_step"])
+    best = dict(best, table=table)
+    _cache[key] = best
+    _apply(bd.ctx, best)
     return best
 
 
@@ -99,4 +181,4 @@ def clear_cache() -> None:
     _cache.clear()
 
 
-__all__ = ["candidates", "tune", "tuned_delaunay", "clear_cache"]
+__all__ = ["candidates", "md_candidates", "tune", "tune_md", "tuned_delaunay", "clear_cache"]

@@ -24,6 +24,9 @@ ARCH = os.environ.get("GDT_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["celllist.hip", "onering.hip", "testrepair.hip", "voronoi.hip", "mdstep.hip",
            "capi.hip", "noalloc.hip"]
+# The LDS-window kernels, once per tile edge: (source, object, defines).
+TILE_SOURCES = [(src, f"{src[:-4]}_ts{ts}", [f"-DGDT_TILE_SHIFT={ts}", f"-DGDT_WIN_MAXPTS={pts}"])
+                for src in ("tile.hip", "ringtest.hip") for ts, pts in ((4, 1024), (5, 2048))]
 # CPU mode: the same per-point routines compiled for the host (g++, OpenMP)
 HOST_SOURCES = ["hostpath.cpp"]
 
@@ -41,12 +44,13 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-W
               f"-I{CSRC}", f"-I{INCLUDE}"]
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+def _compile(item) -> str:
+    src, name, defines = item if isinstance(item, tuple) else (item, os.path.splitext(item)[0], [])
+    obj = os.path.join(BUILD, name + ".o")
     if src.endswith(".cpp"):
         cmd = [os.environ.get("CXX", "g++"), *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     else:
-        cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC, *FLAGS, *defines, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
@@ -83,7 +87,7 @@ def build(verbose: bool = False) -> str:
     """Compile every HIP source and link the shared library (and the C++
     drop-in check program against it); returns the library's path."""
     os.makedirs(BUILD, exist_ok=True)
-    srcs = SOURCES + HOST_SOURCES
+    srcs = SOURCES + TILE_SOURCES + HOST_SOURCES
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"

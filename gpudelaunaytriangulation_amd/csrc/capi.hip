@@ -53,7 +53,7 @@ struct DevGuard {
 struct gdt_context {
     int device = 0;
     int num_cu = 256;                    // compute units of the device
-    gdt_config cfg = {512, 1, 1.0, 0};
+    gdt_config cfg = {512, 1, 1.0, 0, 0, 0, 0};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
     std::vector<GdtSet> sets;
@@ -163,7 +163,8 @@ static int configure(gdt_context* c, std::vector<GdtSet> sets)
     std::vector<GdtChunk> tch;
     for (int si = 0; si < (int)c->sets.size(); ++si)
         if (c->sets[si].tiled)
-            for (int t = 0; t < c->sets[si].nkeys / GDT_TILE_CELLS; ++t) tch.push_back(GdtChunk{si, t});
+            for (int t = 0; t < c->sets[si].nkeys >> (2 * c->sets[si].tshift); ++t)
+                tch.push_back(GdtChunk{si, t});
     GDT_CHECK(upload(c->tch, tch));
     c->ntch = (int)tch.size();
     GDT_CHECK(c->tpart.reserve(std::max(pch.size(), 2 * tch.size())));   // k_tri_count / k_ring_test_tile
@@ -195,13 +196,43 @@ static int ring_chunks(gdt_context* c)
     return GDT_OK;
 }
 
+// The tile edge of the configuration, or the automatic one: 16x16-cell tiles
+// when the 32x32 tiling of the sets would not give every compute unit a
+// second tile (small inputs: four times the blocks, each a quarter of the
+// work), else 32x32.
+static int tile_shift_for(const gdt_context* c, const std::vector<GdtSet>& sets)
+{
+    if (c->cfg.tile_shift) return c->cfg.tile_shift;
+    long long tiles = 0;
+    for (const auto& s : sets) tiles += (long long)(s.nkeys >> (2 * s.tshift));
+    return tiles <= c->num_cu ? 4 : 5;
+}
+
+// Lanes per point of the global triangulation: the configuration's, or two
+// when 16x16 tiles do not outnumber the compute units (every block then holds
+// one tile's points with lanes to spare), else one.
+static int lanes_for(const gdt_context* c)
+{
+    if (c->cfg.lanes) return c->cfg.lanes;
+    return c->sets[0].tshift == 4 && c->ntch <= c->num_cu ? 2 : 1;
+}
+
+// Threads per block of the tile kernels: the configured count, at most 256
+// for 16x16 tiles (about 256 points each).
+static int tile_block(const gdt_context* c)
+{
+    return c->sets[0].tshift == 4 ? std::min(c->cfg.block, 256) : c->cfg.block;
+}
+
 static int single_set(gdt_context* c, int n, double lx, double ly, double cell_size)
 {
     if (n < 4 || !(lx > 0.) || !(ly > 0.) || !(cell_size > 0.) || !std::isfinite(lx) ||
         !std::isfinite(ly) || !std::isfinite(cell_size))
         return GDT_ERR_INVALID;
-    std::vector<GdtSet> sets(1, gdt_make_set(0, n, 0, lx, ly, cell_size * c->cfg.cell_scale,
-                                             c->cfg.tiled));
+    const double a = cell_size * c->cfg.cell_scale;
+    std::vector<GdtSet> sets(1, gdt_make_set(0, n, 0, lx, ly, a, c->cfg.tiled, GDT_TSHIFT_MAX));
+    const int ts = tile_shift_for(c, sets);
+    if (ts != GDT_TSHIFT_MAX) sets[0] = gdt_make_set(0, n, 0, lx, ly, a, c->cfg.tiled, ts);
     return configure(c, sets);
 }
 
@@ -220,8 +251,14 @@ static int batch_sets(gdt_context* c, int nsets, const int* offsets, const doubl
         if (n < 4 || !(lx > 0.) || !(ly > 0.) || !std::isfinite(lx) || !std::isfinite(ly))
             return GDT_ERR_INVALID;
         sets.push_back(gdt_make_set(offsets[k], n, 0, lx, ly, cell_size * c->cfg.cell_scale,
-                                    c->cfg.tiled));
+                                    c->cfg.tiled, GDT_TSHIFT_MAX));
     }
+    const int ts = tile_shift_for(c, sets);
+    if (ts != GDT_TSHIFT_MAX)
+        for (int k = 0; k < nsets; ++k)
+            sets[(size_t)k] = gdt_make_set(offsets[k], sets[(size_t)k].n, 0, boxes[2 * k],
+                                           boxes[2 * k + 1], cell_size * c->cfg.cell_scale,
+                                           c->cfg.tiled, ts);
     return configure(c, sets);
 }
 
@@ -378,6 +415,10 @@ int gdt_set_config(gdt_context* c, const gdt_config* cfg)
     if (!c || !cfg) return GDT_ERR_INVALID;
     if (cfg->block < 64 || cfg->block > 1024 || cfg->block % 64 != 0) return GDT_ERR_INVALID;
     if (!(cfg->cell_scale >= 0.25 && cfg->cell_scale <= 8.0)) return GDT_ERR_INVALID;
+    if (cfg->tile_shift != 0 && (cfg->tile_shift < GDT_TSHIFT_MIN || cfg->tile_shift > GDT_TSHIFT_MAX))
+        return GDT_ERR_INVALID;
+    if (cfg->lanes < 0 || cfg->lanes > 2 || cfg->repair_lanes < 0 || cfg->repair_lanes > 2)
+        return GDT_ERR_INVALID;
     c->cfg = *cfg;
     c->cfg.tiled = cfg->tiled == 2 ? 2 : (cfg->tiled ? 1 : 0);
     c->sets.clear();          // force a new set table (cell size / order may change)
@@ -467,14 +508,19 @@ static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int*
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
     if (c->cfg.tiled) {
         // A grid of at most one tile per CU runs one wave of blocks: each lane
-        // then takes a single point (1024 threads for about 1024 points), so
-        // the block ends after its longest ring instead of after two.
-        const int block = c->ntch <= c->num_cu ? std::max(c->cfg.block, 1024) : c->cfg.block;
-        GDT_CHECK(gdt_launch_one_ring_tile(precision, c->d_sets.p, (int)c->sets.size(), c->tch.p,
-                                           c->ntch, block, c->total_n, nullptr, nullptr,
-                                           ring_io(c, d_nbrs, stride), d_counts, c->status.p,
-                                           c->counters.p, c->defer.p,
-                                           c->cfg.profile ? c->ev[2] : nullptr, s));
+        // (or lane group) then takes a single point (1024 threads for about
+        // 1024 points of a 32x32 tile), so the block ends after its longest
+        // ring instead of after two.
+        const int ts = c->sets[0].tshift, lanes = lanes_for(c);
+        c->stats.tile_shift = ts;
+        c->stats.lanes = lanes;
+        const int one_each = std::min(1024, (1 << (2 * ts)) * lanes);
+        const int block = c->ntch <= c->num_cu ? std::max(tile_block(c), one_each) : tile_block(c);
+        GDT_CHECK(gdt_tiles::one_ring(ts, precision, c->d_sets.p, (int)c->sets.size(), c->tch.p,
+                                      c->ntch, block, lanes, c->total_n, nullptr, nullptr,
+                                      ring_io(c, d_nbrs, stride), d_counts, c->status.p,
+                                      c->counters.p, c->defer.p,
+                                      c->cfg.profile ? c->ev[2] : nullptr, s));
     } else {
         GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), c->rch.p, c->nrch,
                                       c->cfg.block, nullptr, nullptr, 0, 0, ring_io(c, d_nbrs, stride),
@@ -569,9 +615,9 @@ static int triangles_and_test(gdt_context* c, const double* d_xy, const int* d_n
     }
     GDT_CHECK(c->toff.reserve((size_t)n + 1));
     GDT_CHECK(gdt_fill_i32(c->repair.p, -1, (size_t)n, s));
-    GDT_CHECK(gdt_launch_ring_test_tile(c->d_sets.p, c->tch.p, c->ntch, d_nbrs, d_counts, stride, n,
-                                       d_xy, c->cells(), c->repair.p, c->tpart.p, c->toff.p + n,
-                                       c->counters.p, c->status.p, s));
+    GDT_CHECK(gdt_tiles::ring_test(c->sets[0].tshift, c->d_sets.p, c->tch.p, c->ntch, d_nbrs,
+                                   d_counts, stride, n, d_xy, c->cells(), c->repair.p, c->tpart.p,
+                                   c->toff.p + n, c->counters.p, c->status.p, s));
     return GDT_OK;
 }
 
@@ -642,10 +688,12 @@ static int repair_flagged(gdt_context* c, const int* d_repair, int stride, int* 
     GDT_CHECK(gdt_launch_compact(d_repair, c->perm.p, c->total_n, c->mark.p, c->markscan.p,
                                  c->worklist.p, worklen, c->scan_tmp.p, c->halt, s));
     if (tiled_sets(c)) {
-        GDT_CHECK(gdt_launch_one_ring_tile(precision, c->d_sets.p, (int)c->sets.size(), c->tch.p,
-                                           c->ntch, c->cfg.block, c->total_n, c->worklist.p,
-                                           c->markscan.p, ring_io(c, d_nbrs, stride), d_counts,
-                                           c->status.p, c->counters.p, c->defer.p, nullptr, s));
+        GDT_CHECK(gdt_tiles::one_ring(c->sets[0].tshift, precision, c->d_sets.p,
+                                      (int)c->sets.size(), c->tch.p, c->ntch, tile_block(c),
+                                      c->cfg.repair_lanes ? c->cfg.repair_lanes : 2, c->total_n,
+                                      c->worklist.p,
+                                      c->markscan.p, ring_io(c, d_nbrs, stride), d_counts,
+                                      c->status.p, c->counters.p, c->defer.p, nullptr, s));
         return GDT_OK;
     }
     GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), nullptr, 0,

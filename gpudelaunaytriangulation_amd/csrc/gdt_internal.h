@@ -65,7 +65,7 @@ struct GdtSet {
     int xsize, ysize;
     int tiles_x, tiled;
     int rmax;            // largest Chebyshev search radius in cells
-    int pad0;
+    int tshift;          // tiles of 2^tshift x 2^tshift cells (tiled orders)
     double lx, ly;       // box
     double bsx, bsy;     // cell sides
     double ibsx, ibsy;   // their reciprocals
@@ -77,20 +77,23 @@ struct GdtSet {
 
 struct GdtChunk { int set; int start; };   // a block's work: set id and first local item
 
-// Cell order.  tiled = 1: GDT_TILE x GDT_TILE-cell tiles (32x32) in row-major
-// tile order, cells
-// row-major inside a tile, so one tile's points are one contiguous slot range
-// (the unit of work of the LDS-staged one-ring kernel); tiled = 2: the same
-// tiles with the cells of a tile in Morton (Z) order, so that consecutive
-// slots (the points of one wavefront) form square patches; tiled = 0:
-// row-major cells.
+// Cell order.  tiled = 1: tiles of 2^tshift x 2^tshift cells (32x32 or 16x16)
+// in row-major tile order, cells row-major inside a tile, so one tile's points
+// are one contiguous slot range (the unit of work of the LDS-staged kernels);
+// tiled = 2: the same tiles with the cells of a tile in Morton (Z) order, so
+// that consecutive slots (the points of one wavefront) form square patches;
+// tiled = 0: row-major cells.  The tile shift is a field of the set; the
+// LDS-staged kernels are compiled once per supported shift, GDT_TILE_SHIFT
+// being the one a translation unit is compiled for.
+#define GDT_TSHIFT_MIN 4
+#define GDT_TSHIFT_MAX 5
 #ifndef GDT_TILE_SHIFT
 #define GDT_TILE_SHIFT 5
 #endif
 #define GDT_TILE (1 << GDT_TILE_SHIFT)
 #define GDT_TILE_CELLS (GDT_TILE * GDT_TILE)
 
-// Bits of v spread to the even positions (v < 2^GDT_TILE_SHIFT <= 2^8).
+// Bits of v spread to the even positions (v < 2^8).
 GDT_HD inline int gdt_spread_bits(int v)
 {
     v = (v | (v << 4)) & 0x0f0f;
@@ -102,19 +105,21 @@ GDT_HD inline int gdt_spread_bits(int v)
 GDT_HD inline int gdt_cell_key(const GdtSet& s, int cx, int cy)
 {
     if (s.tiled) {
-        const int tile = ((cy >> GDT_TILE_SHIFT) * s.tiles_x) + (cx >> GDT_TILE_SHIFT);
-        const int ix = cx & (GDT_TILE - 1), iy = cy & (GDT_TILE - 1);
+        const int ts = s.tshift, m = (1 << ts) - 1;
+        const int tile = ((cy >> ts) * s.tiles_x) + (cx >> ts);
+        const int ix = cx & m, iy = cy & m;
         const int inner = s.tiled == 2 ? (gdt_spread_bits(ix) | (gdt_spread_bits(iy) << 1))
-                                       : ((iy << GDT_TILE_SHIFT) | ix);
-        return (tile << (2 * GDT_TILE_SHIFT)) | inner;
+                                       : ((iy << ts) | ix);
+        return (tile << (2 * ts)) | inner;
     }
     return cy * s.xsize + cx;
 }
 
-GDT_HD inline int gdt_nkeys(int xsize, int ysize, int tiled)
+GDT_HD inline int gdt_nkeys(int xsize, int ysize, int tiled, int tshift)
 {
     if (!tiled) return xsize * ysize;
-    return ((xsize + GDT_TILE - 1) / GDT_TILE) * ((ysize + GDT_TILE - 1) / GDT_TILE) * GDT_TILE_CELLS;
+    const int t = 1 << tshift;
+    return ((xsize + t - 1) / t) * ((ysize + t - 1) / t) * t * t;
 }
 
 // Offsets (dx,dy) of the i-th cell of the Chebyshev ring of radius r (8r cells).
@@ -188,17 +193,19 @@ GDT_HD inline void gdt_grid(double lx, double ly, double a, int& xsize, int& ysi
 
 // Geometry of one set; c0 is its first cell key.  rmax keeps every searched
 // cell distinct on the torus (2*rmax+1 <= cells per axis).
-inline GdtSet gdt_make_set(int p0, int n, int c0, double lx, double ly, double cell_size, int tiled)
+// tshift: tiles of 2^tshift cells a side (GDT_TSHIFT_MIN..GDT_TSHIFT_MAX).
+inline GdtSet gdt_make_set(int p0, int n, int c0, double lx, double ly, double cell_size, int tiled,
+                           int tshift = GDT_TILE_SHIFT)
 {
     GdtSet s;
     s.p0 = p0; s.n = n; s.c0 = c0;
     gdt_grid(lx, ly, cell_size, s.xsize, s.ysize);
     s.tiled = tiled == 2 ? 2 : (tiled ? 1 : 0);
-    s.tiles_x = (s.xsize + GDT_TILE - 1) / GDT_TILE;
-    s.nkeys = gdt_nkeys(s.xsize, s.ysize, s.tiled);
+    s.tshift = tshift;
+    s.tiles_x = (s.xsize + (1 << tshift) - 1) >> tshift;
+    s.nkeys = gdt_nkeys(s.xsize, s.ysize, s.tiled, tshift);
     const int rx = (s.xsize - 1) / 2, ry = (s.ysize - 1) / 2;
     s.rmax = rx < ry ? rx : ry;
-    s.pad0 = 0;
     s.lx = lx; s.ly = ly;
     s.bsx = lx / s.xsize;
     s.bsy = ly / s.ysize;

@@ -80,15 +80,32 @@ hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
 // Deferred points a tile block finishes itself (the rest go to the list).
 #define GDT_TILE_DEFER 64
 
-// One-ring of every point, one block per GDT_TILE x GDT_TILE-cell tile (tiled
-// cell order).
-// Points the tile kernel cannot finish from LDS are deferred (second and
-// third launches over `defer`, size total_n, count in status[ST_NDEFER]).
-hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nsets,
-                                    const GdtChunk* tiles, int ntiles, int block, int total_n,
-                                    const int* list, const uint32_t* lscan, const GdtRingIO& io,
-                                    int* counts, int* status, unsigned long long* counters,
-                                    int* defer, hipEvent_t after_tile, hipStream_t s);
+// Deferred points of the tile kernel (defer[0..status[ST_NDEFER]), count read
+// on the device): the wavefront kernel, then one lane each beyond GDT_WAVE_CAP.
+hipError_t gdt_launch_deferred(int precision, const GdtSet* d_sets, int nsets, int total_n,
+                               const GdtRingIO& io, int* counts, int* status,
+                               unsigned long long* counters, int* defer, hipStream_t s);
+
+// The LDS-window kernels, compiled once per tile edge into namespaces ts4
+// (16x16-cell tiles) and ts5 (32x32); gdt_tiles::* picks by the sets' shift.
+#define GDT_TS_NS GDT_TS_NS_(GDT_TILE_SHIFT)
+#define GDT_TS_NS_(x) GDT_TS_NS__(x)
+#define GDT_TS_NS__(x) ts##x
+#define GDT_DECLARE_TILE_LAUNCHERS(ns)                                                            \
+    namespace ns {                                                                                \
+    hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nsets,          \
+                                        const GdtChunk* tiles, int ntiles, int block, int lanes, \
+                                        int total_n, const int* list, const uint32_t* lscan,     \
+                                        const GdtRingIO& io, int* counts, int* status,           \
+                                        unsigned long long* counters, int* defer,                \
+                                        hipEvent_t after_tile, hipStream_t s);                   \
+    hipError_t gdt_launch_ring_test_tile(const GdtSet* d_sets, const GdtChunk* tiles, int ntiles,\
+                                         const int* nbrs, const int* counts, int stride,         \
+                                         int total_n, const double* d_xy, const GdtCells& c,     \
+                                         int* repair, unsigned long long* partials,              \
+                                         uint32_t* ntri, unsigned long long* counters,           \
+                                         int* status, hipStream_t s);                            \
+    }
 
 // Triangles owned by their smallest vertex, written at toff (deterministic);
 // counters[CT_INSUM] += sum of counts (partials holds npch words).  perm null:
@@ -107,15 +124,42 @@ hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, con
                            hipStream_t s);
 
 
+// One-ring of every point (list null) or of the slots of a slot-ordered list
+// (lscan its exclusive scan over slots), one block per tile of the tiled cell
+// order, `lanes` (1 or 2) lanes per point.  Points the tile kernel cannot
+// finish from LDS are deferred (gdt_launch_deferred over `defer`, size
+// total_n, count in status[ST_NDEFER]).
+//
 // Enumeration and empty-circle test in one pass over the rings, one block per
 // tile (tiled cell order): triangles owned by their smallest caller index,
 // flags as gdt_launch_test, counters[CT_INSUM] += sum of counts, *ntri =
 // number of owned triangles; partials holds 2 * ntiles words.
-hipError_t gdt_launch_ring_test_tile(const GdtSet* d_sets, const GdtChunk* tiles, int ntiles,
-                                     const int* nbrs, const int* counts, int stride, int total_n,
-                                     const double* d_xy, const GdtCells& c, int* repair,
-                                     unsigned long long* partials, uint32_t* ntri,
-                                     unsigned long long* counters, int* status, hipStream_t s);
+GDT_DECLARE_TILE_LAUNCHERS(ts4)
+GDT_DECLARE_TILE_LAUNCHERS(ts5)
+static_assert(GDT_TSHIFT_MIN == 4 && GDT_TSHIFT_MAX == 5, "a launcher per supported tile edge");
+
+namespace gdt_tiles {
+inline hipError_t one_ring(int tshift, int precision, const GdtSet* d_sets, int nsets,
+                           const GdtChunk* tiles, int ntiles, int block, int lanes, int total_n,
+                           const int* list, const uint32_t* lscan, const GdtRingIO& io,
+                           int* counts, int* status, unsigned long long* counters, int* defer,
+                           hipEvent_t after_tile, hipStream_t s)
+{
+    return (tshift == 4 ? ts4::gdt_launch_one_ring_tile : ts5::gdt_launch_one_ring_tile)(
+        precision, d_sets, nsets, tiles, ntiles, block, lanes, total_n, list, lscan, io, counts,
+        status, counters, defer, after_tile, s);
+}
+inline hipError_t ring_test(int tshift, const GdtSet* d_sets, const GdtChunk* tiles, int ntiles,
+                            const int* nbrs, const int* counts, int stride, int total_n,
+                            const double* d_xy, const GdtCells& c, int* repair,
+                            unsigned long long* partials, uint32_t* ntri,
+                            unsigned long long* counters, int* status, hipStream_t s)
+{
+    return (tshift == 4 ? ts4::gdt_launch_ring_test_tile : ts5::gdt_launch_ring_test_tile)(
+        d_sets, tiles, ntiles, nbrs, counts, stride, total_n, d_xy, c, repair, partials, ntri,
+        counters, status, s);
+}
+}  // namespace gdt_tiles
 
 // Slots of the flagged points, in slot order -> worklist; markscan (total_n+1
 // words) = exclusive scan of the flags with the count at markscan[total_n];

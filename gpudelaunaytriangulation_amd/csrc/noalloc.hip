@@ -108,10 +108,10 @@ extern "C" int gdt_triangulate_noalloc(const double* d_xy, int n, double lx, dou
     if (n < 4 || !(lx > 0.) || !(ly > 0.) || !(cell_size > 0.) || !std::isfinite(lx) ||
         !std::isfinite(ly) || !std::isfinite(cell_size))
         return GDT_ERR_INVALID;
-    const GdtSet S = gdt_make_set(0, n, 0, lx, ly, cell_size, 1);
+    const GdtSet S = gdt_make_set(0, n, 0, lx, ly, cell_size, 1, GDT_TILE_SHIFT);
     if (S.xsize > 65535 || S.ysize > 65535 || S.xsize < 3 || S.ysize < 3) return GDT_ERR_INVALID;
     const size_t N = (size_t)n, K = (size_t)S.nkeys;
-    const int ntiles = S.nkeys / GDT_TILE_CELLS;
+    const int ntiles = S.nkeys >> (2 * GDT_TILE_SHIFT);
     if (!gdt_cells_two_level(N, K)) return GDT_ERR_SCRATCH;
     const Layout L = layout_of(N, K, (size_t)ntiles);
     if (L.total > (size_t)GDT_NOALLOC_SCRATCH_BYTES) return GDT_ERR_SCRATCH;
@@ -159,9 +159,9 @@ extern "C" int gdt_triangulate_noalloc(const double* d_xy, int n, double lx, dou
     io.stride = stride;
     // the block size of gdt_triangulate's default configuration
     const int block = ntiles <= cus ? 1024 : 512;
-    if (gdt_launch_one_ring_tile(precision, d_sets, 1, d_tiles, ntiles, block, n, nullptr, nullptr,
-                                 io, d_counts, status, counters,
-                                 reinterpret_cast<int*>(base + L.defer), nullptr, s) != hipSuccess)
+    if (gdt_tiles::one_ring(GDT_TILE_SHIFT, precision, d_sets, 1, d_tiles, ntiles, block, 1, n,
+                            nullptr, nullptr, io, d_counts, status, counters,
+                            reinterpret_cast<int*>(base + L.defer), nullptr, s) != hipSuccess)
         return GDT_ERR_DEVICE;
     int h_status[ST_WORDS];
     if (hipMemcpyAsync(h_status, status, sizeof(h_status), hipMemcpyDeviceToHost, s) != hipSuccess ||

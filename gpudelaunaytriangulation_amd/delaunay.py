@@ -181,8 +181,16 @@ class Context:
         return c
 
     def set_config(self, block: int | None = None, tiled: int | None = None,
-                   cell_scale: float | None = None, profile: int | None = None) -> None:
+                   cell_scale: float | None = None, profile: int | None = None,
+                   tile_shift: int | None = None, lanes: int | None = None,
+                   repair_lanes: int | None = None) -> None:
         c = self.config()
+        if tile_shift is not None:
+            c.tile_shift = int(tile_shift)
+        if lanes is not None:
+            c.lanes = int(lanes)
+        if repair_lanes is not None:
+            c.repair_lanes = int(repair_lanes)
         if profile is not None:
             c.profile = int(profile)
         if block is not None:

@@ -55,6 +55,14 @@ typedef struct {
                            (direct per-point kernel) */
     double cell_scale;  /* multiplies the caller's cell size (1.0 = as given) */
     int profile;        /* 1: time the cell list and one-ring kernels with HIP events */
+    int tile_shift;     /* LDS tile of the tiled orders: 2^tile_shift cells a side, 4 (16x16)
+                           or 5 (32x32); 0 (default) = automatic: 16x16 when the 32x32
+                           tiles would not outnumber the compute units, else 32x32 */
+    int lanes;          /* lanes per point of the global triangulation's tile kernel, 1 or 2;
+                           0 (default) = automatic: 2 with 16x16 tiles that do not outnumber
+                           the compute units, else 1 */
+    int repair_lanes;   /* lanes per point of the repairs' tile kernel, 1 or 2; 0 (default)
+                           = 2 */
 } gdt_config;
 
 /* Statistics of the last triangulate / repair call. */
@@ -75,6 +83,8 @@ typedef struct {
      * loop without reading the statistics inside it) */
     unsigned long long profiled_calls;
     double ms_cells_total, ms_ring_total, ms_defer_total;
+    int tile_shift;                  /* tile edge of the last global triangulation (tiled order) */
+    int lanes;                       /* its lanes per point */
 } gdt_stats;
 
 /* ---- context ---------------------------------------------------------- */

@@ -6,7 +6,8 @@ points through the wavefront routine) and the wavefront routine (gdt_wave.h,
 paths must reproduce the oracle's rings bit-exactly and be byte-identical to
 each other, on uniform, near-degenerate (jittered lattice), rectangular, small,
 fine-grid (cells much smaller than the spacing: circle-box extensions and
-deferrals) and clustered (windows too full for LDS) inputs."""
+deferrals) and clustered (windows too full for LDS) inputs; the tile routine
+for both compiled tile edges (32x32 and 16x16 cells)."""
 import os
 import subprocess
 
@@ -21,11 +22,18 @@ SRC = os.path.join(ROOT, "tests", "cpu_emul", "emul.cpp")
 CSRC = os.path.join(ROOT, "gpudelaunaytriangulation_amd", "csrc")
 
 
-@pytest.fixture(scope="module")
-def emul(tmp_path_factory):
-    exe = str(tmp_path_factory.mktemp("emul") / "emul")
+# the LDS tile routine's two compiled tile edges: 32x32 cells (window budget
+# 2048 points) and 16x16 (1024), as the library builds them
+TILE_EDGES = {5: 2048, 4: 1024}
+
+
+@pytest.fixture(scope="module", params=sorted(TILE_EDGES, reverse=True), ids=lambda t: f"ts{t}")
+def emul(request, tmp_path_factory):
+    ts = request.param
+    exe = str(tmp_path_factory.mktemp("emul") / f"emul_ts{ts}")
     subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined",
                     "-fno-sanitize-recover=all", "-fno-omit-frame-pointer", "-ffp-contract=off",
+                    f"-DGDT_TILE_SHIFT={ts}", f"-DGDT_WIN_MAXPTS={TILE_EDGES[ts]}",
                     f"-I{CSRC}", SRC, "-o", exe], check=True)
     return exe
 
