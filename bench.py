@@ -299,6 +299,27 @@ def bench_voronoi(torch):
         ms = e0.elapsed_time(e1) / 20
         out[name] = {"ms_forward_adjoint": ms, "points_per_s": n / ms * 1e3,
                      "roofline_frac": 60.0 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        out[name]["_grad"] = once().double()
+    # measured gradient errors (topology fixed): the FP64 adjoint against
+    # central differences of the FP64 energy (h = 1e-6) on 64 sampled
+    # coordinates, and the FP32 adjoint against the FP64 adjoint; both relative
+    # to the largest gradient entry
+    g64, g32 = out["fp64"].pop("_grad"), out["fp32"].pop("_grad")
+    gmax = g64.abs().max().item()
+    h = 1e-6
+    idx = np.random.default_rng(0).choice(2 * n, size=64, replace=False)
+    fd_err = 0.0
+    for k in idx:
+        i, d = divmod(int(k), 2)
+        e = []
+        for sgn in (1.0, -1.0):
+            xp = xy.clone()
+            xp[i, d] += sgn * h
+            e.append(voronoi_forward(ctx, xp, tri, num, (L, L), precision=PREC_FP64)[2].item())
+        fd = (e[0] - e[1]) / (2 * h)
+        fd_err = max(fd_err, abs(fd - g64[i, d].item()))
+    out["grad_rel_err_fp64_vs_central_differences"] = fd_err / gmax
+    out["grad_rel_err_fp32_vs_fp64"] = (g32 - g64).abs().max().item() / gmax
     out["workload"] = "65536 uniform points, box 256, E = sum (A-1)^2 + (P-3.8)^2"
     return out
 
