@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--md-precision", default="fp64", choices=("fp64", "fp32f"),
                     help="predicates of the Brownian loop's repairs")
     ap.add_argument("--no-voronoi", action="store_true", help="skip the Voronoi energy timing")
+    ap.add_argument("--no-general", action="store_true",
+                    help="skip the general (sheared) box timing")
     ap.add_argument("--no-c1", action="store_true",
                     help="skip the 4,096-point case (GPU and CPU oracle timings)")
     ap.add_argument("--no-c3b", action="store_true",
@@ -187,6 +189,61 @@ def bench_c3b(torch, steps, warmup):
         res["matches_golden_oracle_hash"] = (json.load(fh)["workloads"]["c3b"]["ring_sha256"]
                                              == hashes["fp32f"])
     return res
+
+
+def bench_general(torch, steps, warmup):
+    """A general (triclinic) box: 2^20 uniform points in the sheared box with
+    lattice vectors (1024, 0) and (320, 1024) (density 1, cell size 1), FP64
+    (general boxes run the row-major per-point kernels): points/s of the
+    global triangulation, time of a test-and-repair after a Gaussian move of
+    sigma 0.01 (wrapped by whole lattice vectors), and whether the rings equal
+    the CPU mode's (the same routines on the host)."""
+    import ctypes
+    import numpy as np
+    from gpudelaunaytriangulation_amd import PREC_FP64, _lib, inputs, rings
+    from gpudelaunaytriangulation_amd.delaunay import DelaunayGPU, PeriodicBoundaries
+    box = (1024.0, 320.0, 0.0, 1024.0)
+    n = 1 << 20
+    xy = inputs.general_points(n, box, seed=5)
+    pts = torch.from_numpy(xy).cuda()
+    dg = DelaunayGPU(n, 16, 1.0, PeriodicBoundaries(*box))
+    dg.setPrecision(PREC_FP64)
+    tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device="cuda")
+    num = torch.zeros(n, dtype=torch.int32, device="cuda")
+    ms, _, _, _, _, st = time_path(dg, pts, tri, num, steps, warmup, torch)
+    nb = tri.cpu().numpy().reshape(n, dg.MaxSize)
+    cnt = num.cpu().numpy()
+    L = _lib.load()
+    hn = np.full((n, dg.MaxSize), -1, dtype=np.int32)
+    hc = np.zeros(n, dtype=np.int32)
+    req = ctypes.c_int(0)
+    bp = (ctypes.c_double * 4)(*box)
+    hst = L.gdt_host_triangulate_general(xy.ctypes.data, n, bp, 1.0, dg.MaxSize, hn.ctypes.data,
+                                         hc.ctypes.data, PREC_FP64, 0, ctypes.byref(req))
+    moved = inputs.wrap_into_general_box(xy + 0.01 * inputs.gaussian(6, 2 * n).reshape(n, 2), box)
+    mpts = torch.from_numpy(moved).cuda()
+    times = []
+    for _ in range(5):
+        t = tri.clone()
+        c = num.clone()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dg.testAndRepairDelaunayTriangulation(mpts, t, c)
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    return {
+        "workload": "2^20 uniform points, general box a=(1024,0) b=(320,1024), cell 1.0, fp64",
+        "points_per_s": n / (ms * 1e-3),
+        "ms_per_step": ms,
+        "deferred_points": st["deferred"],
+        "cells": [st["xsize"], st["ysize"]],
+        "test_and_repair_ms": float(np.median(times)),
+        "repaired_points": dg.last_repaired,
+        "euler_ok": int(cnt.sum()) == 6 * n,
+        "rings_equal_cpu_mode": hst == _lib.GDT_OK and rings.ring_hash(nb, cnt) == rings.ring_hash(hn, hc),
+    }
 
 
 def bench_small_n(torch, sizes=(16384, 65536, 262144)):
@@ -448,6 +505,12 @@ def main():
         except Exception as e:  # reported, never hides the headline number
             line["c3b"] = {"error": repr(e)}
         print(f"[bench] c3b: {line['c3b']}", file=sys.stderr, flush=True)
+    if not args.no_general:
+        try:
+            line["general_box"] = bench_general(torch, args.steps, args.warmup)
+        except Exception as e:  # reported, never hides the headline number
+            line["general_box"] = {"error": repr(e)}
+        print(f"[bench] general box: {line['general_box']}", file=sys.stderr, flush=True)
     if args.md_steps > 0:
         try:
             line["md_c4"] = bench_md(args.md_steps, torch, args.md_precision)

@@ -37,6 +37,10 @@ EXPORTS = (
     "gdt_test_and_repair_batch", "gdt_md_run", "gdt_voronoi_vertices",
     "gdt_host_celllist", "gdt_host_triangulate", "gdt_host_test_and_repair",
     "gdt_host_voronoi_vertices", "gdt_triangulate_noalloc",
+    "gdt_celllist_build_general", "gdt_triangulate_general", "gdt_test_and_repair_general",
+    "gdt_md_run_general", "gdt_voronoi_vertices_general", "gdt_voronoi_forward_general",
+    "gdt_voronoi_adjoint_general", "gdt_host_celllist_general", "gdt_host_triangulate_general",
+    "gdt_host_test_and_repair_general", "gdt_host_voronoi_vertices_general",
 )
 
 
@@ -148,6 +152,24 @@ def load():
     _sig(L, "gdt_host_triangulate", [_vp, _i, _d, _d, _d, _i, _vp, _vp, _i, _i, _ip])
     _sig(L, "gdt_host_test_and_repair", [_vp, _i, _d, _d, _d, _i, _vp, _vp, _i, _i, _ip, _ip])
     _sig(L, "gdt_host_voronoi_vertices", [_vp, _i, _d, _d, _vp, _vp, _i, _vp, _i])
+    _dp = ctypes.POINTER(_d)
+    _sig(L, "gdt_celllist_build_general", [_vp, _vp, _i, _dp, _d, _vp])
+    _sig(L, "gdt_triangulate_general", [_vp, _vp, _i, _ip, _dp, _d, _i, _vp, _vp, _i, _ip, _vp])
+    _sig(L, "gdt_test_and_repair_general", [_vp, _vp, _i, _ip, _dp, _d, _i, _vp, _vp, _i, _ip,
+                                            _ip, _vp])
+    _sig(L, "gdt_md_run_general", [_vp, _vp, _i, _ip, _dp, _d, _i, _vp, _vp, _i, _d,
+                                   ctypes.c_ulonglong, _i, _i, _i, _i, ctypes.POINTER(MdReport),
+                                   _vp])
+    _sig(L, "gdt_voronoi_vertices_general", [_vp, _vp, _i, _dp, _vp, _vp, _i, _vp, _vp])
+    _sig(L, "gdt_voronoi_forward_general", [_vp, _vp, _i, _ip, _dp, _vp, _vp, _i, _d, _d, _i,
+                                            _vp, _vp, _vp, _vp])
+    _sig(L, "gdt_voronoi_adjoint_general", [_vp, _vp, _i, _ip, _dp, _vp, _vp, _i, _d, _d, _i,
+                                            _vp, _vp, _vp, _vp, _vp])
+    _sig(L, "gdt_host_celllist_general", [_vp, _i, _dp, _d, _vp, _vp, _i, _ip, _ip, _ip])
+    _sig(L, "gdt_host_triangulate_general", [_vp, _i, _dp, _d, _i, _vp, _vp, _i, _i, _ip])
+    _sig(L, "gdt_host_test_and_repair_general", [_vp, _i, _dp, _d, _i, _vp, _vp, _i, _i, _ip,
+                                                 _ip])
+    _sig(L, "gdt_host_voronoi_vertices_general", [_vp, _i, _dp, _vp, _vp, _i, _vp, _i])
     _lib = L
     return L
 

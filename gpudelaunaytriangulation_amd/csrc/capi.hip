@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -224,41 +225,95 @@ static int tile_block(const gdt_context* c)
     return c->sets[0].tshift == 4 ? std::min(c->cfg.block, 256) : c->cfg.block;
 }
 
-static int single_set(gdt_context* c, int n, double lx, double ly, double cell_size)
+// Box s of a caller's array: {lx, ly} pairs (words 2) or box matrices
+// {x11, x12, x21, x22} with the lattice vectors as columns (words 4).
+static void box_of(const double* boxes, int words, int s, double b[4])
 {
-    if (n < 4 || !(lx > 0.) || !(ly > 0.) || !(cell_size > 0.) || !std::isfinite(lx) ||
-        !std::isfinite(ly) || !std::isfinite(cell_size))
+    if (words == 2) {
+        b[0] = boxes[2 * s];
+        b[1] = b[2] = 0.;
+        b[3] = boxes[2 * s + 1];
+        return;
+    }
+    for (int k = 0; k < 4; ++k) b[k] = boxes[4 * (size_t)s + k];
+}
+
+// A usable box: finite, and a rectangle with positive sides (words 2) or an
+// invertible matrix (words 4).
+static bool box_ok(const double b[4], int words)
+{
+    for (int k = 0; k < 4; ++k)
+        if (!std::isfinite(b[k])) return false;
+    if (words == 2) return b[0] > 0. && b[3] > 0.;
+    const double det = b[0] * b[3] - b[1] * b[2];
+    return std::isfinite(det) && std::fabs(det) > 0.;
+}
+
+// General boxes run the row-major cell order and FP64 predicates (the LDS tile
+// kernels and the FP32 filter's bounds are for rectangular cells).
+static bool any_general(const gdt_context* c)
+{
+    for (const auto& s : c->sets)
+        if (s.general) return true;
+    return false;
+}
+static int eff_precision(const gdt_context* c, int precision)
+{
+    return any_general(c) ? GDT_PREC_FP64 : precision;
+}
+
+static int single_set_box(gdt_context* c, int n, const double* box, int words, double cell_size)
+{
+    if (n < 4 || !box_ok(box, words) || !(cell_size > 0.) || !std::isfinite(cell_size))
         return GDT_ERR_INVALID;
     const double a = cell_size * c->cfg.cell_scale;
+    if (!gdt_box_is_rect(box))
+        return configure(c, std::vector<GdtSet>(1, gdt_make_set_box(0, n, 0, box, a, 0)));
+    const double lx = box[0], ly = box[3];
     std::vector<GdtSet> sets(1, gdt_make_set(0, n, 0, lx, ly, a, c->cfg.tiled, GDT_TSHIFT_MAX));
     const int ts = tile_shift_for(c, sets);
     if (ts != GDT_TSHIFT_MAX) sets[0] = gdt_make_set(0, n, 0, lx, ly, a, c->cfg.tiled, ts);
     return configure(c, sets);
 }
 
-// Set s = caller points [offsets[s], offsets[s+1]) in a boxes[2s] x boxes[2s+1] box.
+static int single_set(gdt_context* c, int n, double lx, double ly, double cell_size)
+{
+    const double b[4] = {lx, 0., 0., ly};
+    return single_set_box(c, n, b, 2, cell_size);
+}
+
+// Set s = caller points [offsets[s], offsets[s+1]) in box s of `boxes`
+// (words per box: 2 or 4, as box_of).
 static int batch_sets(gdt_context* c, int nsets, const int* offsets, const double* boxes,
-                      double cell_size)
+                      double cell_size, int words = 2)
 {
     if (nsets <= 0 || !offsets || !boxes || offsets[0] != 0 || !(cell_size > 0.) ||
         !std::isfinite(cell_size))
         return GDT_ERR_INVALID;
+    bool general = false;
+    for (int k = 0; k < nsets; ++k) {
+        double b[4];
+        box_of(boxes, words, k, b);
+        if (offsets[k + 1] - offsets[k] < 4 || !box_ok(b, words)) return GDT_ERR_INVALID;
+        general = general || !gdt_box_is_rect(b);
+    }
+    const int tiled = general ? 0 : c->cfg.tiled;
     std::vector<GdtSet> sets;
     sets.reserve((size_t)nsets);
     for (int k = 0; k < nsets; ++k) {
-        const int n = offsets[k + 1] - offsets[k];
-        const double lx = boxes[2 * k], ly = boxes[2 * k + 1];
-        if (n < 4 || !(lx > 0.) || !(ly > 0.) || !std::isfinite(lx) || !std::isfinite(ly))
-            return GDT_ERR_INVALID;
-        sets.push_back(gdt_make_set(offsets[k], n, 0, lx, ly, cell_size * c->cfg.cell_scale,
-                                    c->cfg.tiled, GDT_TSHIFT_MAX));
+        double b[4];
+        box_of(boxes, words, k, b);
+        sets.push_back(gdt_make_set_box(offsets[k], offsets[k + 1] - offsets[k], 0, b,
+                                        cell_size * c->cfg.cell_scale, tiled, GDT_TSHIFT_MAX));
     }
     const int ts = tile_shift_for(c, sets);
-    if (ts != GDT_TSHIFT_MAX)
-        for (int k = 0; k < nsets; ++k)
-            sets[(size_t)k] = gdt_make_set(offsets[k], sets[(size_t)k].n, 0, boxes[2 * k],
-                                           boxes[2 * k + 1], cell_size * c->cfg.cell_scale,
-                                           c->cfg.tiled, ts);
+    if (!general && ts != GDT_TSHIFT_MAX)
+        for (int k = 0; k < nsets; ++k) {
+            double b[4];
+            box_of(boxes, words, k, b);
+            sets[(size_t)k] = gdt_make_set_box(offsets[k], sets[(size_t)k].n, 0, b,
+                                               cell_size * c->cfg.cell_scale, tiled, ts);
+        }
     return configure(c, sets);
 }
 
@@ -452,6 +507,16 @@ int gdt_celllist_build(gdt_context* c, const double* d_xy, int n, double lx, dou
     return build_cells(c, d_xy, s);
 }
 
+int gdt_celllist_build_general(gdt_context* c, const double* d_xy, int n, const double* box,
+                               double cell_size, void* stream)
+{
+    if (!c || !d_xy || !box) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    int rc = single_set_box(c, n, box, 4, cell_size);
+    if (rc != GDT_OK) return rc;
+    return build_cells(c, d_xy, (hipStream_t)stream);
+}
+
 int gdt_celllist_info(gdt_context* c, int* xsize, int* ysize, double* bsx, double* bsy,
                       int* max_occupancy, void* stream)
 {
@@ -497,16 +562,22 @@ static int finish_rings(gdt_context* c, int stride, int* required, hipStream_t s
     return req > stride ? GDT_OVERFLOW : GDT_OK;
 }
 
+static bool tiled_sets(const gdt_context* c)
+{
+    return !c->sets.empty() && c->sets[0].tiled && c->ntch > 0;
+}
+
 // Global triangulation of the installed set table.
 static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int* d_nbrs,
                             int* d_counts, int precision, int* required, hipStream_t s)
 {
     int rc;
+    precision = eff_precision(c, precision);
     if ((rc = ring_chunks(c)) != GDT_OK) return rc;
     if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
     if ((rc = reset_status(c, s)) != GDT_OK) return rc;
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
-    if (c->cfg.tiled) {
+    if (tiled_sets(c)) {
         // A grid of at most one tile per CU runs one wave of blocks: each lane
         // (or lane group) then takes a single point (1024 threads for about
         // 1024 points of a 32x32 tile), so the block ends after its longest
@@ -522,10 +593,15 @@ static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int*
                                       c->counters.p, c->defer.p,
                                       c->cfg.profile ? c->ev[2] : nullptr, s));
     } else {
+        c->stats.tile_shift = 0;
+        c->stats.lanes = 1;
+        const GdtRingIO io = ring_io(c, d_nbrs, stride);
         GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), c->rch.p, c->nrch,
-                                      c->cfg.block, nullptr, nullptr, 0, 0, ring_io(c, d_nbrs, stride),
-                                      d_counts, c->status.p, c->counters.p, s));
+                                      c->cfg.block, nullptr, nullptr, 0, 0, io, d_counts,
+                                      c->status.p, c->counters.p, c->defer.p, s));
         if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[2], s));
+        GDT_CHECK(gdt_launch_deferred(precision, c->d_sets.p, (int)c->sets.size(), c->total_n, io,
+                                      d_counts, c->status.p, c->counters.p, c->defer.p, s));
     }
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[3], s));
     rc = finish_rings(c, stride, required, s);
@@ -571,6 +647,19 @@ int gdt_triangulate_batch(gdt_context* c, const double* d_xy, int nsets, const i
                             (hipStream_t)stream);
 }
 
+int gdt_triangulate_general(gdt_context* c, const double* d_xy, int nsets, const int* offsets,
+                            const double* boxes, double cell_size, int stride, int* d_nbrs,
+                            int* d_counts, int precision, int* required, void* stream)
+{
+    if (!c || !d_xy || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
+    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    int rc = batch_sets(c, nsets, offsets, boxes, cell_size, 4);
+    if (rc != GDT_OK) return rc;
+    return triangulate_sets(c, d_xy, stride, d_nbrs, d_counts, precision, required,
+                            (hipStream_t)stream);
+}
+
 // Owned triangles into `tris` (capacity cap); count stays on the device at
 // toff[N].  slot_order: owners in cell-slot order (the current cell list's).
 static int enumerate(gdt_context* c, const int* d_nbrs, const int* d_counts, int stride,
@@ -593,11 +682,6 @@ static int test_triangles(gdt_context* c, const double* d_xy, int cap, hipStream
     GDT_CHECK(gdt_launch_test(c->d_sets.p, (int)c->sets.size(), c->tris.p, c->toff.p + n, cap,
                               n, d_xy, c->cells(), c->repair.p, s));
     return GDT_OK;
-}
-
-static bool tiled_sets(const gdt_context* c)
-{
-    return !c->sets.empty() && c->sets[0].tiled && c->ntch > 0;
 }
 
 // Triangles of the current rings and their empty-circle test: with the tiled
@@ -696,10 +780,12 @@ static int repair_flagged(gdt_context* c, const int* d_repair, int stride, int* 
                                       c->status.p, c->counters.p, c->defer.p, nullptr, s));
         return GDT_OK;
     }
+    const GdtRingIO io = ring_io(c, d_nbrs, stride);
     GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), nullptr, 0,
-                                  c->cfg.block, c->worklist.p, worklen, 0, c->total_n,
-                                  ring_io(c, d_nbrs, stride), d_counts, c->status.p,
-                                  c->counters.p, s));
+                                  c->cfg.block, c->worklist.p, worklen, 0, c->total_n, io,
+                                  d_counts, c->status.p, c->counters.p, c->defer.p, s));
+    GDT_CHECK(gdt_launch_deferred(precision, c->d_sets.p, (int)c->sets.size(), c->total_n, io,
+                                  d_counts, c->status.p, c->counters.p, c->defer.p, s));
     return GDT_OK;
 }
 
@@ -711,6 +797,7 @@ int gdt_repair(gdt_context* c, const double* d_xy, int n, const int* d_repair, i
     if (!c->cells_valid || c->total_n != n) return GDT_ERR_INVALID;
     DevGuard g(c->device);
     hipStream_t s = (hipStream_t)stream;
+    precision = eff_precision(c, precision);
     int rc = reset_status(c, s);
     if (rc != GDT_OK) return rc;
     if ((rc = repair_flagged(c, d_repair, stride, d_nbrs, d_counts, precision, s)) != GDT_OK) return rc;
@@ -726,6 +813,7 @@ static int test_and_repair_sets(gdt_context* c, const double* d_xy, int stride, 
 {
     const int n = c->total_n;
     const int cap = 2 * n + 64;
+    precision = eff_precision(c, precision);
     GDT_CHECK(c->tris.reserve(3 * (size_t)cap));
     GDT_CHECK(c->repair.reserve((size_t)n));
     int rc;
@@ -782,6 +870,20 @@ int gdt_test_and_repair_batch(gdt_context* c, const double* d_xy, int nsets, con
                                 (hipStream_t)stream);
 }
 
+int gdt_test_and_repair_general(gdt_context* c, const double* d_xy, int nsets, const int* offsets,
+                                const double* boxes, double cell_size, int stride, int* d_nbrs,
+                                int* d_counts, int precision, int* nflagged, int* required,
+                                void* stream)
+{
+    if (!c || !d_xy || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
+    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    int rc = batch_sets(c, nsets, offsets, boxes, cell_size, 4);
+    if (rc != GDT_OK) return rc;
+    return test_and_repair_sets(c, d_xy, stride, d_nbrs, d_counts, precision, nflagged, required,
+                                (hipStream_t)stream);
+}
+
 // ---- Brownian test-and-repair loop ------------------------------------------
 }  // extern "C"
 
@@ -824,10 +926,12 @@ struct ProfileOff {
 
 extern "C" {
 
-int gdt_md_run(gdt_context* c, double* d_xy, int nsets, const int* offsets, const double* boxes,
-               double cell_size, int stride, int* d_nbrs, int* d_counts, int precision,
-               double sigma, unsigned long long seed, int step0, int nsteps, int use_graph,
-               int check_every, gdt_md_report* rep, void* stream)
+}  // extern "C"
+
+static int md_run(gdt_context* c, double* d_xy, int nsets, const int* offsets, const double* boxes,
+                  int words, double cell_size, int stride, int* d_nbrs, int* d_counts,
+                  int precision, double sigma, unsigned long long seed, int step0, int nsteps,
+                  int use_graph, int check_every, gdt_md_report* rep, void* stream)
 {
     if (!c || !d_xy || !d_nbrs || !d_counts || !rep || stride < 3 || nsteps < 0 || step0 < 0 ||
         check_every < 1 || !(sigma >= 0.) || !std::isfinite(sigma))
@@ -836,8 +940,9 @@ int gdt_md_run(gdt_context* c, double* d_xy, int nsets, const int* offsets, cons
     std::memset(rep, 0, sizeof(*rep));
     rep->first_bad_step = -1;
     DevGuard g(c->device);
-    int rc = batch_sets(c, nsets, offsets, boxes, cell_size);
+    int rc = batch_sets(c, nsets, offsets, boxes, cell_size, words);
     if (rc != GDT_OK) return rc;
+    precision = eff_precision(c, precision);
     const size_t n = (size_t)c->total_n, cap = 2 * n + 64;
     GDT_CHECK(c->tris.reserve(3 * cap));
     GDT_CHECK(c->repair.reserve(n));
@@ -910,22 +1015,51 @@ int gdt_md_run(gdt_context* c, double* d_xy, int nsets, const int* offsets, cons
     return status_from_flags(m.errflags);
 }
 
+extern "C" {
+
+int gdt_md_run(gdt_context* c, double* d_xy, int nsets, const int* offsets, const double* boxes,
+               double cell_size, int stride, int* d_nbrs, int* d_counts, int precision,
+               double sigma, unsigned long long seed, int step0, int nsteps, int use_graph,
+               int check_every, gdt_md_report* rep, void* stream)
+{
+    return md_run(c, d_xy, nsets, offsets, boxes, 2, cell_size, stride, d_nbrs, d_counts,
+                  precision, sigma, seed, step0, nsteps, use_graph, check_every, rep, stream);
+}
+
+int gdt_md_run_general(gdt_context* c, double* d_xy, int nsets, const int* offsets,
+                       const double* boxes, double cell_size, int stride, int* d_nbrs,
+                       int* d_counts, int precision, double sigma, unsigned long long seed,
+                       int step0, int nsteps, int use_graph, int check_every,
+                       gdt_md_report* rep, void* stream)
+{
+    return md_run(c, d_xy, nsets, offsets, boxes, 4, cell_size, stride, d_nbrs, d_counts,
+                  precision, sigma, seed, step0, nsteps, use_graph, check_every, rep, stream);
+}
+
 // ---- Voronoi energy and adjoint -------------------------------------------
 }  // extern "C"
 
-static int voronoi_sets(gdt_context* c, int nsets, const int* offsets, const double* boxes)
+static int voronoi_sets(gdt_context* c, int nsets, const int* offsets, const double* boxes,
+                        int words = 2)
 {
     if (nsets <= 0 || !offsets || !boxes || offsets[0] != 0) return GDT_ERR_INVALID;
     std::vector<GdtSet> v((size_t)nsets);
     for (int s = 0; s < nsets; ++s) {
+        double b[4];
+        box_of(boxes, words, s, b);
+        const int n = offsets[s + 1] - offsets[s];
+        if (n < 3 || !box_ok(b, words)) return GDT_ERR_INVALID;
         GdtSet S;
-        std::memset(&S, 0, sizeof(S));
+        if (gdt_box_is_rect(b)) {
+            std::memset(&S, 0, sizeof(S));
+            S.lx = b[0];
+            S.ly = b[3];
+        } else {
+            // minimum images need the reduced basis only (a one-cell grid)
+            S = gdt_make_set_box(0, n, 0, b, std::numeric_limits<double>::infinity(), 0);
+        }
         S.p0 = offsets[s];
-        S.n = offsets[s + 1] - offsets[s];
-        S.lx = boxes[2 * s];
-        S.ly = boxes[2 * s + 1];
-        if (S.n < 3 || !(S.lx > 0.) || !(S.ly > 0.) || !std::isfinite(S.lx) || !std::isfinite(S.ly))
-            return GDT_ERR_INVALID;
+        S.n = n;
         v[(size_t)s] = S;
     }
     if (same_sets(v, c->vsets) && c->d_vsets.p) return GDT_OK;
@@ -935,7 +1069,7 @@ static int voronoi_sets(gdt_context* c, int nsets, const int* offsets, const dou
 }
 
 static int voronoi_forward(gdt_context* c, const double* d_xy, int nsets, const int* offsets,
-                           const double* boxes, const int* d_nbrs, const int* d_counts, int stride,
+                           const double* boxes, int words, const int* d_nbrs, const int* d_counts, int stride,
                            double a0, double p0, int precision, double* d_area, double* d_perim,
                            double* d_energy, void* stream)
 {
@@ -943,7 +1077,7 @@ static int voronoi_forward(gdt_context* c, const double* d_xy, int nsets, const 
         return GDT_ERR_INVALID;
     if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
     DevGuard g(c->device);
-    int rc = voronoi_sets(c, nsets, offsets, boxes);
+    int rc = voronoi_sets(c, nsets, offsets, boxes, words);
     if (rc != GDT_OK) return rc;
     const int total = offsets[nsets];
     GDT_CHECK(c->ecell.reserve((size_t)total));
@@ -955,7 +1089,7 @@ static int voronoi_forward(gdt_context* c, const double* d_xy, int nsets, const 
 }
 
 static int voronoi_adjoint(gdt_context* c, const double* d_xy, int nsets, const int* offsets,
-                           const double* boxes, const int* d_nbrs, const int* d_counts, int stride,
+                           const double* boxes, int words, const int* d_nbrs, const int* d_counts, int stride,
                            double a0, double p0, int precision, const double* d_area,
                            const double* d_perim, const double* d_gscale, double* d_grad,
                            void* stream)
@@ -964,7 +1098,7 @@ static int voronoi_adjoint(gdt_context* c, const double* d_xy, int nsets, const 
         return GDT_ERR_INVALID;
     if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
     DevGuard g(c->device);
-    int rc = voronoi_sets(c, nsets, offsets, boxes);
+    int rc = voronoi_sets(c, nsets, offsets, boxes, words);
     if (rc != GDT_OK) return rc;
     const int total = offsets[nsets];
     GDT_CHECK(c->vslots.reserve(2 * (size_t)total * (size_t)stride));
@@ -998,7 +1132,7 @@ int gdt_voronoi_forward(gdt_context* c, const double* d_xy, int n, double lx, do
 {
     const int off[2] = {0, n};
     const double box[2] = {lx, ly};
-    return voronoi_forward(c, d_xy, 1, off, box, d_nbrs, d_counts, stride, a0, p0, precision,
+    return voronoi_forward(c, d_xy, 1, off, box, 2, d_nbrs, d_counts, stride, a0, p0, precision,
                            d_area, d_perim, d_energy, stream);
 }
 
@@ -1009,7 +1143,7 @@ int gdt_voronoi_adjoint(gdt_context* c, const double* d_xy, int n, double lx, do
 {
     const int off[2] = {0, n};
     const double box[2] = {lx, ly};
-    return voronoi_adjoint(c, d_xy, 1, off, box, d_nbrs, d_counts, stride, a0, p0, precision,
+    return voronoi_adjoint(c, d_xy, 1, off, box, 2, d_nbrs, d_counts, stride, a0, p0, precision,
                            d_area, d_perim, d_gscale, d_grad, stream);
 }
 
@@ -1018,7 +1152,7 @@ int gdt_voronoi_forward_batch(gdt_context* c, const double* d_xy, int nsets, con
                               int stride, double a0, double p0, int precision, double* d_area,
                               double* d_perim, double* d_energy, void* stream)
 {
-    return voronoi_forward(c, d_xy, nsets, offsets, boxes, d_nbrs, d_counts, stride, a0, p0,
+    return voronoi_forward(c, d_xy, nsets, offsets, boxes, 2, d_nbrs, d_counts, stride, a0, p0,
                            precision, d_area, d_perim, d_energy, stream);
 }
 
@@ -1028,7 +1162,41 @@ int gdt_voronoi_adjoint_batch(gdt_context* c, const double* d_xy, int nsets, con
                               const double* d_area, const double* d_perim, const double* d_gscale,
                               double* d_grad, void* stream)
 {
-    return voronoi_adjoint(c, d_xy, nsets, offsets, boxes, d_nbrs, d_counts, stride, a0, p0,
+    return voronoi_adjoint(c, d_xy, nsets, offsets, boxes, 2, d_nbrs, d_counts, stride, a0, p0,
+                           precision, d_area, d_perim, d_gscale, d_grad, stream);
+}
+
+int gdt_voronoi_vertices_general(gdt_context* c, const double* d_xy, int n, const double* box,
+                                 const int* d_nbrs, const int* d_counts, int stride,
+                                 double* d_vertices, void* stream)
+{
+    if (!c || !d_xy || !box || !d_nbrs || !d_counts || !d_vertices || stride < 3)
+        return GDT_ERR_INVALID;
+    DevGuard g(c->device);
+    const int off[2] = {0, n};
+    int rc = voronoi_sets(c, 1, off, box, 4);
+    if (rc != GDT_OK) return rc;
+    GDT_CHECK(gdt_launch_voronoi_vertices(c->d_vsets.p, 1, n, d_xy, d_nbrs, d_counts, stride,
+                                          d_vertices, (hipStream_t)stream));
+    return GDT_OK;
+}
+
+int gdt_voronoi_forward_general(gdt_context* c, const double* d_xy, int nsets, const int* offsets,
+                                const double* boxes, const int* d_nbrs, const int* d_counts,
+                                int stride, double a0, double p0, int precision, double* d_area,
+                                double* d_perim, double* d_energy, void* stream)
+{
+    return voronoi_forward(c, d_xy, nsets, offsets, boxes, 4, d_nbrs, d_counts, stride, a0, p0,
+                           precision, d_area, d_perim, d_energy, stream);
+}
+
+int gdt_voronoi_adjoint_general(gdt_context* c, const double* d_xy, int nsets, const int* offsets,
+                                const double* boxes, const int* d_nbrs, const int* d_counts,
+                                int stride, double a0, double p0, int precision,
+                                const double* d_area, const double* d_perim,
+                                const double* d_gscale, double* d_grad, void* stream)
+{
+    return voronoi_adjoint(c, d_xy, nsets, offsets, boxes, 4, d_nbrs, d_counts, stride, a0, p0,
                            precision, d_area, d_perim, d_gscale, d_grad, stream);
 }
 

@@ -18,8 +18,9 @@ struct GdtTriCircle {
     double R2, lim;      // squared radius and the conservative test bound
 };
 
-// Minimum images use the inverse box sides (ilx, ily): neighbours are far
-// closer than half a box, so the rounded quotient picks the same image.  The
+// Minimum images use the inverse box sides (ilx, ily; general boxes: the
+// rounded unit-cell coordinates): neighbours are far closer than half a box,
+// so the rounded quotient picks the same image.  The
 // centre uses the hardware reciprocal with one Newton step (relative error
 // ~1e-15, far inside the test's 1e-9 margin).
 GDT_HD inline GdtTriCircle tri_circle(const GdtSet& S, double ilx, double ily,
@@ -28,10 +29,15 @@ GDT_HD inline GdtTriCircle tri_circle(const GdtSet& S, double ilx, double ily,
     GdtTriCircle c;
     const gdt_d2 v = xy[i], pa = xy[a], pb = xy[b];
     double ax = pa.x - v.x, ay = pa.y - v.y, bx = pb.x - v.x, by = pb.y - v.y;
-    ax -= S.lx * rint(ax * ilx);
-    ay -= S.ly * rint(ay * ily);
-    bx -= S.lx * rint(bx * ilx);
-    by -= S.ly * rint(by * ily);
+    if (S.general) {
+        gdt_min_image(S, ax, ay);
+        gdt_min_image(S, bx, by);
+    } else {
+        ax -= S.lx * rint(ax * ilx);
+        ay -= S.ly * rint(ay * ily);
+        bx -= S.lx * rint(bx * ilx);
+        by -= S.ly * rint(by * ily);
+    }
     const double D = ax * by - ay * bx;
     c.bad = !(D > 0.);
     c.qx = c.qy = c.R2 = c.lim = 0.;
@@ -59,8 +65,8 @@ GDT_HD inline int tri_scan_global(const GdtSet& S, const gdt_d2* __restrict__ xy
     int qcx, qcy;
     double oqx, oqy;
     gdt_bin_point(S, v.x + c.qx, v.y + c.qy, qcx, qcy, oqx, oqy);
-    const double covb = GDT_FMIN(GDT_FMIN(oqx, S.bsx - oqx), GDT_FMIN(oqy, S.bsy - oqy));
-    const double bsmin = GDT_FMIN(S.bsx, S.bsy);
+    const double covb = gdt_cover0(S, oqx, oqy);
+    const double bsmin = S.hmin;
     int intruder = -1;
     covered = false;
     for (int r = 0; r <= S.rmax && intruder < 0; ++r) {
@@ -71,7 +77,7 @@ GDT_HD inline int tri_scan_global(const GdtSet& S, const gdt_d2* __restrict__ xy
             const int x = gdt_wrap(qcx + dx, S.xsize), y = gdt_wrap(qcy + dy, S.ysize);
             const int key = S.c0 + gdt_cell_key(S, x, y);
             const uint32_t cb = cstart[key], ce = cstart[key + 1];
-            const double ebx = gdt_base64(dx, S.bsx, oqx), eby = gdt_base64(dy, S.bsy, oqy);
+            const double ebx = gdt_cbase_x(S, dx, dy, oqx), eby = gdt_cbase_y(S, dx, dy, oqy);
             for (uint32_t k = cb; k < ce; ++k) {
                 const gdt_d2 o = off64[k];
                 const double ex = gdt_disp64(ebx, o.x), ey = gdt_disp64(eby, o.y);

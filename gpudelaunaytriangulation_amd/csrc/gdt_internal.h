@@ -14,6 +14,7 @@
 #pragma once
 
 #include <stdint.h>
+#include <string.h>
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define GDT_HD __host__ __device__
@@ -43,7 +44,7 @@ enum {
     ST_ERRFLAGS = 1,     // bit 0: search radius exhausted, bit 1: ring did not close, bit 2: no neighbour
     ST_NFLAGGED = 2,     // number of points flagged by the circumcircle test
     ST_NTRI = 3,         // number of triangles enumerated
-    ST_NDEFER = 4,       // points the tile kernel deferred to the wavefront kernel
+    ST_NDEFER = 4,       // points deferred to the wavefront kernel (tile or per-point kernels)
     ST_NDEFER_TILE = 5,  // deferred points a tile block finished itself with a wavefront
     ST_WORDS = 8
 };
@@ -73,6 +74,19 @@ struct GdtSet {
     float bsmax32, bsmin32;
     float ibsx32, ibsy32;
     float pad1, pad2;
+    // Cell geometry of every box shape.  Rectangles: ex = (bsx, 0),
+    // ey = (0, bsy).  General boxes (general = 1; periodicBoundaries::setGeneral,
+    // periodicBoundaries.h:86-101): the cells are parallelograms spanned by
+    // ex = a/xsize and ey = b/ysize, (a, b) the Lagrange-Gauss reduced basis
+    // of the box's lattice (the same torus, its shortest and most orthogonal
+    // basis), and bsx, bsy hold the cell heights.
+    int general, pad3;
+    double ex_x, ex_y, ey_x, ey_y;   // cell edge vectors
+    double rxx, rxy, ryx, ryy;       // inverse of [ex ey]: cell coordinates of a displacement
+    double nrx, nry;                 // norms of its rows: 1 / cell height
+    double hmin;                     // smaller cell height
+    double lat[4], ilat[4];          // reduced basis (a.x, b.x, a.y, b.y) and its inverse
+    double box[4], ibox[4];          // the caller's box matrix and its inverse (wrapping)
 };
 
 struct GdtChunk { int set; int start; };   // a block's work: set id and first local item
@@ -153,12 +167,37 @@ GDT_HD inline int gdt_set_of(const GdtSet* sets, int nsets, int p)
     return lo;
 }
 
-// Wrap a point into [0,lx) x [0,ly), find its cell and its FP64 offset inside
-// the cell.  Every kernel that needs a point's cell calls this one routine, so
-// all of them agree bit for bit.
+// Wrap a point into the box (rectangles: [0,lx) x [0,ly); general boxes: the
+// unit cell of the reduced basis), find its cell and its FP64 offset inside
+// the cell (general boxes: relative to the cell's corner cx*ex + cy*ey).
+// Every kernel that needs a point's cell calls this one routine, so all of
+// them agree bit for bit.
 GDT_HD inline void gdt_bin_point(const GdtSet& S, double x, double y, int& cx, int& cy,
                                  double& ox, double& oy)
 {
+    if (S.general) {
+        double u = S.ilat[0] * x + S.ilat[1] * y, v = S.ilat[2] * x + S.ilat[3] * y;
+        if (!(u >= 0. && u < 1. && v >= 0. && v < 1.)) {
+            if (!(fabs(u) < 1e15 && fabs(v) < 1e15)) {
+                x = y = u = v = 0.;
+            } else {
+                const double ku = floor(u), kv = floor(v);
+                x -= ku * S.lat[0] + kv * S.lat[1];
+                y -= ku * S.lat[2] + kv * S.lat[3];
+                u -= ku;
+                v -= kv;
+            }
+        }
+        cx = (int)floor(u * (double)S.xsize);
+        cy = (int)floor(v * (double)S.ysize);
+        if (cx > S.xsize - 1) cx = S.xsize - 1;
+        if (cy > S.ysize - 1) cy = S.ysize - 1;
+        if (cx < 0) cx = 0;
+        if (cy < 0) cy = 0;
+        ox = x - ((double)cx * S.ex_x + (double)cy * S.ey_x);
+        oy = y - ((double)cx * S.ex_y + (double)cy * S.ey_y);
+        return;
+    }
     if (!(x >= 0. && x < S.lx)) {
         x -= floor(x / S.lx) * S.lx;
         if (x >= S.lx) x -= S.lx;
@@ -179,6 +218,52 @@ GDT_HD inline void gdt_bin_point(const GdtSet& S, double x, double y, int& cx, i
     oy = y - (double)cy * S.bsy;
 }
 
+// Displacement of the corner of the cell (dx, dy) cells away minus the offset
+// o of a point in the home cell: the candidate at offset p in that cell lies
+// at base + p.  One fixed operation order for every box shape: for
+// rectangles (ey_x = ex_y = 0) the fused term adds an exact zero, so this is
+// dx*bsx - o.x, dy*bsy - o.y bit for bit.
+GDT_HD inline double gdt_cbase_x(const GdtSet& S, int dx, int dy, double ox)
+{
+    return fma((double)dy, S.ey_x, (double)dx * S.ex_x) - ox;
+}
+GDT_HD inline double gdt_cbase_y(const GdtSet& S, int dx, int dy, double oy)
+{
+    return fma((double)dx, S.ex_y, (double)dy * S.ey_y) - oy;
+}
+
+// Distance from a point at offset (ox, oy) in its cell to the cell's walls:
+// after the Chebyshev rings 0..r around the cell every point closer than
+// r * hmin + this distance has been seen.
+GDT_HD inline double gdt_cover0(const GdtSet& S, double ox, double oy)
+{
+    if (!S.general) {
+        const double a = ox < S.bsx - ox ? ox : S.bsx - ox;
+        const double b = oy < S.bsy - oy ? oy : S.bsy - oy;
+        return a < b ? a : b;
+    }
+    const double fx = S.rxx * ox + S.rxy * oy, fy = S.ryx * ox + S.ryy * oy;
+    const double a = (fx < 1. - fx ? fx : 1. - fx) * S.bsx;
+    const double b = (fy < 1. - fy ? fy : 1. - fy) * S.bsy;
+    return a < b ? a : b;
+}
+
+// Minimum image of a displacement between two nearby points (much closer than
+// half a box thickness): rectangles by the rounded quotient of each side,
+// general boxes by the rounded unit-cell coordinates in the reduced basis.
+GDT_HD inline void gdt_min_image(const GdtSet& S, double& x, double& y)
+{
+    if (!S.general) {
+        x -= S.lx * rint(x * (1.0 / S.lx));
+        y -= S.ly * rint(y * (1.0 / S.ly));
+        return;
+    }
+    const double ku = rint(S.ilat[0] * x + S.ilat[1] * y);
+    const double kv = rint(S.ilat[2] * x + S.ilat[3] * y);
+    x -= ku * S.lat[0] + kv * S.lat[1];
+    y -= ku * S.lat[2] + kv * S.lat[3];
+}
+
 // Cell grid of a box (cellListGPU::setGridSize, cellListGPU.cpp:97-117): the
 // count per axis is floor(L/a) rounded up to even; sides are L/count per axis.
 GDT_HD inline void gdt_grid(double lx, double ly, double a, int& xsize, int& ysize)
@@ -193,11 +278,12 @@ GDT_HD inline void gdt_grid(double lx, double ly, double a, int& xsize, int& ysi
 
 // Geometry of one set; c0 is its first cell key.  rmax keeps every searched
 // cell distinct on the torus (2*rmax+1 <= cells per axis).
-// tshift: tiles of 2^tshift cells a side (GDT_TSHIFT_MIN..GDT_TSHIFT_MAX).
+// tshift: tiles of 2^tshift x 2^tshift cells (GDT_TSHIFT_MIN..GDT_TSHIFT_MAX).
 inline GdtSet gdt_make_set(int p0, int n, int c0, double lx, double ly, double cell_size, int tiled,
                            int tshift = GDT_TILE_SHIFT)
 {
     GdtSet s;
+    memset(&s, 0, sizeof(s));
     s.p0 = p0; s.n = n; s.c0 = c0;
     gdt_grid(lx, ly, cell_size, s.xsize, s.ysize);
     s.tiled = tiled == 2 ? 2 : (tiled ? 1 : 0);
@@ -218,7 +304,101 @@ inline GdtSet gdt_make_set(int p0, int n, int c0, double lx, double ly, double c
     s.pad1 = s.pad2 = 0.f;
     s.bsmax32 = (float)(s.bsx > s.bsy ? s.bsx : s.bsy);
     s.bsmin32 = (float)(s.bsx < s.bsy ? s.bsx : s.bsy);
+    s.general = 0;
+    s.ex_x = s.bsx; s.ey_y = s.bsy;
+    s.rxx = s.ibsx; s.ryy = s.ibsy;
+    s.nrx = s.ibsx; s.nry = s.ibsy;
+    s.hmin = s.bsx < s.bsy ? s.bsx : s.bsy;
+    s.lat[0] = s.box[0] = lx; s.lat[3] = s.box[3] = ly;
+    s.ilat[0] = s.ibox[0] = 1.0 / lx; s.ilat[3] = s.ibox[3] = 1.0 / ly;
     return s;
+}
+
+// Whether a box matrix {x11, x12, x21, x22} is a rectangle with positive sides.
+inline bool gdt_box_is_rect(const double* box)
+{
+    return box[1] == 0. && box[2] == 0. && box[0] > 0. && box[3] > 0.;
+}
+
+// Lagrange-Gauss reduction of the lattice basis (a, b): the shortest basis
+// of the same lattice, |a| <= |b| and |a.b| <= |a|^2 / 2, oriented so that
+// a x b > 0.  A sheared box and its reduced form are the same torus.
+inline void gdt_reduce_basis(double& ax, double& ay, double& bx, double& by)
+{
+    for (int it = 0; it < 200; ++it) {
+        if (ax * ax + ay * ay > bx * bx + by * by) {
+            const double tx = ax, ty = ay;
+            ax = bx; ay = by;
+            bx = tx; by = ty;
+        }
+        const double mu = rint((ax * bx + ay * by) / (ax * ax + ay * ay));
+        if (mu == 0.) break;
+        bx -= mu * ax;
+        by -= mu * ay;
+    }
+    if (ax * by - ay * bx < 0.) {
+        bx = -bx;
+        by = -by;
+    }
+}
+
+// A set in a general box {x11, x12, x21, x22} (lattice vectors as columns),
+// cells binned in the unit-cell coordinates of the reduced basis: cell
+// counts from the thicknesses T of the unit cell across each pair of walls
+// (the rectangle's side lengths for a rectangle), by the rule of gdt_grid.
+// Row-major cell order (the LDS tile kernels handle rectangles only).
+// A rectangular box gives gdt_make_set's set.
+inline GdtSet gdt_make_set_box(int p0, int n, int c0, const double* box, double cell_size,
+                               int tiled, int tshift = GDT_TILE_SHIFT)
+{
+    if (gdt_box_is_rect(box)) return gdt_make_set(p0, n, c0, box[0], box[3], cell_size, tiled, tshift);
+    double ax = box[0], ay = box[2], bx = box[1], by = box[3];
+    gdt_reduce_basis(ax, ay, bx, by);
+    const double det = ax * by - ay * bx;
+    const double tx = det / sqrt(bx * bx + by * by), ty = det / sqrt(ax * ax + ay * ay);
+    GdtSet s = gdt_make_set(p0, n, c0, tx, ty, cell_size, 0, tshift);
+    s.general = 1;
+    s.lat[0] = ax; s.lat[1] = bx; s.lat[2] = ay; s.lat[3] = by;
+    s.ilat[0] = by / det; s.ilat[1] = -bx / det; s.ilat[2] = -ay / det; s.ilat[3] = ax / det;
+    const double fx = (double)s.xsize, fy = (double)s.ysize;
+    s.ex_x = ax / fx; s.ex_y = ay / fx;
+    s.ey_x = bx / fy; s.ey_y = by / fy;
+    s.rxx = fx * s.ilat[0]; s.rxy = fx * s.ilat[1];
+    s.ryx = fy * s.ilat[2]; s.ryy = fy * s.ilat[3];
+    s.nrx = sqrt(s.rxx * s.rxx + s.rxy * s.rxy);
+    s.nry = sqrt(s.ryx * s.ryx + s.ryy * s.ryy);
+    s.bsx = tx / fx;            // cell heights
+    s.bsy = ty / fy;
+    s.ibsx = s.nrx;
+    s.ibsy = s.nry;
+    s.hmin = s.bsx < s.bsy ? s.bsx : s.bsy;
+    for (int k = 0; k < 4; ++k) s.box[k] = box[k];
+    const double bd = box[0] * box[3] - box[1] * box[2];
+    s.ibox[0] = box[3] / bd; s.ibox[1] = -box[1] / bd;
+    s.ibox[2] = -box[2] / bd; s.ibox[3] = box[0] / bd;
+    return s;
+}
+
+// Wrap a moved point into the caller's box (periodicBoundaries::putInBoxReal,
+// periodicBoundaries.h:131-152): rectangles per axis into [0, L), general
+// boxes by whole lattice vectors of the caller's basis into its unit cell.
+GDT_HD inline void gdt_wrap_into_box(const GdtSet& S, double& x, double& y)
+{
+    if (!S.general) {
+        x = x - floor(x / S.lx) * S.lx;
+        if (x >= S.lx) x -= S.lx;
+        if (x < 0.0) x = 0.0;
+        y = y - floor(y / S.ly) * S.ly;
+        if (y >= S.ly) y -= S.ly;
+        if (y < 0.0) y = 0.0;
+        return;
+    }
+    const double ku = floor(S.ibox[0] * x + S.ibox[1] * y);
+    const double kv = floor(S.ibox[2] * x + S.ibox[3] * y);
+    if (fabs(ku) < 1e15 && fabs(kv) < 1e15) {
+        x -= ku * S.box[0] + kv * S.box[1];
+        y -= ku * S.box[2] + kv * S.box[3];
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -66,12 +66,14 @@ hipError_t gdt_export_cells(const GdtSet* d_sets, const GdtSet& h_set, const Gdt
 hipError_t gdt_max_occupancy(const uint32_t* ccount, int total_keys, int* d_out, hipStream_t s);
 
 // One-ring of every point (chunks of `block` points) or of the worklist slots
-// [skip, *worklen) (max_work bounds *worklen).
+// [skip, *worklen) (max_work bounds *worklen).  With `defer`, points whose
+// search the grid's search radius cannot certify are appended there (count
+// in status[ST_NDEFER], zero on entry) for gdt_launch_deferred.
 hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
                                const GdtChunk* chunks, int nch, int block, const int* worklist,
                                const uint32_t* worklen, int skip, int max_work,
                                const GdtRingIO& io, int* counts, int* status,
-                               unsigned long long* counters, hipStream_t s);
+                               unsigned long long* counters, int* defer, hipStream_t s);
 
 // Deferred points: the first GDT_WAVE_CAP one wavefront each (GDT_WAVE_BLOCKS
 // blocks of 256 threads), the rest one lane each.

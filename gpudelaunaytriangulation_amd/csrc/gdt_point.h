@@ -94,8 +94,9 @@ GDT_HD inline void gdt_disp64_of(const GdtSet& S, const gdt_d2& ov, const gdt_d2
                                  int dxy, double& x, double& y)
 {
     const gdt_d2 o = off64[k];
-    x = gdt_disp64(gdt_base64(gdt_dx_of(dxy), S.bsx, ov.x), o.x);
-    y = gdt_disp64(gdt_base64(gdt_dy_of(dxy), S.bsy, ov.y), o.y);
+    const int dx = gdt_dx_of(dxy), dy = gdt_dy_of(dxy);
+    x = gdt_disp64(gdt_cbase_x(S, dx, dy, ov.x), o.x);
+    y = gdt_disp64(gdt_cbase_y(S, dx, dy, ov.y), o.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -107,8 +108,8 @@ GDT_HD inline GdtRingResult gdt_one_ring_f64(const GdtSet& S, int j, const GdtRi
     const uint32_t pc = io.scell[j];
     const int cx = (int)(pc & 0xffffu), cy = (int)(pc >> 16);
     const gdt_d2 ov = io.off64[j];
-    const double covb = GDT_FMIN(GDT_FMIN(ov.x, S.bsx - ov.x), GDT_FMIN(ov.y, S.bsy - ov.y));
-    const double bsmin = GDT_FMIN(S.bsx, S.bsy);
+    const double covb = gdt_cover0(S, ov.x, ov.y);
+    const double bsmin = S.hmin;
     const size_t row = (size_t)io.perm[j] * (size_t)io.stride;
     unsigned npred = 0;
 
@@ -116,7 +117,7 @@ GDT_HD inline GdtRingResult gdt_one_ring_f64(const GdtSet& S, int j, const GdtRi
     int nn = -1;
     double nd2 = 0., nx = 0., ny = 0.;
     auto nn_cell = [&](int dx, int dy, int, uint32_t b, uint32_t e) {
-        const double bx = gdt_base64(dx, S.bsx, ov.x), by = gdt_base64(dy, S.bsy, ov.y);
+        const double bx = gdt_cbase_x(S, dx, dy, ov.x), by = gdt_cbase_y(S, dx, dy, ov.y);
         for (uint32_t k = b; k < e; ++k) {
             if ((int)k == j) continue;
             const gdt_d2 o = io.off64[k];
@@ -150,7 +151,7 @@ GDT_HD inline GdtRingResult gdt_one_ring_f64(const GdtSet& S, int j, const GdtRi
         double bx_ = 0., by_ = 0.;
         Circ64 K = {0., 0., 0.};
         auto sc_cell = [&](int dx, int dy, int, uint32_t b, uint32_t e) {
-            const double bx = gdt_base64(dx, S.bsx, ov.x), by = gdt_base64(dy, S.bsy, ov.y);
+            const double bx = gdt_cbase_x(S, dx, dy, ov.x), by = gdt_cbase_y(S, dx, dy, ov.y);
             for (uint32_t k = b; k < e; ++k) {
                 if ((int)k == j || (int)k == a) continue;
                 const gdt_d2 o = io.off64[k];
@@ -442,4 +443,31 @@ GDT_HD inline void gdt_circle_cells(T qxlo, T qxhi, T qylo, T qyhi, T R, T ox, T
     dx1 = gdt_cell_floor((double)((ox + (qxhi + R)) * ibsx + margin));
     dy0 = gdt_cell_floor((double)((oy + (qylo - R)) * ibsy - margin));
     dy1 = gdt_cell_floor((double)((oy + (qyhi + R)) * ibsy + margin));
+}
+
+// The same cell box for every box shape: cells (relative to the home cell of
+// a point at offset (ox, oy)) that a disc of radius R, centred in
+// [qxlo,qxhi] x [qylo,qyhi] relative to the point, can touch, with `margin`
+// cells of slack.  General boxes: the cell coordinates of o + q + w are
+// R(o + q) + Rw, and |(Rw)_x| <= |w| * (norm of R's first row).
+GDT_HD inline void gdt_disc_cells(const GdtSet& S, double qxlo, double qxhi, double qylo,
+                                  double qyhi, double R, double ox, double oy, double margin,
+                                  int& dx0, int& dx1, int& dy0, int& dy1)
+{
+    if (!S.general) {
+        gdt_circle_cells<double>(qxlo, qxhi, qylo, qyhi, R, ox, oy, S.ibsx, S.ibsy, margin, dx0,
+                                 dx1, dy0, dy1);
+        return;
+    }
+    const double fx = S.rxx * ox + S.rxy * oy, fy = S.ryx * ox + S.ryy * oy;
+    const double ax0 = S.rxx * qxlo, ax1 = S.rxx * qxhi, ay0 = S.rxy * qylo, ay1 = S.rxy * qyhi;
+    const double bx0 = S.ryx * qxlo, bx1 = S.ryx * qxhi, by0 = S.ryy * qylo, by1 = S.ryy * qyhi;
+    const double xlo = fx + GDT_FMIN(ax0, ax1) + GDT_FMIN(ay0, ay1) - R * S.nrx;
+    const double xhi = fx + GDT_FMAX(ax0, ax1) + GDT_FMAX(ay0, ay1) + R * S.nrx;
+    const double ylo = fy + GDT_FMIN(bx0, bx1) + GDT_FMIN(by0, by1) - R * S.nry;
+    const double yhi = fy + GDT_FMAX(bx0, bx1) + GDT_FMAX(by0, by1) + R * S.nry;
+    dx0 = gdt_cell_floor(xlo - margin);
+    dx1 = gdt_cell_floor(xhi + margin);
+    dy0 = gdt_cell_floor(ylo - margin);
+    dy1 = gdt_cell_floor(yhi + margin);
 }

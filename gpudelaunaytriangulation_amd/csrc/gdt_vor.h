@@ -13,12 +13,17 @@ struct Disp {
     double x, y;
 };
 
-// Minimum-image displacement x_j - x_i (points inside the box).
+// Minimum-image displacement x_j - x_i (points inside the box; general boxes:
+// neighbours much closer than half a box thickness).
 GDT_HD inline Disp vor_disp(const GdtSet& S, const double* __restrict__ xy, int i, int j)
 {
     Disp d;
     d.x = xy[2 * (size_t)j] - xy[2 * (size_t)i];
     d.y = xy[2 * (size_t)j + 1] - xy[2 * (size_t)i + 1];
+    if (S.general) {
+        gdt_min_image(S, d.x, d.y);
+        return d;
+    }
     const double hx = 0.5 * S.lx, hy = 0.5 * S.ly;
     if (d.x > hx) d.x -= S.lx;
     else if (d.x < -hx) d.x += S.lx;

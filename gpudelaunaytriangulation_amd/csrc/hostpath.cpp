@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <new>
 #include <vector>
 
@@ -51,18 +52,23 @@ struct HostCells {
 
 int nthreads_of(int nthreads) { return nthreads > 0 ? nthreads : omp_get_max_threads(); }
 
-bool geometry_ok(int n, double lx, double ly, double a)
+// A usable box {x11, x12, x21, x22}: finite and invertible.
+bool box_ok(const double* b)
 {
-    return n >= 4 && lx > 0. && ly > 0. && a > 0. && std::isfinite(lx) && std::isfinite(ly) &&
-           std::isfinite(a);
+    for (int k = 0; k < 4; ++k)
+        if (!std::isfinite(b[k])) return false;
+    const double det = b[0] * b[3] - b[1] * b[2];
+    return std::isfinite(det) && std::fabs(det) > 0.;
 }
 
-// Same checks and the same binning as the device build (gdt_make_set,
+bool rect_ok(double lx, double ly) { return lx > 0. && ly > 0.; }
+
+// Same checks and the same binning as the device build (gdt_make_set_box,
 // gdt_bin_point), so both modes see the same cells.
-int build_cells(HostCells& h, const double* xy, int n, double lx, double ly, double a, int nt)
+int build_cells(HostCells& h, const double* xy, int n, const double* box, double a, int nt)
 {
-    if (!geometry_ok(n, lx, ly, a)) return GDT_ERR_INVALID;
-    h.S = gdt_make_set(0, n, 0, lx, ly, a, 0);
+    if (n < 4 || !box_ok(box) || !(a > 0.) || !std::isfinite(a)) return GDT_ERR_INVALID;
+    h.S = gdt_make_set_box(0, n, 0, box, a, 0);
     const GdtSet& S = h.S;
     if (S.xsize > 65535 || S.ysize > 65535 || S.xsize < 3 || S.ysize < 3) return GDT_ERR_INVALID;
     const int K = S.nkeys;
@@ -120,6 +126,7 @@ int rings_of(const HostCells& h, const int* slots, int nslots, int* nbrs, int* c
              int precision, int nt, int* maxc)
 {
     const GdtRingIO io = h.io(nbrs, stride);
+    if (h.S.general) precision = GDT_PREC_FP64;   // the FP32 filter's bounds are for rectangles
     int mx = 0, err = 0;
 #pragma omp parallel for num_threads(nt) schedule(dynamic, 256) reduction(max : mx) reduction(| : err)
     for (int t = 0; t < nslots; ++t) {
@@ -142,13 +149,13 @@ int rings_of(const HostCells& h, const int* slots, int nslots, int* nbrs, int* c
 
 extern "C" {
 
-int gdt_host_celllist(const double* xy, int n, double lx, double ly, double cell_size,
-                      unsigned int* cell_sizes, int* idxs, int nmax, int* xsize, int* ysize,
-                      int* max_occupancy)
+int gdt_host_celllist_general(const double* xy, int n, const double* box, double cell_size,
+                              unsigned int* cell_sizes, int* idxs, int nmax, int* xsize,
+                              int* ysize, int* max_occupancy)
 {
-    if (!xy || (cell_sizes && (!idxs || nmax < 1))) return GDT_ERR_INVALID;
+    if (!xy || !box || (cell_sizes && (!idxs || nmax < 1))) return GDT_ERR_INVALID;
     HostCells h;
-    const int rc = build_cells(h, xy, n, lx, ly, cell_size, nthreads_of(0));
+    const int rc = build_cells(h, xy, n, box, cell_size, nthreads_of(0));
     if (rc != GDT_OK) return rc;
     const GdtSet& S = h.S;
     int occ = 0;
@@ -170,15 +177,25 @@ int gdt_host_celllist(const double* xy, int n, double lx, double ly, double cell
     return GDT_OK;
 }
 
-int gdt_host_triangulate(const double* xy, int n, double lx, double ly, double cell_size,
-                         int stride, int* nbrs, int* counts, int precision, int nthreads,
-                         int* required)
+int gdt_host_celllist(const double* xy, int n, double lx, double ly, double cell_size,
+                      unsigned int* cell_sizes, int* idxs, int nmax, int* xsize, int* ysize,
+                      int* max_occupancy)
 {
-    if (!xy || !nbrs || !counts || stride < 3) return GDT_ERR_INVALID;
+    if (!rect_ok(lx, ly)) return GDT_ERR_INVALID;
+    const double box[4] = {lx, 0., 0., ly};
+    return gdt_host_celllist_general(xy, n, box, cell_size, cell_sizes, idxs, nmax, xsize, ysize,
+                                     max_occupancy);
+}
+
+int gdt_host_triangulate_general(const double* xy, int n, const double* box, double cell_size,
+                                 int stride, int* nbrs, int* counts, int precision, int nthreads,
+                                 int* required)
+{
+    if (!xy || !box || !nbrs || !counts || stride < 3) return GDT_ERR_INVALID;
     if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
     const int nt = nthreads_of(nthreads);
     HostCells h;
-    int rc = build_cells(h, xy, n, lx, ly, cell_size, nt);
+    int rc = build_cells(h, xy, n, box, cell_size, nt);
     if (rc != GDT_OK) return rc;
     int mx = 0;
     const int err = rings_of(h, nullptr, n, nbrs, counts, stride, precision, nt, &mx);
@@ -186,18 +203,28 @@ int gdt_host_triangulate(const double* xy, int n, double lx, double ly, double c
     return status_of(err, mx, stride);
 }
 
-int gdt_host_test_and_repair(const double* xy, int n, double lx, double ly, double cell_size,
-                             int stride, int* nbrs, int* counts, int precision, int nthreads,
-                             int* nflagged, int* required)
+int gdt_host_triangulate(const double* xy, int n, double lx, double ly, double cell_size,
+                         int stride, int* nbrs, int* counts, int precision, int nthreads,
+                         int* required)
 {
-    if (!xy || !nbrs || !counts || stride < 3) return GDT_ERR_INVALID;
+    if (!rect_ok(lx, ly)) return GDT_ERR_INVALID;
+    const double box[4] = {lx, 0., 0., ly};
+    return gdt_host_triangulate_general(xy, n, box, cell_size, stride, nbrs, counts, precision,
+                                        nthreads, required);
+}
+
+int gdt_host_test_and_repair_general(const double* xy, int n, const double* box, double cell_size,
+                                     int stride, int* nbrs, int* counts, int precision,
+                                     int nthreads, int* nflagged, int* required)
+{
+    if (!xy || !box || !nbrs || !counts || stride < 3) return GDT_ERR_INVALID;
     if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
     const int nt = nthreads_of(nthreads);
     if (nflagged) *nflagged = 0;
     if (required) *required = 0;
     // 1. cell list of the moved points (DelaunayGPU.cpp:580-586)
     HostCells h;
-    int rc = build_cells(h, xy, n, lx, ly, cell_size, nt);
+    int rc = build_cells(h, xy, n, box, cell_size, nt);
     if (rc != GDT_OK) return rc;
     const GdtSet& S = h.S;
     const gdt_d2* p = reinterpret_cast<const gdt_d2*>(xy);
@@ -263,15 +290,30 @@ int gdt_host_test_and_repair(const double* xy, int n, double lx, double ly, doub
     return status_of(err, mx, stride);
 }
 
-int gdt_host_voronoi_vertices(const double* xy, int n, double lx, double ly, const int* nbrs,
-                              const int* counts, int stride, double* vertices, int nthreads)
+int gdt_host_test_and_repair(const double* xy, int n, double lx, double ly, double cell_size,
+                             int stride, int* nbrs, int* counts, int precision, int nthreads,
+                             int* nflagged, int* required)
 {
-    if (!xy || !nbrs || !counts || !vertices || stride < 3 || n < 3 || !(lx > 0.) || !(ly > 0.))
+    if (!rect_ok(lx, ly)) return GDT_ERR_INVALID;
+    const double box[4] = {lx, 0., 0., ly};
+    return gdt_host_test_and_repair_general(xy, n, box, cell_size, stride, nbrs, counts, precision,
+                                            nthreads, nflagged, required);
+}
+
+int gdt_host_voronoi_vertices_general(const double* xy, int n, const double* box,
+                                      const int* nbrs, const int* counts, int stride,
+                                      double* vertices, int nthreads)
+{
+    if (!xy || !box || !nbrs || !counts || !vertices || stride < 3 || n < 3 || !box_ok(box))
         return GDT_ERR_INVALID;
     using namespace gdt_vor;
     GdtSet S{};
-    S.lx = lx;
-    S.ly = ly;
+    if (gdt_box_is_rect(box)) {
+        S.lx = box[0];
+        S.ly = box[3];
+    } else {
+        S = gdt_make_set_box(0, n, 0, box, std::numeric_limits<double>::infinity(), 0);
+    }
     const int nt = nthreads_of(nthreads);
 #pragma omp parallel for num_threads(nt) schedule(static)
     for (int i = 0; i < n; ++i) {
@@ -293,6 +335,14 @@ int gdt_host_voronoi_vertices(const double* xy, int n, double lx, double ly, con
         }
     }
     return GDT_OK;
+}
+
+int gdt_host_voronoi_vertices(const double* xy, int n, double lx, double ly, const int* nbrs,
+                              const int* counts, int stride, double* vertices, int nthreads)
+{
+    if (!rect_ok(lx, ly)) return GDT_ERR_INVALID;
+    const double box[4] = {lx, 0., 0., ly};
+    return gdt_host_voronoi_vertices_general(xy, n, box, nbrs, counts, stride, vertices, nthreads);
 }
 
 }  // extern "C"

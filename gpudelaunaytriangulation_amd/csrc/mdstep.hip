@@ -30,8 +30,10 @@ __global__ __launch_bounds__(GDT_BLOCK) void k_md_displace(const GdtSet* __restr
     double g0, g1;
     gdt_gauss_pair(key, (uint64_t)li, g0, g1);
     const size_t i = (size_t)(S.p0 + li);
-    xy[2 * i] = gdt_wrap_coord(xy[2 * i] + sigma * g0, S.lx);
-    xy[2 * i + 1] = gdt_wrap_coord(xy[2 * i + 1] + sigma * g1, S.ly);
+    double x = xy[2 * i] + sigma * g0, y = xy[2 * i + 1] + sigma * g1;
+    gdt_wrap_into_box(S, x, y);
+    xy[2 * i] = x;
+    xy[2 * i + 1] = y;
 }
 
 // End of a step (one thread): fold the step's status into the sticky words,

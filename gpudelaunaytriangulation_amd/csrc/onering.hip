@@ -13,15 +13,26 @@
 
 #include <type_traits>
 
+// A point whose search the grid's search radius cannot certify (boxes of a
+// few cells) is appended to `defer` (when given) for the wavefront kernel,
+// whose searches may stop at the cell box of their circle; its row is then
+// rewritten there.
 template <int PREC>
 __device__ inline void one_ring_lane(bool active, const GdtSet& S, int j, const GdtRingIO& io,
                                      int* __restrict__ counts, int* __restrict__ status,
-                                     unsigned long long* __restrict__ counters)
+                                     unsigned long long* __restrict__ counters,
+                                     int* __restrict__ defer)
 {
     GdtRingResult r = {0, 0, 0u, 0u};
     if (active) {
         r = (PREC == 32) ? gdt_one_ring_f32(S, j, io) : gdt_one_ring_f64(S, j, io);
-        counts[io.perm[j]] = r.count;
+        if (defer && (r.err & ERR_SEARCH)) {
+            defer[atomicAdd(reinterpret_cast<unsigned*>(status + ST_NDEFER), 1u)] = j;
+            r.count = 0;
+            r.err = 0;
+        } else {
+            counts[io.perm[j]] = r.count;
+        }
     }
     // every lane of the wave reaches the reductions below
     const int mx = wave_max_i32(r.count);
@@ -44,13 +55,14 @@ __global__ __launch_bounds__(1024) void k_one_ring(const GdtSet* __restrict__ se
                                                    const GdtChunk* __restrict__ chunks,
                                                    GdtRingIO io, int* __restrict__ counts,
                                                    int* __restrict__ status,
-                                                   unsigned long long* __restrict__ counters)
+                                                   unsigned long long* __restrict__ counters,
+                                                   int* __restrict__ defer)
 {
     const GdtChunk ch = chunks[blockIdx.x];
     const GdtSet S = sets[ch.set];
     const int li = ch.start + (int)threadIdx.x;
     const bool active = li < S.n;
-    one_ring_lane<PREC>(active, S, S.p0 + li, io, counts, status, counters);
+    one_ring_lane<PREC>(active, S, S.p0 + li, io, counts, status, counters, defer);
 }
 
 // Worklist of slots (count in device memory): grid-stride, set found per lane.
@@ -62,7 +74,8 @@ __global__ __launch_bounds__(1024) void k_one_ring_list(const GdtSet* __restrict
                                                         int skip, GdtRingIO io,
                                                         int* __restrict__ counts,
                                                         int* __restrict__ status,
-                                                        unsigned long long* __restrict__ counters)
+                                                        unsigned long long* __restrict__ counters,
+                                                        int* __restrict__ defer)
 {
     const int len = max((int)*worklen - skip, 0);
     worklist += skip;
@@ -79,7 +92,7 @@ __global__ __launch_bounds__(1024) void k_one_ring_list(const GdtSet* __restrict
             s = gdt_set_of(sets, nsets, j);
         }
         const GdtSet S = sets[s];
-        one_ring_lane<PREC>(active, S, j, io, counts, status, counters);
+        one_ring_lane<PREC>(active, S, j, io, counts, status, counters, defer);
     }
 }
 
@@ -137,23 +150,23 @@ hipError_t gdt_launch_deferred(int precision, const GdtSet* d_sets, int nsets, i
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return gdt_launch_one_ring(precision, d_sets, nsets, nullptr, 0, 256, defer, ndefer,
-                               GDT_WAVE_CAP, total_n, io, counts, status, counters, s);
+                               GDT_WAVE_CAP, total_n, io, counts, status, counters, nullptr, s);
 }
 
 hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
                                const GdtChunk* chunks, int nch, int block, const int* worklist,
                                const uint32_t* worklen, int skip, int max_work,
                                const GdtRingIO& io, int* counts, int* status,
-                               unsigned long long* counters, hipStream_t s)
+                               unsigned long long* counters, int* defer, hipStream_t s)
 {
     if (!worklist) {
         if (nch <= 0) return hipSuccess;
         if (precision == 32)
             hipLaunchKernelGGL(k_one_ring<32>, dim3(nch), dim3(block), 0, s, d_sets, chunks, io,
-                               counts, status, counters);
+                               counts, status, counters, defer);
         else
             hipLaunchKernelGGL(k_one_ring<64>, dim3(nch), dim3(block), 0, s, d_sets, chunks, io,
-                               counts, status, counters);
+                               counts, status, counters, defer);
         return hipGetLastError();
     }
     max_work -= skip;
@@ -162,9 +175,9 @@ hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
     if (grid > 4096) grid = 4096;
     if (precision == 32)
         hipLaunchKernelGGL(k_one_ring_list<32>, dim3(grid), dim3(block), 0, s, d_sets, nsets,
-                           worklist, worklen, skip, io, counts, status, counters);
+                           worklist, worklen, skip, io, counts, status, counters, defer);
     else
         hipLaunchKernelGGL(k_one_ring_list<64>, dim3(grid), dim3(block), 0, s, d_sets, nsets,
-                           worklist, worklen, skip, io, counts, status, counters);
+                           worklist, worklen, skip, io, counts, status, counters, defer);
     return hipGetLastError();
 }

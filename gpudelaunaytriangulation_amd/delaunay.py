@@ -57,37 +57,106 @@ def _dev_ptr(t, dtype, name: str) -> int:
 
 
 class PeriodicBoundaries:
-    """Rectangular periodic box (periodicBoundaries::setSquare, periodicBoundaries.h:75-84)."""
+    """Periodic box (periodicBoundaries, periodicBoundaries.h:25-73): a
+    rectangle (``PeriodicBoundaries(x, y)``, setSquare, 75-84) or a general
+    box (``PeriodicBoundaries(a, b, c, d)``, setGeneral, 86-101) whose matrix
+    ``[[a, b], [c, d]]`` has the lattice vectors as columns."""
 
-    def __init__(self, x: float = 1.0, y: float | None = None):
-        self.setSquare(x, x if y is None else y)
+    def __init__(self, x: float = 1.0, y: float | None = None, c: float | None = None,
+                 d: float | None = None):
+        if c is None and d is None:
+            self.setSquare(x, x if y is None else y)
+        else:
+            self.setGeneral(x, 0.0 if y is None else y, 0.0 if c is None else c,
+                            1.0 if d is None else d)
 
     def setSquare(self, x: float, y: float) -> None:
         if not (x > 0 and y > 0):
             raise ValueError("box sides must be positive")
-        self.x11, self.x22 = float(x), float(y)
+        self.x11, self.x12, self.x21, self.x22 = float(x), 0.0, 0.0, float(y)
+        self._inverse()
+        self.isSquare = True
+
+    def setGeneral(self, a: float, b: float, c: float, d: float) -> None:
+        m = [float(a), float(b), float(c), float(d)]
+        det = m[0] * m[3] - m[1] * m[2]
+        if not (all(math.isfinite(v) for v in m) and math.isfinite(det) and det != 0.0):
+            raise ValueError("the box matrix must be finite and invertible")
+        self.x11, self.x12, self.x21, self.x22 = m
+        self._inverse()
+        self.isSquare = False
+
+    def _inverse(self) -> None:
+        det = self.x11 * self.x22 - self.x12 * self.x21
+        self.xi11, self.xi12 = self.x22 / det, -self.x12 / det
+        self.xi21, self.xi22 = -self.x21 / det, self.x11 / det
 
     def getBoxDims(self):
-        return self.x11, 0.0, 0.0, self.x22
+        return self.x11, self.x12, self.x21, self.x22
+
+    def getBoxInvDims(self):
+        return self.xi11, self.xi12, self.xi21, self.xi22
 
     def isBoxSquare(self) -> bool:
-        return True
+        return self.isSquare
+
+    def is_rectangle(self) -> bool:
+        """Whether the library runs this box on its rectangular path
+        (off-diagonal entries zero, positive sides)."""
+        return self.x12 == 0.0 and self.x21 == 0.0 and self.x11 > 0 and self.x22 > 0
+
+    def box_matrix(self) -> np.ndarray:
+        """(x11, x12, x21, x22) as the general-box C calls take it."""
+        return np.array([self.x11, self.x12, self.x21, self.x22], dtype=np.float64)
+
+    def Trans(self, p):
+        """Unit-cell coordinates to the box (periodicBoundaries.h:103-115)."""
+        p = np.asarray(p, dtype=np.float64)
+        return np.stack([self.x11 * p[..., 0] + self.x12 * p[..., 1],
+                         self.x21 * p[..., 0] + self.x22 * p[..., 1]], -1)
+
+    def invTrans(self, p):
+        """Box to unit-cell coordinates (periodicBoundaries.h:117-129)."""
+        p = np.asarray(p, dtype=np.float64)
+        return np.stack([self.xi11 * p[..., 0] + self.xi12 * p[..., 1],
+                         self.xi21 * p[..., 0] + self.xi22 * p[..., 1]], -1)
 
     def minDist(self, p1, p2):
-        """Nearest-image displacement p1 - p2 (periodicBoundaries.h:154-164)."""
-        d = np.asarray(p1, dtype=np.float64) - np.asarray(p2, dtype=np.float64)
-        L = np.array([self.x11, self.x22])
-        return d - L * np.round(d / L)
+        """Nearest-image displacement p1 - p2 (periodicBoundaries.h:154-179):
+        rectangles per axis, general boxes on unit-cell coordinates wrapped
+        into [-1/2, 1/2]."""
+        if self.isSquare:
+            d = np.asarray(p1, dtype=np.float64) - np.asarray(p2, dtype=np.float64)
+            L = np.array([self.x11, self.x22])
+            return d - L * np.round(d / L)
+        u = self.invTrans(p1) - self.invTrans(p2)
+        u = u - np.round(u)
+        return self.Trans(u)
 
     def putInBoxReal(self, p):
-        """Wrap into [0,Lx) x [0,Ly) (periodicBoundaries.h:131-152)."""
+        """Wrap into the unit cell (periodicBoundaries.h:131-152)."""
         p = np.asarray(p, dtype=np.float64)
-        L = np.array([self.x11, self.x22])
-        q = p - np.floor(p / L) * L
-        return np.where(q >= L, q - L, q)
+        if self.isSquare:
+            L = np.array([self.x11, self.x22])
+            q = p - np.floor(p / L) * L
+            return np.where(q >= L, q - L, q)
+        u = self.invTrans(p)
+        u = u - np.floor(u)
+        u = np.where(u >= 1.0, u - 1.0, u)
+        return self.Trans(u)
+
+    def move(self, p, disp):
+        """p + disp, put back in the box (periodicBoundaries.h:181-189)."""
+        return self.putInBoxReal(np.asarray(p, dtype=np.float64) + np.asarray(disp, dtype=np.float64))
 
 
 PeriodicBoxPtr = PeriodicBoundaries
+
+
+def _box_arg(box) -> ctypes.Array:
+    """A host double array of box matrices (4 doubles per box) for the C calls."""
+    flat = np.asarray(box, dtype=np.float64).reshape(-1)
+    return (ctypes.c_double * len(flat))(*flat.tolist())
 
 
 class MultiProfiler:
@@ -240,9 +309,14 @@ class CellListGPU:
         self.Box = bx
 
     def setGridSize(self, a: float) -> None:
-        """cellListGPU::setGridSize (cellListGPU.cpp:97-117)."""
+        """cellListGPU::setGridSize (cellListGPU.cpp:97-117).  A general box's
+        grid (cells of its reduced lattice basis) is known after compute()."""
         x11, _, _, x22 = self.Box.getBoxDims()
         self.a = float(a)
+        if not self.Box.is_rectangle():
+            self.xsize = self.ysize = 0
+            self.boxsize = 0.0
+            return
         self.xsize = int(math.floor(x11 / a))
         self.xsize += self.xsize % 2
         self.ysize = int(math.floor(x22 / a))
@@ -262,7 +336,10 @@ class CellListGPU:
         return self.boxsize
 
     def positionToCellIndex(self, x: float, y: float) -> int:
-        """cellListGPU.cpp:183-189."""
+        """cellListGPU.cpp:183-189 (rectangular boxes, as the reference's)."""
+        if not self.Box.is_rectangle():
+            raise ValueError("positionToCellIndex is defined for rectangular boxes; a general "
+                             "box's cells follow its reduced lattice basis")
         bsy = self.Box.getBoxDims()[3] / self.ysize
         bx = max(0, min(self.xsize - 1, int(math.floor(x / self.boxsize))))
         by = max(0, min(self.ysize - 1, int(math.floor(y / bsy))))
@@ -291,9 +368,15 @@ class CellListGPU:
         x11, _, _, x22 = self.Box.getBoxDims()
         if self.xsize == 0:
             self.setGridSize(self.a)
-        _lib.check(L.gdt_celllist_build(self.ctx.handle, _dev_ptr(points, torch.float64, "points"),
-                                        n, x11, x22, self.a, _stream_ptr(stream)),
-                   "gdt_celllist_build")
+        ptr = _dev_ptr(points, torch.float64, "points")
+        if self.Box.is_rectangle():
+            _lib.check(L.gdt_celllist_build(self.ctx.handle, ptr, n, x11, x22, self.a,
+                                            _stream_ptr(stream)), "gdt_celllist_build")
+        else:
+            _lib.check(L.gdt_celllist_build_general(self.ctx.handle, ptr, n,
+                                                    _box_arg(self.Box.box_matrix()), self.a,
+                                                    _stream_ptr(stream)),
+                       "gdt_celllist_build_general")
         self.Np = n
         xs, ys, mo = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         bx, by = ctypes.c_double(), ctypes.c_double()
@@ -313,20 +396,24 @@ class CellListGPU:
             return
         L = _lib.load()
         n = int(points.shape[0])
-        x11, _, _, x22 = self.Box.getBoxDims()
+        box = self.Box.box_matrix()
+        bp = _box_arg(box)
         ptr = _host_ptr(points, torch.float64, "points")
         xs, ys, mo = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _lib.check(L.gdt_host_celllist(ptr, n, x11, x22, self.a, None, None, 0, ctypes.byref(xs),
-                                       ctypes.byref(ys), ctypes.byref(mo)), "gdt_host_celllist")
+        _lib.check(L.gdt_host_celllist_general(ptr, n, bp, self.a, None, None, 0,
+                                               ctypes.byref(xs), ctypes.byref(ys),
+                                               ctypes.byref(mo)), "gdt_host_celllist")
         self.Np = n
-        self.xsize, self.ysize, self.boxsize = xs.value, ys.value, x11 / xs.value
+        self.xsize, self.ysize = xs.value, ys.value
+        self.boxsize = box[0] / xs.value if self.Box.is_rectangle() else 0.0
         self.Nmax = max(self.Nmax, mo.value)
         nc, nmax = self.xsize * self.ysize, max(1, self.Nmax)
         self._cell_sizes = torch.zeros(nc, dtype=torch.int32)
         self._idxs = torch.zeros(nc * nmax, dtype=torch.int32)
-        _lib.check(L.gdt_host_celllist(ptr, n, x11, x22, self.a, self._cell_sizes.data_ptr(),
-                                       self._idxs.data_ptr(), nmax, ctypes.byref(xs),
-                                       ctypes.byref(ys), ctypes.byref(mo)), "gdt_host_celllist")
+        _lib.check(L.gdt_host_celllist_general(ptr, n, bp, self.a, self._cell_sizes.data_ptr(),
+                                               self._idxs.data_ptr(), nmax, ctypes.byref(xs),
+                                               ctypes.byref(ys), ctypes.byref(mo)),
+                   "gdt_host_celllist")
 
     def _export(self):
         L = _lib.load()
@@ -481,18 +568,26 @@ class DelaunayGPU:
             self._check_host_args(points, GPUTriangulation, cellNeighborNum)
         L = _lib.load()
         x11, _, _, x22 = self.Box.getBoxDims()
+        rect = self.Box.is_rectangle()
+        box, off = _box_arg(self.Box.box_matrix()), (ctypes.c_int * 2)(0, n)
         while True:
             req = ctypes.c_int(0)
-            if self.GPUcompute:
+            if self.GPUcompute and rect:
                 st = L.gdt_triangulate(self.ctx.handle, points.data_ptr(), n, x11, x22,
                                        self.cellsize, self.MaxSize, GPUTriangulation.data_ptr(),
                                        cellNeighborNum.data_ptr(), self.precision,
                                        ctypes.byref(req), _stream_ptr(stream))
+            elif self.GPUcompute:
+                st = L.gdt_triangulate_general(self.ctx.handle, points.data_ptr(), 1, off, box,
+                                               self.cellsize, self.MaxSize,
+                                               GPUTriangulation.data_ptr(),
+                                               cellNeighborNum.data_ptr(), self.precision,
+                                               ctypes.byref(req), _stream_ptr(stream))
             else:
-                st = L.gdt_host_triangulate(points.data_ptr(), n, x11, x22, self.cellsize,
-                                            self.MaxSize, GPUTriangulation.data_ptr(),
-                                            cellNeighborNum.data_ptr(), self.precision,
-                                            self.OMPThreadsNum, ctypes.byref(req))
+                st = L.gdt_host_triangulate_general(points.data_ptr(), n, box, self.cellsize,
+                                                    self.MaxSize, GPUTriangulation.data_ptr(),
+                                                    cellNeighborNum.data_ptr(), self.precision,
+                                                    self.OMPThreadsNum, ctypes.byref(req))
             if st == _lib.GDT_OVERFLOW:
                 self.resize(req.value)
                 GPUTriangulation.resize_(n * self.MaxSize)
@@ -512,19 +607,27 @@ class DelaunayGPU:
             raise ValueError("GPUTriangulation must hold N*MaxSize entries")
         L = _lib.load()
         x11, _, _, x22 = self.Box.getBoxDims()
+        box, off = _box_arg(self.Box.box_matrix()), (ctypes.c_int * 2)(0, n)
         nf, req = ctypes.c_int(0), ctypes.c_int(0)
         self.prof.start("testAndRepair")
-        if self.GPUcompute:
+        if self.GPUcompute and self.Box.is_rectangle():
             st = L.gdt_test_and_repair(self.ctx.handle, points.data_ptr(), n, x11, x22,
                                        self.cellsize, self.MaxSize, GPUTriangulation.data_ptr(),
                                        cellNeighborNum.data_ptr(), self.precision,
                                        ctypes.byref(nf), ctypes.byref(req), _stream_ptr(stream))
+        elif self.GPUcompute:
+            st = L.gdt_test_and_repair_general(self.ctx.handle, points.data_ptr(), 1, off, box,
+                                               self.cellsize, self.MaxSize,
+                                               GPUTriangulation.data_ptr(),
+                                               cellNeighborNum.data_ptr(), self.precision,
+                                               ctypes.byref(nf), ctypes.byref(req),
+                                               _stream_ptr(stream))
         else:
-            st = L.gdt_host_test_and_repair(points.data_ptr(), n, x11, x22, self.cellsize,
-                                            self.MaxSize, GPUTriangulation.data_ptr(),
-                                            cellNeighborNum.data_ptr(), self.precision,
-                                            self.OMPThreadsNum, ctypes.byref(nf),
-                                            ctypes.byref(req))
+            st = L.gdt_host_test_and_repair_general(points.data_ptr(), n, box, self.cellsize,
+                                                    self.MaxSize, GPUTriangulation.data_ptr(),
+                                                    cellNeighborNum.data_ptr(), self.precision,
+                                                    self.OMPThreadsNum, ctypes.byref(nf),
+                                                    ctypes.byref(req))
         self.prof.end("testAndRepair")
         self.last_repaired = nf.value
         if st == _lib.GDT_OVERFLOW:
@@ -535,6 +638,25 @@ class DelaunayGPU:
         _lib.check(st, "testAndRepairDelaunayTriangulation")
 
 
+def _box_tuple(b) -> tuple:
+    """A batch box as (lx, ly) when it is a rectangle, else (x11, x12, x21, x22)."""
+    if isinstance(b, PeriodicBoundaries):
+        b = b.getBoxDims()
+    b = tuple(float(v) for v in b)
+    if len(b) == 4 and b[1] == 0.0 and b[2] == 0.0 and b[0] > 0 and b[3] > 0:
+        return (b[0], b[3])
+    if len(b) not in (2, 4):
+        raise ValueError("a box is (lx, ly) or (x11, x12, x21, x22)")
+    return b
+
+
+def _entry_point(name: str, general: bool) -> str:
+    """The C entry point of a batched call for rectangles or general boxes."""
+    if general:
+        return name + "_general"
+    return name if name == "gdt_md_run" else name + "_batch"
+
+
 MD_HEADROOM = 4   # row slots beyond the longest ring, given before and after the Brownian loop runs
 
 
@@ -542,8 +664,10 @@ class BatchedDelaunayGPU:
     """Many independent periodic sets in one launch per kernel (no reference
     counterpart: the reference runs one DelaunayGPU per set).
 
-    Set s is rows ``offsets[s]:offsets[s+1]`` of the concatenated points, in a
-    ``boxes[s] = (lx, ly)`` box; ring entries index the concatenated array.
+    Set s is rows ``offsets[s]:offsets[s+1]`` of the concatenated points, in
+    the box ``boxes[s]``: ``(lx, ly)``, a box matrix ``(x11, x12, x21, x22)``
+    (lattice vectors as columns) or a PeriodicBoundaries; ring entries index
+    the concatenated array.
     Same triangulation / test-and-repair semantics as DelaunayGPU, including
     the growth of ``MaxSize`` (the row stride shared by every set)."""
 
@@ -555,7 +679,7 @@ class BatchedDelaunayGPU:
         self.offsets = [0]
         for s in sizes:
             self.offsets.append(self.offsets[-1] + s)
-        self.boxes = [(float(b[0]), float(b[1])) for b in boxes]
+        self.boxes = [_box_tuple(b) for b in boxes]
         self.N = self.offsets[-1]
         self.nsets = len(sizes)
         self.MaxSize = max(4, int(maximumNeighborsGuess))
@@ -564,8 +688,15 @@ class BatchedDelaunayGPU:
         self.precision = int(precision)
         self.last_repaired = 0
         self._off = (ctypes.c_int * len(self.offsets))(*self.offsets)
-        flat = [v for b in self.boxes for v in b]
+        # every box a rectangle: (lx, ly) pairs and the rectangular calls;
+        # else box matrices (4 doubles per set) and the general calls
+        self.general = not all(len(b) == 2 for b in self.boxes)
+        mats = [(b[0], 0.0, 0.0, b[1]) if len(b) == 2 else b for b in self.boxes]
+        flat = [v for b in (mats if self.general else self.boxes) for v in b]
         self._box = (ctypes.c_double * len(flat))(*flat)
+        self._tri = _entry_point("gdt_triangulate", self.general)
+        self._tar = _entry_point("gdt_test_and_repair", self.general)
+        self._md = _entry_point("gdt_md_run", self.general)
 
     def setPrecision(self, precision: int) -> None:
         if precision not in (_lib.PREC_FP64, _lib.PREC_FP32F):
@@ -592,7 +723,7 @@ class BatchedDelaunayGPU:
         L = _lib.load()
         while True:
             req = ctypes.c_int(0)
-            st = L.gdt_triangulate_batch(self.ctx.handle, points.data_ptr(), self.nsets, self._off,
+            st = getattr(L, self._tri)(self.ctx.handle, points.data_ptr(), self.nsets, self._off,
                                          self._box, self.cellsize, self.MaxSize,
                                          GPUTriangulation.data_ptr(), cellNeighborNum.data_ptr(),
                                          self.precision, ctypes.byref(req), _stream_ptr(stream))
@@ -610,7 +741,7 @@ class BatchedDelaunayGPU:
             raise ValueError("GPUTriangulation must hold N*MaxSize entries")
         L = _lib.load()
         nf, req = ctypes.c_int(0), ctypes.c_int(0)
-        st = L.gdt_test_and_repair_batch(self.ctx.handle, points.data_ptr(), self.nsets, self._off,
+        st = getattr(L, self._tar)(self.ctx.handle, points.data_ptr(), self.nsets, self._off,
                                          self._box, self.cellsize, self.MaxSize,
                                          GPUTriangulation.data_ptr(), cellNeighborNum.data_ptr(),
                                          self.precision, ctypes.byref(nf), ctypes.byref(req),
@@ -663,7 +794,8 @@ class BatchedDelaunayGPU:
         step, left = int(step0), int(nsteps)
         while left > 0:
             rep = _lib.MdReport()
-            st = L.gdt_md_run(self.ctx.handle, points.data_ptr(), self.nsets, self._off, self._box,
+            st = getattr(L, self._md)(self.ctx.handle, points.data_ptr(), self.nsets, self._off,
+                                      self._box,
                               self.cellsize, self.MaxSize, GPUTriangulation.data_ptr(),
                               cellNeighborNum.data_ptr(), self.precision, float(sigma),
                               int(seed) & 0xFFFFFFFFFFFFFFFF, step, left, int(bool(use_graph)),

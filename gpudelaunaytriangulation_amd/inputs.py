@@ -185,6 +185,42 @@ def md_step(xy: np.ndarray, lx: float, ly: float, sigma: float, seed: int, step:
     return wrap_into_box(xy + sigma * md_noise(seed, step, set_index, xy.shape[0]), lx, ly)
 
 
+def box_inverse(box) -> tuple:
+    """Inverse of a box matrix (x11, x12, x21, x22), in the library's operation
+    order (gdt_make_set_box)."""
+    b0, b1, b2, b3 = (float(v) for v in box)
+    bd = b0 * b3 - b1 * b2
+    return (b3 / bd, -b1 / bd, -b2 / bd, b0 / bd)
+
+
+def general_points(n: int, box, seed: int = 1) -> np.ndarray:
+    """n points uniform in the unit cell of the general box (x11, x12, x21, x22)
+    (lattice vectors as columns): uniform unit-cell coordinates mapped by the
+    box matrix."""
+    X = np.asarray(box, dtype=np.float64).reshape(2, 2)
+    return uniform_points(n, 1.0, 1.0, seed) @ X.T
+
+
+def wrap_into_general_box(xy: np.ndarray, box) -> np.ndarray:
+    """Wrap points into the unit cell of a general box by whole lattice vectors
+    (bit-identical to gdt_wrap_into_box): k = floor(unit-cell coordinates),
+    p -= k_u a + k_v b."""
+    b0, b1, b2, b3 = (float(v) for v in box)
+    i0, i1, i2, i3 = box_inverse(box)
+    x, y = xy[:, 0].copy(), xy[:, 1].copy()
+    ku = np.floor(i0 * x + i1 * y)
+    kv = np.floor(i2 * x + i3 * y)
+    x = x - (ku * b0 + kv * b1)
+    y = y - (ku * b2 + kv * b3)
+    return np.stack([x, y], 1)
+
+
+def md_step_general(xy: np.ndarray, box, sigma: float, seed: int, step: int,
+                    set_index: int) -> np.ndarray:
+    """md_step in a general box."""
+    return wrap_into_general_box(xy + sigma * md_noise(seed, step, set_index, xy.shape[0]), box)
+
+
 def sha256(a: np.ndarray) -> str:
     """SHA-256 of an array's bytes in C order."""
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()

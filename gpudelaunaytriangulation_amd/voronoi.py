@@ -16,7 +16,9 @@ function of the positions while the triangulation does not change.
 
 Batched form: several sets concatenated along dim 0, ``offsets`` (host ints,
 nsets+1) and ``boxes`` [(lx, ly)] per set; ring entries index the
-concatenated array and the energy is one value per set.
+concatenated array and the energy is one value per set.  A box may also be a
+box matrix (x11, x12, x21, x22) with the lattice vectors as columns (a
+general periodic box; minimum images by rounded unit-cell coordinates).
 """
 from __future__ import annotations
 
@@ -25,7 +27,7 @@ import ctypes
 import torch
 
 from . import _lib
-from .delaunay import Context, _dev_ptr, _stream_ptr
+from .delaunay import Context, _box_tuple, _dev_ptr, _stream_ptr
 
 _D = ctypes.c_double
 
@@ -42,13 +44,20 @@ def _check_rings(xy, nbrs, counts):
 
 
 def _sets(n, offsets, boxes):
+    """(nsets, offsets, boxes, general): (lx, ly) pairs when every box is a
+    rectangle, else box matrices (4 doubles per set) for the general calls."""
     offs = [int(o) for o in offsets]
     if len(offs) != len(boxes) + 1 or offs[0] != 0 or offs[-1] != n:
         raise ValueError("offsets must run from 0 to N with one box per set")
     off_arr = (ctypes.c_int * len(offs))(*offs)
-    flat = [float(v) for b in boxes for v in (b[0], b[1])]
+    tb = [_box_tuple(b) for b in boxes]
+    general = not all(len(b) == 2 for b in tb)
+    if general:
+        flat = [v for b in tb for v in ((b[0], 0.0, 0.0, b[1]) if len(b) == 2 else b)]
+    else:
+        flat = [v for b in tb for v in b]
     box_arr = (_D * len(flat))(*flat)
-    return len(boxes), off_arr, box_arr
+    return len(boxes), off_arr, box_arr, general
 
 
 def voronoi_forward(ctx: Context, xy, nbrs, counts, boxes, offsets=None, a0=1.0, p0=3.8,
@@ -58,8 +67,10 @@ def voronoi_forward(ctx: Context, xy, nbrs, counts, boxes, offsets=None, a0=1.0,
     area = torch.empty(n, dtype=torch.float64, device=xy.device)
     perim = torch.empty_like(area)
     L = _lib.load()
+    if offsets is None and len(_box_tuple(boxes)) == 4:
+        offsets, boxes = [0, n], [boxes]
     if offsets is None:
-        lx, ly = boxes
+        lx, ly = _box_tuple(boxes)
         energy = torch.empty(1, dtype=torch.float64, device=xy.device)
         st = L.gdt_voronoi_forward(ctx.handle, _dev_ptr(xy, torch.float64, "points"), n, float(lx),
                                    float(ly), _dev_ptr(nbrs, torch.int32, "rings"),
@@ -67,9 +78,10 @@ def voronoi_forward(ctx: Context, xy, nbrs, counts, boxes, offsets=None, a0=1.0,
                                    float(p0), int(precision), area.data_ptr(), perim.data_ptr(),
                                    energy.data_ptr(), _stream_ptr())
     else:
-        nsets, off_arr, box_arr = _sets(n, offsets, boxes)
+        nsets, off_arr, box_arr, general = _sets(n, offsets, boxes)
         energy = torch.empty(nsets, dtype=torch.float64, device=xy.device)
-        st = L.gdt_voronoi_forward_batch(ctx.handle, _dev_ptr(xy, torch.float64, "points"), nsets,
+        fn = L.gdt_voronoi_forward_general if general else L.gdt_voronoi_forward_batch
+        st = fn(ctx.handle, _dev_ptr(xy, torch.float64, "points"), nsets,
                                          off_arr, box_arr, _dev_ptr(nbrs, torch.int32, "rings"),
                                          _dev_ptr(counts, torch.int32, "counts"), stride, float(a0),
                                          float(p0), int(precision), area.data_ptr(),
@@ -88,16 +100,19 @@ def voronoi_adjoint(ctx: Context, xy, nbrs, counts, boxes, area, perim, gscale=N
         gscale = gscale.detach().to(torch.float64).contiguous()
         gs = _dev_ptr(gscale, torch.float64, "gscale")
     L = _lib.load()
+    if offsets is None and len(_box_tuple(boxes)) == 4:
+        offsets, boxes = [0, n], [boxes]
     if offsets is None:
-        lx, ly = boxes
+        lx, ly = _box_tuple(boxes)
         st = L.gdt_voronoi_adjoint(ctx.handle, _dev_ptr(xy, torch.float64, "points"), n, float(lx),
                                    float(ly), _dev_ptr(nbrs, torch.int32, "rings"),
                                    _dev_ptr(counts, torch.int32, "counts"), stride, float(a0),
                                    float(p0), int(precision), area.data_ptr(), perim.data_ptr(),
                                    gs or None, grad.data_ptr(), _stream_ptr())
     else:
-        nsets, off_arr, box_arr = _sets(n, offsets, boxes)
-        st = L.gdt_voronoi_adjoint_batch(ctx.handle, _dev_ptr(xy, torch.float64, "points"), nsets,
+        nsets, off_arr, box_arr, general = _sets(n, offsets, boxes)
+        fn = L.gdt_voronoi_adjoint_general if general else L.gdt_voronoi_adjoint_batch
+        st = fn(ctx.handle, _dev_ptr(xy, torch.float64, "points"), nsets,
                                          off_arr, box_arr, _dev_ptr(nbrs, torch.int32, "rings"),
                                          _dev_ptr(counts, torch.int32, "counts"), stride, float(a0),
                                          float(p0), int(precision), area.data_ptr(),

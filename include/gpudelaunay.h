@@ -83,7 +83,7 @@ typedef struct {
      * loop without reading the statistics inside it) */
     unsigned long long profiled_calls;
     double ms_cells_total, ms_ring_total, ms_defer_total;
-    int tile_shift;                  /* tile edge of the last global triangulation (tiled order) */
+    int tile_shift;                  /* tile edge of the last global triangulation (0: row-major) */
     int lanes;                       /* its lanes per point */
 } gdt_stats;
 
@@ -298,6 +298,66 @@ int gdt_host_test_and_repair(const double* xy, int n, double lx, double ly, doub
  * DelaunayGPU.cu:921-922); rows that are not rings give NaN. */
 int gdt_host_voronoi_vertices(const double* xy, int n, double lx, double ly, const int* nbrs,
                               const int* counts, int stride, double* vertices, int nthreads);
+
+/* ---- general (triclinic) boxes -----------------------------------------
+ *
+ * box = {x11, x12, x21, x22}, the box matrix of periodicBoundaries::setGeneral
+ * (periodicBoundaries.h:86-101): its columns are the lattice vectors
+ * a = (x11, x21), b = (x12, x22); any invertible matrix.  The reference keeps
+ * general boxes in periodicBoundaries only; its cell list, and so DelaunayGPU,
+ * handles rectangles (cellListGPU.cpp:97-117).  Here every call of the path
+ * takes them: points are binned in the unit-cell coordinates of the
+ * Lagrange-Gauss reduced basis (a', b') of the box's lattice (the same torus;
+ * parallelogram cells spanned by a'/xsize and b'/ysize, the counts given by
+ * setGridSize's rule applied to the unit cell's thickness across each pair of
+ * walls), displacements are cell-relative as for rectangles, and minimum
+ * images round unit-cell coordinates (periodicBoundaries::minDist,
+ * periodicBoundaries.h:165-178).  Points may lie anywhere (they are wrapped);
+ * the Brownian loop wraps moved points into the caller's own box
+ * (putInBoxReal, periodicBoundaries.h:131-152).  General boxes run the
+ * row-major cell order and FP64 predicates whatever the configuration and
+ * precision arguments (the rings are the same); gdt_celllist_info then
+ * reports the cell heights as bsx, bsy.  A matrix with x12 = x21 = 0 and a
+ * positive diagonal runs the rectangular path.  Batched forms take four
+ * doubles per set in `boxes`; otherwise as the rectangular calls. */
+int gdt_celllist_build_general(gdt_context* ctx, const double* d_xy, int n, const double* box,
+                               double cell_size, void* stream);
+int gdt_triangulate_general(gdt_context* ctx, const double* d_xy, int nsets, const int* offsets,
+                            const double* boxes, double cell_size, int stride, int* d_nbrs,
+                            int* d_counts, int precision, int* required, void* stream);
+int gdt_test_and_repair_general(gdt_context* ctx, const double* d_xy, int nsets, const int* offsets,
+                                const double* boxes, double cell_size, int stride, int* d_nbrs,
+                                int* d_counts, int precision, int* nflagged, int* required,
+                                void* stream);
+int gdt_md_run_general(gdt_context* ctx, double* d_xy, int nsets, const int* offsets,
+                       const double* boxes, double cell_size, int stride, int* d_nbrs,
+                       int* d_counts, int precision, double sigma, unsigned long long seed,
+                       int step0, int nsteps, int use_graph, int check_every,
+                       gdt_md_report* report, void* stream);
+int gdt_voronoi_vertices_general(gdt_context* ctx, const double* d_xy, int n, const double* box,
+                                 const int* d_nbrs, const int* d_counts, int stride,
+                                 double* d_vertices, void* stream);
+int gdt_voronoi_forward_general(gdt_context* ctx, const double* d_xy, int nsets, const int* offsets,
+                                const double* boxes, const int* d_nbrs, const int* d_counts,
+                                int stride, double a0, double p0, int precision, double* d_area,
+                                double* d_perim, double* d_energy, void* stream);
+int gdt_voronoi_adjoint_general(gdt_context* ctx, const double* d_xy, int nsets, const int* offsets,
+                                const double* boxes, const int* d_nbrs, const int* d_counts,
+                                int stride, double a0, double p0, int precision,
+                                const double* d_area, const double* d_perim,
+                                const double* d_gscale, double* d_grad, void* stream);
+int gdt_host_celllist_general(const double* xy, int n, const double* box, double cell_size,
+                              unsigned int* cell_sizes, int* idxs, int nmax, int* xsize,
+                              int* ysize, int* max_occupancy);
+int gdt_host_triangulate_general(const double* xy, int n, const double* box, double cell_size,
+                                 int stride, int* nbrs, int* counts, int precision, int nthreads,
+                                 int* required);
+int gdt_host_test_and_repair_general(const double* xy, int n, const double* box, double cell_size,
+                                     int stride, int* nbrs, int* counts, int precision,
+                                     int nthreads, int* nflagged, int* required);
+int gdt_host_voronoi_vertices_general(const double* xy, int n, const double* box,
+                                      const int* nbrs, const int* counts, int stride,
+                                      double* vertices, int nthreads);
 
 #ifdef __cplusplus
 }

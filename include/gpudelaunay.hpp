@@ -120,43 +120,135 @@ private:
 };
 
 // ---------------------------------------------------------------------------
-//! Rectangular periodic box (periodicBoundaries::setSquare / minDist / putInBoxReal).
+//! Periodic box (periodicBoundaries.h:25-73): a rectangle (setSquare, 75-84)
+//! or a general box (setGeneral, 86-101) whose matrix [[x11, x12], [x21, x22]]
+//! has the lattice vectors as columns.
 class periodicBoundaries {
 public:
     periodicBoundaries() { setSquare(1.0, 1.0); }
     periodicBoundaries(double x, double y) { setSquare(x, y); }
+    periodicBoundaries(double a, double b, double c, double d) { setGeneral(a, b, c, d); }
     void setSquare(double x, double y)
     {
         if (!(x > 0.) || !(y > 0.)) throw std::invalid_argument("box sides must be positive");
         x11 = x;
+        x12 = x21 = 0.;
         x22 = y;
+        invert();
+        isSquare = true;
+    }
+    void setGeneral(double a, double b, double c, double d)
+    {
+        const double det = a * d - b * c;
+        if (!std::isfinite(a) || !std::isfinite(b) || !std::isfinite(c) || !std::isfinite(d) ||
+            !std::isfinite(det) || det == 0.)
+            throw std::invalid_argument("the box matrix must be finite and invertible");
+        x11 = a;
+        x12 = b;
+        x21 = c;
+        x22 = d;
+        invert();
+        isSquare = false;
     }
     void getBoxDims(double& xx, double& xy, double& yx, double& yy) const
     {
         xx = x11;
-        xy = 0.;
-        yx = 0.;
+        xy = x12;
+        yx = x21;
         yy = x22;
     }
-    bool isBoxSquare() const { return true; }
-    //! nearest-image displacement p1 - p2
+    void getBoxInvDims(double& xx, double& xy, double& yx, double& yy) const
+    {
+        xx = xi11;
+        xy = xi12;
+        yx = xi21;
+        yy = xi22;
+    }
+    bool isBoxSquare() const { return isSquare; }
+    //! whether the library runs this box on its rectangular path
+    bool isRectangle() const { return x12 == 0. && x21 == 0. && x11 > 0. && x22 > 0.; }
+    //! {x11, x12, x21, x22}, as the general-box calls take it
+    void boxMatrix(double* m) const
+    {
+        m[0] = x11;
+        m[1] = x12;
+        m[2] = x21;
+        m[3] = x22;
+    }
+    //! unit-cell coordinates to the box and back (periodicBoundaries.h:103-129)
+    void Trans(const double2& p, double2& out) const
+    {
+        const double x = x11 * p.x + x12 * p.y, y = x21 * p.x + x22 * p.y;
+        out.x = x;
+        out.y = y;
+    }
+    void invTrans(const double2& p, double2& out) const
+    {
+        const double x = xi11 * p.x + xi12 * p.y, y = xi21 * p.x + xi22 * p.y;
+        out.x = x;
+        out.y = y;
+    }
+    //! nearest-image displacement p1 - p2 (periodicBoundaries.h:154-179): per
+    //! axis for rectangles, on unit-cell coordinates wrapped into [-1/2, 1/2]
+    //! for general boxes
     void minDist(const double2& p1, const double2& p2, double2& out) const
     {
-        double dx = p1.x - p2.x, dy = p1.y - p2.y;
-        dx -= x11 * std::nearbyint(dx / x11);
-        dy -= x22 * std::nearbyint(dy / x22);
-        out.x = dx;
-        out.y = dy;
+        if (isSquare) {
+            double dx = p1.x - p2.x, dy = p1.y - p2.y;
+            dx -= x11 * std::nearbyint(dx / x11);
+            dy -= x22 * std::nearbyint(dy / x22);
+            out.x = dx;
+            out.y = dy;
+            return;
+        }
+        double2 a, b;
+        invTrans(p1, a);
+        invTrans(p2, b);
+        double2 u;
+        u.x = a.x - b.x;
+        u.y = a.y - b.y;
+        u.x -= std::nearbyint(u.x);
+        u.y -= std::nearbyint(u.y);
+        Trans(u, out);
     }
-    //! wrap into [0, Lx) x [0, Ly)
+    //! wrap into the unit cell (periodicBoundaries.h:131-152)
     void putInBoxReal(double2& p) const
     {
-        p.x -= std::floor(p.x / x11) * x11;
-        p.y -= std::floor(p.y / x22) * x22;
-        if (p.x >= x11) p.x -= x11;
-        if (p.y >= x22) p.y -= x22;
+        if (isSquare) {
+            p.x -= std::floor(p.x / x11) * x11;
+            p.y -= std::floor(p.y / x22) * x22;
+            if (p.x >= x11) p.x -= x11;
+            if (p.y >= x22) p.y -= x22;
+            return;
+        }
+        double2 u;
+        invTrans(p, u);
+        u.x -= std::floor(u.x);
+        u.y -= std::floor(u.y);
+        if (u.x >= 1.) u.x -= 1.;
+        if (u.y >= 1.) u.y -= 1.;
+        Trans(u, p);
     }
-    double x11 = 1., x22 = 1.;
+    //! p + disp, put back in the box (periodicBoundaries.h:181-189)
+    void move(double2& p, const double2& disp) const
+    {
+        p.x += disp.x;
+        p.y += disp.y;
+        putInBoxReal(p);
+    }
+    double x11 = 1., x12 = 0., x21 = 0., x22 = 1.;
+    double xi11 = 1., xi12 = 0., xi21 = 0., xi22 = 1.;
+    bool isSquare = true;
+
+private:
+    void invert()
+    {
+        const double det = x11 * x22 - x12 * x21;
+        xi11 = x22 / det;
+        xi12 = -x12 / det;
+        xi21 = -x21 / det;
+        xi22 = x11 / det;
+    }
 };
 typedef std::shared_ptr<periodicBoundaries> PeriodicBoxPtr;
 
@@ -456,10 +548,16 @@ public:
     void setBox(PeriodicBoxPtr bx) { Box = std::move(bx); }
 
     //! cellListGPU::setGridSize (cellListGPU.cpp:97-117): cells per axis
-    //! floor(L/a) rounded up to even; the y side uses Ly/ysize.
+    //! floor(L/a) rounded up to even; the y side uses Ly/ysize.  A general
+    //! box's grid (cells of its reduced lattice basis) is known after compute.
     void setGridSize(double a)
     {
         m_a = a;
+        if (!Box->isRectangle()) {
+            xsize = ysize = totalCells = 0;
+            boxsize = 0.;
+            return;
+        }
         xsize = (int)std::floor(Box->x11 / a);
         if (xsize % 2) ++xsize;
         ysize = (int)std::floor(Box->x22 / a);
@@ -475,8 +573,12 @@ public:
     int getYsize() { return ysize; }
     double getBoxsize() { return boxsize; }
 
+    //! rectangular boxes (the reference's cell list)
     int positionToCellIndex(double x, double y)
     {
+        if (!Box->isRectangle())
+            throw std::invalid_argument("positionToCellIndex: a general box's cells follow its "
+                                        "reduced lattice basis");
         const double bsy = Box->x22 / ysize;
         const int cx = std::min(xsize - 1, std::max(0, (int)std::floor(x / boxsize)));
         const int cy = std::min(ysize - 1, std::max(0, (int)std::floor(y / bsy)));
@@ -511,9 +613,14 @@ public:
         if (xsize == 0) setGridSize(m_a);
         {
             ArrayHandle<double2> p(points, access_location::device, access_mode::read);
-            gdt_detail::status_check(gdt_celllist_build(m_ctx->get(), reinterpret_cast<const double*>(p.data),
-                                                        n, Box->x11, Box->x22, m_a, nullptr),
-                                     "cellListGPU::computeGPU");
+            const double* xy = reinterpret_cast<const double*>(p.data);
+            double bm[4];
+            Box->boxMatrix(bm);
+            gdt_detail::status_check(
+                Box->isRectangle()
+                    ? gdt_celllist_build(m_ctx->get(), xy, n, Box->x11, Box->x22, m_a, nullptr)
+                    : gdt_celllist_build_general(m_ctx->get(), xy, n, bm, m_a, nullptr),
+                "cellListGPU::computeGPU");
         }
         Np = n;
         int xs = 0, ys = 0, mo = 0;
@@ -544,13 +651,15 @@ public:
         ArrayHandle<double2> p(points, access_location::host, access_mode::read);
         const double* xy = reinterpret_cast<const double*>(p.data);
         int xs = 0, ys = 0, mo = 0;
-        gdt_detail::status_check(gdt_host_celllist(xy, n, Box->x11, Box->x22, m_a, nullptr, nullptr,
-                                                   0, &xs, &ys, &mo),
+        double bm[4];
+        Box->boxMatrix(bm);
+        gdt_detail::status_check(gdt_host_celllist_general(xy, n, bm, m_a, nullptr, nullptr, 0, &xs,
+                                                           &ys, &mo),
                                  "cellListGPU::compute");
         Np = n;
         xsize = xs;
         ysize = ys;
-        boxsize = Box->x11 / xs;
+        boxsize = Box->isRectangle() ? Box->x11 / xs : 0.;
         totalCells = xs * ys;
         Nmax = std::max(Nmax, mo);
         cell_indexer = Index2D(xsize, ysize);
@@ -559,8 +668,8 @@ public:
         if ((int)idxs.getNumElements() != totalCells * Nmax) idxs.resize(totalCells * Nmax);
         ArrayHandle<unsigned int> cs(cell_sizes, access_location::host, access_mode::overwrite);
         ArrayHandle<int> ix(idxs, access_location::host, access_mode::overwrite);
-        gdt_detail::status_check(gdt_host_celllist(xy, n, Box->x11, Box->x22, m_a, cs.data, ix.data,
-                                                   Nmax, &xs, &ys, &mo),
+        gdt_detail::status_check(gdt_host_celllist_general(xy, n, bm, m_a, cs.data, ix.data, Nmax,
+                                                           &xs, &ys, &mo),
                                  "cellListGPU::compute");
     }
     void computeGPU() { computeGPU(particles); }
@@ -654,11 +763,18 @@ public:
                 ArrayHandle<int> t(GPUTriangulation, where, access_mode::overwrite);
                 ArrayHandle<int> c(cellNeighborNum, where, access_mode::overwrite);
                 const double* xy = reinterpret_cast<const double*>(p.data);
-                st = GPUcompute
-                         ? gdt_triangulate(m_ctx->get(), xy, n, Box->x11, Box->x22, cellsize, MaxSize,
-                                           t.data, c.data, precision, &req, nullptr)
-                         : gdt_host_triangulate(xy, n, Box->x11, Box->x22, cellsize, MaxSize, t.data,
-                                                c.data, precision, (int)OMPThreadsNum, &req);
+                double bm[4];
+                Box->boxMatrix(bm);
+                const int off[2] = {0, n};
+                if (!GPUcompute)
+                    st = gdt_host_triangulate_general(xy, n, bm, cellsize, MaxSize, t.data, c.data,
+                                                      precision, (int)OMPThreadsNum, &req);
+                else if (Box->isRectangle())
+                    st = gdt_triangulate(m_ctx->get(), xy, n, Box->x11, Box->x22, cellsize, MaxSize,
+                                         t.data, c.data, precision, &req, nullptr);
+                else
+                    st = gdt_triangulate_general(m_ctx->get(), xy, 1, off, bm, cellsize, MaxSize,
+                                                 t.data, c.data, precision, &req, nullptr);
             }
             if (st == GDT_OVERFLOW) {
                 MaxSize = req;
@@ -688,12 +804,18 @@ public:
             ArrayHandle<int> t(GPUTriangulation, where, access_mode::readwrite);
             ArrayHandle<int> c(cellNeighborNum, where, access_mode::readwrite);
             const double* xy = reinterpret_cast<const double*>(p.data);
-            st = GPUcompute
-                     ? gdt_test_and_repair(m_ctx->get(), xy, n, Box->x11, Box->x22, cellsize,
-                                           MaxSize, t.data, c.data, precision, &nf, &req, nullptr)
-                     : gdt_host_test_and_repair(xy, n, Box->x11, Box->x22, cellsize, MaxSize,
-                                                t.data, c.data, precision, (int)OMPThreadsNum, &nf,
-                                                &req);
+            double bm[4];
+            Box->boxMatrix(bm);
+            const int off[2] = {0, n};
+            if (!GPUcompute)
+                st = gdt_host_test_and_repair_general(xy, n, bm, cellsize, MaxSize, t.data, c.data,
+                                                      precision, (int)OMPThreadsNum, &nf, &req);
+            else if (Box->isRectangle())
+                st = gdt_test_and_repair(m_ctx->get(), xy, n, Box->x11, Box->x22, cellsize,
+                                         MaxSize, t.data, c.data, precision, &nf, &req, nullptr);
+            else
+                st = gdt_test_and_repair_general(m_ctx->get(), xy, 1, off, bm, cellsize, MaxSize,
+                                                 t.data, c.data, precision, &nf, &req, nullptr);
         }
         prof.end("testAndRepair");
         lastRepaired = nf;
@@ -734,11 +856,18 @@ protected:
         ArrayHandle<double2> v(GPUVoroCur, where, access_mode::overwrite);
         const double* xy = reinterpret_cast<const double*>(p.data);
         double* out = reinterpret_cast<double*>(v.data);
-        const int st = GPUcompute ? gdt_voronoi_vertices(m_ctx->get(), xy, n, Box->x11, Box->x22,
-                                                         t.data, c.data, MaxSize, out, nullptr)
-                                  : gdt_host_voronoi_vertices(xy, n, Box->x11, Box->x22, t.data,
-                                                              c.data, MaxSize, out,
-                                                              (int)OMPThreadsNum);
+        double bm[4];
+        Box->boxMatrix(bm);
+        int st;
+        if (!GPUcompute)
+            st = gdt_host_voronoi_vertices_general(xy, n, bm, t.data, c.data, MaxSize, out,
+                                                   (int)OMPThreadsNum);
+        else if (Box->isRectangle())
+            st = gdt_voronoi_vertices(m_ctx->get(), xy, n, Box->x11, Box->x22, t.data, c.data,
+                                      MaxSize, out, nullptr);
+        else
+            st = gdt_voronoi_vertices_general(m_ctx->get(), xy, n, bm, t.data, c.data, MaxSize,
+                                              out, nullptr);
         gdt_detail::status_check(st, "GPUVoroCur");
     }
 

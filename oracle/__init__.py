@@ -55,6 +55,19 @@ def lib():
                                          _ip, _ip, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          _dp, _dp, _dp, ctypes.c_int]
         L.orc_voronoi_energy.restype = ctypes.c_double
+        L.orc_triangulate_box.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int,
+                                          _ip, _ip, _ip, ctypes.c_int]
+        L.orc_triangulate_box.restype = ctypes.c_int
+        L.orc_test_circumcircles_box.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_double, _ip,
+                                                 ctypes.c_int, _ip, ctypes.c_int]
+        L.orc_test_circumcircles_box.restype = ctypes.c_int
+        L.orc_test_and_repair_box.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_double,
+                                              ctypes.c_int, _ip, _ip, _ip, _ip, ctypes.c_int]
+        L.orc_test_and_repair_box.restype = ctypes.c_int
+        L.orc_voronoi_energy_box.argtypes = [_dp, ctypes.c_int, _dp, _ip, _ip, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_double, _dp, _dp, _dp,
+                                             ctypes.c_int]
+        L.orc_voronoi_energy_box.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -143,4 +156,75 @@ def voronoi_energy(xy, lx, ly, nbrs, counts, a0=1.0, p0=3.8, nthreads=0):
     grad = np.zeros((n, 2))
     E = lib().orc_voronoi_energy(_d(xy), n, lx, ly, _i(nbrs), _i(counts), stride, a0, p0,
                                  _d(area), _d(perim), _d(grad), nthreads)
+    return E, area, perim, grad
+
+
+# ---- general (triclinic) boxes: box = (x11, x12, x21, x22), lattice vectors
+# as columns (periodicBoundaries::setGeneral, periodicBoundaries.h:86-101) ----
+
+def _box(box):
+    b = np.ascontiguousarray(np.asarray(box, dtype=np.float64).reshape(4))
+    return b
+
+
+def triangulate_box(xy, box, cell_size=1.0, max_nbrs=16, nthreads=0):
+    """triangulate() in a general box."""
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    b = _box(box)
+    n = xy.shape[0]
+    m = max(4, int(max_nbrs))
+    while True:
+        nbrs = np.full((n, m), -1, dtype=np.int32)
+        counts = np.zeros(n, dtype=np.int32)
+        req = np.zeros(1, dtype=np.int32)
+        rc = lib().orc_triangulate_box(_d(xy), n, _d(b), cell_size, m, _i(nbrs), _i(counts),
+                                       _i(req), nthreads)
+        if rc == 1:
+            m = int(req[0])
+            continue
+        if rc != 0:
+            raise RuntimeError(f"oracle triangulate failed ({rc})")
+        return nbrs, counts
+
+
+def test_circumcircles_box(xy, box, tris, cell_size=1.0, nthreads=0):
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    tris = np.ascontiguousarray(tris, dtype=np.int32)
+    b = _box(box)
+    rep = np.zeros(xy.shape[0], dtype=np.int32)
+    lib().orc_test_circumcircles_box(_d(xy), xy.shape[0], _d(b), cell_size, _i(tris),
+                                     tris.shape[0], _i(rep), nthreads)
+    return rep
+
+
+def test_and_repair_box(xy, box, nbrs, counts, cell_size=1.0, nthreads=0):
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    nbrs = np.ascontiguousarray(nbrs, dtype=np.int32).copy()
+    counts = np.ascontiguousarray(counts, dtype=np.int32).copy()
+    b = _box(box)
+    n, m = nbrs.shape
+    req = np.zeros(1, dtype=np.int32)
+    nf = np.zeros(1, dtype=np.int32)
+    rc = lib().orc_test_and_repair_box(_d(xy), n, _d(b), cell_size, m, _i(nbrs), _i(counts),
+                                       _i(req), _i(nf), nthreads)
+    if rc == -2:
+        raise RuntimeError("Euler check failed")
+    if rc == 1:
+        raise RuntimeError(f"repair needs max_nbrs={int(req[0])}")
+    if rc != 0:
+        raise RuntimeError(f"oracle test_and_repair failed ({rc})")
+    return nbrs, counts, int(nf[0])
+
+
+def voronoi_energy_box(xy, box, nbrs, counts, a0=1.0, p0=3.8, nthreads=0):
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    nbrs = np.ascontiguousarray(nbrs, dtype=np.int32)
+    counts = np.ascontiguousarray(counts, dtype=np.int32)
+    b = _box(box)
+    n, stride = nbrs.shape
+    area = np.zeros(n)
+    perim = np.zeros(n)
+    grad = np.zeros((n, 2))
+    E = lib().orc_voronoi_energy_box(_d(xy), n, _d(b), _i(nbrs), _i(counts), stride, a0, p0,
+                                     _d(area), _d(perim), _d(grad), nthreads)
     return E, area, perim, grad

@@ -30,25 +30,84 @@ typedef struct { double x, y; } d2;
 /* ------------------------------------------------------------------ */
 
 /* cellListGPU::setGridSize (cellListGPU.cpp:97-117): the cell count per
- * axis is floor(L/a), bumped to the next even number. */
-void orc_make_grid(double lx, double ly, double cell_size, orc_grid* g)
+ * axis is floor(T/a), bumped to the next even number (T: box side, or box
+ * thickness for a general box). */
+static void grid_counts(double tx, double ty, double cell_size, orc_grid* g)
 {
-    g->lx = lx; g->ly = ly;
-    g->hx = lx * 0.5; g->hy = ly * 0.5;           /* periodicBoundaries.h:82-83 */
-    g->ilx = 1. / lx; g->ily = 1. / ly;           /* periodicBoundaries.h:79 */
-    g->xsize = (int)floor(lx / cell_size);
+    g->xsize = (int)floor(tx / cell_size);
     if (g->xsize % 2 == 1) g->xsize += 1;
-    g->ysize = (int)floor(ly / cell_size);
+    g->ysize = (int)floor(ty / cell_size);
     if (g->ysize % 2 == 1) g->ysize += 1;
     if (g->xsize < 2) g->xsize = 2;
     if (g->ysize < 2) g->ysize = 2;
-    g->bsx = lx / g->xsize;
-    g->bsy = ly / g->ysize;   /* reference uses lx/xsize here (cellListGPU.cpp:106) */
 }
 
-/* periodicBoundaries::minDist, rectangular branch (periodicBoundaries.h:154-164). */
+void orc_make_grid(double lx, double ly, double cell_size, orc_grid* g)
+{
+    memset(g, 0, sizeof(*g));
+    g->general = 0;
+    g->x11 = lx; g->x22 = ly;                      /* setSquare (periodicBoundaries.h:75-84) */
+    g->xi11 = 1. / lx; g->xi22 = 1. / ly;
+    g->lx = lx; g->ly = ly;
+    g->hx = lx * 0.5; g->hy = ly * 0.5;           /* periodicBoundaries.h:82-83 */
+    g->ilx = 1. / lx; g->ily = 1. / ly;           /* periodicBoundaries.h:79 */
+    grid_counts(lx, ly, cell_size, g);
+    g->bsx = lx / g->xsize;
+    g->bsy = ly / g->ysize;   /* reference uses lx/xsize here (cellListGPU.cpp:106) */
+    g->hcx = g->bsx;
+    g->hcy = g->bsy;
+    g->lhalf = (g->xsize * g->bsx) * 0.5;
+}
+
+/* setGeneral (periodicBoundaries.h:86-101); the cells are the unit-cell grid
+ * mapped by the box matrix, parallelograms of heights T/xsize, T/ysize. */
+void orc_make_grid_box(const double* box, double cell_size, orc_grid* g)
+{
+    const double a = box[0], b = box[1], c = box[2], d = box[3];
+    if (b == 0. && c == 0. && a > 0. && d > 0.) {
+        orc_make_grid(a, d, cell_size, g);
+        return;
+    }
+    memset(g, 0, sizeof(*g));
+    g->general = 1;
+    g->x11 = a; g->x12 = b; g->x21 = c; g->x22 = d;
+    const double det = a * d - b * c, pre = 1.0 / det;
+    g->xi11 = pre * d; g->xi12 = -pre * b;
+    g->xi21 = -pre * c; g->xi22 = pre * a;
+    const double adet = fabs(det);
+    const double tx = adet / sqrt(b * b + d * d);   /* distance between the walls along b */
+    const double ty = adet / sqrt(a * a + c * c);   /* distance between the walls along a */
+    grid_counts(tx, ty, cell_size, g);
+    g->hcx = tx / g->xsize;
+    g->hcy = ty / g->ysize;
+    g->lhalf = 0.5 * (tx < ty ? tx : ty);
+}
+
+/* periodicBoundaries::invTrans / Trans (periodicBoundaries.h:103-129). */
+static inline d2 inv_trans(const orc_grid* g, d2 p)
+{
+    d2 u = { g->xi11 * p.x + g->xi12 * p.y, g->xi21 * p.x + g->xi22 * p.y };
+    return u;
+}
+static inline d2 trans(const orc_grid* g, d2 u)
+{
+    d2 p = { g->x11 * u.x + g->x12 * u.y, g->x21 * u.x + g->x22 * u.y };
+    return p;
+}
+
+/* periodicBoundaries::minDist (periodicBoundaries.h:154-179): rectangular
+ * branch on the coordinates, general branch on unit-cell coordinates. */
 static inline d2 min_dist(const orc_grid* g, d2 p1, d2 p2)
 {
+    if (g->general) {
+        const d2 va = inv_trans(g, p1), vb = inv_trans(g, p2);
+        d2 u = { va.x - vb.x, va.y - vb.y };
+        while (u.x < -0.5) u.x += 1.0;
+        while (u.y < -0.5) u.y += 1.0;
+        while (u.x > 0.5) u.x -= 1.0;
+        while (u.y > 0.5) u.y -= 1.0;
+        return trans(g, u);
+    }
     d2 r = { p1.x - p2.x, p1.y - p2.y };
     while (r.x < -g->hx) r.x += g->lx;
     while (r.y < -g->hy) r.y += g->ly;
@@ -61,6 +120,14 @@ static inline d2 min_dist(const orc_grid* g, d2 p1, d2 p2)
  * unit coordinates, wrap into [0,1), map back. */
 static inline d2 put_in_box(const orc_grid* g, d2 p)
 {
+    if (g->general) {
+        d2 u = inv_trans(g, p);
+        while (u.x < 0) u.x += 1.0;
+        while (u.y < 0) u.y += 1.0;
+        while (u.x >= 1.0) u.x -= 1.0;
+        while (u.y >= 1.0) u.y -= 1.0;
+        return trans(g, u);
+    }
     double ux = g->ilx * p.x, uy = g->ily * p.y;
     while (ux < 0) ux += 1.0;
     while (uy < 0) uy += 1.0;
@@ -72,13 +139,18 @@ static inline d2 put_in_box(const orc_grid* g, d2 p)
 
 static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-static inline int cell_x_of(const orc_grid* g, double x)
+/* Cell of a point inside the box: rectangles by side length
+ * (cellListGPU.cpp:314-316), general boxes by unit-cell coordinates. */
+static inline void cell_of(const orc_grid* g, d2 p, int* cx, int* cy)
 {
-    return clampi((int)floor(x / g->bsx), 0, g->xsize - 1);
-}
-static inline int cell_y_of(const orc_grid* g, double y)
-{
-    return clampi((int)floor(y / g->bsy), 0, g->ysize - 1);
+    if (g->general) {
+        const d2 u = inv_trans(g, p);
+        *cx = clampi((int)floor(u.x * g->xsize), 0, g->xsize - 1);
+        *cy = clampi((int)floor(u.y * g->ysize), 0, g->ysize - 1);
+        return;
+    }
+    *cx = clampi((int)floor(p.x / g->bsx), 0, g->xsize - 1);
+    *cy = clampi((int)floor(p.y / g->bsy), 0, g->ysize - 1);
 }
 
 /* positiveModulo (DelaunayGPU.cu:26-31). */
@@ -99,8 +171,9 @@ int orc_cell_list(const double* xy, int n, const orc_grid* g, int* cell_start, i
     if (!fill || !key) { free(fill); free(key); return -1; }
     memset(cell_start, 0, sizeof(int) * (size_t)(ncell + 1));
     for (int i = 0; i < n; ++i) {
-        int cx = cell_x_of(g, xy[2 * i]);
-        int cy = cell_y_of(g, xy[2 * i + 1]);
+        int cx, cy;
+        d2 p = { xy[2 * i], xy[2 * i + 1] };
+        cell_of(g, p, &cx, &cy);
         key[i] = cy * g->xsize + cx;            /* Index2D(xsize,ysize)(cx,cy), indexer.h:38-41 */
         cell_start[key[i] + 1]++;
     }
@@ -267,7 +340,7 @@ static inline int skip_candidate(const ring_t* r, int cand, int self, int jj)
 
 static inline int spiral_half_width(const orc_grid* g, double R)
 {
-    double bs = g->bsx < g->bsy ? g->bsx : g->bsy;
+    double bs = g->hcx < g->hcy ? g->hcx : g->hcy;
     int cap = (g->xsize < g->ysize ? g->xsize : g->ysize) / 2;
     int w = (int)ceil(R / bs);
     return w < cap ? w : cap;
@@ -308,7 +381,7 @@ static int search_and_clip(const orc_ctx* C, ring_t* r, int self, d2 v, int jj,
 static int candidate_polygon(const orc_ctx* C, int k, ring_t* r)
 {
     const orc_grid* g = C->g;
-    const double Lmax = (g->xsize * g->bsx) * 0.5;
+    const double Lmax = g->lhalf;
     const double LL = Lmax / 1.414213562373095 - ORC_EPS;
     r->n = 4;
     r->P[0].x = LL;  r->P[0].y = LL;
@@ -321,7 +394,8 @@ static int candidate_polygon(const orc_ctx* C, int k, ring_t* r)
     }
     const d2 v = C->pt[k];
     const d2 vb = put_in_box(g, v);
-    const int cx = cell_x_of(g, vb.x), cy = cell_y_of(g, vb.y);
+    int cx, cy;
+    cell_of(g, vb, &cx, &cy);
     for (int jj = 0; jj < r->n; ++jj) {
         int res = search_and_clip(C, r, k, v, jj, cx, cy, 0, 1);
         if (res == CLIP_FULL) return -1;
@@ -339,7 +413,8 @@ static int refine_ring(const orc_ctx* C, int k, ring_t* r)
     for (int jj = 0; jj < r->n; ++jj) {
         d2 c = { v.x + r->Q[jj].x, v.y + r->Q[jj].y };
         c = put_in_box(g, c);
-        int cx = cell_x_of(g, c.x), cy = cell_y_of(g, c.y);
+        int cx, cy;
+        cell_of(g, c, &cx, &cy);
         int res = search_and_clip(C, r, k, v, jj, cx, cy, jj, jj == 0);
         if (res == CLIP_FULL) return -1;
         if (res == CLIP_DONE) jj--;
@@ -353,9 +428,9 @@ typedef struct {
     int* cidx;
 } orc_cells;
 
-static int build_cells(const double* xy, int n, double lx, double ly, double cell_size, orc_cells* cl)
+static int build_cells(const double* xy, int n, const double* box, double cell_size, orc_cells* cl)
 {
-    orc_make_grid(lx, ly, cell_size, &cl->g);
+    orc_make_grid_box(box, cell_size, &cl->g);
     int ncell = cl->g.xsize * cl->g.ysize;
     cl->cstart = (int*)malloc(sizeof(int) * (size_t)(ncell + 1));
     cl->cidx = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
@@ -405,9 +480,16 @@ static int triangulate_selected(const double* xy, int n, const orc_cells* cl, co
 int orc_triangulate(const double* xy, int n, double lx, double ly, double cell_size,
                     int max_nbrs, int* nbrs, int* counts, int* required, int nthreads)
 {
+    const double box[4] = { lx, 0., 0., ly };
+    return orc_triangulate_box(xy, n, box, cell_size, max_nbrs, nbrs, counts, required, nthreads);
+}
+
+int orc_triangulate_box(const double* xy, int n, const double* box, double cell_size,
+                        int max_nbrs, int* nbrs, int* counts, int* required, int nthreads)
+{
     if (n <= 0) return -1;
     orc_cells cl;
-    if (build_cells(xy, n, lx, ly, cell_size, &cl) != 0) { free_cells(&cl); return -4; }
+    if (build_cells(xy, n, box, cell_size, &cl) != 0) { free_cells(&cl); return -4; }
     int rc = triangulate_selected(xy, n, &cl, NULL, max_nbrs, nbrs, counts, required, nthreads);
     free_cells(&cl);
     return rc;
@@ -446,8 +528,15 @@ int orc_triangles(const int* nbrs, const int* counts, int n, int stride, int* tr
 int orc_test_circumcircles(const double* xy, int n, double lx, double ly, double cell_size,
                            const int* tris, int ntri, int* repair, int nthreads)
 {
+    const double box[4] = { lx, 0., 0., ly };
+    return orc_test_circumcircles_box(xy, n, box, cell_size, tris, ntri, repair, nthreads);
+}
+
+int orc_test_circumcircles_box(const double* xy, int n, const double* box, double cell_size,
+                               const int* tris, int ntri, int* repair, int nthreads)
+{
     orc_cells cl;
-    if (build_cells(xy, n, lx, ly, cell_size, &cl) != 0) { free_cells(&cl); return -4; }
+    if (build_cells(xy, n, box, cell_size, &cl) != 0) { free_cells(&cl); return -4; }
     const orc_grid* g = &cl.g;
     const d2* pt = (const d2*)xy;
     for (int i = 0; i < n; ++i) repair[i] = -1;
@@ -466,9 +555,10 @@ int orc_test_circumcircles(const double* xy, int n, double lx, double ly, double
         const double R = sqrt(q.x * q.x + q.y * q.y);     /* Circumcircle, functions.h:185-189 */
         d2 qb = { v.x + q.x, v.y + q.y };
         qb = put_in_box(g, qb);
-        const int cx = cell_x_of(g, qb.x), cy = cell_y_of(g, qb.y);
+        int cx, cy;
+        cell_of(g, qb, &cx, &cy);
         /* the test caps the half-width by max(xsize,ysize)/2 (DelaunayGPU.cu:227-228) */
-        const double bs = g->bsx < g->bsy ? g->bsx : g->bsy;
+        const double bs = g->hcx < g->hcy ? g->hcx : g->hcy;
         int cap = (g->xsize > g->ysize ? g->xsize : g->ysize) / 2;
         int w = (int)ceil(R / bs);
         if (w > cap) w = cap;
@@ -507,8 +597,17 @@ int orc_test_circumcircles(const double* xy, int n, double lx, double ly, double
 int orc_repair(const double* xy, int n, double lx, double ly, double cell_size, const int* repair,
                int max_nbrs, int* nbrs, int* counts, int* required, int nthreads)
 {
+    const double box[4] = { lx, 0., 0., ly };
+    return orc_repair_box(xy, n, box, cell_size, repair, max_nbrs, nbrs, counts, required,
+                          nthreads);
+}
+
+int orc_repair_box(const double* xy, int n, const double* box, double cell_size,
+                   const int* repair, int max_nbrs, int* nbrs, int* counts, int* required,
+                   int nthreads)
+{
     orc_cells cl;
-    if (build_cells(xy, n, lx, ly, cell_size, &cl) != 0) { free_cells(&cl); return -4; }
+    if (build_cells(xy, n, box, cell_size, &cl) != 0) { free_cells(&cl); return -4; }
     int rc = triangulate_selected(xy, n, &cl, repair, max_nbrs, nbrs, counts, required, nthreads);
     free_cells(&cl);
     return rc;
@@ -518,6 +617,15 @@ int orc_test_and_repair(const double* xy, int n, double lx, double ly, double ce
                         int max_nbrs, int* nbrs, int* counts, int* required, int* nflagged,
                         int nthreads)
 {
+    const double box[4] = { lx, 0., 0., ly };
+    return orc_test_and_repair_box(xy, n, box, cell_size, max_nbrs, nbrs, counts, required,
+                                   nflagged, nthreads);
+}
+
+int orc_test_and_repair_box(const double* xy, int n, const double* box, double cell_size,
+                            int max_nbrs, int* nbrs, int* counts, int* required, int* nflagged,
+                            int nthreads)
+{
     int cap = 2 * n;
     int* tris = (int*)malloc(sizeof(int) * 3 * (size_t)cap);
     int* rep = (int*)malloc(sizeof(int) * (size_t)n);
@@ -525,10 +633,11 @@ int orc_test_and_repair(const double* xy, int n, double lx, double ly, double ce
     long long total = 0;
     int nt = orc_triangles(nbrs, counts, n, max_nbrs, tris, cap, &total);
     if (nt != 2 * n || total != 6LL * n) { free(tris); free(rep); return -2; }
-    int nf = orc_test_circumcircles(xy, n, lx, ly, cell_size, tris, nt, rep, nthreads);
+    int nf = orc_test_circumcircles_box(xy, n, box, cell_size, tris, nt, rep, nthreads);
     if (nflagged) *nflagged = nf;
     int rc = 0;
-    if (nf > 0) rc = orc_repair(xy, n, lx, ly, cell_size, rep, max_nbrs, nbrs, counts, required, nthreads);
+    if (nf > 0)
+        rc = orc_repair_box(xy, n, box, cell_size, rep, max_nbrs, nbrs, counts, required, nthreads);
     free(tris);
     free(rep);
     return rc;
@@ -569,8 +678,17 @@ double orc_voronoi_energy(const double* xy, int n, double lx, double ly, const i
                           const int* counts, int stride, double a0, double p0, double* area,
                           double* perim, double* grad, int nthreads)
 {
+    const double box[4] = { lx, 0., 0., ly };
+    return orc_voronoi_energy_box(xy, n, box, nbrs, counts, stride, a0, p0, area, perim, grad,
+                                  nthreads);
+}
+
+double orc_voronoi_energy_box(const double* xy, int n, const double* box, const int* nbrs,
+                              const int* counts, int stride, double a0, double p0, double* area,
+                              double* perim, double* grad, int nthreads)
+{
     orc_grid g;
-    orc_make_grid(lx, ly, dmin(lx, ly), &g);
+    orc_make_grid_box(box, dmin(fabs(box[0]), fabs(box[3])), &g);   /* only minimum images are used */
     const d2* pt = (const d2*)xy;
     double* G = (double*)malloc(sizeof(double) * 2 * (size_t)n * stride);  /* dE_c/dH_l */
     double* Ei = (double*)malloc(sizeof(double) * (size_t)n);

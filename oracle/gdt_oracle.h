@@ -40,16 +40,32 @@
 extern "C" {
 #endif
 
-/* Grid geometry as chosen by cellListGPU::setGridSize (cellListGPU.cpp:97-117). */
+/* Box and cell grid.  Rectangular boxes (periodicBoundaries::setSquare,
+ * periodicBoundaries.h:75-84) and general ones (setGeneral, 86-101), whose
+ * matrix has the lattice vectors as columns: a = (x11, x21), b = (x12, x22);
+ * a point's unit-cell coordinates are u = X^-1 p (invTrans, 117-129).  The
+ * grid follows cellListGPU::setGridSize (cellListGPU.cpp:97-117): the cell
+ * count per axis is floor(T/a) bumped to the next even number, with T the
+ * box side (rectangles) or the box thickness across the walls spanned by the
+ * other lattice vector (general boxes, whose cells are the unit-cell grid
+ * mapped by X). */
 typedef struct {
-    double lx, ly;      /* box side lengths */
+    int general;                    /* 0: rectangle lx x ly */
+    double x11, x12, x21, x22;      /* box matrix */
+    double xi11, xi12, xi21, xi22;  /* its inverse */
+    double lx, ly;      /* rectangle side lengths */
     double hx, hy;      /* half box */
     double ilx, ily;    /* inverse box lengths (periodicBoundaries.h:79) */
     int xsize, ysize;   /* number of cells */
-    double bsx, bsy;    /* cell side lengths */
+    double bsx, bsy;    /* cell side lengths (rectangles) */
+    double hcx, hcy;    /* cell heights: distance between opposite cell walls */
+    double lhalf;       /* half box width: size of the initial virtual polygon */
 } orc_grid;
 
 void orc_make_grid(double lx, double ly, double cell_size, orc_grid* g);
+/* box = {x11, x12, x21, x22}, invertible; a diagonal matrix with positive
+ * entries gives the rectangular grid. */
+void orc_make_grid_box(const double* box, double cell_size, orc_grid* g);
 
 /* Bin points into cells in point-index order (CSR form of the reference's
  * padded cell_sizes/idxs arrays).  cell_start has xsize*ysize+1 entries,
@@ -98,6 +114,25 @@ double orc_voronoi_energy(const double* xy, int n, double lx, double ly,
                           const int* nbrs, const int* counts, int stride,
                           double a0, double p0, double* area, double* perim,
                           double* grad, int nthreads);
+
+/* The same functions for a general box = {x11, x12, x21, x22}: minimum
+ * images and wrapping follow periodicBoundaries' general branch
+ * (periodicBoundaries.h:117-178), cells are binned in unit-cell coordinates,
+ * and spiral half-widths count cell heights.  A diagonal box matrix with
+ * positive entries reduces to the rectangular functions above, bit for bit. */
+int orc_triangulate_box(const double* xy, int n, const double* box, double cell_size,
+                        int max_nbrs, int* nbrs, int* counts, int* required, int nthreads);
+int orc_test_circumcircles_box(const double* xy, int n, const double* box, double cell_size,
+                               const int* tris, int ntri, int* repair, int nthreads);
+int orc_repair_box(const double* xy, int n, const double* box, double cell_size,
+                   const int* repair, int max_nbrs, int* nbrs, int* counts, int* required,
+                   int nthreads);
+int orc_test_and_repair_box(const double* xy, int n, const double* box, double cell_size,
+                            int max_nbrs, int* nbrs, int* counts, int* required, int* nflagged,
+                            int nthreads);
+double orc_voronoi_energy_box(const double* xy, int n, const double* box, const int* nbrs,
+                              const int* counts, int stride, double a0, double p0, double* area,
+                              double* perim, double* grad, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -4,6 +4,7 @@
 // the points on the host, test-and-repair; cellListGPU binning; GPUVoroCur.
 //
 // Usage: dropin_check <points.bin> <n> <lx> <ly> <moved.bin> <precision> <out.bin> [cpu]
+// A general box: <lx> = "box" and <ly> = "x11,x12,x21,x22" (setGeneral).
 // With `cpu`, DelaunayGPU runs in CPU mode (setGPUcompute(false), 4 OpenMP
 // threads) and the cell list is binned by cellListGPU::compute on the host.
 // Either way the profiler (del.prof) must have timed both calls.
@@ -41,7 +42,11 @@ int main(int argc, char** argv)
     if (argc != 8 && argc != 9) { std::fprintf(stderr, "bad args\n"); return 2; }
     const bool cpu = argc == 9 && std::string(argv[8]) == "cpu";
     const int n = std::atoi(argv[2]);
-    const double lx = std::atof(argv[3]), ly = std::atof(argv[4]);
+    const bool general = std::string(argv[3]) == "box";
+    double bm[4] = {0., 0., 0., 0.};
+    if (general && std::sscanf(argv[4], "%lf,%lf,%lf,%lf", &bm[0], &bm[1], &bm[2], &bm[3]) != 4)
+        return 2;
+    const double lx = general ? 0. : std::atof(argv[3]), ly = general ? 0. : std::atof(argv[4]);
     const int prec = std::atoi(argv[6]);
     std::vector<double> xy, moved;
     if (!read_points(argv[1], n, xy) || !read_points(argv[5], n, moved)) return 3;
@@ -51,7 +56,9 @@ int main(int argc, char** argv)
             ArrayHandle<double2> h(points, access_location::host, access_mode::overwrite);
             for (int i = 0; i < n; ++i) h.data[i] = make_double2(xy[2 * i], xy[2 * i + 1]);
         }
-        PeriodicBoxPtr box = std::make_shared<periodicBoundaries>(lx, ly);
+        PeriodicBoxPtr box = general
+                                 ? std::make_shared<periodicBoundaries>(bm[0], bm[1], bm[2], bm[3])
+                                 : std::make_shared<periodicBoundaries>(lx, ly);
         // a small guess on purpose: the call grows MaxSize and the arrays
         DelaunayGPU del(n, 4, 1.0, box);
         del.setPrecision(prec);

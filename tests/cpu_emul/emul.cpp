@@ -6,6 +6,7 @@
 // for every point.
 // Usage: emul <points.bin> <n> <lx> <ly> <cell_size> <tiled> <precision 64|32> <stride>
 //             <out.bin> [mode 0=direct 1=tile 2=wave] [audit.bin]
+// A general box: <lx> = "box:x11,x12,x21,x22" (row-major cells, <ly> ignored).
 // out.bin: int32 counts[n], int32 nbrs[n*stride], uint64 npred, uint64 nfb, int32 errflags
 // Built with -DGDT_AUDIT_FILTER, every decision the FP32 filter takes from its
 // FP32 value alone goes to audit.bin: int32 kind, int32 claimed sign, then the
@@ -31,7 +32,11 @@ int main(int argc, char** argv)
 {
     if (argc < 10 || argc > 12) { std::fprintf(stderr, "bad args\n"); return 2; }
     const int n = std::atoi(argv[2]);
-    const double lx = std::atof(argv[3]), ly = std::atof(argv[4]), a = std::atof(argv[5]);
+    double box[4] = {0., 0., 0., 0.};
+    const bool general = std::sscanf(argv[3], "box:%lf,%lf,%lf,%lf", &box[0], &box[1], &box[2],
+                                     &box[3]) == 4;
+    const double lx = general ? 0. : std::atof(argv[3]), ly = general ? 0. : std::atof(argv[4]);
+    const double a = std::atof(argv[5]);
     const int tiled = std::atoi(argv[6]), prec = std::atoi(argv[7]), stride = std::atoi(argv[8]);
     const int mode = argc >= 11 ? std::atoi(argv[10]) : 0;
     if (mode == 1 && !tiled) { std::fprintf(stderr, "tile mode needs tiled order\n"); return 2; }
@@ -40,7 +45,8 @@ int main(int argc, char** argv)
     if (!f || std::fread(xy.data(), sizeof(double), xy.size(), f) != xy.size()) return 3;
     std::fclose(f);
 
-    GdtSet S = gdt_make_set(0, n, 0, lx, ly, a, tiled);
+    if (general && tiled) { std::fprintf(stderr, "general boxes use row-major cells\n"); return 2; }
+    GdtSet S = general ? gdt_make_set_box(0, n, 0, box, a, 0) : gdt_make_set(0, n, 0, lx, ly, a, tiled);
     const int K = S.nkeys;
     std::vector<uint32_t> cnt(K, 0), cstart(K + 1, 0);
     std::vector<int> key(n), perm(n);

@@ -31,16 +31,48 @@ def _incircle(p, q, r, d):
     return (ax * (by * c2 - b2 * cy) - ay * (bx * c2 - b2 * cx) + a2 * (bx * cy - by * cx))
 
 
-def rings(P, lx_cells, ly_cells, D=3.0):
+def lattice_points(n, box_cells, seed):
+    """n points on the 2^-10 grid inside the parallelogram spanned by the
+    integer lattice vectors a = (x11, x21), b = (x12, x22) of
+    box_cells = (x11, x12, x21, x22) (int64, units of 2^-10; rejection
+    sampling with the exact membership test 0 <= det * u < det per unit-cell
+    coordinate, det * u = adj(X) p)."""
+    x11, x12, x21, x22 = (int(v) for v in box_cells)
+    det = x11 * x22 - x12 * x21
+    assert det > 0
+    xs = [0, x11, x12, x11 + x12]
+    ys = [0, x21, x22, x21 + x22]
+    rng = np.random.default_rng(seed)
+    out = np.zeros((0, 2), dtype=np.int64)
+    while out.shape[0] < n:
+        X = rng.integers(min(xs) * SCALE, max(xs) * SCALE, size=4 * n)
+        Y = rng.integers(min(ys) * SCALE, max(ys) * SCALE, size=4 * n)
+        du = x22 * X - x12 * Y
+        dv = -x21 * X + x11 * Y
+        m = det * SCALE
+        keep = (du >= 0) & (du < m) & (dv >= 0) & (dv < m)
+        out = np.concatenate([out, np.stack([X[keep], Y[keep]], 1)])
+    return out[:n].astype(np.int64)
+
+
+def rings(P, lx_cells, ly_cells, D=3.0, box_cells=None, images=1):
+    """Exact rings of the points P (integer units of 2^-10) in a periodic
+    lx_cells x ly_cells box, or in the general box whose integer lattice
+    vectors are the columns of box_cells = (x11, x12, x21, x22); candidate
+    images p + i a + j b for |i|, |j| <= images."""
     n = P.shape[0]
-    Lx, Ly = lx_cells * SCALE, ly_cells * SCALE
+    if box_cells is None:
+        box_cells = (lx_cells, 0, 0, ly_cells)
+    x11, x12, x21, x22 = (int(v) * SCALE for v in box_cells)
     R2 = int((D * SCALE) ** 2)
     out = []
+    rng_i = range(-images, images + 1)
+    shifts = np.array([[i * x11 + j * x12, i * x21 + j * x22] for i in rng_i for j in rng_i],
+                      dtype=np.int64)
     for v in range(n):
         d = P[None, :, :] - P[v][None, None, :]
-        shifts = np.array([[sx * Lx, sy * Ly] for sx in (-1, 0, 1) for sy in (-1, 0, 1)])
         cand = (d + shifts[:, None, :]).reshape(-1, 2)
-        ids = np.tile(np.arange(n), 9)
+        ids = np.tile(np.arange(n), len(shifts))
         keep = (cand[:, 0] ** 2 + cand[:, 1] ** 2 < R2) & ~((cand[:, 0] == 0) & (cand[:, 1] == 0))
         C, I = cand[keep], ids[keep]
         k = C.shape[0]

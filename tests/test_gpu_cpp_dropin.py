@@ -112,6 +112,59 @@ def test_cpp_dropin_classes(tmp_path, prec, mode):
     assert np.allclose(got[mask], ref[mask], rtol=1e-12, atol=1e-12)
 
 
+SHEAR = (64.0, 16.0, 0.0, 64.0)     # lattice vectors (64, 0) and (16, 64)
+
+
+@pytest.mark.parametrize("mode", ["gpu", "cpu"])
+def test_cpp_dropin_general_box(tmp_path, mode):
+    """periodicBoundaries(a, b, c, d) (setGeneral): the same class calls on
+    4096 points in a sheared box, against the oracle's general-box
+    restatement; cells binned by unit-cell coordinates of the box (whose
+    basis is already reduced: 62 x 64 cells); GPUVoroCur by minimum images of
+    rounded unit-cell coordinates."""
+    X = np.array(SHEAR).reshape(2, 2)
+    xy = inputs.uniform_points(N, 1.0, 1.0, seed=3) @ X.T
+    u = (xy + 0.02 * inputs.gaussian(33, 2 * N).reshape(-1, 2)) @ np.linalg.inv(X).T
+    moved = (u - np.floor(u)) @ X.T
+    p0, p1, po = (str(tmp_path / f) for f in ("p.bin", "m.bin", "o.bin"))
+    np.ascontiguousarray(xy).tofile(p0)
+    np.ascontiguousarray(moved).tofile(p1)
+    print(f"[cpp dropin] general box, {mode}", flush=True)
+    args = [DROPIN_BIN, p0, str(N), "box", ",".join(repr(v) for v in SHEAR), p1, str(PREC_FP64),
+            po]
+    r = subprocess.run(args + (["cpu"] if mode == "cpu" else []), capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = parse(po, N)
+    nb, c = oracle.triangulate_box(xy, SHEAR, 1.0, 16)
+    assert np.array_equal(rings.canonical(out["r0"], out["c0"]), rings.canonical(nb, c))
+    nb1, c1 = oracle.triangulate_box(moved, SHEAR, 1.0, 16)
+    assert out["repaired"] > 0
+    assert np.array_equal(rings.canonical(out["r1"], out["c1"]), rings.canonical(nb1, c1))
+
+    xs, ys, nmax = out["grid"]
+    assert (xs, ys) == (62, 64)
+    um = moved @ np.linalg.inv(X).T
+    cell = (np.clip(np.floor(um[:, 1] * ys), 0, ys - 1) * xs +
+            np.clip(np.floor(um[:, 0] * xs), 0, xs - 1)).astype(int)
+    assert np.array_equal(out["cell_sizes"], np.bincount(cell, minlength=xs * ys))
+
+    Xi = np.linalg.inv(X)
+    mask = np.arange(out["ms"][1])[None, :] < out["c1"][:, None]
+    ref = np.full_like(out["voro"], np.nan)
+    for i in range(N):
+        k = out["c1"][i]
+        d = moved[out["r1"][i, :k]] - moved[i]
+        w = d @ Xi.T
+        d = (w - np.round(w)) @ X.T
+        a, b = d, np.roll(d, -1, axis=0)
+        an, bn = (a * a).sum(1), (b * b).sum(1)
+        den = 2.0 * (a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0])
+        ref[i, :k, 0] = (an * b[:, 1] - bn * a[:, 1]) / den
+        ref[i, :k, 1] = (bn * a[:, 0] - an * b[:, 0]) / den
+    assert np.allclose(out["voro"][mask], ref[mask], rtol=1e-9, atol=1e-9)
+
+
 def test_gpuarray_mirror_matches_reference_semantics():
     """GPUArray/ArrayHandle/ArrayHandleAsync (tests/cpp/gpuarray_check.cpp):
     location state machine, 1-D and 2-D resizes of both copies with the
