@@ -222,33 +222,22 @@ GDT_HD inline unsigned gdt_block_rows(const GdtTileLds& L, int c0, int h, const 
     return n;
 }
 
-// Extension rows: from row dy (advanced) up to dy1, the cells of each row that
-// meet the disk with centre in [qxlo,qxhi] x [qylo,qyhi] and radius R (cell
-// sides bsy and 1/ibsx, v's offset in its cell (ox,oy), margin in cells),
-// clipped to the columns cl keeps, without the scanned square [-hs,hs]^2, as
-// row segments in tbl (at most 4, two rows at a time).  With `outer` (the
-// disk's column range lies within [-hs,hs]) the rows through the square have
-// nothing left and are skipped.  Segments are cut to part sub of W.  Returns
-// the entries.
-template <class T>
-GDT_HD inline unsigned gdt_disk_rows(const GdtTileLds& L, int c0, int& dy, int dy1, int hs,
-                                     bool outer, T qxlo, T qxhi, T qylo, T qyhi, T R, T ox, T oy,
-                                     T bsy, T ibsx, T margin, const GdtClip& cl, uint32_t* tbl,
-                                     int sub = 0, int W = 1)
+// Extension rows: from row dy (advanced) up to dy1, the columns [dx0, dx1] of
+// the circle's cell box, clipped to the columns cl keeps, without the scanned
+// square [-hs,hs]^2, as row segments in tbl (at most 5: rows are added while
+// two more segments fit).  Rows through the square contribute the parts left
+// and right of it.  Segments are cut to part sub of W.  Returns the entries.
+// (The box holds the circle's part left of v->a with its margins; scanning
+// whole box rows costs a few more entries than the rows' chords would, and
+// no square root per row.)
+GDT_HD inline unsigned gdt_box_rows(const GdtTileLds& L, int c0, int& dy, int dy1, int dx0,
+                                    int dx1, int hs, const GdtClip& cl, uint32_t* tbl,
+                                    int sub = 0, int W = 1)
 {
     unsigned n = 0;
     int ns = 0;
-    for (int rows = 0; rows < 2; ++rows, ++dy) {
-        if (outer && dy >= -hs && dy <= hs) dy = hs + 1;
-        if (dy > dy1) break;
-        const T y0 = (T)dy * bsy - oy, y1 = y0 + bsy;
-        T d = y0 - qyhi;
-        if (qylo - y1 > d) d = qylo - y1;
-        if (d < (T)0) d = (T)0;
-        if (d > R) continue;
-        const T w = (T)gdt_sqrt64((double)(R * R - d * d));
-        int lo = gdt_cell_floor((double)((ox + (qxlo - w)) * ibsx - margin));
-        int hi = gdt_cell_floor((double)((ox + (qxhi + w)) * ibsx + margin));
+    for (; dy <= dy1 && ns <= 3; ++dy) {
+        int lo = dx0, hi = dx1;
         if (cl.mode) {
             float X = fmaf(cl.s, (float)dy, cl.c);
             X = GDT_FMINF(GDT_FMAXF(X, -1e6f), 1e6f);
@@ -257,9 +246,9 @@ GDT_HD inline unsigned gdt_disk_rows(const GdtTileLds& L, int c0, int& dy, int d
             else lo = bnd > lo ? bnd : lo;
         }
         const int rb = c0 + dy * GDT_WIN;
-        auto seg = [&](int a, int b) {
-            if (a > b) return;
-            uint32_t k = L.cs[rb + a], e = L.cs[rb + b + 1];
+        auto seg = [&](int sa, int sb) {
+            if (sa > sb) return;
+            uint32_t k = L.cs[rb + sa], e = L.cs[rb + sb + 1];
             gdt_range_part(k, e, sub, W);
             if (e > k) tbl[ns++] = k | (e << 16);
             n += e - k;
@@ -574,7 +563,6 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
 
     bool ok = false, boxed = false;
     int dx0 = 0, dx1 = 0, dy0 = 0, dy1 = 0;
-    double cqx = 0., cqy = 0., cm = 0., crad = 0.;      // the certified circle's box
     // Certification against the scanned block [-hs,hs]^2 (sets the box too),
     // one code path for both searches: the circle is centred at
     // q = -(Kx,Ky)/(2 K0) -- the origin for the nearest neighbour, whose K is
@@ -593,10 +581,6 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
             const double m = 1e-6 * (fabs(qx) + fabs(qy) + rad) + 1e-9 * S.bsx;
             gdt_circle_cells<double>(qx - m, qx + m, qy - m, qy + m, rad, ox, oy, S.ibsx, S.ibsy,
                                      1e-3, dx0, dx1, dy0, dy1);
-            cqx = qx;
-            cqy = qy;
-            cm = m;
-            crad = rad;
             ok = (dx0 >= -hs && dx1 <= hs && dy0 >= -hs && dy1 <= hs) ||
                  (cov > 0. && (wrap ? 4. : 1.) * R2 <= cov * cov * (1.0 - 1e-6));
             boxed = true;
@@ -610,13 +594,10 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
         certify();
     }
     if (!ok && boxed && gdt_in_window(S, st.c0, dx0, dx1, dy0, dy1)) {
-        // the rest of the circle: the cells of its rows that meet it, left of
-        // v->a, outside the scanned square (every later best's circle lies inside)
-        const bool outer = dx0 >= -hs && dx1 <= hs;
+        // the rest of the circle's cell box, left of v->a, outside the scanned
+        // square (every later best's circle lies inside)
         for (int dy = dy0; dy <= dy1;) {
-            const unsigned ne = gdt_disk_rows<double>(L, st.c0, dy, dy1, hs, outer, cqx - cm,
-                                                      cqx + cm, cqy - cm, cqy + cm, crad, ox, oy,
-                                                      S.bsy, S.ibsx, 1e-3, cl, st.tbl, sub, W);
+            const unsigned ne = gdt_box_rows(L, st.c0, dy, dy1, dx0, dx1, hs, cl, st.tbl, sub, W);
             gdt_scan_tbl(st.tbl, ne, load, visit);
             npred += ne;
         }
@@ -840,7 +821,6 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
     const float bsm = S.bsmin32;
     bool ok = false, boxed = false;
     int dx0 = 0, dx1 = 0, dy0 = 0, dy1 = 0;
-    float cq[4] = {0.f, 0.f, 0.f, 0.f}, crad = 0.f;     // the certified circle's box
     // certification against the scanned block [-hs,hs]^2 (sets the box too)
     auto certify = [&]() {
         const float cov = (float)hs * bsm + st.covb - 9.5367431640625e-07f * (float)(hs + 2) * bsM;
@@ -868,11 +848,6 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
             const float rad = gdt_sqrt32(R2) * 1.0001f;
             gdt_circle_cells<float>(qxlo, qxhi, qylo, qyhi, rad, ox, oy, S.ibsx32, S.ibsy32,
                                     1e-2f, dx0, dx1, dy0, dy1);
-            cq[0] = qxlo;
-            cq[1] = qxhi;
-            cq[2] = qylo;
-            cq[3] = qyhi;
-            crad = rad;
             const float gx = fabsf(Kx) + EK, gy = fabsf(Ky) + EK;
             const bool within = wrap ? (gx * gx + gy * gy) * 1.001f <= (cov * cov) * (Klo * Klo)
                                      : R2 <= cov * cov * 0.9999f;
@@ -895,11 +870,8 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
         set_region(-h0 > h1 ? -h0 : h1);
         const GdtClip clx = gdt_left_clip(wrap, ax, ay, ox, oy, S.bsx32, S.bsy32,
                                           wrap ? BO * gdt_rcp32(fabsf(ay) * S.bsx32) + 1e-3f : 0.f);
-        const bool outer = dx0 >= -hs && dx1 <= hs;
         for (int dy = dy0; dy <= dy1;) {
-            const unsigned ne = gdt_disk_rows<float>(L, st.c0, dy, dy1, hs, outer, cq[0], cq[1],
-                                                     cq[2], cq[3], crad, ox, oy, S.bsy32, S.ibsx32,
-                                                     1e-2f, clx, st.tbl, sub, W);
+            const unsigned ne = gdt_box_rows(L, st.c0, dy, dy1, dx0, dx1, hs, clx, st.tbl, sub, W);
             gdt_scan_tbl(st.tbl, ne, load, visit);
             npred += ne;
         }

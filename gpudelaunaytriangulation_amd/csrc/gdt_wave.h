@@ -107,8 +107,11 @@ struct GdtWaveHost {
 // One point's ring.  `lead` is true on the one lane that writes the row.
 // npred counts the predicates this lane evaluated.  Built without the tie rule
 // (TIES false), a point that meets an exact tie returns ERR_TIE (its row
-// partly written) for the caller to hand to the variant with it.
-template <bool TIES = true, class W>
+// partly written) for the caller to hand to the variant with it.  Built for
+// rectangles only (GEN false: the LDS tile kernel, whose sets are rectangles),
+// the geometry takes the rectangular expressions, which the general ones
+// reduce to bit for bit, and the kernel keeps the registers of the others.
+template <bool TIES = true, bool GEN = true, class W>
 GDT_HD inline GdtRingResult gdt_wave_ring(const GdtSet& S, int j, const GdtRingIO& io, W& w,
                                           bool lead)
 {
@@ -116,7 +119,8 @@ GDT_HD inline GdtRingResult gdt_wave_ring(const GdtSet& S, int j, const GdtRingI
     const uint32_t pc = io.scell[j];
     const int cx = (int)(pc & 0xffffu), cy = (int)(pc >> 16);
     const gdt_d2 ov = io.off64[j];
-    const double covb = gdt_cover0(S, ov.x, ov.y);
+    const double covb = GEN ? gdt_cover0(S, ov.x, ov.y)
+                            : GDT_FMIN(GDT_FMIN(ov.x, S.bsx - ov.x), GDT_FMIN(ov.y, S.bsy - ov.y));
     const double bsmin = S.hmin;
     const size_t row = (size_t)io.perm[j] * (size_t)io.stride;
     unsigned npred = 0;
@@ -142,7 +146,8 @@ GDT_HD inline GdtRingResult gdt_wave_ring(const GdtSet& S, int j, const GdtRingI
                     const int x = gdt_wrap(cx + dx, S.xsize), y = gdt_wrap(cy + dy, S.ysize);
                     const int key = S.c0 + gdt_cell_key(S, x, y);
                     const uint32_t b = io.cstart[key], e = io.cstart[key + 1];
-                    const double bx = gdt_cbase_x(S, dx, dy, ov.x), by = gdt_cbase_y(S, dx, dy, ov.y);
+                    const double bx = GEN ? gdt_cbase_x(S, dx, dy, ov.x) : gdt_base64(dx, S.bsx, ov.x);
+                    const double by = GEN ? gdt_cbase_y(S, dx, dy, ov.y) : gdt_base64(dy, S.bsy, ov.y);
                     for (uint32_t k = b; k < e; ++k) {
                         if ((int)k == j || (int)k == a) continue;
                         const gdt_d2 o = io.off64[k];
@@ -195,7 +200,11 @@ GDT_HD inline GdtRingResult gdt_wave_ring(const GdtSet& S, int j, const GdtRingI
                 if (!wrap) {
                     if (cov > 0. && g.d2 <= cov * cov * (1.0 - 1e-12)) return true;
                     const double rad = gdt_sqrt64(g.d2) * (1.0 + 1e-9);
-                    gdt_disc_cells(S, 0., 0., 0., 0., rad, ov.x, ov.y, 1e-3, dx0, dx1, dy0, dy1);
+                    if constexpr (GEN)
+                        gdt_disc_cells(S, 0., 0., 0., 0., rad, ov.x, ov.y, 1e-3, dx0, dx1, dy0, dy1);
+                    else
+                        gdt_circle_cells<double>(0., 0., 0., 0., rad, ov.x, ov.y, S.ibsx, S.ibsy,
+                                                 1e-3, dx0, dx1, dy0, dy1);
                     boxed = true;
                 } else {
                     const Circ64 K = gdt_circ64(ax, ay, g.x, g.y);
@@ -208,8 +217,12 @@ GDT_HD inline GdtRingResult gdt_wave_ring(const GdtSet& S, int j, const GdtRingI
                         const double qx = -K.kx * i2k, qy = -K.ky * i2k;
                         const double rad = gdt_sqrt64(qx * qx + qy * qy);
                         const double m = 1e-6 * (fabs(qx) + fabs(qy) + rad) + 1e-9 * S.bsx;
-                        gdt_disc_cells(S, qx - m, qx + m, qy - m, qy + m, rad, ov.x, ov.y, 1e-3,
-                                       dx0, dx1, dy0, dy1);
+                        if constexpr (GEN)
+                            gdt_disc_cells(S, qx - m, qx + m, qy - m, qy + m, rad, ov.x, ov.y, 1e-3,
+                                           dx0, dx1, dy0, dy1);
+                        else
+                            gdt_circle_cells<double>(qx - m, qx + m, qy - m, qy + m, rad, ov.x,
+                                                     ov.y, S.ibsx, S.ibsy, 1e-3, dx0, dx1, dy0, dy1);
                         boxed = true;
                     }
                 }

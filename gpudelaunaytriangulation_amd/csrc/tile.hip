@@ -136,9 +136,10 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
         wv.lane = tid & 63;
         for (int t = tid >> 6; t < ndl; t += nt >> 6) {
             const int jd = s_dlist[t];
-            // (built without the tie rule, which would cost this kernel's main
-            // loop registers: a point with an exact tie goes to k_one_ring_wave)
-            const GdtRingResult r = gdt_wave_ring<false>(S, jd, io, wv, wv.lane == 0);
+            // (built without the tie rule and for rectangles only, which would
+            // cost this kernel's main loop registers: a point with an exact tie
+            // goes to k_one_ring_wave)
+            const GdtRingResult r = gdt_wave_ring<false, false>(S, jd, io, wv, wv.lane == 0);
             npred += r.npred;
             if (r.err & ERR_TIE) {
                 if (wv.lane == 0) {
