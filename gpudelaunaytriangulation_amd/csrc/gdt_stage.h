@@ -37,28 +37,96 @@ __device__ inline void block_scan_lds(uint32_t* a, int n, uint32_t* wsum)
     if (tid == 0) a[n] = tot;
 }
 
+// Global loads a lane issues before it waits for any (staging is a few
+// dependent rounds of loads per block: batching them keeps each round to one
+// memory latency instead of one per cell or point).
+#define GDT_STAGE_BATCH 4
+
 // Stage the window of tile w into L (cell starts, global starts, frame
 // coordinates, caller indices; FP32 coordinates too when F32).  Returns
-// whether it fits (block-uniform); ends with a barrier.
+// whether it fits (block-uniform); ends with a barrier.  Phase 1 reads the
+// window cells' slot ranges; phase 2 copies the points entry-parallel (lane
+// per LDS entry, so consecutive lanes read consecutive slots of a cell row),
+// each entry first told its window cell through L.orig, which its caller
+// index then overwrites.
 template <bool F32>
 __device__ inline bool gdt_stage_window(const GdtSet& S, const GdtTileWin& w, const GdtRingIO& io,
                                         const GdtTileLds& L, uint32_t* wsum)
 {
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int c = tid; c < GDT_WIN_CELLS; c += nt) {
-        int gs;
-        uint32_t cn;
-        gdt_tile_count(S, w, c, io.cstart, gs, cn);
-        L.gstart[c] = gs;
-        L.cs[c] = cn;
+    for (int c0 = 0; c0 < GDT_WIN_CELLS; c0 += GDT_STAGE_BATCH * nt) {
+        uint32_t b[GDT_STAGE_BATCH], e[GDT_STAGE_BATCH];
+#pragma unroll
+        for (int j = 0; j < GDT_STAGE_BATCH; ++j) {
+            const int c = c0 + j * nt + tid;
+            b[j] = e[j] = 0u;
+            if (c < GDT_WIN_CELLS) {
+                const int key = gdt_tile_key(S, w, c);
+                b[j] = io.cstart[key];
+                e[j] = io.cstart[key + 1];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < GDT_STAGE_BATCH; ++j) {
+            const int c = c0 + j * nt + tid;
+            if (c < GDT_WIN_CELLS) {
+                L.gstart[c] = (int)b[j];
+                L.cs[c] = e[j] - b[j];
+            }
+        }
     }
     __syncthreads();
     block_scan_lds(L.cs, GDT_WIN_CELLS, wsum);
     __syncthreads();
-    const bool fits = L.cs[GDT_WIN_CELLS] <= GDT_WIN_MAXPTS && S.rmax >= GDT_SCAN;
-    if (fits)
+    const int np = (int)L.cs[GDT_WIN_CELLS];
+    const bool fits = np <= GDT_WIN_MAXPTS && S.rmax >= GDT_SCAN;
+    if (fits) {
         for (int c = tid; c < GDT_WIN_CELLS; c += nt)
-            gdt_tile_copy<F32>(S, c, L.gstart[c], L.cs[c], L.cs[c + 1] - L.cs[c], io, L);
+            for (uint32_t k = L.cs[c], ke = L.cs[c + 1]; k < ke; ++k) L.orig[k] = c;
+        __syncthreads();
+        for (int k0 = 0; k0 < np; k0 += GDT_STAGE_BATCH * nt) {
+            int cell[GDT_STAGE_BATCH], slot[GDT_STAGE_BATCH], p[GDT_STAGE_BATCH];
+            gdt_d2 o[GDT_STAGE_BATCH];
+#pragma unroll
+            for (int j = 0; j < GDT_STAGE_BATCH; ++j) {
+                const int k = k0 + j * nt + tid;
+                cell[j] = 0;
+                slot[j] = 0;
+                if (k < np) {
+                    cell[j] = L.orig[k];
+                    slot[j] = L.gstart[cell[j]] + (k - (int)L.cs[cell[j]]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < GDT_STAGE_BATCH; ++j) {
+                const int k = k0 + j * nt + tid;
+                o[j].x = o[j].y = 0.0;
+                p[j] = 0;
+                if (k < np) {
+                    o[j] = io.off64[slot[j]];
+                    p[j] = io.perm[slot[j]];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < GDT_STAGE_BATCH; ++j) {
+                const int k = k0 + j * nt + tid;
+                if (k < np) {
+                    const int fx = cell[j] % GDT_WIN - GDT_FRAME, fy = cell[j] / GDT_WIN - GDT_FRAME;
+                    gdt_d2 X;
+                    X.x = gdt_frame64(fx, S.bsx, o[j].x);
+                    X.y = gdt_frame64(fy, S.bsy, o[j].y);
+                    L.X64[k] = X;
+                    L.orig[k] = p[j];
+                    if (F32) {
+                        gdt_f2 Y;
+                        Y.x = (float)X.x;
+                        Y.y = (float)X.y;
+                        L.X32[k] = Y;
+                    }
+                }
+            }
+        }
+    }
     __syncthreads();
     return fits;
 }

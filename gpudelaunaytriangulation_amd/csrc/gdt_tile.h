@@ -72,13 +72,19 @@ GDT_HD inline GdtTileWin gdt_tile_window(const GdtSet& S, int t)
     return w;
 }
 
+// Cell key of window cell c (wrapped on the torus).
+GDT_HD inline int gdt_tile_key(const GdtSet& S, const GdtTileWin& w, int c)
+{
+    const int wx = c % GDT_WIN, wy = c / GDT_WIN;
+    const int gx = gdt_wrapm(w.gx0 + wx, S.xsize), gy = gdt_wrapm(w.gy0 + wy, S.ysize);
+    return S.c0 + gdt_cell_key(S, gx, gy);
+}
+
 // Staging, phase 1: global slot range of window cell c.
 GDT_HD inline void gdt_tile_count(const GdtSet& S, const GdtTileWin& w, int c,
                                   const uint32_t* cstart, int& gstart, uint32_t& cnt)
 {
-    const int wx = c % GDT_WIN, wy = c / GDT_WIN;
-    const int gx = gdt_wrapm(w.gx0 + wx, S.xsize), gy = gdt_wrapm(w.gy0 + wy, S.ysize);
-    const int key = S.c0 + gdt_cell_key(S, gx, gy);
+    const int key = gdt_tile_key(S, w, c);
     const uint32_t b = cstart[key], e = cstart[key + 1];
     gstart = (int)b;
     cnt = e - b;
