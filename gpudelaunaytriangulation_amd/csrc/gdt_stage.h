@@ -42,13 +42,15 @@ __device__ inline void block_scan_lds(uint32_t* a, int n, uint32_t* wsum)
 // memory latency instead of one per cell or point).
 #define GDT_STAGE_BATCH 4
 
-// Stage the window of tile w into L (cell starts, global starts, frame
-// coordinates, caller indices; FP32 coordinates too when F32).  Returns
-// whether it fits (block-uniform); ends with a barrier.  Phase 1 reads the
-// window cells' slot ranges; phase 2 copies the points entry-parallel (lane
-// per LDS entry, so consecutive lanes read consecutive slots of a cell row),
-// each entry first told its window cell through L.orig, which its caller
-// index then overwrites.
+// Stage the window of tile w into L (cell starts, frame coordinates, caller
+// indices; FP32 coordinates too when F32; the cells' global starts when
+// L.gstart is set).  Returns whether it fits (block-uniform); ends with a
+// barrier.  Phase 1 reads the window cells' slot ranges; phase 2 copies the
+// points entry-parallel (lane per LDS entry, so consecutive lanes read
+// consecutive slots of a cell row), each entry first told its window cell
+// through L.orig, which its caller index then overwrites.  Without L.gstart
+// an entry's cell start is read again from the cell list (a cached read: the
+// block read it in phase 1), which keeps that array out of LDS.
 template <bool F32>
 __device__ inline bool gdt_stage_window(const GdtSet& S, const GdtTileWin& w, const GdtRingIO& io,
                                         const GdtTileLds& L, uint32_t* wsum)
@@ -70,7 +72,7 @@ __device__ inline bool gdt_stage_window(const GdtSet& S, const GdtTileWin& w, co
         for (int j = 0; j < GDT_STAGE_BATCH; ++j) {
             const int c = c0 + j * nt + tid;
             if (c < GDT_WIN_CELLS) {
-                L.gstart[c] = (int)b[j];
+                if (L.gstart) L.gstart[c] = (int)b[j];
                 L.cs[c] = e[j] - b[j];
             }
         }
@@ -94,8 +96,17 @@ __device__ inline bool gdt_stage_window(const GdtSet& S, const GdtTileWin& w, co
                 slot[j] = 0;
                 if (k < np) {
                     cell[j] = L.orig[k];
-                    slot[j] = L.gstart[cell[j]] + (k - (int)L.cs[cell[j]]);
+                    slot[j] = k - (int)L.cs[cell[j]];
+                    if (L.gstart) slot[j] += L.gstart[cell[j]];
                 }
+            }
+            if (!L.gstart) {
+                uint32_t b[GDT_STAGE_BATCH];
+#pragma unroll
+                for (int j = 0; j < GDT_STAGE_BATCH; ++j)
+                    b[j] = k0 + j * nt + tid < np ? io.cstart[gdt_tile_key(S, w, cell[j])] : 0u;
+#pragma unroll
+                for (int j = 0; j < GDT_STAGE_BATCH; ++j) slot[j] += (int)b[j];
             }
 #pragma unroll
             for (int j = 0; j < GDT_STAGE_BATCH; ++j) {

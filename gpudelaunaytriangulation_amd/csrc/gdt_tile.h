@@ -50,7 +50,7 @@ struct GdtTileWin {
 
 struct GdtTileLds {
     uint32_t* cs;        // [GDT_WIN_CELLS+1] first LDS entry of each window cell
-    int* gstart;         // [GDT_WIN_CELLS] first global slot of each window cell
+    int* gstart;         // [GDT_WIN_CELLS] first global slot of each window cell (or null)
     gdt_d2* X64;         // [GDT_WIN_MAXPTS + 1] FP64 frame coordinates
     gdt_f2* X32;         // [GDT_WIN_MAXPTS + 1] FP32 frame coordinates (filtered path)
     int* orig;           // [GDT_WIN_MAXPTS] caller index
@@ -359,6 +359,16 @@ struct GdtTileLane {
     uint32_t* tbl;    // the lane's row table: 5 words, and a sixth readable one after them
 };
 
+// LDS entry of caller index i in window cell c (a cell holds distinct points;
+// the search stays inside the cell's range whatever it finds).
+GDT_HD inline int gdt_tile_entry(const GdtTileLds& L, int c, int i)
+{
+    int k = (int)L.cs[c];
+    const int last = (int)L.cs[c + 1] - 1;
+    while (k < last && L.orig[k] != i) ++k;
+    return k;
+}
+
 template <class Real>
 GDT_HD inline void gdt_tile_lane_init(const GdtTileWin& w, const GdtTileLds& L, int j,
                                       const GdtRingIO& io, const GdtSet& S, GdtTileLane<Real>& st);
@@ -369,10 +379,11 @@ GDT_HD inline void gdt_tile_lane_init<double>(const GdtTileWin& w, const GdtTile
                                               GdtTileLane<double>& st)
 {
     const uint32_t pc = io.scell[j];
+    const int i = io.perm[j];
     const int wxv = (int)(pc & 0xffffu) - w.gx0, wyv = (int)(pc >> 16) - w.gy0;
     st.j = j;
     st.c0 = wyv * GDT_WIN + wxv;
-    st.iv = (int)L.cs[st.c0] + (j - L.gstart[st.c0]);
+    st.iv = gdt_tile_entry(L, st.c0, i);
     const gdt_d2 X = L.X64[st.iv];
     st.vx = X.x;
     st.vy = X.y;
@@ -381,7 +392,7 @@ GDT_HD inline void gdt_tile_lane_init<double>(const GdtTileWin& w, const GdtTile
     st.ak = st.n0 = -1;
     st.ax = st.ay = 0.;
     st.count = 0;
-    st.row = io.nbrs + (size_t)L.orig[st.iv] * (size_t)io.stride;
+    st.row = io.nbrs + (size_t)i * (size_t)io.stride;
 }
 
 template <>
@@ -390,10 +401,11 @@ GDT_HD inline void gdt_tile_lane_init<float>(const GdtTileWin& w, const GdtTileL
                                              GdtTileLane<float>& st)
 {
     const uint32_t pc = io.scell[j];
+    const int i = io.perm[j];
     const int wxv = (int)(pc & 0xffffu) - w.gx0, wyv = (int)(pc >> 16) - w.gy0;
     st.j = j;
     st.c0 = wyv * GDT_WIN + wxv;
-    st.iv = (int)L.cs[st.c0] + (j - L.gstart[st.c0]);
+    st.iv = gdt_tile_entry(L, st.c0, i);
     const gdt_f2 X = L.X32[st.iv];
     st.vx = X.x;
     st.vy = X.y;
@@ -402,7 +414,7 @@ GDT_HD inline void gdt_tile_lane_init<float>(const GdtTileWin& w, const GdtTileL
     st.ak = st.n0 = -1;
     st.ax = st.ay = 0.f;
     st.count = 0;
-    st.row = io.nbrs + (size_t)L.orig[st.iv] * (size_t)io.stride;
+    st.row = io.nbrs + (size_t)i * (size_t)io.stride;
 }
 
 // Offset of v inside its cell, recovered from the frame coordinate.

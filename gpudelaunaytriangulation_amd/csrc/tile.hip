@@ -34,7 +34,6 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
                      unsigned long long* __restrict__ counters, int* __restrict__ defer)
 {
     __shared__ uint32_t s_cs[GDT_WIN_CELLS + 1];
-    __shared__ int s_gstart[GDT_WIN_CELLS];
     __shared__ gdt_d2 s_X64[GDT_WIN_MAXPTS + 1];     // (+1: read by the scan's last prefetch)
     __shared__ gdt_f2 s_X32[PREC == 32 ? GDT_WIN_MAXPTS + 1 : 1];
     __shared__ int s_orig[GDT_WIN_MAXPTS];
@@ -48,7 +47,7 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
     const GdtSet S = sets[ch.set];
     const GdtTileWin w = gdt_tile_window(S, ch.start);
     GdtTileLds L;
-    L.cs = s_cs; L.gstart = s_gstart; L.X64 = s_X64; L.X32 = s_X32; L.orig = s_orig;
+    L.cs = s_cs; L.gstart = nullptr; L.X64 = s_X64; L.X32 = s_X32; L.orig = s_orig;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int k0 = S.c0 + ch.start * GDT_TILE_CELLS;
     const int T0 = (int)io.cstart[k0], T1 = (int)io.cstart[k0 + GDT_TILE_CELLS];
