@@ -5,8 +5,9 @@ kernels (the reference fixes 128, DelaunayGPU.cu:16), the cell-size scale of
 the internal grid (the reference takes the caller's cell size,
 triangulation.cpp:172), the cell order (LDS tiles or row-major), the LDS
 tile's edge (16x16 or 32x32 cells: the window every block stages) and the
-lanes that share one point (one or two, for the global triangulation and for
-repairs).  Every candidate is timed on the caller's own points with HIP
+lanes that share one point (one, two or four, for the global triangulation
+and for repairs; the FP32-filtered kernel shares a point between at most
+two).  Every candidate is timed on the caller's own points with HIP
 events; the rings of every candidate must be identical (canonical hash) --
 the output never depends on the configuration -- and the fastest is applied
 to the context.  ``tune`` times global triangulations; ``tune_md`` times the
@@ -38,7 +39,7 @@ def _as_config(c) -> dict:
     return dict(zip(KNOBS, c))
 
 
-def candidates(blocks=DEFAULT_BLOCKS, scales=DEFAULT_SCALES, tile_shifts=(4, 5), lanes=(1, 2),
+def candidates(blocks=DEFAULT_BLOCKS, scales=DEFAULT_SCALES, tile_shifts=(4, 5), lanes=(1, 2, 4),
                row_major=True):
     """Global-triangulation candidates: every block x scale x tile edge x
     lanes per point in tile order, plus the row-major order at the caller's
@@ -50,7 +51,7 @@ def candidates(blocks=DEFAULT_BLOCKS, scales=DEFAULT_SCALES, tile_shifts=(4, 5),
     return out
 
 
-def md_candidates(blocks=DEFAULT_BLOCKS, tile_shifts=(4, 5), repair_lanes=(1, 2)):
+def md_candidates(blocks=DEFAULT_BLOCKS, tile_shifts=(4, 5), repair_lanes=(1, 2, 4)):
     """Brownian-loop candidates: block x tile edge x lanes per repaired point."""
     return [dict(block=b, tiled=1, tile_shift=t, repair_lanes=r)
             for b, t, r in itertools.product(blocks, tile_shifts, repair_lanes)]

@@ -472,8 +472,8 @@ int gdt_set_config(gdt_context* c, const gdt_config* cfg)
     if (!(cfg->cell_scale >= 0.25 && cfg->cell_scale <= 8.0)) return GDT_ERR_INVALID;
     if (cfg->tile_shift != 0 && (cfg->tile_shift < GDT_TSHIFT_MIN || cfg->tile_shift > GDT_TSHIFT_MAX))
         return GDT_ERR_INVALID;
-    if (cfg->lanes < 0 || cfg->lanes > 2 || cfg->repair_lanes < 0 || cfg->repair_lanes > 2)
-        return GDT_ERR_INVALID;
+    auto lanes_ok = [](int w) { return w == 0 || w == 1 || w == 2 || w == 4; };
+    if (!lanes_ok(cfg->lanes) || !lanes_ok(cfg->repair_lanes)) return GDT_ERR_INVALID;
     c->cfg = *cfg;
     c->cfg.tiled = cfg->tiled == 2 ? 2 : (cfg->tiled ? 1 : 0);
     c->sets.clear();          // force a new set table (cell size / order may change)
@@ -584,7 +584,7 @@ static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int*
         // ring instead of after two.
         const int ts = c->sets[0].tshift, lanes = lanes_for(c);
         c->stats.tile_shift = ts;
-        c->stats.lanes = lanes;
+        c->stats.lanes = lanes == 4 && precision == GDT_PREC_FP32F ? 2 : lanes;
         const int one_each = std::min(1024, (1 << (2 * ts)) * lanes);
         const int block = c->ntch <= c->num_cu ? std::max(tile_block(c), one_each) : tile_block(c);
         GDT_CHECK(gdt_tiles::one_ring(ts, precision, c->d_sets.p, (int)c->sets.size(), c->tch.p,

@@ -23,9 +23,10 @@ namespace GDT_TS_NS {
 // staging.  The points of a window that does not fit the LDS budget, or of a
 // grid too small for the scan block, and points with a step whose circle
 // leaves the window, are appended to `defer` and handled by k_one_ring_wave
-// in the next launch.  With W = 2 two neighbouring lanes share each point, each
-// scanning half of every row (a list leaves most lanes of a block without a
-// point otherwise; the launcher takes W from the caller).
+// in the next launch.  With W = 2 or 4 (FP64) neighbouring lanes share each
+// point, each scanning its part of every row (a list leaves most lanes of a
+// block without a point otherwise, and a small grid leaves most of the chip
+// idle; the launcher takes W from the caller).
 
 template <int PREC, int W>
 __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict__ sets, const GdtChunk* __restrict__ tiles,
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
     // Lanes take points of the tile and advance them one search step per
     // iteration; a lane whose point is finished takes the next one, so a
     // wave's iterations track the total number of steps, not the longest ring.
-    static_assert(W == 1 || W == 2, "one or two lanes per point");
+    static_assert(W == 1 || W == 2 || (W == 4 && PREC == 64), "1, 2 or (FP64) 4 lanes per point");
     if (tid == 0) {
         s_next = Q0 + nt / W;
         s_dn = 0u;
@@ -197,8 +198,12 @@ hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nse
     if (ntiles <= 0) return hipSuccess;
     if (block > 1024) block = 1024;
     const size_t tbl = (5 * (size_t)block + 1) * sizeof(uint32_t);
-    const bool two = lanes == 2;
-    if (precision == 32 && two)
+    // (the FP32-filtered step shares a point between at most two lanes)
+    const bool two = lanes == 2 || (lanes == 4 && precision == 32);
+    if (precision == 64 && lanes == 4)
+        hipLaunchKernelGGL((k_one_ring_tile<64, 4>), dim3(ntiles), dim3(block), tbl, s, d_sets,
+                           tiles, list, lscan, io, counts, status, counters, defer);
+    else if (precision == 32 && two)
         hipLaunchKernelGGL((k_one_ring_tile<32, 2>), dim3(ntiles), dim3(block), tbl, s, d_sets,
                            tiles, list, lscan, io, counts, status, counters, defer);
     else if (precision == 32)

@@ -58,11 +58,12 @@ typedef struct {
     int tile_shift;     /* LDS tile of the tiled orders: 2^tile_shift cells a side, 4 (16x16)
                            or 5 (32x32); 0 (default) = automatic: 16x16 when the 32x32
                            tiles would not outnumber the compute units, else 32x32 */
-    int lanes;          /* lanes per point of the global triangulation's tile kernel, 1 or 2;
+    int lanes;          /* lanes per point of the global triangulation's tile kernel, 1, 2
+                           or 4 (4: FP64; the FP32-filtered kernel takes 2 then);
                            0 (default) = automatic: 2 with 16x16 tiles that do not outnumber
                            the compute units, else 1 */
-    int repair_lanes;   /* lanes per point of the repairs' tile kernel, 1 or 2; 0 (default)
-                           = 2 */
+    int repair_lanes;   /* lanes per point of the repairs' tile kernel, 1, 2 or 4 (as
+                           lanes); 0 (default) = 2 */
 } gdt_config;
 
 /* Statistics of the last triangulate / repair call. */
@@ -84,7 +85,7 @@ typedef struct {
     unsigned long long profiled_calls;
     double ms_cells_total, ms_ring_total, ms_defer_total;
     int tile_shift;                  /* tile edge of the last global triangulation (0: row-major) */
-    int lanes;                       /* its lanes per point */
+    int lanes;                       /* its lanes per point (as launched) */
 } gdt_stats;
 
 /* ---- context ---------------------------------------------------------- */
