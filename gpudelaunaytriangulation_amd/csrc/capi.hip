@@ -606,6 +606,10 @@ static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int*
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[3], s));
     rc = finish_rings(c, stride, required, s);
     if (c->cfg.profile) {
+        // The status words reach the host before the runtime need have seen
+        // the events complete (an elapsed time read then is "not ready"): wait
+        // for the last one, which the stream completes after the others.
+        GDT_CHECK(hipEventSynchronize(c->ev[3]));
         float a = 0.f, b = 0.f, d = 0.f;
         GDT_CHECK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
         GDT_CHECK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));

@@ -55,18 +55,16 @@ template <bool F32>
 __device__ inline bool gdt_stage_window(const GdtSet& S, const GdtTileWin& w, const GdtRingIO& io,
                                         const GdtTileLds& L, uint32_t* wsum)
 {
+    // (indices past the end are clamped to the last valid one, so every load
+    // of a batch is issued without a branch and the batch waits once)
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int c0 = 0; c0 < GDT_WIN_CELLS; c0 += GDT_STAGE_BATCH * nt) {
         uint32_t b[GDT_STAGE_BATCH], e[GDT_STAGE_BATCH];
 #pragma unroll
         for (int j = 0; j < GDT_STAGE_BATCH; ++j) {
-            const int c = c0 + j * nt + tid;
-            b[j] = e[j] = 0u;
-            if (c < GDT_WIN_CELLS) {
-                const int key = gdt_tile_key(S, w, c);
-                b[j] = io.cstart[key];
-                e[j] = io.cstart[key + 1];
-            }
+            const int key = gdt_tile_key(S, w, min(c0 + j * nt + tid, GDT_WIN_CELLS - 1));
+            b[j] = io.cstart[key];
+            e[j] = io.cstart[key + 1];
         }
 #pragma unroll
         for (int j = 0; j < GDT_STAGE_BATCH; ++j) {
@@ -82,7 +80,7 @@ __device__ inline bool gdt_stage_window(const GdtSet& S, const GdtTileWin& w, co
     __syncthreads();
     const int np = (int)L.cs[GDT_WIN_CELLS];
     const bool fits = np <= GDT_WIN_MAXPTS && S.rmax >= GDT_SCAN;
-    if (fits) {
+    if (fits && np > 0) {
         for (int c = tid; c < GDT_WIN_CELLS; c += nt)
             for (uint32_t k = L.cs[c], ke = L.cs[c + 1]; k < ke; ++k) L.orig[k] = c;
         __syncthreads();
@@ -91,33 +89,26 @@ __device__ inline bool gdt_stage_window(const GdtSet& S, const GdtTileWin& w, co
             gdt_d2 o[GDT_STAGE_BATCH];
 #pragma unroll
             for (int j = 0; j < GDT_STAGE_BATCH; ++j) {
-                const int k = k0 + j * nt + tid;
-                cell[j] = 0;
-                slot[j] = 0;
-                if (k < np) {
-                    cell[j] = L.orig[k];
-                    slot[j] = k - (int)L.cs[cell[j]];
-                    if (L.gstart) slot[j] += L.gstart[cell[j]];
-                }
+                const int k = min(k0 + j * nt + tid, np - 1);
+                cell[j] = L.orig[k];
+                slot[j] = k - (int)L.cs[cell[j]];
+                if (L.gstart) slot[j] += L.gstart[cell[j]];
             }
             if (!L.gstart) {
                 uint32_t b[GDT_STAGE_BATCH];
 #pragma unroll
-                for (int j = 0; j < GDT_STAGE_BATCH; ++j)
-                    b[j] = k0 + j * nt + tid < np ? io.cstart[gdt_tile_key(S, w, cell[j])] : 0u;
+                for (int j = 0; j < GDT_STAGE_BATCH; ++j) b[j] = io.cstart[gdt_tile_key(S, w, cell[j])];
 #pragma unroll
                 for (int j = 0; j < GDT_STAGE_BATCH; ++j) slot[j] += (int)b[j];
             }
 #pragma unroll
             for (int j = 0; j < GDT_STAGE_BATCH; ++j) {
-                const int k = k0 + j * nt + tid;
-                o[j].x = o[j].y = 0.0;
-                p[j] = 0;
-                if (k < np) {
-                    o[j] = io.off64[slot[j]];
-                    p[j] = io.perm[slot[j]];
-                }
+                o[j] = io.off64[slot[j]];
+                p[j] = io.perm[slot[j]];
             }
+            // (every lane read its entries' cells before any is overwritten:
+            // a clamped lane reads entry np - 1, which another lane writes)
+            __syncthreads();
 #pragma unroll
             for (int j = 0; j < GDT_STAGE_BATCH; ++j) {
                 const int k = k0 + j * nt + tid;
