@@ -589,7 +589,8 @@ static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int*
         const int block = c->ntch <= c->num_cu ? std::max(tile_block(c), one_each) : tile_block(c);
         GDT_CHECK(gdt_tiles::one_ring(ts, precision, c->d_sets.p, (int)c->sets.size(), c->tch.p,
                                       c->ntch, block, lanes, c->total_n, nullptr, nullptr,
-                                      ring_io(c, d_nbrs, stride), d_counts, c->status.p,
+                                      nullptr, nullptr, ring_io(c, d_nbrs, stride), d_counts,
+                                      c->status.p,
                                       c->counters.p, c->defer.p,
                                       c->cfg.profile ? c->ev[2] : nullptr, s));
     } else {
@@ -761,13 +762,23 @@ int gdt_test_circumcircles(gdt_context* c, const double* d_xy, int n, const int*
     return GDT_OK;
 }
 
-// Compact the flags (slot order) and redo the rings of the flagged points:
-// with the tiled cell order through the LDS tile kernel, restricted to each
-// tile's flagged slots (tiles without any exit at once), else one lane each.
-// status[ST_NDEFER] must be zero on entry (reset_status).
+// Redo the rings of the flagged points: with the tiled cell order through
+// the LDS tile kernel, each tile reading its own slots' flags (tiles without
+// any exit before staging), else the flags compacted (slot order) and one
+// lane each.  status[ST_NDEFER] and status[ST_NFLAGGED] must be zero on entry
+// (reset_status).
 static int repair_flagged(gdt_context* c, const int* d_repair, int stride, int* d_nbrs,
                           int* d_counts, int precision, hipStream_t s)
 {
+    if (tiled_sets(c)) {
+        GDT_CHECK(gdt_tiles::one_ring(c->sets[0].tshift, precision, c->d_sets.p,
+                                      (int)c->sets.size(), c->tch.p, c->ntch, tile_block(c),
+                                      c->cfg.repair_lanes ? c->cfg.repair_lanes : 2, c->total_n,
+                                      nullptr, nullptr, d_repair, c->halt,
+                                      ring_io(c, d_nbrs, stride), d_counts, c->status.p,
+                                      c->counters.p, c->defer.p, nullptr, s));
+        return GDT_OK;
+    }
     const size_t n = (size_t)c->total_n;
     GDT_CHECK(c->mark.reserve(n));
     GDT_CHECK(c->markscan.reserve(n + 1));
@@ -775,15 +786,6 @@ static int repair_flagged(gdt_context* c, const int* d_repair, int stride, int* 
     uint32_t* worklen = reinterpret_cast<uint32_t*>(c->status.p + ST_NFLAGGED);
     GDT_CHECK(gdt_launch_compact(d_repair, c->perm.p, c->total_n, c->mark.p, c->markscan.p,
                                  c->worklist.p, worklen, c->scan_tmp.p, c->halt, s));
-    if (tiled_sets(c)) {
-        GDT_CHECK(gdt_tiles::one_ring(c->sets[0].tshift, precision, c->d_sets.p,
-                                      (int)c->sets.size(), c->tch.p, c->ntch, tile_block(c),
-                                      c->cfg.repair_lanes ? c->cfg.repair_lanes : 2, c->total_n,
-                                      c->worklist.p,
-                                      c->markscan.p, ring_io(c, d_nbrs, stride), d_counts,
-                                      c->status.p, c->counters.p, c->defer.p, nullptr, s));
-        return GDT_OK;
-    }
     const GdtRingIO io = ring_io(c, d_nbrs, stride);
     GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), nullptr, 0,
                                   c->cfg.block, c->worklist.p, worklen, 0, c->total_n, io,

@@ -98,6 +98,7 @@ hipError_t gdt_launch_deferred(int precision, const GdtSet* d_sets, int nsets, i
     hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nsets,          \
                                         const GdtChunk* tiles, int ntiles, int block, int lanes, \
                                         int total_n, const int* list, const uint32_t* lscan,     \
+                                        const int* flags, const int* halt,                       \
                                         const GdtRingIO& io, int* counts, int* status,           \
                                         unsigned long long* counters, int* defer,                \
                                         hipEvent_t after_tile, hipStream_t s);                   \
@@ -126,11 +127,14 @@ hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, con
                            hipStream_t s);
 
 
-// One-ring of every point (list null) or of the slots of a slot-ordered list
-// (lscan its exclusive scan over slots), one block per tile of the tiled cell
-// order, `lanes` (1 or 2) lanes per point.  Points the tile kernel cannot
-// finish from LDS are deferred (gdt_launch_deferred over `defer`, size
-// total_n, count in status[ST_NDEFER]).
+// One-ring of every point (list and flags null), of the slots of a
+// slot-ordered list (lscan its exclusive scan over slots), or of the points
+// flagged in a caller-order array (flags[i] >= 0; each tile finds its own
+// flagged slots, status[ST_NFLAGGED] += their count, which must start at zero;
+// nothing with *halt set), one block per tile of the tiled cell order,
+// `lanes` (1, 2 or 4) lanes per point.  Points the tile kernel cannot finish
+// from LDS are deferred (gdt_launch_deferred over `defer`, size total_n, count
+// in status[ST_NDEFER]).
 //
 // Enumeration and empty-circle test in one pass over the rings, one block per
 // tile (tiled cell order): triangles owned by their smallest caller index,
@@ -143,13 +147,14 @@ static_assert(GDT_TSHIFT_MIN == 4 && GDT_TSHIFT_MAX == 5, "a launcher per suppor
 namespace gdt_tiles {
 inline hipError_t one_ring(int tshift, int precision, const GdtSet* d_sets, int nsets,
                            const GdtChunk* tiles, int ntiles, int block, int lanes, int total_n,
-                           const int* list, const uint32_t* lscan, const GdtRingIO& io,
-                           int* counts, int* status, unsigned long long* counters, int* defer,
-                           hipEvent_t after_tile, hipStream_t s)
+                           const int* list, const uint32_t* lscan, const int* flags,
+                           const int* halt, const GdtRingIO& io, int* counts, int* status,
+                           unsigned long long* counters, int* defer, hipEvent_t after_tile,
+                           hipStream_t s)
 {
     return (tshift == 4 ? ts4::gdt_launch_one_ring_tile : ts5::gdt_launch_one_ring_tile)(
-        precision, d_sets, nsets, tiles, ntiles, block, lanes, total_n, list, lscan, io, counts,
-        status, counters, defer, after_tile, s);
+        precision, d_sets, nsets, tiles, ntiles, block, lanes, total_n, list, lscan, flags, halt,
+        io, counts, status, counters, defer, after_tile, s);
 }
 inline hipError_t ring_test(int tshift, const GdtSet* d_sets, const GdtChunk* tiles, int ntiles,
                             const int* nbrs, const int* counts, int stride, int total_n,

@@ -160,7 +160,7 @@ extern "C" int gdt_triangulate_noalloc(const double* d_xy, int n, double lx, dou
     // the block size of gdt_triangulate's default configuration
     const int block = ntiles <= cus ? 1024 : 512;
     if (gdt_tiles::one_ring(GDT_TILE_SHIFT, precision, d_sets, 1, d_tiles, ntiles, block, 1, n,
-                            nullptr, nullptr, io, d_counts, status, counters,
+                            nullptr, nullptr, nullptr, nullptr, io, d_counts, status, counters,
                             reinterpret_cast<int*>(base + L.defer), nullptr, s) != hipSuccess)
         return GDT_ERR_DEVICE;
     int h_status[ST_WORDS];

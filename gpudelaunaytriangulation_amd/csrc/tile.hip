@@ -16,11 +16,13 @@
 namespace GDT_TS_NS {
 
 // One block per tile (GDT_TILE x GDT_TILE cells, this file's compiled tile edge): stage the tile window in LDS, then the lanes
-// advance the tile's points one search step at a time.  With `list` null the
-// tile's points are all its slots [T0, T1); otherwise they are the slots
-// list[lscan[T0] .. lscan[T1]) (a slot-ordered list of the points to redo, with
-// lscan its exclusive scan over slots), and a tile with none exits before
-// staging.  The points of a window that does not fit the LDS budget, or of a
+// advance the tile's points one search step at a time.  With `list` and
+// `flags` null the tile's points are all its slots [T0, T1); with `list` they
+// are the slots list[lscan[T0] .. lscan[T1]) (a slot-ordered list of the
+// points to redo, with lscan its exclusive scan over slots); with `flags` (a
+// caller-order array, flagged >= 0) they are the tile's flagged slots, which
+// the block gathers first into a bit mask in LDS (status[ST_NFLAGGED] += their
+// count).  A tile with none exits before staging.  The points of a window that does not fit the LDS budget, or of a
 // grid too small for the scan block, and points with a step whose circle
 // leaves the window, are appended to `defer` and handled by k_one_ring_wave
 // in the next launch.  With W = 2 or 4 (FP64) neighbouring lanes share each
@@ -28,9 +30,31 @@ namespace GDT_TS_NS {
 // block without a point otherwise, and a small grid leaves most of the chip
 // idle; the launcher takes W from the caller).
 
+// The tile's flagged slots (flags mode): slot T0 + 64 w + b is flagged when bit
+// b of mask[w] is set; pre[w] = flagged slots before word w.
+struct TileFlags {
+    const unsigned long long* mask;
+    const uint32_t* pre;
+    int nw;     // words
+};
+
+// Slot of the q-th flagged slot (q below the tile's count).
+__device__ inline int tile_flagged_slot(const TileFlags& f, int T0, int q)
+{
+    int lo = 0, hi = f.nw - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int)f.pre[mid] <= q) lo = mid; else hi = mid - 1;
+    }
+    unsigned long long m = f.mask[lo];
+    for (int r = q - (int)f.pre[lo]; r > 0; --r) m &= m - 1ull;
+    return T0 + 64 * lo + __ffsll((long long)m) - 1;
+}
+
 template <int PREC, int W>
 __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict__ sets, const GdtChunk* __restrict__ tiles,
                      const int* __restrict__ list, const uint32_t* __restrict__ lscan,
+                     const int* __restrict__ flags, const int* __restrict__ halt,
                      GdtRingIO io, int* __restrict__ counts, int* __restrict__ status,
                      unsigned long long* __restrict__ counters, int* __restrict__ defer)
 {
@@ -42,6 +66,8 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
     __shared__ int s_next;
     __shared__ int s_dlist[GDT_TILE_DEFER];
     __shared__ uint32_t s_dn;
+    __shared__ unsigned long long s_fmask[GDT_WIN_MAXPTS / 64];
+    __shared__ uint32_t s_fpre[GDT_WIN_MAXPTS / 64 + 1];
     extern __shared__ uint32_t s_tbl[];     // [5 * blockDim + 1] the lanes' row tables
 
     const GdtChunk ch = tiles[blockIdx.x];
@@ -52,8 +78,52 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
     const int tid = threadIdx.x, nt = blockDim.x;
     const int k0 = S.c0 + ch.start * GDT_TILE_CELLS;
     const int T0 = (int)io.cstart[k0], T1 = (int)io.cstart[k0 + GDT_TILE_CELLS];
-    // the tile's items q in [Q0, Q1): point j = list ? list[q] : q
-    const int Q0 = list ? (int)lscan[T0] : T0, Q1 = list ? (int)lscan[T1] : T1;
+    // the tile's items q in [Q0, Q1): point j = item(q)
+    int Q0 = list ? (int)lscan[T0] : T0, Q1 = list ? (int)lscan[T1] : T1;
+    TileFlags tf;
+    tf.mask = s_fmask;
+    tf.pre = s_fpre;
+    tf.nw = 0;
+    if (flags) {
+        if (halt && *halt) return;
+        const int npt = T1 - T0;
+        if (npt > GDT_WIN_MAXPTS) {
+            // (more points than a window holds: the window cannot fit either)
+            unsigned nf = 0;
+            for (int q = T0 + tid; q < T1; q += nt)
+                if (flags[io.perm[q]] >= 0) {
+                    ++nf;
+                    const uint32_t pos = atomicAdd(reinterpret_cast<unsigned int*>(&status[ST_NDEFER]), 1u);
+                    defer[pos] = q;
+                }
+            if (nf) atomicAdd(&status[ST_NFLAGGED], (int)nf);
+            return;
+        }
+        tf.nw = (npt + 63) >> 6;
+        for (int t0 = 0; t0 < 64 * tf.nw; t0 += nt) {
+            const int t = t0 + tid;
+            const bool f = t < npt && flags[io.perm[T0 + t]] >= 0;
+            const unsigned long long b = __ballot(f);
+            if ((tid & 63) == 0 && (t >> 6) < tf.nw) s_fmask[t >> 6] = b;
+        }
+        __syncthreads();
+        if (tid < 64) {
+            const uint32_t c = tid < tf.nw ? (uint32_t)__popcll(s_fmask[tid]) : 0u;
+            uint32_t x = c;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(x, d, 64);
+                if (tid >= d) x += y;
+            }
+            if (tid < tf.nw) s_fpre[tid + 1] = x;
+            if (tid == 0) s_fpre[0] = 0u;
+        }
+        __syncthreads();
+        Q0 = 0;
+        Q1 = (int)s_fpre[tf.nw];
+        if (tid == 0 && Q1 > 0) atomicAdd(&status[ST_NFLAGGED], Q1);
+    }
+    auto item = [&](int q) { return flags ? tile_flagged_slot(tf, T0, q) : (list ? list[q] : q); };
     if (Q0 >= Q1) return;    // block-uniform: nothing to do in this tile
 
     const bool fits = gdt_stage_window<PREC == 32>(S, w, io, L, s_wsum);
@@ -61,7 +131,7 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
     if (!fits) {
         for (int q = Q0 + tid; q < Q1; q += nt) {
             const uint32_t pos = atomicAdd(reinterpret_cast<unsigned int*>(&status[ST_NDEFER]), 1u);
-            defer[pos] = list ? list[q] : q;
+            defer[pos] = item(q);
         }
         return;
     }
@@ -80,7 +150,7 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
     const int sub = tid & (W - 1);      // the lane's part of its point's rows
     int q = Q0 + tid / W;
     bool have = q < Q1;
-    int j = have ? (list ? list[q] : q) : 0;
+    int j = have ? item(q) : 0;
     if (have) gdt_tile_lane_init<Real>(w, L, j, io, S, st);
     unsigned npred = 0, nfb = 0, nsum = 0;
     int mx = 0, er = 0;
@@ -119,7 +189,7 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
                 }
                 have = q < Q1;
                 if (have) {
-                    j = list ? list[q] : q;
+                    j = item(q);
                     gdt_tile_lane_init<Real>(w, L, j, io, S, st);
                 }
             }
@@ -191,6 +261,7 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
 hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nsets,
                                     const GdtChunk* tiles, int ntiles, int block, int lanes,
                                     int total_n, const int* list, const uint32_t* lscan,
+                                    const int* flags, const int* halt,
                                     const GdtRingIO& io, int* counts, int* status,
                                     unsigned long long* counters, int* defer,
                                     hipEvent_t after_tile, hipStream_t s)
@@ -202,19 +273,19 @@ hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nse
     const bool two = lanes == 2 || (lanes == 4 && precision == 32);
     if (precision == 64 && lanes == 4)
         hipLaunchKernelGGL((k_one_ring_tile<64, 4>), dim3(ntiles), dim3(block), tbl, s, d_sets,
-                           tiles, list, lscan, io, counts, status, counters, defer);
+                           tiles, list, lscan, flags, halt, io, counts, status, counters, defer);
     else if (precision == 32 && two)
         hipLaunchKernelGGL((k_one_ring_tile<32, 2>), dim3(ntiles), dim3(block), tbl, s, d_sets,
-                           tiles, list, lscan, io, counts, status, counters, defer);
+                           tiles, list, lscan, flags, halt, io, counts, status, counters, defer);
     else if (precision == 32)
         hipLaunchKernelGGL((k_one_ring_tile<32, 1>), dim3(ntiles), dim3(block), tbl, s, d_sets,
-                           tiles, list, lscan, io, counts, status, counters, defer);
+                           tiles, list, lscan, flags, halt, io, counts, status, counters, defer);
     else if (two)
         hipLaunchKernelGGL((k_one_ring_tile<64, 2>), dim3(ntiles), dim3(block), tbl, s, d_sets,
-                           tiles, list, lscan, io, counts, status, counters, defer);
+                           tiles, list, lscan, flags, halt, io, counts, status, counters, defer);
     else
         hipLaunchKernelGGL((k_one_ring_tile<64, 1>), dim3(ntiles), dim3(block), tbl, s, d_sets,
-                           tiles, list, lscan, io, counts, status, counters, defer);
+                           tiles, list, lscan, flags, halt, io, counts, status, counters, defer);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (after_tile && (e = hipEventRecord(after_tile, s)) != hipSuccess) return e;
