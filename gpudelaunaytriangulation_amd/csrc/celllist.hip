@@ -338,17 +338,43 @@ bool gdt_cells_two_level(size_t n, size_t k)
     return kb_plan(n, k).B <= KB_MAXSHIFT;
 }
 
-// Cell of caller point i (its set found by binary search over the sets).
-__device__ inline void kb_bin(const GdtSet* __restrict__ sets, int nsets, const gdt_d2* __restrict__ xy,
-                              int i, uint32_t& key, uint32_t& packed, double& ox, double& oy)
+// A lane's current set while it walks increasing caller indices (the sets are
+// contiguous index ranges in order): one binary search for its first point,
+// then a step forward whenever a point lies past the set's end.
+struct KbSetCursor {
+    const GdtSet* sets;
+    int nsets, k, end;
+    GdtSet S;
+
+    __device__ void start(const GdtSet* __restrict__ all, int count, int i)
+    {
+        sets = all;
+        nsets = count;
+        k = gdt_set_of(all, count, i);
+        S = all[k];
+        end = S.p0 + S.n;
+    }
+    __device__ void advance(int i)
+    {
+        while (i >= end && k + 1 < nsets) {
+            S = sets[++k];
+            end = S.p0 + S.n;
+        }
+    }
+};
+
+// Cell of caller point i at p in set S.
+__device__ inline void kb_bin_in(const GdtSet& S, const gdt_d2 p, uint32_t& key, uint32_t& packed,
+                                 double& ox, double& oy)
 {
-    const GdtSet S = sets[gdt_set_of(sets, nsets, i)];
-    const gdt_d2 p = xy[i];
     int cx, cy;
     gdt_bin_point(S, p.x, p.y, cx, cy, ox, oy);
     key = (uint32_t)(S.c0 + gdt_cell_key(S, cx, cy));
     packed = (uint32_t)cx | ((uint32_t)cy << 16);
 }
+
+// Points a lane reads before it bins any (their loads overlap).
+#define KB_BATCH 4
 
 // Pass 1: bmat[bucket * nb + block] = points of the block in the bucket.
 __global__ __launch_bounds__(KB_THREADS) void k_kb_count(const GdtSet* __restrict__ sets, int nsets,
@@ -362,11 +388,25 @@ __global__ __launch_bounds__(KB_THREADS) void k_kb_count(const GdtSet* __restric
     for (int q = tid; q < pl.nkb; q += KB_THREADS) hist[q] = 0u;
     __syncthreads();
     const int i0 = b * pl.per, i1 = min(n, i0 + pl.per);
-    for (int i = i0 + tid; i < i1; i += KB_THREADS) {
-        uint32_t key, packed;
-        double ox, oy;
-        kb_bin(sets, nsets, xy, i, key, packed, ox, oy);
-        atomicAdd(&hist[key >> pl.B], 1u);
+    if (i0 + tid < i1) {
+        KbSetCursor cur;
+        cur.start(sets, nsets, i0 + tid);
+        for (int ib = i0 + tid; ib < i1; ib += KB_BATCH * KB_THREADS) {
+            gdt_d2 p[KB_BATCH];
+#pragma unroll
+            for (int u = 0; u < KB_BATCH; ++u) p[u] = xy[min(ib + u * KB_THREADS, i1 - 1)];
+#pragma unroll
+            for (int u = 0; u < KB_BATCH; ++u) {
+                const int i = ib + u * KB_THREADS;
+                if (i < i1) {
+                    cur.advance(i);
+                    uint32_t key, packed;
+                    double ox, oy;
+                    kb_bin_in(cur.S, p[u], key, packed, ox, oy);
+                    atomicAdd(&hist[key >> pl.B], 1u);
+                }
+            }
+        }
     }
     __syncthreads();
     for (int q = tid; q < pl.nkb; q += KB_THREADS) bmat[(size_t)q * pl.nb + b] = hist[q];
@@ -455,12 +495,26 @@ __global__ __launch_bounds__(KB_THREADS) void k_kb_scatter(const GdtSet* __restr
     for (int q = tid; q < pl.nkb; q += KB_THREADS) cursor[q] += bscan[(size_t)q * pl.nb + b];
     __syncthreads();
     const int i0 = b * pl.per, i1 = min(n, i0 + pl.per);
-    for (int i = i0 + tid; i < i1; i += KB_THREADS) {
-        GdtKbRec r;
-        kb_bin(sets, nsets, xy, i, r.key, r.cell, r.off.x, r.off.y);
-        r.idx = i;
-        r.pad = 0u;
-        bkt[atomicAdd(&cursor[r.key >> pl.B], 1u)] = r;
+    if (i0 + tid < i1) {
+        KbSetCursor cur;
+        cur.start(sets, nsets, i0 + tid);
+        for (int ib = i0 + tid; ib < i1; ib += KB_BATCH * KB_THREADS) {
+            gdt_d2 p[KB_BATCH];
+#pragma unroll
+            for (int u = 0; u < KB_BATCH; ++u) p[u] = xy[min(ib + u * KB_THREADS, i1 - 1)];
+#pragma unroll
+            for (int u = 0; u < KB_BATCH; ++u) {
+                const int i = ib + u * KB_THREADS;
+                if (i < i1) {
+                    cur.advance(i);
+                    GdtKbRec r;
+                    kb_bin_in(cur.S, p[u], r.key, r.cell, r.off.x, r.off.y);
+                    r.idx = i;
+                    r.pad = 0u;
+                    bkt[atomicAdd(&cursor[r.key >> pl.B], 1u)] = r;
+                }
+            }
+        }
     }
 }
 
