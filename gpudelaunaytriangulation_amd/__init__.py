@@ -8,11 +8,11 @@ implemented as hand-written HIP kernels behind the C ABI in
 ``python -m gpudelaunaytriangulation_amd.build``).
 """
 from . import inputs, rings
-from ._lib import (GDT_OK, GDT_OVERFLOW, PREC_FP32F, PREC_FP64, GdtError, LibraryNotBuilt,
+from ._lib import (GDT_OK, GDT_OVERFLOW, PREC_EXACT, PREC_FP32F, PREC_FP64, GdtError, LibraryNotBuilt,
                    load as load_library)
 
 __all__ = [
-    "inputs", "rings", "GDT_OK", "GDT_OVERFLOW", "PREC_FP32F", "PREC_FP64", "GdtError",
+    "inputs", "rings", "GDT_OK", "GDT_OVERFLOW", "PREC_EXACT", "PREC_FP32F", "PREC_FP64", "GdtError",
     "LibraryNotBuilt", "load_library", "DelaunayGPU", "CellListGPU", "PeriodicBoundaries",
     "Context", "BatchedDelaunayGPU",
 ]

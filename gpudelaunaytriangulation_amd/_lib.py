@@ -26,6 +26,7 @@ GDT_ERR_SCRATCH = -7
 
 PREC_FP64 = 64
 PREC_FP32F = 32
+PREC_EXACT = 128       # every decision exact (rectangular boxes)
 
 # Every symbol include/gpudelaunay.h declares (checked by the CPU tests).
 EXPORTS = (

@@ -60,6 +60,7 @@ struct gdt_context {
     std::vector<GdtSet> sets;
     int total_n = 0, total_keys = 0;
     bool cells_valid = false;
+    const double* cells_xy = nullptr;            // the coordinates of the current cell list
     DBuf<GdtSet> d_sets;
     DBuf<GdtChunk> pch, kch, rch, tch;
     int npch = 0, nkch = 0, nrch = 0, rch_block = 0, ntch = 0;
@@ -257,10 +258,18 @@ static bool any_general(const gdt_context* c)
         if (s.general) return true;
     return false;
 }
+// (general boxes run FP64; the exact mode reports them as ERR_RANGE)
 static int eff_precision(const gdt_context* c, int precision)
 {
-    return any_general(c) ? GDT_PREC_FP64 : precision;
+    return any_general(c) && precision != GDT_PREC_EXACT ? GDT_PREC_FP64 : precision;
 }
+
+// Precision codes of the ring entry points.
+static bool ring_precision_ok(int p)
+{
+    return p == GDT_PREC_FP64 || p == GDT_PREC_FP32F || p == GDT_PREC_EXACT;
+}
+static_assert((int)GDT_PREC_EXACT == (int)GDT_EXACT, "one code for the exact mode");
 
 static int single_set_box(gdt_context* c, int n, const double* box, int words, double cell_size)
 {
@@ -347,6 +356,7 @@ static int read_status(gdt_context* c, hipStream_t s)
 
 static int status_from_flags(int flags)
 {
+    if (flags & ERR_RANGE) return GDT_ERR_INVALID;
     if (flags & (ERR_NOCLOSE | ERR_NONEIGHBOUR)) return GDT_ERR_NONGENERIC;
     if (flags & ERR_SEARCH) return GDT_ERR_SEARCH;
     return GDT_OK;
@@ -377,6 +387,7 @@ static GdtRingIO ring_io(gdt_context* c, int* nbrs, int stride)
     io.scell = c->scell.p;
     io.nbrs = nbrs;
     io.stride = stride;
+    io.xy = reinterpret_cast<const gdt_d2*>(c->cells_xy);
     return io;
 }
 
@@ -389,6 +400,7 @@ static int build_cells(gdt_context* c, const double* d_xy, hipStream_t s)
                               c->nkch, c->total_n, c->total_keys, d_xy, cl, s));
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
     c->cells_valid = true;
+    c->cells_xy = d_xy;
     return GDT_OK;
 }
 
@@ -577,7 +589,7 @@ static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int*
     if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
     if ((rc = reset_status(c, s)) != GDT_OK) return rc;
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
-    if (tiled_sets(c)) {
+    if (tiled_sets(c) && precision != GDT_PREC_EXACT) {
         // A grid of at most one tile per CU runs one wave of blocks: each lane
         // (or lane group) then takes a single point (1024 threads for about
         // 1024 points of a 32x32 tile), so the block ends after its longest
@@ -597,12 +609,15 @@ static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int*
         c->stats.tile_shift = 0;
         c->stats.lanes = 1;
         const GdtRingIO io = ring_io(c, d_nbrs, stride);
+        // (the exact mode hands no point on: one beyond the search radius is an error)
+        const bool exact = precision == GDT_PREC_EXACT;
         GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), c->rch.p, c->nrch,
                                       c->cfg.block, nullptr, nullptr, 0, 0, io, d_counts,
-                                      c->status.p, c->counters.p, c->defer.p, s));
+                                      c->status.p, c->counters.p, exact ? nullptr : c->defer.p, s));
         if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[2], s));
-        GDT_CHECK(gdt_launch_deferred(precision, c->d_sets.p, (int)c->sets.size(), c->total_n, io,
-                                      d_counts, c->status.p, c->counters.p, c->defer.p, s));
+        if (!exact)
+            GDT_CHECK(gdt_launch_deferred(precision, c->d_sets.p, (int)c->sets.size(), c->total_n,
+                                          io, d_counts, c->status.p, c->counters.p, c->defer.p, s));
     }
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[3], s));
     rc = finish_rings(c, stride, required, s);
@@ -631,7 +646,7 @@ int gdt_triangulate(gdt_context* c, const double* d_xy, int n, double lx, double
                     int* required, void* stream)
 {
     if (!c || !d_xy || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
-    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    if (!ring_precision_ok(precision)) return GDT_ERR_INVALID;
     DevGuard g(c->device);
     int rc = single_set(c, n, lx, ly, cell_size);
     if (rc != GDT_OK) return rc;
@@ -644,7 +659,7 @@ int gdt_triangulate_batch(gdt_context* c, const double* d_xy, int nsets, const i
                           int* d_counts, int precision, int* required, void* stream)
 {
     if (!c || !d_xy || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
-    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    if (!ring_precision_ok(precision)) return GDT_ERR_INVALID;
     DevGuard g(c->device);
     int rc = batch_sets(c, nsets, offsets, boxes, cell_size);
     if (rc != GDT_OK) return rc;
@@ -657,7 +672,7 @@ int gdt_triangulate_general(gdt_context* c, const double* d_xy, int nsets, const
                             int* d_counts, int precision, int* required, void* stream)
 {
     if (!c || !d_xy || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
-    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    if (!ring_precision_ok(precision)) return GDT_ERR_INVALID;
     DevGuard g(c->device);
     int rc = batch_sets(c, nsets, offsets, boxes, cell_size, 4);
     if (rc != GDT_OK) return rc;
@@ -770,7 +785,8 @@ int gdt_test_circumcircles(gdt_context* c, const double* d_xy, int n, const int*
 static int repair_flagged(gdt_context* c, const int* d_repair, int stride, int* d_nbrs,
                           int* d_counts, int precision, hipStream_t s)
 {
-    if (tiled_sets(c)) {
+    const bool exact = precision == GDT_PREC_EXACT;
+    if (tiled_sets(c) && !exact) {
         GDT_CHECK(gdt_tiles::one_ring(c->sets[0].tshift, precision, c->d_sets.p,
                                       (int)c->sets.size(), c->tch.p, c->ntch, tile_block(c),
                                       c->cfg.repair_lanes ? c->cfg.repair_lanes : 2, c->total_n,
@@ -789,9 +805,11 @@ static int repair_flagged(gdt_context* c, const int* d_repair, int stride, int* 
     const GdtRingIO io = ring_io(c, d_nbrs, stride);
     GDT_CHECK(gdt_launch_one_ring(precision, c->d_sets.p, (int)c->sets.size(), nullptr, 0,
                                   c->cfg.block, c->worklist.p, worklen, 0, c->total_n, io,
-                                  d_counts, c->status.p, c->counters.p, c->defer.p, s));
-    GDT_CHECK(gdt_launch_deferred(precision, c->d_sets.p, (int)c->sets.size(), c->total_n, io,
-                                  d_counts, c->status.p, c->counters.p, c->defer.p, s));
+                                  d_counts, c->status.p, c->counters.p,
+                                  exact ? nullptr : c->defer.p, s));
+    if (!exact)
+        GDT_CHECK(gdt_launch_deferred(precision, c->d_sets.p, (int)c->sets.size(), c->total_n, io,
+                                      d_counts, c->status.p, c->counters.p, c->defer.p, s));
     return GDT_OK;
 }
 
@@ -799,7 +817,7 @@ int gdt_repair(gdt_context* c, const double* d_xy, int n, const int* d_repair, i
                int* d_nbrs, int* d_counts, int precision, int* required, void* stream)
 {
     if (!c || !d_xy || !d_repair || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
-    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    if (!ring_precision_ok(precision)) return GDT_ERR_INVALID;
     if (!c->cells_valid || c->total_n != n) return GDT_ERR_INVALID;
     DevGuard g(c->device);
     hipStream_t s = (hipStream_t)stream;
@@ -854,7 +872,7 @@ int gdt_test_and_repair(gdt_context* c, const double* d_xy, int n, double lx, do
                         int* nflagged, int* required, void* stream)
 {
     if (!c || !d_xy || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
-    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    if (!ring_precision_ok(precision)) return GDT_ERR_INVALID;
     DevGuard g(c->device);
     int rc = single_set(c, n, lx, ly, cell_size);
     if (rc != GDT_OK) return rc;
@@ -868,7 +886,7 @@ int gdt_test_and_repair_batch(gdt_context* c, const double* d_xy, int nsets, con
                               void* stream)
 {
     if (!c || !d_xy || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
-    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    if (!ring_precision_ok(precision)) return GDT_ERR_INVALID;
     DevGuard g(c->device);
     int rc = batch_sets(c, nsets, offsets, boxes, cell_size);
     if (rc != GDT_OK) return rc;
@@ -882,7 +900,7 @@ int gdt_test_and_repair_general(gdt_context* c, const double* d_xy, int nsets, c
                                 void* stream)
 {
     if (!c || !d_xy || !d_nbrs || !d_counts || stride < 3) return GDT_ERR_INVALID;
-    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    if (!ring_precision_ok(precision)) return GDT_ERR_INVALID;
     DevGuard g(c->device);
     int rc = batch_sets(c, nsets, offsets, boxes, cell_size, 4);
     if (rc != GDT_OK) return rc;
@@ -942,7 +960,7 @@ static int md_run(gdt_context* c, double* d_xy, int nsets, const int* offsets, c
     if (!c || !d_xy || !d_nbrs || !d_counts || !rep || stride < 3 || nsteps < 0 || step0 < 0 ||
         check_every < 1 || !(sigma >= 0.) || !std::isfinite(sigma))
         return GDT_ERR_INVALID;
-    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    if (!ring_precision_ok(precision)) return GDT_ERR_INVALID;
     std::memset(rep, 0, sizeof(*rep));
     rep->first_bad_step = -1;
     DevGuard g(c->device);

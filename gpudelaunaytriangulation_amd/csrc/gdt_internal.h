@@ -51,7 +51,10 @@ enum {
 // ERR_DEFER is internal: the LDS routine hands the point to the direct routine.
 // ERR_TIE is internal: the wavefront routine built without the tie rule met
 // an exact tie and hands the point to the one built with it.
-enum { ERR_SEARCH = 1, ERR_NOCLOSE = 2, ERR_NONEIGHBOUR = 4, ERR_DEFER = 8, ERR_TIE = 16 };
+// The precision code of the exact mode (GDT_PREC_EXACT of the public header).
+enum { GDT_EXACT = 128 };
+enum { ERR_SEARCH = 1, ERR_NOCLOSE = 2, ERR_NONEIGHBOUR = 4, ERR_DEFER = 8, ERR_TIE = 16,
+       ERR_RANGE = 32 /* exact mode: coordinates too far outside the box, or a general box */ };
 // 64-bit counters (gdt_counters[...]).
 // CT_NBRSUM: sum of ring lengths written; CT_INSUM: sum of input counts seen by
 // the triangle enumeration (Euler check).

@@ -56,6 +56,7 @@ struct GdtRingIO {
     const uint32_t* scell;    // [N] packed (cx | cy << 16) of each slot
     int* nbrs;                // [N][stride] rings, caller order
     int stride;
+    const gdt_d2* xy;         // [N] caller coordinates (exact mode; else may be null)
 };
 
 struct GdtRingResult {

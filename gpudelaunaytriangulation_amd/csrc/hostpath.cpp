@@ -12,6 +12,7 @@
 // host (g++, -ffp-contract=off like the kernels) and called from OpenMP loops.
 // Host pointers only; no HIP call is made.
 #include "gdt_circle.h"
+#include "gdt_exact_ring.h"
 #include "gdt_vor.h"
 #include "gdt_wave.h"
 #include "gpudelaunay.h"
@@ -35,6 +36,7 @@ struct HostCells {
     std::vector<gdt_d2> off64;
     std::vector<gdt_f2> off32;
     std::vector<uint32_t> scell;
+    const double* xy = nullptr;     // the caller's coordinates (the exact mode reads them)
 
     GdtRingIO io(int* nbrs, int stride) const
     {
@@ -46,6 +48,7 @@ struct HostCells {
         r.scell = scell.data();
         r.nbrs = nbrs;
         r.stride = stride;
+        r.xy = reinterpret_cast<const gdt_d2*>(xy);
         return r;
     }
 };
@@ -69,6 +72,7 @@ int build_cells(HostCells& h, const double* xy, int n, const double* box, double
 {
     if (n < 4 || !box_ok(box) || !(a > 0.) || !std::isfinite(a)) return GDT_ERR_INVALID;
     h.S = gdt_make_set_box(0, n, 0, box, a, 0);
+    h.xy = xy;
     const GdtSet& S = h.S;
     if (S.xsize > 65535 || S.ysize > 65535 || S.xsize < 3 || S.ysize < 3) return GDT_ERR_INVALID;
     const int K = S.nkeys;
@@ -111,6 +115,7 @@ int build_cells(HostCells& h, const double* xy, int n, const double* box, double
 
 int status_of(int err, int required, int stride)
 {
+    if (err & ERR_RANGE) return GDT_ERR_INVALID;
     if (err & (ERR_NOCLOSE | ERR_NONEIGHBOUR)) return GDT_ERR_NONGENERIC;
     if (err & ERR_SEARCH) return GDT_ERR_SEARCH;
     return required > stride ? GDT_OVERFLOW : GDT_OK;
@@ -120,20 +125,22 @@ int status_of(int err, int required, int stride)
 // routine in the requested precision; a point whose search reaches the grid's
 // search radius (boxes of a few cells) is redone by the routine the device
 // path hands such points to (gdt_wave.h, its 64 lanes emulated), as the
-// device's deferred points are.  Returns the OR of the error bits and the
+// device's deferred points are (the exact mode, as on the device, reports it).  Returns the OR of the error bits and the
 // longest ring in *maxc.
 int rings_of(const HostCells& h, const int* slots, int nslots, int* nbrs, int* counts, int stride,
              int precision, int nt, int* maxc)
 {
     const GdtRingIO io = h.io(nbrs, stride);
-    if (h.S.general) precision = GDT_PREC_FP64;   // the FP32 filter's bounds are for rectangles
+    // the FP32 filter's bounds are for rectangles (the exact mode reports a general box)
+    if (h.S.general && precision != GDT_PREC_EXACT) precision = GDT_PREC_FP64;
     int mx = 0, err = 0;
 #pragma omp parallel for num_threads(nt) schedule(dynamic, 256) reduction(max : mx) reduction(| : err)
     for (int t = 0; t < nslots; ++t) {
         const int j = slots ? slots[t] : t;
-        GdtRingResult r = precision == GDT_PREC_FP32F ? gdt_one_ring_f32(h.S, j, io)
-                                                      : gdt_one_ring_f64(h.S, j, io);
-        if (r.err & ERR_SEARCH) {
+        GdtRingResult r = precision == GDT_PREC_FP32F   ? gdt_one_ring_f32(h.S, j, io)
+                          : precision == GDT_PREC_EXACT ? gdt_one_ring_exact(h.S, j, io)
+                                                        : gdt_one_ring_f64(h.S, j, io);
+        if ((r.err & ERR_SEARCH) && precision != GDT_PREC_EXACT) {
             GdtWaveHost w;
             r = gdt_wave_ring<true>(h.S, j, io, w, true);
         }
@@ -192,7 +199,8 @@ int gdt_host_triangulate_general(const double* xy, int n, const double* box, dou
                                  int* required)
 {
     if (!xy || !box || !nbrs || !counts || stride < 3) return GDT_ERR_INVALID;
-    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F && precision != GDT_PREC_EXACT)
+        return GDT_ERR_INVALID;
     const int nt = nthreads_of(nthreads);
     HostCells h;
     int rc = build_cells(h, xy, n, box, cell_size, nt);
@@ -218,7 +226,8 @@ int gdt_host_test_and_repair_general(const double* xy, int n, const double* box,
                                      int nthreads, int* nflagged, int* required)
 {
     if (!xy || !box || !nbrs || !counts || stride < 3) return GDT_ERR_INVALID;
-    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F) return GDT_ERR_INVALID;
+    if (precision != GDT_PREC_FP64 && precision != GDT_PREC_FP32F && precision != GDT_PREC_EXACT)
+        return GDT_ERR_INVALID;
     const int nt = nthreads_of(nthreads);
     if (nflagged) *nflagged = 0;
     if (required) *required = 0;

@@ -157,6 +157,7 @@ extern "C" int gdt_triangulate_noalloc(const double* d_xy, int n, double lx, dou
     io.scell = c.scell;
     io.nbrs = d_nbrs;
     io.stride = stride;
+    io.xy = nullptr;
     // the block size of gdt_triangulate's default configuration
     const int block = ntiles <= cus ? 1024 : 512;
     if (gdt_tiles::one_ring(GDT_TILE_SHIFT, precision, d_sets, 1, d_tiles, ntiles, block, 1, n,

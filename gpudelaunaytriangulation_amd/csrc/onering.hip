@@ -8,6 +8,7 @@
 // (DelaunayGPU.cu:649-675, 981-1006) and their masked repair variants
 // (DelaunayGPU.cu:611-640, 945-975), which instead launch every point and
 // mask most of them out.
+#include "gdt_exact_ring.h"
 #include "gdt_kernels.h"
 #include "gdt_wavedev.h"
 
@@ -25,7 +26,10 @@ __device__ inline void one_ring_lane(bool active, const GdtSet& S, int j, const 
 {
     GdtRingResult r = {0, 0, 0u, 0u};
     if (active) {
-        r = (PREC == 32) ? gdt_one_ring_f32(S, j, io) : gdt_one_ring_f64(S, j, io);
+        if constexpr (PREC == GDT_EXACT)
+            r = gdt_one_ring_exact(S, j, io);
+        else
+            r = (PREC == 32) ? gdt_one_ring_f32(S, j, io) : gdt_one_ring_f64(S, j, io);
         if (defer && (r.err & ERR_SEARCH)) {
             defer[atomicAdd(reinterpret_cast<unsigned*>(status + ST_NDEFER), 1u)] = j;
             r.count = 0;
@@ -164,6 +168,9 @@ hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
         if (precision == 32)
             hipLaunchKernelGGL(k_one_ring<32>, dim3(nch), dim3(block), 0, s, d_sets, chunks, io,
                                counts, status, counters, defer);
+        else if (precision == GDT_EXACT)
+            hipLaunchKernelGGL(k_one_ring<GDT_EXACT>, dim3(nch), dim3(block), 0, s, d_sets,
+                               chunks, io, counts, status, counters, defer);
         else
             hipLaunchKernelGGL(k_one_ring<64>, dim3(nch), dim3(block), 0, s, d_sets, chunks, io,
                                counts, status, counters, defer);
@@ -176,6 +183,9 @@ hipError_t gdt_launch_one_ring(int precision, const GdtSet* d_sets, int nsets,
     if (precision == 32)
         hipLaunchKernelGGL(k_one_ring_list<32>, dim3(grid), dim3(block), 0, s, d_sets, nsets,
                            worklist, worklen, skip, io, counts, status, counters, defer);
+    else if (precision == GDT_EXACT)
+        hipLaunchKernelGGL(k_one_ring_list<GDT_EXACT>, dim3(grid), dim3(block), 0, s, d_sets,
+                           nsets, worklist, worklen, skip, io, counts, status, counters, defer);
     else
         hipLaunchKernelGGL(k_one_ring_list<64>, dim3(grid), dim3(block), 0, s, d_sets, nsets,
                            worklist, worklen, skip, io, counts, status, counters, defer);

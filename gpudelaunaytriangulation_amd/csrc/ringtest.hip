@@ -233,6 +233,7 @@ hipError_t gdt_launch_ring_test_tile(const GdtSet* d_sets, const GdtChunk* tiles
     io.scell = c.scell;
     io.nbrs = nullptr;
     io.stride = 0;
+    io.xy = nullptr;
     hipLaunchKernelGGL(k_ring_test_tile, dim3(ntiles), dim3(GDT_RT_THREADS), 0, s, d_sets, tiles, nbrs,
                        counts, stride, total_n, reinterpret_cast<const gdt_d2*>(d_xy), io,
                        repair, partials, status, c.halt);

@@ -519,9 +519,13 @@ class DelaunayGPU:
         self.OMPThreadsNum = int(num)
 
     def setPrecision(self, precision: int) -> None:
-        """64: all-FP64 predicates; 32: FP32 with certified bound + FP64 fallback."""
-        if precision not in (_lib.PREC_FP64, _lib.PREC_FP32F):
-            raise ValueError("precision must be 64 or 32")
+        """64: all-FP64 predicates; 32: FP32 with certified bound + FP64
+        fallback; 128 (PREC_EXACT): every decision exact -- FP64 where its
+        error bound certifies it, expansion arithmetic otherwise, exact ties
+        by the symbolic perturbation (rectangular boxes, coordinates within
+        four box lengths of the box)."""
+        if precision not in (_lib.PREC_FP64, _lib.PREC_FP32F, _lib.PREC_EXACT):
+            raise ValueError("precision must be 64, 32 or 128")
         self.precision = int(precision)
 
     def stats(self) -> dict:
@@ -699,8 +703,8 @@ class BatchedDelaunayGPU:
         self._md = _entry_point("gdt_md_run", self.general)
 
     def setPrecision(self, precision: int) -> None:
-        if precision not in (_lib.PREC_FP64, _lib.PREC_FP32F):
-            raise ValueError("precision must be 64 or 32")
+        if precision not in (_lib.PREC_FP64, _lib.PREC_FP32F, _lib.PREC_EXACT):
+            raise ValueError("precision must be 64, 32 or 128")
         self.precision = int(precision)
 
     def stats(self) -> dict:

@@ -42,7 +42,12 @@ typedef enum {
 
 typedef enum {
     GDT_PREC_FP64 = 64,        /* all predicates in FP64 */
-    GDT_PREC_FP32F = 32        /* FP32 with certified error bound, FP64 fallback */
+    GDT_PREC_FP32F = 32,       /* FP32 with certified error bound, FP64 fallback */
+    GDT_PREC_EXACT = 128       /* every decision exact (FP64 where its error bound
+                                  certifies it, else expansion arithmetic; exact ties
+                                  by the symbolic perturbation): the per-point
+                                  kernels, rectangular boxes, caller coordinates
+                                  within four box lengths of the box */
 } gdt_precision;
 
 typedef struct gdt_context gdt_context;

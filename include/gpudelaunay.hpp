@@ -737,7 +737,8 @@ public:
     //! 64: all-FP64 predicates; 32: FP32 with a certified bound + FP64 fallback
     void setPrecision(int p)
     {
-        if (p != GDT_PREC_FP64 && p != GDT_PREC_FP32F) throw std::invalid_argument("precision");
+        if (p != GDT_PREC_FP64 && p != GDT_PREC_FP32F && p != GDT_PREC_EXACT)
+            throw std::invalid_argument("precision");
         precision = p;
     }
     //! GPUVoroCur is filled after every call while this is on (default).

@@ -74,6 +74,7 @@ int main(int argc, char** argv)
     GdtRingIO io;
     io.cstart = cstart.data(); io.perm = perm.data(); io.off64 = off64.data();
     io.off32 = off32.data(); io.scell = scell.data(); io.nbrs = nbrs.data(); io.stride = stride;
+    io.xy = nullptr;
     unsigned long long npred = 0, nfb = 0;
     int err = 0, ndefer = 0;
     auto acc = [&](int j, const GdtRingResult& r) {
