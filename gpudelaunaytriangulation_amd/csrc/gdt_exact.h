@@ -244,17 +244,23 @@ GDT_HD inline int gdt_x_incircle(const GdtXVec& a, const GdtXVec& b, const GdtXV
 // plus the rounding of their FP64 evaluation), each doubled below.
 #define GDT_XU 1.1102230246251565e-16
 
-GDT_HD inline double gdt_x_err(double L, double d) { return 64. * GDT_XU * (L + fabs(d)); }
+#ifndef GDT_XTOL_SCALE     // (a host test shrinks it to show that a too-tight bound is caught)
+#define GDT_XTOL_SCALE 1.0
+#endif
+GDT_HD inline double gdt_x_err(double L, double d)
+{
+    return GDT_XTOL_SCALE * 64. * GDT_XU * (L + fabs(d));
+}
 
 GDT_HD inline double gdt_x_tol2(double e, double M)
 {
-    return 2. * (4. * (e * M + e * e) + 4. * GDT_XU * M * M);
+    return 2. * (4. * (e * M + e * e) + GDT_XTOL_SCALE * 4. * GDT_XU * M * M);
 }
 
 GDT_HD inline double gdt_x_tol4(double e, double M)
 {
     const double Me = M + e;
-    return 2. * (48. * e * Me * Me * Me + 32. * GDT_XU * M * M * M * M);
+    return 2. * (48. * e * Me * Me * Me + GDT_XTOL_SCALE * 32. * GDT_XU * M * M * M * M);
 }
 
 // a - b, componentwise.

@@ -17,6 +17,16 @@
 #include "gdt_exact.h"
 #include "gdt_point.h"
 
+// Audit hook of host test builds: every decision taken from its FP64 value
+// alone (kind 0 orientation, 1 in-circle, 2 distance), the sign it claims and
+// the exact sign of the same predicate.
+#ifdef GDT_AUDIT_EXACT
+void gdt_exact_audit(int kind, int claimed, int exact);
+#define GDT_XAUDIT(kind, claimed, exact_sign) gdt_exact_audit(kind, claimed, exact_sign)
+#else
+#define GDT_XAUDIT(kind, claimed, exact_sign) ((void)0)
+#endif
+
 // A candidate: slot, FP64 displacement, and (once needed) its exact one.
 struct GdtXCand {
     int k;
@@ -79,7 +89,10 @@ GDT_HD inline GdtRingResult gdt_one_ring_exact(const GdtSet& S, int j, const Gdt
                 const double tol = gdt_x_tol2(gdt_x_err(L, mc), mc) + gdt_x_tol2(gdt_x_err(L, mb), mb);
                 if (d2 < nd2 - tol) {
                     take = true;
-                } else if (d2 <= nd2 + tol) {
+                    GDT_XAUDIT(2, -1, gdt_x_dist_cmp(exact(cand), exact(nn), xc));
+                } else if (d2 > nd2 + tol) {
+                    GDT_XAUDIT(2, 1, gdt_x_dist_cmp(exact(cand), exact(nn), xc));
+                } else {
                     const int s = gdt_x_dist_cmp(exact(cand), exact(nn), xc);
                     take = s < 0 || (s == 0 && gdt_nn_tie(io.perm[k], io.perm[nn.k]));
                 }
@@ -126,7 +139,9 @@ GDT_HD inline GdtRingResult gdt_one_ring_exact(const GdtSet& S, int j, const Gdt
                     const double M = fmax(mag(a), mag(cand));
                     const double tol = gdt_x_tol2(gdt_x_err(L, M), M);
                     bool left = O > tol;
-                    if (!left && O >= -tol) left = gdt_x_orient(exact(a), exact(cand), xc) > 0;
+                    if (left) GDT_XAUDIT(0, 1, gdt_x_orient(exact(a), exact(cand), xc));
+                    else if (O < -tol) GDT_XAUDIT(0, -1, gdt_x_orient(exact(a), exact(cand), xc));
+                    else left = gdt_x_orient(exact(a), exact(cand), xc) > 0;
                     if (!left) continue;
                 }
                 if (best.k >= 0) {
@@ -135,7 +150,11 @@ GDT_HD inline GdtRingResult gdt_one_ring_exact(const GdtSet& S, int j, const Gdt
                     const double M = fmax(fmax(mag(a), mag(best)), mag(cand));
                     const double tol = gdt_x_tol4(gdt_x_err(L, M), M);
                     bool inside = P < -tol;
-                    if (!inside && P <= tol) {
+                    if (inside)
+                        GDT_XAUDIT(1, -1, gdt_x_incircle(exact(a), exact(best), exact(cand), xc));
+                    else if (P > tol)
+                        GDT_XAUDIT(1, 1, gdt_x_incircle(exact(a), exact(best), exact(cand), xc));
+                    else {
                         const int s = gdt_x_incircle(exact(a), exact(best), exact(cand), xc);
                         inside = s < 0 ||
                                  (s == 0 && gdt_x_incircle_tie(io.perm[j], io.perm[a.k],

@@ -190,3 +190,41 @@ def test_cpu_exact_mode_refuses_what_it_cannot_bound():
                                         1.0, 32, nb.ctypes.data, cnt.ctypes.data, PREC_EXACT, 4,
                                         ctypes.byref(req))
     assert st == _lib.GDT_ERR_INVALID
+
+
+def build_exact_audit(path, extra=()):
+    subprocess.run(["g++", "-std=c++17", "-O2", "-fopenmp", "-ffp-contract=off", "-DGDT_AUDIT_EXACT",
+                    *extra, f"-I{CSRC}", f"-I{os.path.join(ROOT, 'include')}",
+                    os.path.join(CSRC, "hostpath.cpp"),
+                    os.path.join(ROOT, "tests", "cpp", "exact_audit.cpp"), "-o", path], check=True)
+    return path
+
+
+def run_exact_audit(exe, xy, lx, ly, cell, tmp_path):
+    f = tmp_path / "pts.bin"
+    with open(f, "wb") as fh:
+        fh.write(np.int32(xy.shape[0]).tobytes())
+        fh.write(np.array([lx, ly, cell], dtype=np.float64).tobytes())
+        fh.write(np.ascontiguousarray(xy, dtype=np.float64).tobytes())
+    out = subprocess.run([exe, str(f)], capture_output=True, text=True, check=True).stdout.split()
+    st, v = int(out[0]), [int(x) for x in out[1:]]
+    return st, v[0::2], v[1::2]
+
+
+def test_exact_mode_fp64_decisions_audited(tmp_path):
+    """Every decision the exact routine takes from its FP64 value alone agrees
+    with the exact sign of the predicate (the error bounds hold), on the
+    near-degenerate lattices and generic points; with the bounds shrunk a
+    million times the same audit finds disagreements, so it can fail."""
+    exe = build_exact_audit(str(tmp_path / "audit"))
+    tight = build_exact_audit(str(tmp_path / "audit_tight"), ["-DGDT_XTOL_SCALE=1e-6"])
+    cases = [NEAR[k] for k in sorted(NEAR)]
+    cases.append(((inputs.uniform_points(2000, 45.0, 45.0, seed=4), 45.0, 45.0), 1.0))
+    wrong_tight = 0
+    for (xy, lx, ly), cell in cases:
+        st, taken, wrong = run_exact_audit(exe, xy, lx, ly, cell, tmp_path)
+        assert st == 0 and min(taken) > 0
+        assert wrong == [0, 0, 0], (taken, wrong)
+        _, _, wt = run_exact_audit(tight, xy, lx, ly, cell, tmp_path)
+        wrong_tight += sum(wt)
+    assert wrong_tight > 0
