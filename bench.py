@@ -70,6 +70,8 @@ def parse():
                     help="skip the general (sheared) box timing")
     ap.add_argument("--no-c1", action="store_true",
                     help="skip the 4,096-point case (GPU and CPU oracle timings)")
+    ap.add_argument("--no-exact", action="store_true",
+                    help="skip the exact precision mode's timings")
     ap.add_argument("--no-c3b", action="store_true",
                     help="skip the jittered-lattice case (2^20 points, both precisions)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -198,6 +200,51 @@ def bench_c3b(torch, steps, warmup):
     with open(gpath) as fh:
         res["matches_golden_oracle_hash"] = (json.load(fh)["workloads"]["c3b"]["ring_sha256"]
                                              == hashes["fp32f"])
+    return res
+
+
+def bench_exact(torch, steps, warmup):
+    """The exact precision mode (every decision exact): points/s on the 2^20
+    uniform workload (rings must equal the committed oracle hash) and on a
+    near-degenerate lattice, 1024 x 1024 centres of a square lattice of
+    spacing 0.1 (rounded to FP64: every square's four points nearly
+    cocircular), box 102.4, cell 0.1, with the FP64 path beside it (its rings
+    there are not a triangulation: their counts need not sum to 6N)."""
+    import numpy as np
+    from gpudelaunaytriangulation_amd import PREC_EXACT, PREC_FP64, _lib, inputs, rings
+    from gpudelaunaytriangulation_amd.delaunay import DelaunayGPU, PeriodicBoundaries
+    res = {}
+    n = N_POINTS
+    xy = inputs.uniform_points(n, BOX, BOX, seed=2)
+    dg = DelaunayGPU(n, 16, 1.0, PeriodicBoundaries(BOX, BOX))
+    dg.setPrecision(PREC_EXACT)
+    pts = torch.from_numpy(xy).cuda()
+    tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device="cuda")
+    num = torch.zeros(n, dtype=torch.int32, device="cuda")
+    ms, *_ = time_path(dg, pts, tri, num, steps, warmup, torch)
+    h = rings.ring_hash(tri.cpu().numpy().reshape(n, dg.MaxSize), num.cpu().numpy())
+    with open(os.path.join(ROOT, "tests", "golden", "golden.json")) as fh:
+        golden = json.load(fh)["workloads"]["c2"]["ring_sha256"]
+    res["uniform_1M_points_per_s"] = n / (ms * 1e-3)
+    res["uniform_1M_ms_per_step"] = ms
+    res["uniform_1M_matches_golden_oracle_hash"] = h == golden
+    m, hsp = 1024, 0.1
+    X, Y = np.meshgrid(np.arange(m), np.arange(m), indexing="xy")
+    lat = np.ascontiguousarray(np.stack([(X.reshape(-1) + 0.5) * hsp, (Y.reshape(-1) + 0.5) * hsp], 1))
+    L = m * hsp
+    pts = torch.from_numpy(lat).cuda()
+    res["lattice_workload"] = f"1024x1024 square lattice of spacing 0.1 (FP64), box {L:g}, cell 0.1"
+    for name, prec in (("exact", PREC_EXACT), ("fp64", PREC_FP64)):
+        dg = DelaunayGPU(lat.shape[0], 16, hsp, PeriodicBoundaries(L, L))
+        dg.setPrecision(prec)
+        tri = torch.zeros(lat.shape[0] * dg.MaxSize, dtype=torch.int32, device="cuda")
+        num = torch.zeros(lat.shape[0], dtype=torch.int32, device="cuda")
+        try:
+            ms, *_ = time_path(dg, pts, tri, num, steps, warmup, torch)
+            res[f"lattice_{name}_points_per_s"] = lat.shape[0] / (ms * 1e-3)
+            res[f"lattice_{name}_sum_counts_eq_6N"] = int(num.sum().item()) == 6 * lat.shape[0]
+        except _lib.GdtError as e:
+            res[f"lattice_{name}_error"] = str(e)
     return res
 
 
@@ -526,6 +573,12 @@ def main():
         except Exception as e:  # reported, never hides the headline number
             line["small_n"] = {"error": repr(e)}
         print(f"[bench] small_n: {line['small_n']}", file=sys.stderr, flush=True)
+    if not args.no_exact:
+        try:
+            line["exact_mode"] = bench_exact(torch, args.steps, args.warmup)
+        except Exception as e:  # reported, never hides the headline number
+            line["exact_mode"] = {"error": repr(e)}
+        print(f"[bench] exact mode: {line['exact_mode']}", file=sys.stderr, flush=True)
     if not args.no_c3b:
         try:
             line["c3b"] = bench_c3b(torch, args.steps, args.warmup)

@@ -27,12 +27,14 @@ void gdt_exact_audit(int kind, int claimed, int exact);
 #define GDT_XAUDIT(kind, claimed, exact_sign) ((void)0)
 #endif
 
-// A candidate: slot, FP64 displacement, and (once needed) its exact one.
+// A candidate: slot, FP64 displacement, and the pool entry that holds its
+// exact displacement once needed.  Taking another candidate's role swaps the
+// pool entries instead of copying expansions.
 struct GdtXCand {
     int k;
     double x, y;
-    bool have;
-    GdtXVec X;
+    int xs;         // pool entry owned by this role
+    bool have;      // entry xs holds the exact displacement of slot k
 };
 
 GDT_HD inline void gdt_xc_set(GdtXCand& c, int k, double x, double y)
@@ -41,6 +43,19 @@ GDT_HD inline void gdt_xc_set(GdtXCand& c, int k, double x, double y)
     c.x = x;
     c.y = y;
     c.have = false;
+}
+
+// dst takes src's candidate (src's role is reset by its next set)
+GDT_HD inline void gdt_xc_take(GdtXCand& dst, GdtXCand& src)
+{
+    const int t = dst.xs;
+    dst.k = src.k;
+    dst.x = src.x;
+    dst.y = src.y;
+    dst.have = src.have;
+    dst.xs = src.xs;
+    src.xs = t;
+    src.have = false;
 }
 
 GDT_HD inline GdtRingResult gdt_one_ring_exact(const GdtSet& S, int j, const GdtRingIO& io)
@@ -61,18 +76,25 @@ GDT_HD inline GdtRingResult gdt_one_ring_exact(const GdtSet& S, int j, const Gdt
     unsigned npred = 0;
     GdtXCtx xc;
     xc.overflow = false;
+    GdtXVec pool[4];
     auto exact = [&](GdtXCand& c) -> const GdtXVec& {
+        GdtXVec& X = pool[c.xs];
         if (!c.have) {
-            gdt_x_disp(pv, io.xy[io.perm[c.k]], S.lx, S.ly, c.x, c.y, c.X.x, c.X.y, xc);
+            gdt_x_disp(pv, io.xy[io.perm[c.k]], S.lx, S.ly, c.x, c.y, X.x, X.y, xc);
             c.have = true;
         }
-        return c.X;
+        return X;
     };
     auto mag = [](const GdtXCand& c) { return fmax(fabs(c.x), fabs(c.y)); };
 
     // 1. nearest neighbour: c beats b when |c|^2 < |b|^2, exact ties by caller index
-    GdtXCand nn, cand;
+    GdtXCand nn, cand, a, best;
+    nn.xs = 0;
+    cand.xs = 1;
+    a.xs = 2;
+    best.xs = 3;
     nn.k = -1;
+    nn.have = cand.have = a.have = best.have = false;
     double nd2 = 0.;
     auto nn_cell = [&](int dx, int dy, int, uint32_t b, uint32_t e) {
         const double bx = gdt_cbase_x(S, dx, dy, ov.x), by = gdt_cbase_y(S, dx, dy, ov.y);
@@ -98,7 +120,7 @@ GDT_HD inline GdtRingResult gdt_one_ring_exact(const GdtSet& S, int j, const Gdt
                 }
             }
             if (take) {
-                nn = cand;
+                gdt_xc_take(nn, cand);
                 nd2 = d2;
             }
         }
@@ -117,7 +139,8 @@ GDT_HD inline GdtRingResult gdt_one_ring_exact(const GdtSet& S, int j, const Gdt
     }
 
     // 2. walk the ring counter-clockwise
-    GdtXCand a = nn, best;
+    gdt_xc_take(a, nn);
+    nn.k = a.k;         // (only its slot is read from here on)
     int count = 0;
     for (;;) {
         if (count < io.stride) io.nbrs[row + count] = io.perm[a.k];
@@ -163,7 +186,7 @@ GDT_HD inline GdtRingResult gdt_one_ring_exact(const GdtSet& S, int j, const Gdt
                     }
                     if (!inside) continue;
                 }
-                best = cand;
+                gdt_xc_take(best, cand);
                 K = gdt_circ64(a.x, a.y, px, py);
             }
         };
@@ -179,7 +202,7 @@ GDT_HD inline GdtRingResult gdt_one_ring_exact(const GdtSet& S, int j, const Gdt
             break;
         }
         if (best.k == nn.k) break;
-        a = best;
+        gdt_xc_take(a, best);
     }
     if (xc.overflow) res.err |= ERR_NONEIGHBOUR;
     res.count = count;
