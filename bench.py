@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--tile-shift", type=int, default=None,
                     help="LDS tile edge 2^k cells (4 or 5; default: automatic)")
     ap.add_argument("--lanes", type=int, default=None,
-                    help="lanes per point of the tile kernel (1 or 2; default: automatic)")
+                    help="lanes per point of the tile kernel (1, 2, 4 or 64; default: automatic)")
     ap.add_argument("--no-small-n", action="store_true",
                     help="skip the 16,384 / 65,536 / 262,144-point timings")
     ap.add_argument("--precisions", default="fp32f,fp64", help="comma list of fp32f,fp64")
@@ -136,7 +136,7 @@ def cpu_model():
 
 def bench_c1(torch, cpu_runs, with_cpu):
     """4,096 uniform points in a box of side 64 (the reference's CPU-runnable
-    case): GPU points/s per precision (HIP events, 20 calls after 3) and,
+    case): GPU points/s per precision (HIP events, 100 calls after 20) and,
     unless skipped, the CPU oracle on the same points (median of cpu_runs)."""
     from gpudelaunaytriangulation_amd import PREC_FP32F, PREC_FP64, inputs
     from gpudelaunaytriangulation_amd.delaunay import DelaunayGPU, PeriodicBoundaries
@@ -149,7 +149,7 @@ def bench_c1(torch, cpu_runs, with_cpu):
         dg.setPrecision(prec)
         tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device="cuda")
         num = torch.zeros(n, dtype=torch.int32, device="cuda")
-        ms = time_path(dg, pts, tri, num, 20, 3, torch)[0]
+        ms = time_path(dg, pts, tri, num, 100, 20, torch)[0]
         res[f"gpu_{name}_points_per_s"] = n / (ms * 1e-3)
     if with_cpu:
         import oracle
@@ -305,7 +305,7 @@ def bench_general(torch, steps, warmup):
 
 def bench_small_n(torch, sizes=(16384, 65536, 262144)):
     """Uniform points at density 1 (box sqrt(N)), default (automatic)
-    configuration: points/s per precision over 20 calls after 3 (HIP events),
+    configuration: points/s per precision over 100 calls after 20 (HIP events),
     with the tile edge and lanes per point the library chose."""
     from gpudelaunaytriangulation_amd import PREC_FP32F, PREC_FP64, inputs
     from gpudelaunaytriangulation_amd.delaunay import DelaunayGPU, PeriodicBoundaries
@@ -320,7 +320,7 @@ def bench_small_n(torch, sizes=(16384, 65536, 262144)):
             dg.setPrecision(prec)
             tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device="cuda")
             num = torch.zeros(n, dtype=torch.int32, device="cuda")
-            ms, _, _, _, _, st = time_path(dg, pts, tri, num, 20, 3, torch)
+            ms, _, _, _, _, st = time_path(dg, pts, tri, num, 100, 20, torch)
             row[f"{name}_points_per_s"] = n / (ms * 1e-3)
             row["tile_cells"] = 1 << st["tile_shift"]
             row["lanes_per_point"] = st["lanes"]

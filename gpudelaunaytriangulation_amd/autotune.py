@@ -7,7 +7,8 @@ triangulation.cpp:172), the cell order (LDS tiles or row-major), the LDS
 tile's edge (16x16 or 32x32 cells: the window every block stages) and the
 lanes that share one point (one, two or four, for the global triangulation
 and for repairs; the FP32-filtered kernel shares a point between at most
-two).  Every candidate is timed on the caller's own points with HIP
+two; 64 for the global triangulation: the star kernel, one wavefront per
+point).  Every candidate is timed on the caller's own points with HIP
 events; the rings of every candidate must be identical (canonical hash) --
 the output never depends on the configuration -- and the fastest is applied
 to the context.  ``tune`` times global triangulations; ``tune_md`` times the
@@ -39,8 +40,8 @@ def _as_config(c) -> dict:
     return dict(zip(KNOBS, c))
 
 
-def candidates(blocks=DEFAULT_BLOCKS, scales=DEFAULT_SCALES, tile_shifts=(4, 5), lanes=(1, 2, 4),
-               row_major=True):
+def candidates(blocks=DEFAULT_BLOCKS, scales=DEFAULT_SCALES, tile_shifts=(4, 5),
+               lanes=(1, 2, 4, 64), row_major=True):
     """Global-triangulation candidates: every block x scale x tile edge x
     lanes per point in tile order, plus the row-major order at the caller's
     cell size."""

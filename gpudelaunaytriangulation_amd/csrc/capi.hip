@@ -210,13 +210,19 @@ static int tile_shift_for(const gdt_context* c, const std::vector<GdtSet>& sets)
     return tiles <= c->num_cu ? 4 : 5;
 }
 
-// Lanes per point of the global triangulation: the configuration's, or two
-// when 16x16 tiles do not outnumber the compute units (every block then holds
-// one tile's points with lanes to spare), else one.
+// Lanes per point of the global triangulation: the configuration's; else a
+// wavefront (the star kernel) when 16x16 tiles number at most a quarter of
+// the compute units and cells hold at most 1.25 points on average (so that
+// the 7x7 blocks of its lanes mostly hold at most 64); two when the tiles do
+// not outnumber the compute units (every block then holds one tile's points
+// with lanes to spare); else one.
 static int lanes_for(const gdt_context* c)
 {
     if (c->cfg.lanes) return c->cfg.lanes;
-    return c->sets[0].tshift == 4 && c->ntch <= c->num_cu ? 2 : 1;
+    if (c->sets[0].tshift != 4 || c->ntch > c->num_cu) return 1;
+    if (4 * (long long)c->ntch <= c->num_cu && 4 * (long long)c->total_n <= 5LL * c->total_keys)
+        return GDT_STAR_LANES;
+    return 2;
 }
 
 // Threads per block of the tile kernels: the configured count, at most 256
@@ -392,12 +398,14 @@ static GdtRingIO ring_io(gdt_context* c, int* nbrs, int stride)
 }
 
 // The cell list of d_xy (no caller index -> slot map: nothing reads one).
-static int build_cells(gdt_context* c, const double* d_xy, hipStream_t s)
+// (reset: zero the status words and counters on the way)
+static int build_cells(gdt_context* c, const double* d_xy, hipStream_t s, bool reset = false)
 {
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[0], s));
     GdtCells cl = c->cells();
     GDT_CHECK(gdt_build_cells(c->d_sets.p, (int)c->sets.size(), c->pch.p, c->npch, c->kch.p,
-                              c->nkch, c->total_n, c->total_keys, d_xy, cl, s));
+                              c->nkch, c->total_n, c->total_keys, d_xy, cl, s,
+                              reset ? c->status.p : nullptr, reset ? c->counters.p : nullptr));
     if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
     c->cells_valid = true;
     c->cells_xy = d_xy;
@@ -485,7 +493,8 @@ int gdt_set_config(gdt_context* c, const gdt_config* cfg)
     if (cfg->tile_shift != 0 && (cfg->tile_shift < GDT_TSHIFT_MIN || cfg->tile_shift > GDT_TSHIFT_MAX))
         return GDT_ERR_INVALID;
     auto lanes_ok = [](int w) { return w == 0 || w == 1 || w == 2 || w == 4; };
-    if (!lanes_ok(cfg->lanes) || !lanes_ok(cfg->repair_lanes)) return GDT_ERR_INVALID;
+    if (!(lanes_ok(cfg->lanes) || cfg->lanes == GDT_STAR_LANES) || !lanes_ok(cfg->repair_lanes))
+        return GDT_ERR_INVALID;
     c->cfg = *cfg;
     c->cfg.tiled = cfg->tiled == 2 ? 2 : (cfg->tiled ? 1 : 0);
     c->sets.clear();          // force a new set table (cell size / order may change)
@@ -586,9 +595,7 @@ static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int*
     int rc;
     precision = eff_precision(c, precision);
     if ((rc = ring_chunks(c)) != GDT_OK) return rc;
-    if ((rc = build_cells(c, d_xy, s)) != GDT_OK) return rc;
-    if ((rc = reset_status(c, s)) != GDT_OK) return rc;
-    if (c->cfg.profile) GDT_CHECK(hipEventRecord(c->ev[1], s));
+    if ((rc = build_cells(c, d_xy, s, true)) != GDT_OK) return rc;
     if (tiled_sets(c) && precision != GDT_PREC_EXACT) {
         // A grid of at most one tile per CU runs one wave of blocks: each lane
         // (or lane group) then takes a single point (1024 threads for about
@@ -596,15 +603,30 @@ static int triangulate_sets(gdt_context* c, const double* d_xy, int stride, int*
         // ring instead of after two.
         const int ts = c->sets[0].tshift, lanes = lanes_for(c);
         c->stats.tile_shift = ts;
-        c->stats.lanes = lanes == 4 && precision == GDT_PREC_FP32F ? 2 : lanes;
-        const int one_each = std::min(1024, (1 << (2 * ts)) * lanes);
-        const int block = c->ntch <= c->num_cu ? std::max(tile_block(c), one_each) : tile_block(c);
-        GDT_CHECK(gdt_tiles::one_ring(ts, precision, c->d_sets.p, (int)c->sets.size(), c->tch.p,
-                                      c->ntch, block, lanes, c->total_n, nullptr, nullptr,
-                                      nullptr, nullptr, ring_io(c, d_nbrs, stride), d_counts,
-                                      c->status.p,
-                                      c->counters.p, c->defer.p,
-                                      c->cfg.profile ? c->ev[2] : nullptr, s));
+        if (lanes == GDT_STAR_LANES) {
+            // one wavefront per point: about one point per wave, at most four
+            // blocks per compute unit (each stages its tile's window)
+            c->stats.lanes = lanes;
+            const int block = 256, nw = block / 64;
+            const long long avg = (c->total_n + c->ntch - 1) / c->ntch;
+            int nsplit = (int)std::max(1LL, (avg + nw - 1) / nw);
+            nsplit = std::min(nsplit, std::max(1, 4 * c->num_cu / c->ntch));
+            GDT_CHECK(gdt_tiles::one_ring_star(ts, c->d_sets.p, (int)c->sets.size(), c->tch.p,
+                                               c->ntch, nsplit, block, c->total_n,
+                                               ring_io(c, d_nbrs, stride), d_counts, c->status.p,
+                                               c->counters.p, c->defer.p,
+                                               c->cfg.profile ? c->ev[2] : nullptr, s));
+        } else {
+            c->stats.lanes = lanes == 4 && precision == GDT_PREC_FP32F ? 2 : lanes;
+            const int one_each = std::min(1024, (1 << (2 * ts)) * lanes);
+            const int block =
+                c->ntch <= c->num_cu ? std::max(tile_block(c), one_each) : tile_block(c);
+            GDT_CHECK(gdt_tiles::one_ring(ts, precision, c->d_sets.p, (int)c->sets.size(),
+                                          c->tch.p, c->ntch, block, lanes, c->total_n, nullptr,
+                                          nullptr, nullptr, nullptr, ring_io(c, d_nbrs, stride),
+                                          d_counts, c->status.p, c->counters.p, c->defer.p,
+                                          c->cfg.profile ? c->ev[2] : nullptr, s));
+        }
     } else {
         c->stats.tile_shift = 0;
         c->stats.lanes = 1;

@@ -380,8 +380,14 @@ __device__ inline void kb_bin_in(const GdtSet& S, const gdt_d2 p, uint32_t& key,
 __global__ __launch_bounds__(KB_THREADS) void k_kb_count(const GdtSet* __restrict__ sets, int nsets,
                                                          const gdt_d2* __restrict__ xy, int n,
                                                          KbPlan pl, uint32_t* __restrict__ bmat,
-                                                         const int* __restrict__ halt)
+                                                         const int* __restrict__ halt,
+                                                         int* __restrict__ st_zero,
+                                                         unsigned long long* __restrict__ ct_zero)
 {
+    if (st_zero && blockIdx.x == 0) {
+        if (threadIdx.x < ST_WORDS) st_zero[threadIdx.x] = 0;
+        if (threadIdx.x < CT_WORDS) ct_zero[threadIdx.x] = 0ull;
+    }
     if (halt && *halt) return;
     extern __shared__ uint32_t hist[];
     const int tid = threadIdx.x, b = blockIdx.x;
@@ -659,14 +665,20 @@ static hipError_t build_cells_single_level(const GdtSet* d_sets, const GdtChunk*
 
 hipError_t gdt_build_cells(const GdtSet* d_sets, int nsets, const GdtChunk* pchunks, int npch,
                            const GdtChunk* kchunks, int nkch, int total_n, int total_keys,
-                           const double* d_xy, GdtCells& c, hipStream_t s)
+                           const double* d_xy, GdtCells& c, hipStream_t s, int* status,
+                           unsigned long long* counters)
 {
     const gdt_d2* xy = reinterpret_cast<const gdt_d2*>(d_xy);
     const KbPlan pl = kb_plan((size_t)total_n, (size_t)total_keys);
-    if (pl.B > KB_MAXSHIFT)
+    if (pl.B > KB_MAXSHIFT) {
+        if (status) {
+            const hipError_t e = gdt_status_reset(status, ST_WORDS, counters, CT_WORDS, s);
+            if (e != hipSuccess) return e;
+        }
         return build_cells_single_level(d_sets, pchunks, npch, kchunks, nkch, total_keys, xy, c, s);
+    }
     hipLaunchKernelGGL(k_kb_count, dim3(pl.nb), dim3(KB_THREADS), pl.nkb * sizeof(uint32_t), s,
-                       d_sets, nsets, xy, total_n, pl, c.bmat, c.halt);
+                       d_sets, nsets, xy, total_n, pl, c.bmat, c.halt, status, counters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_kb_rowscan, dim3(pl.nkb), dim3(KB_NB), 0, s, pl, c.bmat, c.bscan);

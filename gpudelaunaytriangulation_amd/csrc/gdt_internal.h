@@ -53,6 +53,8 @@ enum {
 // an exact tie and hands the point to the one built with it.
 // The precision code of the exact mode (GDT_PREC_EXACT of the public header).
 enum { GDT_EXACT = 128 };
+// Lanes per point of the global triangulation's star kernel (one wavefront).
+enum { GDT_STAR_LANES = 64 };
 enum { ERR_SEARCH = 1, ERR_NOCLOSE = 2, ERR_NONEIGHBOUR = 4, ERR_DEFER = 8, ERR_TIE = 16,
        ERR_RANGE = 32 /* exact mode: coordinates too far outside the box, or a general box */ };
 // 64-bit counters (gdt_counters[...]).

@@ -56,9 +56,12 @@ hipError_t gdt_status_publish(const int* st, int nst, const unsigned long long* 
 hipError_t gdt_fill_i32(int* p, int v, size_t n, hipStream_t s);
 
 // Cell list over all sets.  pchunks: blocks of 256 points, kchunks: blocks of 256 keys.
+// With status non-null its ST_WORDS words and the CT_WORDS counters are
+// zeroed on the way (as gdt_status_reset, without a launch of its own).
 hipError_t gdt_build_cells(const GdtSet* d_sets, int nsets, const GdtChunk* pchunks, int npch,
                            const GdtChunk* kchunks, int nkch, int total_n, int total_keys,
-                           const double* d_xy, GdtCells& c, hipStream_t s);
+                           const double* d_xy, GdtCells& c, hipStream_t s,
+                           int* status = nullptr, unsigned long long* counters = nullptr);
 
 
 hipError_t gdt_export_cells(const GdtSet* d_sets, const GdtSet& h_set, const GdtCells& c,
@@ -102,6 +105,11 @@ hipError_t gdt_launch_deferred(int precision, const GdtSet* d_sets, int nsets, i
                                         const GdtRingIO& io, int* counts, int* status,           \
                                         unsigned long long* counters, int* defer,                \
                                         hipEvent_t after_tile, hipStream_t s);                   \
+    hipError_t gdt_launch_one_ring_star(const GdtSet* d_sets, int nsets, const GdtChunk* tiles,  \
+                                        int ntiles, int nsplit, int block, int total_n,          \
+                                        const GdtRingIO& io, int* counts, int* status,           \
+                                        unsigned long long* counters, int* defer,                \
+                                        hipEvent_t after_tile, hipStream_t s);                   \
     hipError_t gdt_launch_ring_test_tile(const GdtSet* d_sets, const GdtChunk* tiles, int ntiles,\
                                          const int* nbrs, const int* counts, int stride,         \
                                          int total_n, const double* d_xy, const GdtCells& c,     \
@@ -136,6 +144,12 @@ hipError_t gdt_launch_test(const GdtSet* d_sets, int nsets, const int* tris, con
 // from LDS are deferred (gdt_launch_deferred over `defer`, size total_n, count
 // in status[ST_NDEFER]).
 //
+// The same rings of every point with one wavefront per point (the star
+// kernel, FP64 predicates): the lanes run the search steps from every
+// candidate of the point's block at once and the ring is read off their
+// results; blocks (tile, 0..nsplit-1) share each tile's points, `block`
+// threads (a multiple of 64) each.
+//
 // Enumeration and empty-circle test in one pass over the rings, one block per
 // tile (tiled cell order): triangles owned by their smallest caller index,
 // flags as gdt_launch_test, counters[CT_INSUM] += sum of counts, *ntri =
@@ -155,6 +169,16 @@ inline hipError_t one_ring(int tshift, int precision, const GdtSet* d_sets, int 
     return (tshift == 4 ? ts4::gdt_launch_one_ring_tile : ts5::gdt_launch_one_ring_tile)(
         precision, d_sets, nsets, tiles, ntiles, block, lanes, total_n, list, lscan, flags, halt,
         io, counts, status, counters, defer, after_tile, s);
+}
+inline hipError_t one_ring_star(int tshift, const GdtSet* d_sets, int nsets,
+                                const GdtChunk* tiles, int ntiles, int nsplit, int block,
+                                int total_n, const GdtRingIO& io, int* counts, int* status,
+                                unsigned long long* counters, int* defer, hipEvent_t after_tile,
+                                hipStream_t s)
+{
+    return (tshift == 4 ? ts4::gdt_launch_one_ring_star : ts5::gdt_launch_one_ring_star)(
+        d_sets, nsets, tiles, ntiles, nsplit, block, total_n, io, counts, status, counters, defer,
+        after_tile, s);
 }
 inline hipError_t ring_test(int tshift, const GdtSet* d_sets, const GdtChunk* tiles, int ntiles,
                             const int* nbrs, const int* counts, int stride, int total_n,

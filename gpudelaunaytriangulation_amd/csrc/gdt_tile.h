@@ -199,7 +199,7 @@ GDT_HD inline void gdt_range_part(uint32_t& k, uint32_t& e, int sub, int W)
     }
 }
 
-// Row table of the block [-h,h]^2 around window cell c0 (h <= 2), each row
+// Row table of the block [-h,h]^2 around window cell c0 (h <= GDT_SCAN), each row
 // clipped to the columns cl keeps (and cut to part sub of W): one word per
 // non-empty row, first LDS entry | end << 16.  Returns the number of entries.
 GDT_HD inline unsigned gdt_block_rows(const GdtTileLds& L, int c0, int h, const GdtClip& cl,
@@ -312,6 +312,57 @@ GDT_HD inline void gdt_scan_tbl(const uint32_t* tbl, unsigned n, Load& load, Fn&
         XA = load((int)(ke & 0xffffu));
         fn((int)kc, XB);
         if (!more) break;
+    }
+}
+
+// A value the caller knows to be the same in every active lane, made visibly
+// uniform (scalar registers on the device).
+GDT_HD inline uint32_t gdt_uniform(uint32_t v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+#else
+    return v;
+#endif
+}
+
+// gdt_scan_tbl for a table that every active lane of the wave holds the same
+// (the star kernel's): the same visits in the same order, with the row switch
+// and the entry index uniform, so that they cost scalar instructions and
+// every lane reads the same coordinates (one LDS broadcast per entry).
+template <class Fn>
+GDT_HD inline void gdt_scan_tbl_uniform(const uint32_t* tbl, unsigned n, const gdt_d2* X, Fn& fn)
+{
+    n = gdt_uniform(n);
+    if (n == 0) return;
+    int r = 0;
+    uint32_t t = gdt_uniform(tbl[0]);
+    uint32_t k = t & 0xffffu, e = t >> 16;
+    unsigned i = 1;     // k is entry i of n
+    auto next = [&]() {
+        uint32_t kn = k + 1;
+        if (kn == e && i < n) {
+            t = gdt_uniform(tbl[++r]);
+            kn = t & 0xffffu;
+            e = t >> 16;
+        }
+        return kn;      // (after the last entry: its row's end, which exists)
+    };
+    // two entries a trip, so that the prefetched coordinates need no copy
+    gdt_d2 XA = X[k], XB;
+    for (;;) {
+        uint32_t kn = next();
+        XB = X[kn];
+        fn((int)k, XA);
+        if (i == n) break;
+        ++i;
+        k = kn;
+        kn = next();
+        XA = X[kn];
+        fn((int)k, XB);
+        if (i == n) break;
+        ++i;
+        k = kn;
     }
 }
 
@@ -455,16 +506,30 @@ GDT_HD inline int gdt_tile_advance(const GdtTileLds& L, const GdtRingIO& io, Gdt
     return STEP_RUN;
 }
 
+// The search of one FP64 step: the next neighbour bk after st.ak (st.ak < 0:
+// the nearest neighbour) and its displacement (bx, by), certified
+// (STEP_RUN), or STEP_DEFER.  ext = false gives up (STEP_DEFER) where the
+// search would scan beyond its first block or the rest of a circle's box.
+// The lane state is only read (its row table is written).  With `pre` (a row
+// table of the whole block [-ph, ph]^2, ph = GDT_SCAN0 or GDT_SCAN, npre
+// entries, the same in every active lane of the wave) both searches scan that
+// block first, unclipped, and certify
+// against it: the nearest neighbour is the same, and the successor is the
+// same best of the same left candidates (the clip only drops points that are
+// not strictly left of v->a).
 // W lanes may share the point (W = 2 or 4, device only): each scans its part
 // of every row segment, and the group then agrees on the best of their bests,
 // so all end every step in the same state.
 template <int W = 1>
-GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const GdtRingIO& io,
-                                    GdtTileLane<double>& st, unsigned& npred, int sub = 0)
+GDT_HD inline int gdt_tile_search_f64(const GdtSet& S, const GdtTileLds& L,
+                                      const GdtTileLane<double>& st, unsigned& npred, int sub,
+                                      bool ext, int& bk, double& bx, double& by,
+                                      const uint32_t* pre = nullptr, unsigned npre = 0,
+                                      int ph = GDT_SCAN0)
 {
     static_assert(W == 1 || W == 2 || W == 4, "1, 2 or 4 lanes per point");
     const bool wrap = st.ak >= 0;
-    int bk = -1;
+    bk = -1;
     const double ax = st.ax, ay = st.ay, vx = st.vx, vy = st.vy;
     const int wx = st.c0 % GDT_WIN, wy = st.c0 / GDT_WIN;
     // Both searches as one test, c taken when O(c) > Othr, P(c) < Pthr and
@@ -555,7 +620,8 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
 #endif
     };
     // the best's displacement and squared distance (the loop keeps only its entry)
-    double bx = 0., by = 0., bd2 = 0.;
+    double bd2 = 0.;
+    bx = by = 0.;
     auto best = [&]() {
         if (bk >= 0) {
             const gdt_d2 X = L.X64[bk];
@@ -566,15 +632,22 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
     };
     // nearest neighbour: the 3x3 block; successor: the 5x5 block, clipped to
     // the columns that may hold points left of v->a
-    int hs = wrap ? GDT_SCAN0 : GDT_SCAN_NN;
+    int hs = pre ? ph : (wrap ? GDT_SCAN0 : GDT_SCAN_NN);
     const double ox = gdt_cell_offset(vx, wx, S.bsx), oy = gdt_cell_offset(vy, wy, S.bsy);
     const GdtClip cl = gdt_left_clip(wrap, (float)ax, (float)ay, (float)ox, (float)oy,
                                      (float)S.bsx, (float)S.bsy, 1e-3f);
-    const unsigned nb = gdt_block_rows(L, st.c0, hs, cl, st.tbl, sub, W);
-    gdt_scan_tbl(st.tbl, nb, load, visit);
+    unsigned nb;
+    if (pre) {
+        nb = npre;
+        gdt_scan_tbl_uniform(pre, nb, L.X64, visit);
+    } else {
+        nb = gdt_block_rows(L, st.c0, hs, cl, st.tbl, sub, W);
+        gdt_scan_tbl(st.tbl, nb, load, visit);
+    }
     npred += nb;
     pair_reduce();
     if (bk < 0 && hs < GDT_SCAN) {
+        if (!ext) return STEP_DEFER;
         npred += gdt_lds_block(L, wx, wy, -GDT_SCAN, GDT_SCAN, -GDT_SCAN, GDT_SCAN, hs, visit_k);
         hs = GDT_SCAN;
     }
@@ -605,6 +678,7 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
         }
     };
     certify();
+    if (!ok && !ext) return STEP_DEFER;
     // a box leaving the window: the full block first (it lies in the window)
     if (!ok && hs < GDT_SCAN && !(boxed && gdt_in_window(S, st.c0, dx0, dx1, dy0, dy1))) {
         npred += gdt_lds_block(L, wx, wy, -GDT_SCAN, GDT_SCAN, -GDT_SCAN, GDT_SCAN, hs, visit_k);
@@ -624,7 +698,18 @@ GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const 
     }
     if (!ok) return STEP_DEFER;     // (a best of GDT_TIE is never certified)
     best();
-    return gdt_tile_advance<double>(L, io, st, wrap, bk, bx, by);
+    return STEP_RUN;
+}
+
+template <int W = 1>
+GDT_HD inline int gdt_tile_step_f64(const GdtSet& S, const GdtTileLds& L, const GdtRingIO& io,
+                                    GdtTileLane<double>& st, unsigned& npred, int sub = 0)
+{
+    int bk;
+    double bx, by;
+    const int rc = gdt_tile_search_f64<W>(S, L, st, npred, sub, true, bk, bx, by);
+    if (rc != STEP_RUN) return rc;
+    return gdt_tile_advance<double>(L, io, st, st.ak >= 0, bk, bx, by);
 }
 
 // W = 2: two neighbouring lanes share the point, as in the FP64 step.

@@ -51,6 +51,45 @@ __device__ inline int tile_flagged_slot(const TileFlags& f, int T0, int q)
     return T0 + 64 * lo + __ffsll((long long)m) - 1;
 }
 
+// Block totals of a ring kernel (every thread calls it): one set of atomics
+// per block, not per wave.
+__device__ inline void ring_block_totals(int mx, int er, unsigned npred, unsigned nfb,
+                                         unsigned nsum, int* status, unsigned long long* counters)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    mx = wave_max_i32(mx);
+    er = wave_or_i32(er);
+    npred = wave_sum_u32(npred);
+    nfb = wave_sum_u32(nfb);
+    nsum = wave_sum_u32(nsum);
+    __shared__ unsigned s_red[5][16];
+    if ((tid & 63) == 0) {
+        const int w = tid >> 6;
+        s_red[0][w] = (unsigned)mx;
+        s_red[1][w] = (unsigned)er;
+        s_red[2][w] = npred;
+        s_red[3][w] = nfb;
+        s_red[4][w] = nsum;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long bp = 0, bf = 0, bs = 0;
+        int bm = 0, be = 0;
+        for (int w = 0; w < (nt >> 6); ++w) {
+            bm = max(bm, (int)s_red[0][w]);
+            be |= (int)s_red[1][w];
+            bp += s_red[2][w];
+            bf += s_red[3][w];
+            bs += s_red[4][w];
+        }
+        if (bm > 0) atomicMax(&status[ST_REQUIRED], bm);
+        if (be) atomicOr(&status[ST_ERRFLAGS], be);
+        if (bp) atomicAdd(&counters[CT_PRED], bp);
+        if (bf) atomicAdd(&counters[CT_FALLBACK], bf);
+        if (bs) atomicAdd(&counters[CT_NBRSUM], bs);
+    }
+}
+
 template <int PREC, int W>
 __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict__ sets, const GdtChunk* __restrict__ tiles,
                      const int* __restrict__ list, const uint32_t* __restrict__ lscan,
@@ -224,38 +263,176 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
             }
         }
     }
-    // block totals: one set of atomics per block, not per wave
-    mx = wave_max_i32(mx);
-    er = wave_or_i32(er);
-    npred = wave_sum_u32(npred);
-    nfb = wave_sum_u32(nfb);
-    nsum = wave_sum_u32(nsum);
-    __shared__ unsigned s_red[5][16];
-    if ((tid & 63) == 0) {
-        const int w = tid >> 6;
-        s_red[0][w] = (unsigned)mx;
-        s_red[1][w] = (unsigned)er;
-        s_red[2][w] = npred;
-        s_red[3][w] = nfb;
-        s_red[4][w] = nsum;
+    ring_block_totals(mx, er, npred, nfb, nsum, status, counters);
+}
+
+// Words of a star lane's row table: one per cell row of the 7x7 block (the
+// word a scan reads after a table's last row is the next lane's first, or the
+// array's extra last word).
+#define STAR_TBL (2 * GDT_SCAN + 1)
+
+// One wavefront per point (small grids; FP64 predicates): lane g takes entry g
+// of the cell block around the point's cell (7x7 cells, or 5x5 when the 7x7
+// block holds more than 64 entries), in the order a step scans it, and runs
+// the step's search (gdt_tile_search_f64) over that block with that
+// candidate as the current neighbour -- every candidate's successor at once;
+// the lane holding the point itself runs the nearest-neighbour search.  The
+// ring is then the walk nearest -> successor -> ... read from the lanes: the
+// sequential walk, each step's result taken from the lane that ran the same
+// search on the same state.  Where the walk reaches a candidate whose search
+// the lanes did not certify (its circle leaves the block: those lanes do not
+// extend) or one outside the block, lane 0 runs that one search in full; a
+// block of more than 64 entries runs the sequential steps in lane 0.  Blocks
+// (tile, y) share a tile's points: wave w of block y takes the points
+// T0 + y nw + w + m gridDim.y nw.  Deferred points go to `defer` (handled by
+// k_one_ring_wave in the next launch).
+__global__ __launch_bounds__(1024) void k_one_ring_star(const GdtSet* __restrict__ sets,
+                                                        const GdtChunk* __restrict__ tiles,
+                                                        GdtRingIO io, int* __restrict__ counts,
+                                                        int* __restrict__ status,
+                                                        unsigned long long* __restrict__ counters,
+                                                        int* __restrict__ defer)
+{
+    __shared__ uint32_t s_cs[GDT_WIN_CELLS + 1];
+    __shared__ gdt_d2 s_X64[GDT_WIN_MAXPTS + 1];
+    __shared__ gdt_f2 s_X32[1];
+    __shared__ int s_orig[GDT_WIN_MAXPTS];
+    __shared__ uint32_t s_wsum[16];
+    extern __shared__ uint32_t s_tbl[];     // [STAR_TBL * blockDim + 1] the lanes' row tables
+
+    const GdtChunk ch = tiles[blockIdx.x];
+    const GdtSet S = sets[ch.set];
+    const GdtTileWin w = gdt_tile_window(S, ch.start);
+    GdtTileLds L;
+    L.cs = s_cs; L.gstart = nullptr; L.X64 = s_X64; L.X32 = s_X32; L.orig = s_orig;
+    const int tid = threadIdx.x, lane = tid & 63, nw = (int)blockDim.x >> 6;
+    const int k0 = S.c0 + ch.start * GDT_TILE_CELLS;
+    const int T0 = (int)io.cstart[k0], T1 = (int)io.cstart[k0 + GDT_TILE_CELLS];
+    const int qstep = (int)gridDim.y * nw;
+    const int qb = T0 + (int)blockIdx.y * nw;
+    if (qb >= T1) return;    // block-uniform: no point for this block
+
+    const bool fits = gdt_stage_window<false>(S, w, io, L, s_wsum);
+    if (!fits) {
+        if (lane == 0)
+            for (int q = qb + (tid >> 6); q < T1; q += qstep) {
+                const uint32_t pos = atomicAdd(reinterpret_cast<unsigned int*>(&status[ST_NDEFER]), 1u);
+                defer[pos] = q;
+            }
+        return;
     }
-    __syncthreads();
-    if (tid == 0) {
-        unsigned long long bp = 0, bf = 0, bs = 0;
-        int bm = 0, be = 0;
-        for (int w = 0; w < (nt >> 6); ++w) {
-            bm = max(bm, (int)s_red[0][w]);
-            be |= (int)s_red[1][w];
-            bp += s_red[2][w];
-            bf += s_red[3][w];
-            bs += s_red[4][w];
+    unsigned npred = 0, nsum = 0;
+    int mx = 0, er = 0;
+    for (int q = qb + (tid >> 6); q < T1; q += qstep) {     // wave-uniform
+        GdtTileLane<double> st;
+        st.tbl = s_tbl + STAR_TBL * tid;
+        gdt_tile_lane_init<double>(w, L, q, io, S, st);
+        // the lanes' block, as one row table (every lane the same): the
+        // (2 GDT_SCAN + 1)^2 cells when they hold at most 64 entries, else
+        // the (2 GDT_SCAN0 + 1)^2 cells; and this lane's entry of it
+        const GdtClip whole = {0.f, 0.f, 0};
+        int ph = GDT_SCAN;
+        int tot = (int)gdt_block_rows(L, st.c0, ph, whole, st.tbl);
+        if (tot > 64) {
+            ph = GDT_SCAN0;
+            tot = (int)gdt_block_rows(L, st.c0, ph, whole, st.tbl);
         }
-        if (bm > 0) atomicMax(&status[ST_REQUIRED], bm);
-        if (be) atomicOr(&status[ST_ERRFLAGS], be);
-        if (bp) atomicAdd(&counters[CT_PRED], bp);
-        if (bf) atomicAdd(&counters[CT_FALLBACK], bf);
-        if (bs) atomicAdd(&counters[CT_NBRSUM], bs);
+        int kg = -1;
+        for (int acc = 0, r = 0; acc < tot; ++r) {      // (the table's non-empty rows)
+            const uint32_t t = st.tbl[r];
+            const int b = (int)(t & 0xffffu), n = (int)(t >> 16) - b;
+            if (lane >= acc && lane < acc + n) kg = b + lane - acc;
+            acc += n;
+        }
+        int bk = -1, rc = STEP_DEFER;
+        if (tot <= 64 && kg >= 0) {
+            GdtTileLane<double> sl = st;
+            if (kg != st.iv) {
+                const gdt_d2 X = L.X64[kg];
+                sl.ak = kg;
+                sl.ax = X.x - st.vx;
+                sl.ay = X.y - st.vy;
+            }
+            double bx, by;
+            rc = gdt_tile_search_f64<1>(S, L, sl, npred, 0, false, bk, bx, by, st.tbl,
+                                        (unsigned)tot, ph);
+        }
+        int res = STEP_RUN, count = 0;
+        if (tot > 64) {
+            if (lane == 0) {
+                int r = STEP_RUN;
+                while (r == STEP_RUN) r = gdt_tile_step_f64<1>(S, L, io, st, npred);
+                res = r;
+                count = st.count;
+            }
+        } else {
+            const unsigned long long mv = __ballot(kg == st.iv);
+            const int hv = __ffsll((long long)mv) - 1;
+            int nn = __shfl(bk, hv, 64), nrc = __shfl(rc, hv, 64);
+            if (nrc != STEP_RUN) {
+                // not certified over the block: lane 0 runs the full search
+                if (lane == 0) {
+                    double bx, by;
+                    nrc = gdt_tile_search_f64<1>(S, L, st, npred, 0, true, nn, bx, by);
+                }
+                nrc = __shfl(nrc, 0, 64);
+                nn = __shfl(nn, 0, 64);
+            }
+            if (nrc != STEP_RUN) {
+                res = STEP_DEFER;
+            } else {
+                if (lane == 0) st.row[0] = L.orig[nn];
+                count = 1;
+                for (int cur = nn;;) {
+                    const unsigned long long m = __ballot(kg == cur && rc == STEP_RUN);
+                    int nx;
+                    if (m) {
+                        nx = __shfl(bk, __ffsll((long long)m) - 1, 64);
+                    } else {
+                        // no lane certified cur's successor: lane 0 runs the
+                        // full search from cur (extension included)
+                        int r = STEP_DEFER, b = -1;
+                        if (lane == 0) {
+                            GdtTileLane<double> sl = st;
+                            const gdt_d2 X = L.X64[cur];
+                            sl.ak = cur;
+                            sl.ax = X.x - st.vx;
+                            sl.ay = X.y - st.vy;
+                            double bx, by;
+                            r = gdt_tile_search_f64<1>(S, L, sl, npred, 0, true, b, bx, by);
+                        }
+                        if (__shfl(r, 0, 64) != STEP_RUN) {
+                            res = STEP_DEFER;
+                            break;
+                        }
+                        nx = __shfl(b, 0, 64);
+                    }
+                    if (nx == nn) {
+                        res = STEP_DONE;
+                        break;
+                    }
+                    if (lane == 0 && count < io.stride) st.row[count] = L.orig[nx];
+                    if (++count > GDT_HARD_CAP) {
+                        res = STEP_NOCLOSE;
+                        break;
+                    }
+                    cur = nx;
+                }
+            }
+        }
+        if (lane == 0) {
+            if (res == STEP_DEFER) {
+                const uint32_t pos = atomicAdd(reinterpret_cast<unsigned int*>(&status[ST_NDEFER]), 1u);
+                defer[pos] = q;
+            } else {
+                counts[L.orig[st.iv]] = count;
+                mx = max(mx, count);
+                nsum += (unsigned)count;
+                if (res == STEP_NOCLOSE) er |= ERR_NOCLOSE;
+            }
+        }
     }
+    ring_block_totals(mx, er, npred, 0u, nsum, status, counters);
 }
 
 hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nsets,
@@ -291,6 +468,24 @@ hipError_t gdt_launch_one_ring_tile(int precision, const GdtSet* d_sets, int nse
     if (after_tile && (e = hipEventRecord(after_tile, s)) != hipSuccess) return e;
     return gdt_launch_deferred(precision, d_sets, nsets, total_n, io, counts, status, counters,
                                defer, s);
+}
+
+hipError_t gdt_launch_one_ring_star(const GdtSet* d_sets, int nsets, const GdtChunk* tiles,
+                                    int ntiles, int nsplit, int block, int total_n,
+                                    const GdtRingIO& io, int* counts, int* status,
+                                    unsigned long long* counters, int* defer,
+                                    hipEvent_t after_tile, hipStream_t s)
+{
+    if (ntiles <= 0) return hipSuccess;
+    if (block < 64 || block > 1024 || (block & 63) || nsplit < 1 || nsplit > 65535)
+        return hipErrorInvalidValue;
+    const size_t tbl = (STAR_TBL * (size_t)block + 1) * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_one_ring_star, dim3(ntiles, nsplit), dim3(block), tbl, s, d_sets, tiles,
+                       io, counts, status, counters, defer);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (after_tile && (e = hipEventRecord(after_tile, s)) != hipSuccess) return e;
+    return gdt_launch_deferred(64, d_sets, nsets, total_n, io, counts, status, counters, defer, s);
 }
 
 }  // namespace GDT_TS_NS

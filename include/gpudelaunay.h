@@ -64,9 +64,14 @@ typedef struct {
                            or 5 (32x32); 0 (default) = automatic: 16x16 when the 32x32
                            tiles would not outnumber the compute units, else 32x32 */
     int lanes;          /* lanes per point of the global triangulation's tile kernel, 1, 2
-                           or 4 (4: FP64; the FP32-filtered kernel takes 2 then);
-                           0 (default) = automatic: 2 with 16x16 tiles that do not outnumber
-                           the compute units, else 1 */
+                           or 4 (4: FP64; the FP32-filtered kernel takes 2 then), or 64:
+                           one wavefront per point whose lanes search from every candidate
+                           of the point's cell block at once (FP64 predicates in either
+                           precision, the same rings; 256-thread blocks; for small grids);
+                           0 (default) = automatic: 64 with 16x16 tiles that number at most
+                           a quarter of the compute units and at most 1.25 points per cell,
+                           2 with 16x16 tiles that do not outnumber the compute units,
+                           else 1 */
     int repair_lanes;   /* lanes per point of the repairs' tile kernel, 1, 2 or 4 (as
                            lanes); 0 (default) = 2 */
 } gdt_config;

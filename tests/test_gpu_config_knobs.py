@@ -1,6 +1,7 @@
 """The tuner's knobs change the work split, never the rings: LDS tiles of
 16x16 cells (tile_shift 4) or 32x32 (5), one, two or four lanes per point in the
-global triangulation (lanes) and in repairs (repair_lanes), each against the
+global triangulation (lanes; 64: the star kernel) and in repairs
+(repair_lanes), each against the
 oracle: uniform points (the 2^20 workload against its golden hash), a
 rectangular box, a cluster that overflows the 16x16 windows, test-and-repair
 and a short Brownian run."""
@@ -20,7 +21,7 @@ from gpudelaunaytriangulation_amd.delaunay import (BatchedDelaunayGPU, DelaunayG
                                                    PeriodicBoundaries)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-KNOBS = [(4, 1), (4, 2), (5, 2), (4, 4)]  # (tile_shift, lanes); (5, 1) is the default
+KNOBS = [(4, 1), (4, 2), (5, 2), (4, 4), (4, 64)]  # (tile_shift, lanes); (5, 1) is the default
 
 
 def triangulate(xy, lx, ly, prec, ts, lanes, repair_lanes=2):

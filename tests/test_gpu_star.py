@@ -1,0 +1,89 @@
+"""The star kernel (lanes = 64: one wavefront per point, every candidate's
+successor searched at once, the ring read off the lanes) gives the rings of
+the sequential tile kernel and of the oracle: uniform points at the small sizes
+it serves and at 2^18, a rectangular box, both tile edges, a cluster whose
+blocks hold more than 64 entries and whose windows overflow (sequential steps
+and deferral), a jittered lattice (near-cocircular quads: uncertified
+successors) and exact square lattices (exact ties, decided by the symbolic
+perturbation after deferral)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import oracle  # noqa: E402
+from exact_delaunay import SCALE, square_lattice  # noqa: E402
+from gpudelaunaytriangulation_amd import PREC_FP32F, PREC_FP64, inputs, rings  # noqa: E402
+from gpudelaunaytriangulation_amd.delaunay import DelaunayGPU, PeriodicBoundaries  # noqa: E402
+
+STAR = 64
+
+
+def triangulate(xy, lx, ly, prec=PREC_FP64, lanes=STAR, ts=0, cell=1.0):
+    n = xy.shape[0]
+    dg = DelaunayGPU(n, 16, cell, PeriodicBoundaries(lx, ly))
+    dg.setPrecision(prec)
+    dg.ctx.set_config(lanes=lanes, tile_shift=ts)
+    pts = torch.from_numpy(np.ascontiguousarray(xy)).cuda()
+    tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device="cuda")
+    num = torch.zeros(n, dtype=torch.int32, device="cuda")
+    dg.globalDelaunayTriangulation(pts, tri, num)
+    torch.cuda.synchronize()
+    nb = tri.cpu().numpy().reshape(n, dg.MaxSize)
+    return nb, num.cpu().numpy(), dg
+
+
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32F])
+@pytest.mark.parametrize("n,lx,ly,seed,ts", [(4096, 64.0, 64.0, 1, 0), (16384, 128.0, 128.0, 5, 0),
+                                             (65536, 256.0, 256.0, 61, 0),
+                                             (20000, 200.0, 100.0, 62, 0),
+                                             (16384, 128.0, 128.0, 6, 5),
+                                             (1 << 18, 512.0, 512.0, 7, 0)])
+def test_star_equals_oracle(n, lx, ly, seed, ts, prec):
+    print(f"[star] n {n} box {lx}x{ly} ts {ts} prec {prec}", flush=True)
+    xy = inputs.uniform_points(n, lx, ly, seed=seed)
+    nb, cnt, dg = triangulate(xy, lx, ly, prec, ts=ts)
+    assert dg.stats()["lanes"] == STAR
+    on, oc = oracle.triangulate(xy, lx, ly, 1.0, 16)
+    assert rings.ring_hash(nb, cnt) == rings.ring_hash(on, oc)
+    assert dg.stats()["neighbours"] == int(oc.sum())
+
+
+def test_star_cluster_runs_sequential_steps_and_defers():
+    """3,000 points in a 6x6 square: cell blocks of hundreds of entries (the
+    lanes cannot hold them) and windows beyond the LDS budget (deferred)."""
+    L = 141.0
+    base = inputs.uniform_points(17000, L, L, seed=41)
+    cluster = 60.0 + np.random.default_rng(42).uniform(0.0, 6.0, size=(3000, 2))
+    xy = np.concatenate([base, cluster])
+    on, oc = oracle.triangulate(xy, L, L, 1.0, 16)
+    for ts in (4, 5):
+        print(f"[star] cluster ts {ts}", flush=True)
+        nb, cnt, dg = triangulate(xy, L, L, PREC_FP64, ts=ts)
+        if ts == 4:
+            assert dg.stats()["deferred"] > 0
+        assert rings.ring_hash(nb, cnt) == rings.ring_hash(on, oc)
+
+
+def test_star_jittered_lattice_equals_oracle():
+    lx, ly = inputs.lattice_box(128, 128)
+    xy = inputs.lattice_points(128, 128)
+    print("[star] jittered lattice 128x128", flush=True)
+    nb, cnt, _ = triangulate(xy, lx, ly, PREC_FP64)
+    on, oc = oracle.triangulate(xy, lx, ly, 1.0, 16)
+    assert rings.ring_hash(nb, cnt) == rings.ring_hash(on, oc)
+
+
+@pytest.mark.parametrize("nx,ny,sp", [(64, 64, 1.0), (96, 96, 0.25)])
+def test_star_exact_lattice_equals_sequential(nx, ny, sp):
+    """Exact ties: the rings equal the sequential kernel's (which the exact
+    tie tests pin to the perturbed Delaunay triangulation)."""
+    P = square_lattice(nx, ny, sp)
+    xy, lx, ly = P / SCALE, nx * sp, ny * sp
+    print(f"[star] exact lattice {nx}x{ny} spacing {sp}", flush=True)
+    nb, cnt, _ = triangulate(xy, lx, ly, PREC_FP64)
+    sb, sc, _ = triangulate(xy, lx, ly, PREC_FP64, lanes=1)
+    assert int(cnt.sum()) == 6 * len(cnt)
+    assert rings.ring_hash(nb, cnt) == rings.ring_hash(sb, sc)
