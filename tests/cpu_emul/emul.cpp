@@ -2,10 +2,13 @@
 // Builds the cell-ordered arrays the GPU cell-list kernels build, then runs
 // the direct per-point routine (gdt_point.h), the LDS-staged tile routine
 // (gdt_tile.h, one emulated block per tile, deferred points through the
-// wavefront routine) or the wavefront routine (gdt_wave.h, 64 emulated lanes)
+// wavefront routine), the wavefront routine (gdt_wave.h, 64 emulated lanes)
+// or the star kernel's steps (tile.hip k_one_ring_star: 64 emulated lanes,
+// each running gdt_tile_search_f64 from one candidate of the shared block
+// table, the ring walked over their results; FP64 whatever the precision)
 // for every point.
 // Usage: emul <points.bin> <n> <lx> <ly> <cell_size> <tiled> <precision 64|32> <stride>
-//             <out.bin> [mode 0=direct 1=tile 2=wave] [audit.bin]
+//             <out.bin> [mode 0=direct 1=tile 2=wave 3=star] [audit.bin]
 // A general box: <lx> = "box:x11,x12,x21,x22" (row-major cells, <ly> ignored).
 // out.bin: int32 counts[n], int32 nbrs[n*stride], uint64 npred, uint64 nfb, int32 errflags
 // Built with -DGDT_AUDIT_FILTER, every decision the FP32 filter takes from its
@@ -28,6 +31,98 @@ static std::vector<AuditRec> g_audit;
 #include <cstdlib>
 #include <vector>
 
+// LDS entry k as the lane's current neighbour, as the star kernel sets it.
+static void gdt_tile_set_a_host(const GdtTileLds& L, GdtTileLane<double>& st, int k)
+{
+    const gdt_d2 X = L.X64[k];
+    st.ak = k;
+    st.ax = X.x - st.vx;
+    st.ay = X.y - st.vy;
+}
+
+// The star kernel's work for the point in slot j, one emulated lane after the
+// other (the kernel's ballots and shuffles become searches over the lanes'
+// arrays).  ERR_DEFER: the point goes on to the wavefront routine.
+static GdtRingResult star_ring(const GdtSet& S, const GdtTileWin& w, const GdtTileLds& L, int j,
+                               const GdtRingIO& io)
+{
+    GdtRingResult res = {0, 0, 0u, 0u};
+    uint32_t pre[2 * GDT_SCAN + 2] = {}, own[2 * GDT_SCAN + 2] = {};
+    GdtTileLane<double> st;
+    st.tbl = own;
+    gdt_tile_lane_init<double>(w, L, j, io, S, st);
+    const GdtClip whole = {0.f, 0.f, 0};
+    int ph = GDT_SCAN;
+    int tot = (int)gdt_block_rows(L, st.c0, ph, whole, pre);
+    if (tot > 64) {
+        ph = GDT_SCAN0;
+        tot = (int)gdt_block_rows(L, st.c0, ph, whole, pre);
+    }
+    int rc = STEP_RUN;
+    if (tot > 64) {
+        while (rc == STEP_RUN) rc = gdt_tile_step_f64<1>(S, L, io, st, res.npred);
+    } else {
+        int kg[64], bk[64], lrc[64];
+        for (int acc = 0, r = 0; acc < tot; ++r) {
+            const int b = (int)(pre[r] & 0xffffu), n = (int)(pre[r] >> 16) - b;
+            for (int g = acc; g < acc + n; ++g) kg[g] = b + g - acc;
+            acc += n;
+        }
+        for (int g = 0; g < tot; ++g) {
+            GdtTileLane<double> sl = st;
+            if (kg[g] != st.iv) gdt_tile_set_a_host(L, sl, kg[g]);
+            double bx, by;
+            lrc[g] = gdt_tile_search_f64<1>(S, L, sl, res.npred, 0, kg[g] == st.iv, bk[g], bx, by,
+                                            pre, (unsigned)tot, ph);
+        }
+        auto lane_of = [&](int k) {
+            for (int g = 0; g < tot; ++g)
+                if (kg[g] == k) return g;
+            return -1;
+        };
+        // the full search from entry a (a < 0: the nearest neighbour)
+        auto full = [&](int a, int& b) {
+            GdtTileLane<double> sl = st;
+            if (a >= 0) gdt_tile_set_a_host(L, sl, a);
+            double bx, by;
+            return gdt_tile_search_f64<1>(S, L, sl, res.npred, 0, true, b, bx, by);
+        };
+        const int hv = lane_of(st.iv);
+        int nn = bk[hv], nrc = lrc[hv];
+        if (nrc != STEP_RUN) nrc = full(-1, nn);
+        if (nrc != STEP_RUN) {
+            rc = STEP_DEFER;
+        } else {
+            st.row[0] = L.orig[nn];
+            st.count = 1;
+            for (int cur = nn;;) {
+                const int h = lane_of(cur);
+                int nx;
+                if (h >= 0 && lrc[h] == STEP_RUN) {
+                    nx = bk[h];
+                } else if (full(cur, nx) != STEP_RUN) {
+                    rc = STEP_DEFER;
+                    break;
+                }
+                if (nx == nn) {
+                    rc = STEP_DONE;
+                    break;
+                }
+                if (st.count < io.stride) st.row[st.count] = L.orig[nx];
+                if (++st.count > GDT_HARD_CAP) {
+                    rc = STEP_NOCLOSE;
+                    break;
+                }
+                cur = nx;
+            }
+        }
+    }
+    res.count = st.count;
+    if (rc == STEP_DEFER) res.err |= ERR_DEFER;
+    if (rc == STEP_NOCLOSE) res.err |= ERR_NOCLOSE;
+    return res;
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 10 || argc > 12) { std::fprintf(stderr, "bad args\n"); return 2; }
@@ -39,7 +134,10 @@ int main(int argc, char** argv)
     const double a = std::atof(argv[5]);
     const int tiled = std::atoi(argv[6]), prec = std::atoi(argv[7]), stride = std::atoi(argv[8]);
     const int mode = argc >= 11 ? std::atoi(argv[10]) : 0;
-    if (mode == 1 && !tiled) { std::fprintf(stderr, "tile mode needs tiled order\n"); return 2; }
+    if ((mode == 1 || mode == 3) && !tiled) {
+        std::fprintf(stderr, "tile and star modes need tiled order\n");
+        return 2;
+    }
     std::vector<double> xy(2 * (size_t)n);
     FILE* f = std::fopen(argv[1], "rb");
     if (!f || std::fread(xy.data(), sizeof(double), xy.size(), f) != xy.size()) return 3;
@@ -112,7 +210,8 @@ int main(int argc, char** argv)
             const int k0 = S.c0 + t * GDT_TILE_CELLS;
             for (int j = (int)io.cstart[k0]; j < (int)io.cstart[k0 + GDT_TILE_CELLS]; ++j) {
                 GdtRingResult r;
-                if (fits) r = prec == 32 ? gdt_tile_ring<32>(S, w, L, j, io) : gdt_tile_ring<64>(S, w, L, j, io);
+                if (fits && mode == 3) r = star_ring(S, w, L, j, io);
+                else if (fits) r = prec == 32 ? gdt_tile_ring<32>(S, w, L, j, io) : gdt_tile_ring<64>(S, w, L, j, io);
                 if (!fits || (r.err & ERR_DEFER)) {
                     const unsigned np = fits ? r.npred : 0u, nf = fits ? r.nfb : 0u;
                     ++ndefer;
@@ -127,7 +226,7 @@ int main(int argc, char** argv)
             }
         }
     }
-    if (mode == 1) std::fprintf(stderr, "deferred %d of %d points\n", ndefer, n);
+    if (mode == 1 || mode == 3) std::fprintf(stderr, "deferred %d of %d points\n", ndefer, n);
     f = std::fopen(argv[9], "wb");
     if (!f) return 4;
     std::fwrite(counts.data(), sizeof(int), n, f);

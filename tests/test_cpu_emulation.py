@@ -1,8 +1,10 @@
 """The device routines compiled for the host with g++ under AddressSanitizer
 and UBSan and run on the CPU: the direct per-point routine (gdt_point.h), the
 LDS-staged tile routine (gdt_tile.h, one emulated block per tile, deferred
-points through the wavefront routine) and the wavefront routine (gdt_wave.h,
-64 emulated lanes and the shuffle reduction tree).  FP64 and FP32-filtered
+points through the wavefront routine), the wavefront routine (gdt_wave.h,
+64 emulated lanes and the shuffle reduction tree) and the star kernel's steps
+(64 emulated lanes each searching from one candidate of the point's block,
+the ring walked over their results).  FP64 and FP32-filtered
 paths must reproduce the oracle's rings bit-exactly and be byte-identical to
 each other, on uniform, near-degenerate (jittered lattice), rectangular, small,
 fine-grid (cells much smaller than the spacing: circle-box extensions and
@@ -82,14 +84,14 @@ def test_emulated_kernels_match_oracle(emul, tmp_path, case):
     outs = {}
     # (cell order, mode): direct routine in tile and row-major order, tile
     # routine in tile and Morton-in-tile order, wavefront routine in tile and
-    # row-major order
-    for tiled, mode in ((1, 0), (0, 0), (1, 1), (2, 1), (1, 2), (0, 2)):
+    # row-major order, the star kernel's steps in tile order
+    for tiled, mode in ((1, 0), (0, 0), (1, 1), (2, 1), (1, 2), (0, 2), (1, 3)):
         for prec in (64, 32):
             nb, c, npred, nfb, err = run(emul, str(tmp_path), xy, lx, ly, tiled, prec, mode=mode,
                                          cell=cell)
             assert err == 0
             assert rings.ring_hash(nb, c) == ref, (prec, tiled, mode)
-            if prec == 64:
+            if prec == 64 or mode == 3:
                 assert nfb == 0
             elif case == "uniform4096" and mode == 1:
                 # the LDS routine's filter decides all but a few per mille in
@@ -138,7 +140,7 @@ def test_emulated_kernels_exact_ties(emul, tmp_path, case):
     P = square_lattice(nx, ny, sp)
     xy, lx, ly = P / SCALE, nx * sp, ny * sp
     ref = None
-    for tiled, mode in ((1, 0), (0, 0), (1, 1), (2, 1), (1, 2), (0, 2)):
+    for tiled, mode in ((1, 0), (0, 0), (1, 1), (2, 1), (1, 2), (0, 2), (1, 3)):
         for prec in (64, 32):
             nb, c, _, _, err = run(emul, str(tmp_path), xy, lx, ly, tiled, prec, mode=mode)
             assert err == 0
