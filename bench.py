@@ -136,7 +136,8 @@ def cpu_model():
 
 def bench_c1(torch, cpu_runs, with_cpu):
     """4,096 uniform points in a box of side 64 (the reference's CPU-runnable
-    case): GPU points/s per precision (HIP events, 100 calls after 20) and,
+    case): GPU points/s per precision (HIP events, 200 calls after 200, so that
+    the clocks have ramped up for calls of a few tens of microseconds) and,
     unless skipped, the CPU oracle on the same points (median of cpu_runs)."""
     from gpudelaunaytriangulation_amd import PREC_FP32F, PREC_FP64, inputs
     from gpudelaunaytriangulation_amd.delaunay import DelaunayGPU, PeriodicBoundaries
@@ -149,7 +150,7 @@ def bench_c1(torch, cpu_runs, with_cpu):
         dg.setPrecision(prec)
         tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device="cuda")
         num = torch.zeros(n, dtype=torch.int32, device="cuda")
-        ms = time_path(dg, pts, tri, num, 100, 20, torch)[0]
+        ms = time_path(dg, pts, tri, num, 200, 200, torch)[0]
         res[f"gpu_{name}_points_per_s"] = n / (ms * 1e-3)
     if with_cpu:
         import oracle
@@ -305,7 +306,7 @@ def bench_general(torch, steps, warmup):
 
 def bench_small_n(torch, sizes=(16384, 65536, 262144)):
     """Uniform points at density 1 (box sqrt(N)), default (automatic)
-    configuration: points/s per precision over 100 calls after 20 (HIP events),
+    configuration: points/s per precision over 200 calls after 200 (HIP events),
     with the tile edge and lanes per point the library chose."""
     from gpudelaunaytriangulation_amd import PREC_FP32F, PREC_FP64, inputs
     from gpudelaunaytriangulation_amd.delaunay import DelaunayGPU, PeriodicBoundaries
@@ -320,7 +321,7 @@ def bench_small_n(torch, sizes=(16384, 65536, 262144)):
             dg.setPrecision(prec)
             tri = torch.zeros(n * dg.MaxSize, dtype=torch.int32, device="cuda")
             num = torch.zeros(n, dtype=torch.int32, device="cuda")
-            ms, _, _, _, _, st = time_path(dg, pts, tri, num, 100, 20, torch)
+            ms, _, _, _, _, st = time_path(dg, pts, tri, num, 200, 200, torch)
             row[f"{name}_points_per_s"] = n / (ms * 1e-3)
             row["tile_cells"] = 1 << st["tile_shift"]
             row["lanes_per_point"] = st["lanes"]
