@@ -4,8 +4,8 @@ the sequential tile kernel and of the oracle: uniform points at the small sizes
 it serves and at 2^18, a rectangular box, both tile edges, a cluster whose
 blocks hold more than 64 entries and whose windows overflow (sequential steps
 and deferral), a jittered lattice (near-cocircular quads: uncertified
-successors) and exact square lattices (exact ties, decided by the symbolic
-perturbation after deferral)."""
+successors), exact square lattices (exact ties, decided by the symbolic
+perturbation after deferral) and batches of a few small sets."""
 import numpy as np
 import pytest
 
@@ -87,3 +87,31 @@ def test_star_exact_lattice_equals_sequential(nx, ny, sp):
     sb, sc, _ = triangulate(xy, lx, ly, PREC_FP64, lanes=1)
     assert int(cnt.sum()) == 6 * len(cnt)
     assert rings.ring_hash(nb, cnt) == rings.ring_hash(sb, sc)
+
+
+def test_star_small_batches_equal_oracle_per_set():
+    """A few small sets (the automatic choice then picks the star kernel; the
+    blocks take their set from the tile they serve), rectangles among them,
+    each set's rings equal the oracle's for that set alone."""
+    from gpudelaunaytriangulation_amd.delaunay import BatchedDelaunayGPU
+    specs = [(500, 25.0, 20.0, 81), (1200, 30.0, 40.0, 82), (3000, 55.0, 55.0, 83), (64, 8.0, 8.0, 84)]
+    pts = [inputs.uniform_points(n, lx, ly, seed=s) for n, lx, ly, s in specs]
+    for lanes in (0, STAR):
+        bd = BatchedDelaunayGPU([n for n, *_ in specs], [(lx, ly) for _, lx, ly, _ in specs], 16, 1.0,
+                                precision=PREC_FP64)
+        bd.ctx.set_config(lanes=lanes)
+        X = torch.from_numpy(np.concatenate(pts)).cuda()
+        tri = torch.zeros(bd.N * bd.MaxSize, dtype=torch.int32, device="cuda")
+        num = torch.zeros(bd.N, dtype=torch.int32, device="cuda")
+        print(f"[star] small batch lanes {lanes}", flush=True)
+        bd.globalDelaunayTriangulation(X, tri, num)
+        torch.cuda.synchronize()
+        assert bd.stats()["lanes"] == STAR
+        nb = tri.cpu().numpy().reshape(bd.N, bd.MaxSize)
+        cnt = num.cpu().numpy()
+        for s, (xy, (n, lx, ly, _)) in enumerate(zip(pts, specs)):
+            a, b = bd.offsets[s], bd.offsets[s + 1]
+            local = nb[a:b].copy()
+            local[local >= 0] -= a
+            on, oc = oracle.triangulate(xy, lx, ly, 1.0, 16)
+            assert rings.ring_hash(local, cnt[a:b]) == rings.ring_hash(on, oc), s
