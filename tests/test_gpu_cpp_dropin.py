@@ -16,7 +16,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 import oracle  # noqa: E402
-from gpudelaunaytriangulation_amd import PREC_FP32F, PREC_FP64, inputs, rings  # noqa: E402
+from gpudelaunaytriangulation_amd import PREC_EXACT, PREC_FP32F, PREC_FP64, inputs, rings  # noqa: E402
 from gpudelaunaytriangulation_amd.build import DROPIN_BIN  # noqa: E402
 
 N, L = 4096, 64.0
@@ -67,10 +67,11 @@ def circumcentres(xy, nb, c):
 
 
 @pytest.mark.parametrize("mode", ["gpu", "cpu"])
-@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32F])
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32F, PREC_EXACT])
 def test_cpp_dropin_classes(tmp_path, prec, mode):
     """mode "cpu": DelaunayGPU with setGPUcompute(false) and 4 OpenMP threads
-    and cellListGPU::compute on the host, the same checks."""
+    and cellListGPU::compute on the host, the same checks.  The exact mode on
+    this generic input gives the same rings (every decision exact)."""
     assert os.path.exists(DROPIN_BIN), "C++ drop-in check not built (run __graft_entry__.build())"
     xy = inputs.uniform_points(N, L, L, seed=1)
     moved = inputs.wrap_into_box(xy + 0.02 * inputs.gaussian(31, 2 * N).reshape(-1, 2), L, L)
