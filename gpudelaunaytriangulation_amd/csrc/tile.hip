@@ -267,8 +267,8 @@ __global__ __launch_bounds__(1024) void k_one_ring_tile(const GdtSet* __restrict
 }
 
 // Words of a star lane's row table: one per cell row of the 7x7 block (the
-// word a scan reads after a table's last row is the next lane's first, or the
-// array's extra last word).
+// uniform scan reads no word beyond its rows; a step's own tables, at most
+// five rows and the word read after them, fit in the first six).
 #define STAR_TBL (2 * GDT_SCAN + 1)
 
 // One wavefront per point (small grids; FP64 predicates): lane g takes entry g
