@@ -482,6 +482,19 @@ GDT_HD inline bool gdt_incircle_tie(int pv, int pa, int pb, int pc, double ax, d
 // offsets, of dcx*bs and of the two sums); all bounds below include a factor 2
 // of slack for the rounding of the bound itself.
 // ---------------------------------------------------------------------------
+// A decision the FP32 filter takes from its FP32 value alone.  A host test
+// build (GDT_FILTER_FORCE_FP64) takes none: every predicate then goes to the
+// FP64 fallback, whose rows must be those of the FP64 path.
+// GDT_FILTER_FORCED(x): a condition that only such a build needs (where the
+// FP32 test would otherwise have decided it).
+#ifdef GDT_FILTER_FORCE_FP64
+#define GDT_FILTER_DECIDES(x) false
+#define GDT_FILTER_FORCED(x) (x)
+#else
+#define GDT_FILTER_DECIDES(x) (x)
+#define GDT_FILTER_FORCED(x) false
+#endif
+
 #ifndef GDT_U32     // (a host test shrinks it to show that a too-tight bound is caught)
 #define GDT_U32 5.9604644775390625e-08f
 #endif

@@ -244,12 +244,12 @@ GDT_HD inline GdtRingResult gdt_one_ring_f32(const GdtSet& S, int j, const GdtRi
             };
 #endif
             if (nn < 0) take = true;
-            else if (d2 + E < nd2 - nE) {
+            else if (GDT_FILTER_DECIDES(d2 + E < nd2 - nE)) {
                 take = true;
 #ifdef GDT_AUDIT_FILTER
                 audit_nn(-1);
 #endif
-            } else if (d2 - E > nd2 + nE) {
+            } else if (GDT_FILTER_DECIDES(d2 - E > nd2 + nE)) {
                 take = false;
 #ifdef GDT_AUDIT_FILTER
                 audit_nn(1);
@@ -329,8 +329,8 @@ GDT_HD inline GdtRingResult gdt_one_ring_f32(const GdtSet& S, int j, const GdtRi
                 if (O < -BO) audit_sc(GDT_AUD_ORIENT, -1);
 #endif
                 bool left;
-                if (O > BO) left = true;
-                else if (O < -BO) left = false;
+                if (GDT_FILTER_DECIDES(O > BO)) left = true;
+                else if (GDT_FILTER_DECIDES(O < -BO)) left = false;
                 else {
                     ++nfb;
                     double a64x, a64y, c64x, c64y;
@@ -348,8 +348,8 @@ GDT_HD inline GdtRingResult gdt_one_ring_f32(const GdtSet& S, int j, const GdtRi
                     if (P < -BP) audit_sc(GDT_AUD_INCIRCLE, -1);
                     if (P > BP) audit_sc(GDT_AUD_INCIRCLE, 1);
 #endif
-                    if (P < -BP) inside = true;
-                    else if (P > BP) inside = false;
+                    if (GDT_FILTER_DECIDES(P < -BP)) inside = true;
+                    else if (GDT_FILTER_DECIDES(P > BP)) inside = false;
                     else {
                         ++nfb;
                         double a64x, a64y, b64x, b64y, c64x, c64y;

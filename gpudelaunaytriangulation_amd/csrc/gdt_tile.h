@@ -821,8 +821,12 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
         const float O = fmaf(ax, py, -(ay * px));
         const float P = fmaf(K0, cc, fmaf(Kx, px, Ky * py));
         // one branch, taken only when the candidate may be a new best
-        if ((O >= Op) & (P <= Pp) & (k != iv) & (k != ak)) {
-            bool take = (O > Oc) & (P < Pc);
+        if (!GDT_FILTER_DECIDES(!((O >= Op) & (P <= Pp))) & (k != iv) & (k != ak)) {
+            // (the nearest-neighbour search's first candidate is always
+            // taken: there is no best to compare it with; the FP32 test
+            // takes it, a build without FP32 verdicts says so)
+            bool take = GDT_FILTER_FORCED(!wrap & (bk < 0)) |
+                        GDT_FILTER_DECIDES((O > Oc) & (P < Pc));
 #ifdef GDT_AUDIT_FILTER
             if (take) audit_fp32(k, O, P, true);
 #endif
@@ -839,11 +843,12 @@ GDT_HD inline int gdt_tile_step_f32(const GdtSet& S, const GdtTileLds& L, const 
                         bk >= 0 && fabsf(P) > BP)
                         audit(GDT_AUD_INCIRCLE, P < -BP ? -1 : 1, k);
 #endif
-                    const bool left = fabsf(O) > BO ? O > BO
-                                                    : gdt_orient64(a64x, a64y, c64x, c64y) > 0.;
+                    const bool left = GDT_FILTER_DECIDES(fabsf(O) > BO)
+                                          ? O > BO
+                                          : gdt_orient64(a64x, a64y, c64x, c64y) > 0.;
                     if (!left) take = false;
                     else if (bk < 0) take = true;
-                    else if (fabsf(P) > BP) take = P < -BP;
+                    else if (GDT_FILTER_DECIDES(fabsf(P) > BP)) take = P < -BP;
                     else {
                         double b64x, b64y;
                         x64(bk, b64x, b64y);

@@ -107,6 +107,39 @@ def test_emulated_kernels_match_oracle(emul, tmp_path, case):
             assert np.array_equal(n64[i, :c64[i]], n32[i, :c64[i]])
 
 
+@pytest.mark.parametrize("case", ["uniform4096", "lattice64", "finecells", "clustered"])
+def test_emulated_filter_forced_to_fp64(tmp_path, case):
+    """The FP32 filter taking no decision of its own (GDT_FILTER_FORCE_FP64: a
+    certain FP32 verdict is never trusted): every predicate goes to the FP64
+    fallback, and the rows equal the FP64 path's byte for byte, and the
+    oracle's, in the direct and the tile routines."""
+    exe = str(tmp_path / "emul_forced")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined",
+                    "-fno-sanitize-recover=all", "-fno-omit-frame-pointer", "-ffp-contract=off",
+                    "-DGDT_TILE_SHIFT=5", "-DGDT_WIN_MAXPTS=2048", "-DGDT_FILTER_FORCE_FP64",
+                    f"-I{CSRC}", SRC, "-o", exe], check=True)
+    gen, lx, ly, cell = CASES[case]
+    xy = gen()
+    on, oc = oracle.triangulate(xy, lx, ly, 1.0, 16)
+    ref = rings.ring_hash(on, oc)
+    for tiled, mode in ((1, 0), (1, 1), (2, 1)):
+        n64, c64, _, _, err = run(exe, str(tmp_path), xy, lx, ly, tiled, 64, mode=mode, cell=cell)
+        assert err == 0
+        n32, c32, npred, nfb, err = run(exe, str(tmp_path), xy, lx, ly, tiled, 32, mode=mode,
+                                        cell=cell)
+        assert err == 0
+        # (the rest: entries of v and a scanned by the tile step, and the
+        # nearest-neighbour search's first candidate, which no value decides;
+        # on fine cells and clusters also the deferred points' FP64-only
+        # wavefront routine, which takes all of the cluster's windows)
+        least = {"uniform4096": 0.85, "lattice64": 0.85, "finecells": 0.2, "clustered": 0.0}
+        assert nfb >= least[case] * npred, (tiled, mode, nfb, npred)
+        assert rings.ring_hash(n32, c32) == ref, (tiled, mode)
+        assert np.array_equal(c64, c32)
+        for i in range(len(c64)):
+            assert np.array_equal(n64[i, :c64[i]], n32[i, :c64[i]])
+
+
 def _flip(nb, c, v, m):
     """Rings after flipping edge (v, b = ring_v[m]) of triangles (v, a, b) and
     (v, b, d) to edge (a, d)."""
