@@ -72,8 +72,9 @@ static GdtRingResult star_ring(const GdtSet& S, const GdtTileWin& w, const GdtTi
             GdtTileLane<double> sl = st;
             if (kg[g] != st.iv) gdt_tile_set_a_host(L, sl, kg[g]);
             double bx, by;
-            lrc[g] = gdt_tile_search_f64<1>(S, L, sl, res.npred, 0, kg[g] == st.iv, bk[g], bx, by,
-                                            pre, (unsigned)tot, ph);
+            // (no lane extends its search, the point's own included: as the kernel)
+            lrc[g] = gdt_tile_search_f64<1>(S, L, sl, res.npred, 0, false, bk[g], bx, by, pre,
+                                            (unsigned)tot, ph);
         }
         auto lane_of = [&](int k) {
             for (int g = 0; g < tot; ++g)
