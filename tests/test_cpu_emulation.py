@@ -71,6 +71,14 @@ CASES = {
     "small300": (lambda: inputs.uniform_points(300, 17.0, 17.0, seed=9), 17.0, 17.0, 1.0),
     "tiny60": (lambda: inputs.uniform_points(60, 8.0, 8.0, seed=9), 8.0, 8.0, 1.0),
     "finecells": (lambda: inputs.uniform_points(2048, 45.0, 45.0, seed=5), 45.0, 45.0, 0.4),
+    # 1.44 points per cell: the star steps' 7x7 blocks mostly exceed 64
+    # entries (their 5x5 blocks), the 16x16-tile windows still fit
+    "coarsecells": (lambda: inputs.uniform_points(3000, 55.0, 55.0, seed=6), 55.0, 55.0, 1.2),
+    # a clump of 150 points in a 2x2 square: blocks beyond 64 entries even
+    # at 5x5 (the star steps' sequential lane), in windows that still fit
+    "bump": (lambda: np.concatenate([inputs.uniform_points(4096, 64.0, 64.0, seed=12),
+                                     20.0 + 2.0 * inputs.uniform_points(150, 1.0, 1.0, seed=13)]),
+             64.0, 64.0, 1.0),
     "clustered": (clustered, 64.0, 64.0, 1.0),
 }
 

@@ -5,7 +5,8 @@ it serves and at 2^18, a rectangular box, both tile edges, a cluster whose
 blocks hold more than 64 entries and whose windows overflow (sequential steps
 and deferral), a jittered lattice (near-cocircular quads: uncertified
 successors), exact square lattices (exact ties, decided by the symbolic
-perturbation after deferral) and batches of a few small sets."""
+perturbation after deferral), denser cells and a clump (the lanes' 5x5
+blocks and lane 0's sequential steps) and batches of a few small sets."""
 import numpy as np
 import pytest
 
@@ -65,6 +66,24 @@ def test_star_cluster_runs_sequential_steps_and_defers():
         if ts == 4:
             assert dg.stats()["deferred"] > 0
         assert rings.ring_hash(nb, cnt) == rings.ring_hash(on, oc)
+
+
+@pytest.mark.parametrize("case", ["coarse", "bump"])
+def test_star_large_blocks_equal_oracle(case):
+    """coarse: 1.44 points per cell (cell 1.2), the lanes' 7x7 blocks mostly
+    exceed 64 entries (their 5x5 blocks); bump: a clump of 300 points in a 2x2
+    square, where even the 5x5 blocks do (the sequential steps in lane 0);
+    16x16-cell tiles, whose windows still fit."""
+    n, L = 16384, 128.0
+    xy = inputs.uniform_points(n, L, L, seed=9)
+    cell = 1.2 if case == "coarse" else 1.0
+    if case == "bump":
+        xy = np.concatenate([xy, 40.0 + 2.0 * inputs.uniform_points(300, 1.0, 1.0, seed=10)])
+    print(f"[star] large blocks: {case}", flush=True)
+    nb, cnt, dg = triangulate(xy, L, L, PREC_FP64, ts=4, cell=cell)
+    assert dg.stats()["lanes"] == STAR
+    on, oc = oracle.triangulate(xy, L, L, 1.0, 16)
+    assert rings.ring_hash(nb, cnt) == rings.ring_hash(on, oc)
 
 
 def test_star_jittered_lattice_equals_oracle():
