@@ -7,8 +7,9 @@ autograd.Function.
 Workloads: 65,536 uniform points in a 256x256 box, and 64 sets of 4,096 points
 in 64x64 boxes (batched).  Tolerances: FP64 relative 1e-11 on areas,
 perimeters and energies, 1e-9 on gradients (scaled by the largest gradient
-entry); FP32 path relative 2e-5 on areas and perimeters, 1e-5 on energies,
-2e-3 on gradients (scaled the same way)."""
+entry); FP32 path relative 5e-6 on areas and perimeters, 1e-7 on energies,
+1e-4 on gradients (scaled the same way; measured at 65,536 points: 4.0e-7,
+1.3e-9 and 7.1e-6)."""
 import numpy as np
 import pytest
 
@@ -22,7 +23,7 @@ from gpudelaunaytriangulation_amd.delaunay import Context  # noqa: E402
 from gpudelaunaytriangulation_amd.voronoi import (voronoi_adjoint, voronoi_energy,  # noqa: E402
                                                   voronoi_forward)
 
-TOL = {PREC_FP64: dict(ap=1e-11, e=1e-11, g=1e-9), PREC_FP32F: dict(ap=2e-5, e=1e-5, g=2e-3)}
+TOL = {PREC_FP64: dict(ap=1e-11, e=1e-11, g=1e-9), PREC_FP32F: dict(ap=5e-6, e=1e-7, g=1e-4)}
 
 
 @pytest.fixture(scope="module")
