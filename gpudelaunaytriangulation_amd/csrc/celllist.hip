@@ -21,6 +21,8 @@
 //      the slot arrays (perm, slot_of, off64, off32, scell) of its key range
 //      in slot order; a bucket above KB_CAP points (dense clusters) takes the
 //      same steps through global memory.
+// Up to 8,192 points over at most 64 buckets the four passes run as one
+// launch (k_kb_small), which writes the same slots.
 // More than 2^25 keys (beyond the LDS histograms) take the single-level path:
 // per-point atomic arrival ranks, scan, scatter, per-cell sort.
 #include "gdt_kernels.h"
@@ -644,6 +646,97 @@ __global__ __launch_bounds__(KB_THREADS) void k_kb_finalize(KbPlan pl, int total
     }
 }
 
+// Small inputs (at most KB_SMALL_N points over at most KB_SMALL_NKB buckets of
+// 2^10 keys): the four passes in one launch, one block per bucket.  Each block
+// bins every point (8 per thread at most, read from L2), counts those of the
+// buckets before its own (its first slot), counting-sorts its own by cell in
+// LDS, orders each cell by caller index and writes cstart/ccount and the slot
+// arrays: the slots of the four-pass build, without its three launch gaps.
+// Measured per call (4,096 / 8,192 / 16,384 points): 0.077 / 0.097 / 0.143 ms
+// against 0.083 / 0.097 / 0.126 ms with the four passes, hence the limit.
+#define KB_SMALL_N 8192
+#define KB_SMALL_NKB 64
+#define KB_SMALL_B 10
+#define KB_SMALL_PER (KB_SMALL_N / KB_THREADS)
+
+__global__ __launch_bounds__(KB_THREADS) void k_kb_small(const GdtSet* __restrict__ sets, int nsets,
+                                                         const gdt_d2* __restrict__ xy, int n,
+                                                         int total_keys, GdtCells c,
+                                                         int* __restrict__ st_zero,
+                                                         unsigned long long* __restrict__ ct_zero)
+{
+    if (st_zero && blockIdx.x == 0) {
+        if (threadIdx.x < ST_WORDS) st_zero[threadIdx.x] = 0;
+        if (threadIdx.x < CT_WORDS) ct_zero[threadIdx.x] = 0ull;
+    }
+    if (c.halt && *c.halt) return;
+    __shared__ uint32_t cst[(1 << KB_SMALL_B) + 1];      // cell counts, then starts
+    __shared__ int idxs[KB_SMALL_N];                     // caller indices by position
+    __shared__ uint32_t wsum[KB_THREADS / 64];
+    __shared__ uint32_t bsum[KB_THREADS / 64];
+    const int tid = threadIdx.x, bin = blockIdx.x;
+    const uint32_t k0 = (uint32_t)bin << KB_SMALL_B;
+    const int nk = min(1 << KB_SMALL_B, total_keys - (int)k0);
+    if (bin == (int)gridDim.x - 1 && tid == 0) c.cstart[total_keys] = (uint32_t)n;
+    for (int q = tid; q <= nk; q += KB_THREADS) cst[q] = 0u;
+    __syncthreads();
+    // pk[u]: (cell in the bucket << 16) | arrival rank in the cell, or ~0
+    uint32_t pk[KB_SMALL_PER];
+    uint32_t below = 0;
+    KbSetCursor cur;
+    if (tid < n) cur.start(sets, nsets, tid);
+#pragma unroll
+    for (int u0 = 0; u0 < KB_SMALL_PER; u0 += KB_BATCH) {
+        gdt_d2 p[KB_BATCH];
+#pragma unroll
+        for (int u = 0; u < KB_BATCH; ++u) p[u] = xy[min(tid + (u0 + u) * KB_THREADS, n - 1)];
+#pragma unroll
+        for (int u = 0; u < KB_BATCH; ++u) {
+            const int i = tid + (u0 + u) * KB_THREADS;
+            pk[u0 + u] = ~0u;
+            if (i < n) {
+                cur.advance(i);
+                uint32_t key, packed;
+                double ox, oy;
+                kb_bin_in(cur.S, p[u], key, packed, ox, oy);
+                const uint32_t kb = key >> KB_SMALL_B;
+                if (kb < (uint32_t)bin) ++below;
+                else if (kb == (uint32_t)bin)
+                    pk[u0 + u] = ((key - k0) << 16) | atomicAdd(&cst[key - k0], 1u);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) below += __shfl_xor(below, o, 64);
+    if ((tid & 63) == 0) bsum[tid >> 6] = below;
+    __syncthreads();
+    uint32_t base = 0;
+#pragma unroll
+    for (int w = 0; w < KB_THREADS / 64; ++w) base += bsum[w];
+    kb_scan_lds(cst, nk, wsum);
+    kb_write_starts(cst, nk, k0, base, c.cstart, c.ccount);
+#pragma unroll
+    for (int u = 0; u < KB_SMALL_PER; ++u)
+        if (pk[u] != ~0u) idxs[cst[pk[u] >> 16] + (pk[u] & 0xffffu)] = tid + u * KB_THREADS;
+    __syncthreads();
+    if (tid < n) cur.start(sets, nsets, tid);
+#pragma unroll
+    for (int u = 0; u < KB_SMALL_PER; ++u) {
+        if (pk[u] == ~0u) continue;
+        const int i = tid + u * KB_THREADS;
+        // rank by caller index inside the cell
+        const uint32_t c0 = cst[pk[u] >> 16], c1 = cst[(pk[u] >> 16) + 1];
+        uint32_t r = 0;
+        for (uint32_t t = c0; t < c1; ++t) r += (uint32_t)(idxs[t] < i);
+        cur.advance(i);
+        GdtKbRec rec;
+        kb_bin_in(cur.S, xy[i], rec.key, rec.cell, rec.off.x, rec.off.y);
+        rec.idx = i;
+        rec.pad = 0u;
+        kb_write_slot(c, base + c0 + r, rec);
+    }
+}
+
 static hipError_t build_cells_single_level(const GdtSet* d_sets, const GdtChunk* pchunks, int npch,
                                            const GdtChunk* kchunks, int nkch, int total_keys,
                                            const gdt_d2* xy, GdtCells& c, hipStream_t s)
@@ -676,6 +769,11 @@ hipError_t gdt_build_cells(const GdtSet* d_sets, int nsets, const GdtChunk* pchu
             if (e != hipSuccess) return e;
         }
         return build_cells_single_level(d_sets, pchunks, npch, kchunks, nkch, total_keys, xy, c, s);
+    }
+    if (pl.B == KB_SMALL_B && pl.nkb <= KB_SMALL_NKB && total_n >= 1 && total_n <= KB_SMALL_N) {
+        hipLaunchKernelGGL(k_kb_small, dim3(pl.nkb), dim3(KB_THREADS), 0, s, d_sets, nsets, xy,
+                           total_n, total_keys, c, status, counters);
+        return hipGetLastError();
     }
     hipLaunchKernelGGL(k_kb_count, dim3(pl.nb), dim3(KB_THREADS), pl.nkb * sizeof(uint32_t), s,
                        d_sets, nsets, xy, total_n, pl, c.bmat, c.halt, status, counters);
