@@ -21,7 +21,7 @@
 //      the slot arrays (perm, slot_of, off64, off32, scell) of its key range
 //      in slot order; a bucket above KB_CAP points (dense clusters) takes the
 //      same steps through global memory.
-// Up to 8,192 points over at most 64 buckets the four passes run as one
+// Up to 16,384 points over at most 64 buckets the four passes run as one
 // launch (k_kb_small), which writes the same slots.
 // More than 2^25 keys (beyond the LDS histograms) take the single-level path:
 // per-point atomic arrival ranks, scan, scatter, per-cell sort.
@@ -648,13 +648,14 @@ __global__ __launch_bounds__(KB_THREADS) void k_kb_finalize(KbPlan pl, int total
 
 // Small inputs (at most KB_SMALL_N points over at most KB_SMALL_NKB buckets of
 // 2^10 keys): the four passes in one launch, one block per bucket.  Each block
-// bins every point (8 per thread at most, read from L2), counts those of the
+// bins every point (16 per thread at most, read from L2), counts those of the
 // buckets before its own (its first slot), counting-sorts its own by cell in
 // LDS, orders each cell by caller index and writes cstart/ccount and the slot
 // arrays: the slots of the four-pass build, without its three launch gaps.
-// Measured per call (4,096 / 8,192 / 16,384 points): 0.077 / 0.097 / 0.143 ms
-// against 0.083 / 0.097 / 0.126 ms with the four passes, hence the limit.
-#define KB_SMALL_N 8192
+// Measured per call (4,096 / 8,192 / 12,288 / 16,384 points): 0.071 / 0.089 /
+// 0.114 / 0.122 ms against 0.083 / 0.099 / 0.122 / 0.127 ms with the four
+// passes.
+#define KB_SMALL_N 16384
 #define KB_SMALL_NKB 64
 #define KB_SMALL_B 10
 #define KB_SMALL_PER (KB_SMALL_N / KB_THREADS)
@@ -672,6 +673,7 @@ __global__ __launch_bounds__(KB_THREADS) void k_kb_small(const GdtSet* __restric
     if (c.halt && *c.halt) return;
     __shared__ uint32_t cst[(1 << KB_SMALL_B) + 1];      // cell counts, then starts
     __shared__ int idxs[KB_SMALL_N];                     // caller indices by position
+    __shared__ uint16_t pcell[KB_SMALL_N];               // their cells in the bucket
     __shared__ uint32_t wsum[KB_THREADS / 64];
     __shared__ uint32_t bsum[KB_THREADS / 64];
     const int tid = threadIdx.x, bin = blockIdx.x;
@@ -717,20 +719,22 @@ __global__ __launch_bounds__(KB_THREADS) void k_kb_small(const GdtSet* __restric
     kb_write_starts(cst, nk, k0, base, c.cstart, c.ccount);
 #pragma unroll
     for (int u = 0; u < KB_SMALL_PER; ++u)
-        if (pk[u] != ~0u) idxs[cst[pk[u] >> 16] + (pk[u] & 0xffffu)] = tid + u * KB_THREADS;
+        if (pk[u] != ~0u) {
+            const uint32_t pos = cst[pk[u] >> 16] + (pk[u] & 0xffffu);
+            idxs[pos] = tid + u * KB_THREADS;
+            pcell[pos] = (uint16_t)(pk[u] >> 16);
+        }
     __syncthreads();
-    if (tid < n) cur.start(sets, nsets, tid);
-#pragma unroll
-    for (int u = 0; u < KB_SMALL_PER; ++u) {
-        if (pk[u] == ~0u) continue;
-        const int i = tid + u * KB_THREADS;
-        // rank by caller index inside the cell
-        const uint32_t c0 = cst[pk[u] >> 16], c1 = cst[(pk[u] >> 16) + 1];
+    // the bucket's points by position (about one per thread), each ranked by
+    // caller index inside its cell and binned again for its slot record
+    const int cnt = (int)cst[nk];
+    for (int t = tid; t < cnt; t += KB_THREADS) {
+        const int i = idxs[t];
+        const uint32_t c0 = cst[pcell[t]], c1 = cst[pcell[t] + 1];
         uint32_t r = 0;
-        for (uint32_t t = c0; t < c1; ++t) r += (uint32_t)(idxs[t] < i);
-        cur.advance(i);
+        for (uint32_t q = c0; q < c1; ++q) r += (uint32_t)(idxs[q] < i);
         GdtKbRec rec;
-        kb_bin_in(cur.S, xy[i], rec.key, rec.cell, rec.off.x, rec.off.y);
+        kb_bin_in(sets[gdt_set_of(sets, nsets, i)], xy[i], rec.key, rec.cell, rec.off.x, rec.off.y);
         rec.idx = i;
         rec.pad = 0u;
         kb_write_slot(c, base + c0 + r, rec);

@@ -16,10 +16,11 @@ from gpudelaunaytriangulation_amd.delaunay import CellListGPU, PeriodicBoundarie
 
 
 @pytest.mark.parametrize("n,L,cell,clump", [(4096, 64.0, 1.0, 0), (4096, 64.0, 0.25, 0),
-                                           (8192, 64.0, 1.0, 2000), (65536, 256.0, 1.0, 0),
+                                           (8192, 64.0, 1.0, 2000), (16384, 128.0, 1.0, 3000),
+                                           (65536, 256.0, 1.0, 0),
                                            (65536, 256.0, 1.0, 5000)])
 def test_cell_list_export_matches_reference_binning(n, L, cell, clump):
-    """Up to 8,192 points the build is one launch (one block per 1,024 keys),
+    """Up to 16,384 points the build is one launch (one block per 1,024 keys),
     above it four passes; cell 0.25 gives the one-launch build its most
     blocks (64), a clump puts hundreds of points in a cell."""
     print(f"[cells] n {n} box {L} cell {cell} clump {clump}", flush=True)
