@@ -688,14 +688,16 @@ __global__ __launch_bounds__(KB_THREADS) void k_kb_small(const GdtSet* __restric
     KbSetCursor cur;
     if (tid < n) cur.start(sets, nsets, tid);
 #pragma unroll
+    for (int u = 0; u < KB_SMALL_PER; ++u) pk[u] = ~0u;
+#pragma unroll
     for (int u0 = 0; u0 < KB_SMALL_PER; u0 += KB_BATCH) {
+        if (tid + u0 * KB_THREADS >= n) break;            // no batch of clamped reads
         gdt_d2 p[KB_BATCH];
 #pragma unroll
         for (int u = 0; u < KB_BATCH; ++u) p[u] = xy[min(tid + (u0 + u) * KB_THREADS, n - 1)];
 #pragma unroll
         for (int u = 0; u < KB_BATCH; ++u) {
             const int i = tid + (u0 + u) * KB_THREADS;
-            pk[u0 + u] = ~0u;
             if (i < n) {
                 cur.advance(i);
                 uint32_t key, packed;
